@@ -1,0 +1,802 @@
+// Host side of libgasx.so: context, table preparation, the C ABI entry points
+// and the fused transport loop.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "gasx_internal.h"
+
+using namespace gasx;
+
+struct gasx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    gasx_params p{};
+    Geom geom{};
+    FdtdArgs fa{};
+    HostTables ht{};
+    DevTables* d_tab = nullptr;
+    float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies
+    double *er_b = nullptr, *ez_b = nullptr, *h_b = nullptr;      // FP64 ping-pong partner
+    unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buf][r/z]
+    double *jr_tmp = nullptr, *jz_tmp = nullptr;
+    unsigned long long* d_counters = nullptr;
+    // append staging
+    int64_t stage_slots = 0;
+    float* stage[6] = {};
+    int32_t* stage_cnt = nullptr;
+    float* istage[6] = {};
+    int32_t* istage_cnt = nullptr;
+    int32_t* scan_tmp = nullptr;
+    int32_t* base_tmp = nullptr;
+    // sort scratch
+    void* sort_tmp = nullptr;
+    size_t sort_bytes = 0;
+    double* d_tally = nullptr;
+    int cu_count = 256;
+    std::string err;
+};
+
+namespace {
+
+gasx_status fail(gasx_ctx* c, gasx_status s, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+gasx_status hip_fail(gasx_ctx* c, hipError_t e, const char* where)
+{
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    if (c) c->err = m;
+    return e == hipErrorOutOfMemory ? GASX_E_NOMEM : GASX_E_HIP;
+}
+
+#define HIPCHK(expr, where)                                   \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, where); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t n)
+{
+    return hipMalloc((void**)p, (n > 0 ? n : 1) * sizeof(T));
+}
+
+double interp1_nan(const double* x, const double* y, int n, double xq)
+{
+    if (!(xq >= x[0]) || !(xq <= x[n - 1])) return NAN;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (x[mid] <= xq) lo = mid; else hi = mid;
+    }
+    double s = (xq - x[lo]) / (x[hi] - x[lo]);
+    return y[lo] + s * (y[hi] - y[lo]);
+}
+
+bool valid_species(const gasx_species* s)
+{
+    return s && s->x && s->y && s->z && s->vx && s->vy && s->vz && s->count && s->capacity > 0 &&
+           s->capacity < (int64_t)INT32_MAX - kChunk && s->mass > 0.0;
+}
+
+bool valid_fields(const gasx_fields* f) { return f && f->er && f->ez && f->hphi; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Tables: electron_collision.m:12-41 merges and the find_CS0.m units.
+// ---------------------------------------------------------------------------
+int gasx::prepare_tables(const gasx_raw_tables* raw, HostTables* h, DevTables* d)
+{
+    const double sqr_a0 = 2.8002852e-21;   // electron_collision.m:12, find_CS0.m:17
+    if (!raw || !raw->t_el || !raw->t_sel || !raw->t_sel1 || !raw->t_del || !raw->t_sdel ||
+        !raw->t_ion || !raw->t_exc)
+        return 1;
+    const int n_small = raw->n_sel1 - 1;                     // electron_collision.m:17
+    const int n_dcs = (raw->w_sdel - 1) + (raw->w_del - 1);  // electron_collision.m:23-25
+    if (raw->n_el < 2 || raw->n_el > kMaxTab || raw->n_sel < 2 || raw->n_sel > kMaxTab ||
+        raw->n_ion < 2 || raw->n_ion > kMaxTab || raw->n_exc < 1 || raw->n_exc > kMaxExc ||
+        n_small < 0 || n_dcs < 2 || n_dcs > kMaxTab || n_small + raw->n_el != n_dcs)
+        return 2;
+    *h = HostTables{};
+    h->n_el = raw->n_el;
+    for (int i = 0; i < raw->n_el; ++i) {
+        h->el_e[i] = raw->t_el[i];
+        h->el_s[i] = raw->t_el[raw->n_el + i];
+    }
+    h->n_sel = raw->n_sel;
+    for (int i = 0; i < raw->n_sel; ++i) {
+        h->sel_e[i] = raw->t_sel[2 * i];
+        h->sel_s[i] = raw->t_sel[2 * i + 1];
+    }
+    h->n_ion = raw->n_ion;
+    for (int i = 0; i < raw->n_ion; ++i) {
+        h->ion_e[i] = raw->t_ion[2 * i];
+        h->ion_s[i] = raw->t_ion[2 * i + 1];
+    }
+    h->n_exc = raw->n_exc;
+    for (int i = 0; i < raw->n_exc; ++i) {   // columns j, A, omega, v, miu
+        h->exc_w[i] = raw->t_exc[5 * i + 0];
+        h->exc_a[i] = raw->t_exc[5 * i + 1];
+        h->exc_om[i] = raw->t_exc[5 * i + 2];
+        h->exc_nu[i] = raw->t_exc[5 * i + 3];
+        h->exc_ga[i] = raw->t_exc[5 * i + 4];
+        if (!(h->exc_w[i] > 0)) return 3;
+    }
+    h->n_dcs = n_dcs;
+    for (int i = 0; i < n_small; ++i) h->dcs_tcs[i] = raw->t_sel1[2 * i + 1] * 1e-20 / sqr_a0 / 2.0;
+    for (int i = 0; i < raw->n_el; ++i) h->dcs_tcs[n_small + i] = h->el_s[i];
+    for (int j = 0; j < n_dcs; ++j) {
+        const double* tab;
+        int w, col;
+        if (j < raw->w_sdel - 1) { tab = raw->t_sdel; w = raw->w_sdel; col = j + 1; }
+        else { tab = raw->t_del; w = raw->w_del; col = j - (raw->w_sdel - 1) + 1; }
+        h->dcs_e[j] = tab[col];
+        if (!(tab[col] > 0)) return 4;
+        h->dcs_loge[j] = std::log(tab[col]);
+        for (int k = 0; k < kNAng; ++k) {
+            double v = tab[(k + 1) * w + col];
+            if (!(v > 0)) return 5;
+            h->dcs_logv[j * kNAng + k] = std::log(v);
+        }
+    }
+    for (int j = 1; j < n_dcs; ++j)
+        if (!(h->dcs_loge[j] > h->dcs_loge[j - 1])) return 6;
+
+    // device image: energies in eV, sigma in m^2
+    std::memset(d, 0, sizeof(*d));
+    d->n_el = h->n_el; d->n_sel = h->n_sel; d->n_ion = h->n_ion; d->n_exc = h->n_exc; d->n_dcs = n_dcs;
+    for (int i = 0; i < h->n_el; ++i) {
+        d->el_e[i] = (float)(h->el_e[i] * 1000.0);
+        d->el_s[i] = (float)(h->el_s[i] * sqr_a0 * 2.0);     // find_CS0.m:21
+    }
+    for (int i = 0; i < h->n_sel; ++i) {
+        d->sel_e[i] = (float)h->sel_e[i];
+        d->sel_s[i] = (float)(h->sel_s[i] * 1e-20);           // find_CS0.m:25
+    }
+    for (int i = 0; i < h->n_ion; ++i) {
+        d->ion_e[i] = (float)h->ion_e[i];
+        d->ion_s[i] = (float)(h->ion_s[i] * 1e-20);           // find_CS0.m:33
+    }
+    const double q0 = 6.542e-14;                              // find_CS0.m:42
+    for (int i = 0; i < h->n_exc; ++i) {
+        d->exc_w[i] = (float)h->exc_w[i];
+        d->exc_q[i] = (float)(q0 * h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]) * 1e-4);
+        d->exc_a[i] = (float)(h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]));
+        d->exc_om[i] = (float)h->exc_om[i];
+        d->exc_nu[i] = (float)h->exc_nu[i];
+        d->exc_ga[i] = (float)h->exc_ga[i];
+    }
+    for (int j = 0; j < n_dcs; ++j) {
+        d->dcs_loge[j] = (float)h->dcs_loge[j];
+        for (int k = 0; k < kNAng; ++k) d->dcs_logv[j * kNAng + k] = (float)h->dcs_logv[j * kNAng + k];
+    }
+    return 0;
+}
+
+// sigma_T(E) * v(E) maximised over the collision window (find_CS0.m forms,
+// FP64) on a dense log grid plus every table node.  Used for the exact
+// rejection bound p_T <= dt * n_max * max(v sigma_T).
+double gasx::max_rate_coefficient(const HostTables& h, const gasx_params& p)
+{
+    auto sig_t = [&](double T) {
+        double se = T >= 50.0 ? interp1_nan(h.el_e, h.el_s, h.n_el, T / 1000.0) * 2.8002852e-21 * 2.0
+                              : interp1_nan(h.sel_e, h.sel_s, h.n_sel, T) * 1e-20;
+        double si = interp1_nan(h.ion_e, h.ion_s, h.n_ion, T) * 1e-20;
+        double sx = 0.0;
+        for (int i = 0; i < h.n_exc; ++i) {
+            double a = 6.542e-14 * h.exc_a[i] / (h.exc_w[i] * h.exc_w[i]);
+            double b = std::pow(h.exc_w[i] / T, h.exc_om[i]);
+            double c = std::pow(1.0 - std::pow(h.exc_w[i] / T, h.exc_ga[i]), h.exc_nu[i]);
+            sx += a * b * c;
+        }
+        return se + si + sx * 1e-4;
+    };
+    auto speed = [&](double T) {   // push-side electron (EM4.cu KE with field6 constants)
+        double g1 = T * p.e_em / (p.me_em * p.c * p.c);
+        return p.c * std::sqrt(g1 * (g1 + 2.0)) / (1.0 + g1);
+    };
+    double best = 0.0;
+    auto visit = [&](double T) {
+        if (T < 15.6 || T > 20000.0) return;
+        double v = speed(T) * sig_t(T);
+        if (v > best) best = v;
+    };
+    const int N = 200000;
+    for (int i = 0; i <= N; ++i) visit(15.6 * std::pow(20000.0 / 15.6, (double)i / N));
+    for (int i = 0; i < h.n_el; ++i) visit(h.el_e[i] * 1000.0);
+    for (int i = 0; i < h.n_sel; ++i) visit(h.sel_e[i]);
+    for (int i = 0; i < h.n_ion; ++i) visit(h.ion_e[i]);
+    return best;
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+extern "C" int gasx_abi_version(void) { return GASX_ABI_VERSION; }
+
+extern "C" const char* gasx_status_string(gasx_status s)
+{
+    switch (s) {
+    case GASX_OK: return "ok";
+    case GASX_E_ARG: return "invalid argument";
+    case GASX_E_CAPACITY: return "capacity exceeded";
+    case GASX_E_HIP: return "HIP runtime error";
+    case GASX_E_NOMEM: return "out of device memory";
+    case GASX_E_STATE: return "invalid call sequence";
+    case GASX_E_NUMERIC: return "numeric range error";
+    }
+    return "unknown status";
+}
+
+extern "C" const char* gasx_last_error(const gasx_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+static void fill_geometry(gasx_ctx* c)
+{
+    const gasx_params& p = c->p;
+    Geom& g = c->geom;
+    g.nr = p.nr; g.nz = p.nz;
+    g.dr = (float)p.dr; g.dz = (float)p.dz;
+    g.inv_dr = (float)(1.0 / p.dr); g.inv_dz = (float)(1.0 / p.dz);
+    g.rp = (float)(p.nr * p.dr); g.lp = (float)(p.nz * p.dz);
+    g.phip = (float)p.phip; g.cphip = (float)std::cos(p.phip); g.sphip = (float)std::sin(p.phip);
+    g.dt = (float)p.ddt;
+    g.c = (float)p.c; g.inv_c2 = (float)(1.0 / (p.c * p.c)); g.mu0 = (float)p.mu0;
+
+    FdtdArgs& a = c->fa;
+    a.nr = p.nr; a.nz = p.nz;
+    a.n_sub = p.n_sub;
+    a.halo = p.n_sub + 3;
+    a.dtf = p.ddt / p.n_sub;
+    a.ch_r = a.dtf / (p.mu0 * p.dr);
+    a.ch_z = a.dtf / (p.mu0 * p.dz);
+    a.ce = a.dtf / p.eps0;
+    a.cez = a.dtf / (p.eps0 * p.dz);
+    a.c4 = 4 * a.dtf / (p.eps0 * p.dr);
+    a.dr = p.dr;
+    a.ez_metric = p.ez_metric;
+    a.axis_sign = p.axis_j_sign >= 0 ? 1 : -1;
+    a.junit = p.fn * p.e_em * p.c / 4294967296.0;
+    a.vcoef = p.phip / 2 * p.dr * p.dr * p.dz;
+    // default tiles: 16 x 32 (r x z), shrunk for tiny grids
+    a.ti = std::min(16, p.nr);
+    a.tk = std::min(32, p.nz);
+    a.ntr = (p.nr + a.ti - 1) / a.ti;
+    a.ntk = (p.nz + a.tk - 1) / a.tk;
+}
+
+extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const gasx_raw_tables* t,
+                                       gasx_ctx** out)
+{
+    if (!out) return GASX_E_ARG;
+    *out = nullptr;
+    if (!p || !t) return GASX_E_ARG;
+    if (device < 0) return GASX_E_ARG;   // the library has no CPU backend
+    if (p->nr < 2 || p->nz < 2 || !(p->dr > 0) || !(p->dz > 0) || !(p->phip > 0) ||
+        !(p->phip < 3.14159265358979 / 2) || !(p->ddt > 0) || p->n_sub < 1 || p->n_sub > 64 ||
+        !(p->c > 0) || !(p->eps0 > 0) || !(p->mu0 > 0) || !(p->e_em > 0) || !(p->me_em > 0) ||
+        !(p->e_mc > 0) || !(p->me_mc > 0) || !(p->m_ion > 0) || !(p->k_b > 0) ||
+        (int64_t)p->nr * p->nz > (int64_t)1 << 28)
+        return GASX_E_ARG;
+    gasx_ctx* ctx = new (std::nothrow) gasx_ctx();
+    if (!ctx) return GASX_E_NOMEM;
+    ctx->device = device;
+    ctx->p = *p;
+    DevTables dt;
+    if (prepare_tables(t, &ctx->ht, &dt) != 0) {
+        delete ctx;
+        return GASX_E_ARG;
+    }
+    fill_geometry(ctx);
+    gasx_status st = GASX_OK;
+    auto chk = [&](hipError_t e, const char* w) {
+        if (e != hipSuccess && st == GASX_OK) st = hip_fail(ctx, e, w);
+    };
+    chk(hipSetDevice(device), "hipSetDevice");
+    if (st == GASX_OK) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+        chk(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        ctx->own_stream = true;
+        const size_t nh = (size_t)p->nr * p->nz, ner = (size_t)p->nr * (p->nz + 1),
+                     nez = (size_t)(p->nr + 1) * p->nz;
+        chk(dalloc(&ctx->d_tab, 1), "alloc tables");
+        chk(dalloc(&ctx->er32, ner), "alloc er32");
+        chk(dalloc(&ctx->ez32, nez), "alloc ez32");
+        chk(dalloc(&ctx->h32, nh), "alloc h32");
+        chk(dalloc(&ctx->er_b, ner), "alloc er_b");
+        chk(dalloc(&ctx->ez_b, nez), "alloc ez_b");
+        chk(dalloc(&ctx->h_b, nh), "alloc h_b");
+        for (int b = 0; b < 2; ++b)
+            for (int k = 0; k < 2; ++k) chk(dalloc(&ctx->q[b][k], nh), "alloc q");
+        chk(dalloc(&ctx->jr_tmp, nh), "alloc jr");
+        chk(dalloc(&ctx->jz_tmp, nh), "alloc jz");
+        chk(dalloc(&ctx->d_counters, kNCounters), "alloc counters");
+        chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
+        chk(dalloc(&ctx->base_tmp, 4), "alloc base");
+        chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
+        if (st == GASX_OK) {
+            chk(hipMemcpy(ctx->d_tab, &dt, sizeof(dt), hipMemcpyHostToDevice), "upload tables");
+            chk(hipMemset(ctx->d_counters, 0, kNCounters * sizeof(unsigned long long)), "zero counters");
+            chk(hipMemset(ctx->er32, 0, ner * sizeof(float)), "zero er32");
+            chk(hipMemset(ctx->ez32, 0, nez * sizeof(float)), "zero ez32");
+            chk(hipMemset(ctx->h32, 0, nh * sizeof(float)), "zero h32");
+        }
+    }
+    if (st != GASX_OK) {
+        std::string m = ctx->err;
+        gasx_ctx_destroy(ctx);
+        return st;
+    }
+    *out = ctx;
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
+{
+    if (!ctx) return GASX_OK;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    void* bufs[] = {ctx->d_tab, ctx->er32, ctx->ez32, ctx->h32, ctx->er_b, ctx->ez_b, ctx->h_b,
+                    ctx->q[0][0], ctx->q[0][1], ctx->q[1][0], ctx->q[1][1], ctx->jr_tmp, ctx->jz_tmp,
+                    ctx->d_counters, ctx->stage_cnt, ctx->istage_cnt, ctx->scan_tmp, ctx->base_tmp,
+                    ctx->sort_tmp, ctx->d_tally};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    for (int c = 0; c < 6; ++c) {
+        if (ctx->stage[c]) hipFree(ctx->stage[c]);
+        if (ctx->istage[c]) hipFree(ctx->istage[c]);
+    }
+    if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_ctx_set_stream(gasx_ctx* ctx, void* s)
+{
+    if (!ctx) return GASX_E_ARG;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
+    ctx->own_stream = false;
+    ctx->stream = (hipStream_t)s;
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_synchronize(gasx_ctx* ctx)
+{
+    if (!ctx) return GASX_E_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    return GASX_OK;
+}
+
+// staging for appends, sized for `cap` slots
+static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
+{
+    const int64_t slots = ((cap + kChunk - 1) / kChunk) * kChunk;
+    if (slots > ctx->stage_slots || (ions && !ctx->istage[0])) {
+        const int64_t ns = std::max(slots, ctx->stage_slots);
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+        for (int c = 0; c < 6; ++c) {
+            if (ctx->stage[c]) hipFree(ctx->stage[c]);
+            if (ctx->istage[c]) hipFree(ctx->istage[c]);
+            ctx->stage[c] = nullptr;
+            ctx->istage[c] = nullptr;
+        }
+        if (ctx->stage_cnt) hipFree(ctx->stage_cnt);
+        if (ctx->istage_cnt) hipFree(ctx->istage_cnt);
+        if (ctx->scan_tmp) hipFree(ctx->scan_tmp);
+        ctx->stage_cnt = ctx->istage_cnt = ctx->scan_tmp = nullptr;
+        for (int c = 0; c < 6; ++c) {
+            HIPCHK(dalloc(&ctx->stage[c], ns), "alloc stage");
+            HIPCHK(dalloc(&ctx->istage[c], ns), "alloc ion stage");
+        }
+        HIPCHK(dalloc(&ctx->stage_cnt, ns / kChunk), "alloc stage cnt");
+        HIPCHK(dalloc(&ctx->istage_cnt, ns / kChunk), "alloc ion stage cnt");
+        HIPCHK(dalloc(&ctx->scan_tmp, ns / kChunk), "alloc scan");
+        ctx->stage_slots = ns;
+    }
+    return GASX_OK;
+}
+
+static PartArgs make_part(gasx_ctx* ctx, const gasx_species* s)
+{
+    PartArgs pa{};
+    pa.x = s->x; pa.y = s->y; pa.z = s->z; pa.vx = s->vx; pa.vy = s->vy; pa.vz = s->vz;
+    pa.count = s->count;
+    pa.cap = (int32_t)s->capacity;
+    pa.q_over_m = (float)(s->qnum * ctx->p.e_em / s->mass);
+    pa.kq = (float)(4294967296.0 / ctx->p.c) * s->qnum;
+    pa.er = ctx->er32; pa.ez = ctx->ez32; pa.h = ctx->h32;
+    for (int c = 0; c < 6; ++c) { pa.sec[c] = ctx->stage[c]; pa.ion[c] = ctx->istage[c]; }
+    pa.sec_cnt = ctx->stage_cnt;
+    pa.ion_cnt = ctx->istage_cnt;
+    pa.counters = ctx->d_counters;
+    return pa;
+}
+
+static gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, uint64_t seed,
+                             uint32_t step, uint32_t run, CollArgs* ca)
+{
+    *ca = CollArgs{};
+    if (!g) return GASX_OK;
+    if (!g->density || g->gnr < 1 || g->gnz < 1 || !(g->gdr > 0) || !(g->gdz > 0) ||
+        !(g->n_max >= 0) || !(dt_coll > 0))
+        return fail(ctx, GASX_E_ARG, "invalid gas description");
+    const gasx_params& p = ctx->p;
+    ca->tab = ctx->d_tab;
+    ca->dens = g->density;
+    ca->temp = g->temperature;
+    ca->gnr = g->gnr; ca->gnz = g->gnz;
+    ca->inv_gdr = (float)(1.0 / g->gdr); ca->inv_gdz = (float)(1.0 / g->gdz);
+    ca->t_scalar = (float)g->t_scalar;
+    ca->dt_coll = (float)dt_coll;
+    double pm = dt_coll * g->n_max * max_rate_coefficient(ctx->ht, p) * 1.02 + 1e-7;
+    ca->pmax = (float)std::min(1.0, pm);
+    ca->key0 = (uint32_t)seed; ca->key1 = (uint32_t)(seed >> 32);
+    ca->step = step; ca->run = run;
+    ca->ev_per_mc2_em = (float)(p.me_em * p.c * p.c / p.e_em);
+    ca->mc2_mc_ev = (float)(p.me_mc * p.c * p.c / p.e_mc);
+    ca->ion_vth_k = (float)std::sqrt(p.k_b / p.m_ion);
+    ca->exc_form = p.exc_form;
+    ca->rot_eps = (float)p.rot_eps;
+    return GASX_OK;
+}
+
+static int particle_blocks(gasx_ctx* ctx, int64_t cap)
+{
+    const int64_t chunks = (cap + kChunk - 1) / kChunk;
+    const int64_t persistent = (int64_t)ctx->cu_count * 16;
+    return (int)std::max<int64_t>(1, std::min(chunks, persistent));
+}
+
+// one field cycle: reads (in) writes (out) and refreshes the FP32 copies
+static gasx_status fdtd_cycle(gasx_ctx* ctx, const double* er_in, const double* ez_in,
+                              const double* h_in, double* er_out, double* ez_out, double* h_out,
+                              const unsigned long long* qr, const unsigned long long* qz,
+                              const double* jr, const double* jz, unsigned long long* qr_clear,
+                              unsigned long long* qz_clear)
+{
+    FdtdRun r{};
+    r.er_in = er_in; r.ez_in = ez_in; r.h_in = h_in;
+    r.er_out = er_out; r.ez_out = ez_out; r.h_out = h_out;
+    r.er32 = ctx->er32; r.ez32 = ctx->ez32; r.h32 = ctx->h32;
+    r.qr = qr; r.qz = qz; r.jr = jr; r.jz = jz;
+    r.qr_clear = qr_clear; r.qz_clear = qz_clear;
+    HIPCHK(launch_fdtd(ctx->fa, r, nullptr, 1, qr != nullptr, ctx->stream), "fdtd");
+    return GASX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FDTD entry points
+// ---------------------------------------------------------------------------
+extern "C" gasx_status gasx_fdtd(gasx_ctx* ctx, gasx_fields* f, const double* jr, const double* jz,
+                                 int n_push)
+{
+    if (!ctx || !valid_fields(f) || !jr || !jz || n_push < 0) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    const gasx_params& p = ctx->p;
+    const size_t nh = (size_t)p.nr * p.nz, ner = (size_t)p.nr * (p.nz + 1), nez = (size_t)(p.nr + 1) * p.nz;
+    for (int s = 0; s < n_push; ++s) {
+        const bool even = (s % 2) == 0;
+        gasx_status st = even ? fdtd_cycle(ctx, f->er, f->ez, f->hphi, ctx->er_b, ctx->ez_b, ctx->h_b,
+                                           nullptr, nullptr, jr, jz, nullptr, nullptr)
+                              : fdtd_cycle(ctx, ctx->er_b, ctx->ez_b, ctx->h_b, f->er, f->ez, f->hphi,
+                                           nullptr, nullptr, jr, jz, nullptr, nullptr);
+        if (st != GASX_OK) return st;
+    }
+    if (n_push % 2 == 1) {
+        HIPCHK(hipMemcpyAsync(f->er, ctx->er_b, ner * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+        HIPCHK(hipMemcpyAsync(f->ez, ctx->ez_b, nez * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+        HIPCHK(hipMemcpyAsync(f->hphi, ctx->h_b, nh * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    }
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_fields_sync(gasx_ctx* ctx, const gasx_fields* f)
+{
+    if (!ctx || !valid_fields(f)) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    const gasx_params& p = ctx->p;
+    HIPCHK(launch_to_f32(f->er, ctx->er32, (int64_t)p.nr * (p.nz + 1), ctx->stream), "to_f32");
+    HIPCHK(launch_to_f32(f->ez, ctx->ez32, (int64_t)(p.nr + 1) * p.nz, ctx->stream), "to_f32");
+    HIPCHK(launch_to_f32(f->hphi, ctx->h32, (int64_t)p.nr * p.nz, ctx->stream), "to_f32");
+    return GASX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Particle entry points
+// ---------------------------------------------------------------------------
+static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_species* s,
+                                 unsigned long long* qr, unsigned long long* qz, const CollArgs& ca)
+{
+    PartArgs pa = make_part(ctx, s);
+    pa.qr = qr; pa.qz = qz;
+    HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity), ctx->stream),
+           "particles");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_half_drift(gasx_ctx* ctx, gasx_species* s)
+{
+    if (!ctx || !valid_species(s)) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    CollArgs ca{};
+    return run_particles(ctx, kDrift, s, nullptr, nullptr, ca);
+}
+
+extern "C" gasx_status gasx_deposit(gasx_ctx* ctx, const gasx_species* s, int64_t* qr, int64_t* qz)
+{
+    if (!ctx || !valid_species(s) || !qr || !qz) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    CollArgs ca{};
+    return run_particles(ctx, kDeposit, s, (unsigned long long*)qr, (unsigned long long*)qz, ca);
+}
+
+extern "C" gasx_status gasx_current_from_fixed(gasx_ctx* ctx, const int64_t* qr, const int64_t* qz,
+                                               double* jr, double* jz)
+{
+    if (!ctx || !qr || !qz || !jr || !jz) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(launch_j_from_fixed(ctx->fa, (const unsigned long long*)qr, (const unsigned long long*)qz,
+                               jr, jz, ctx->stream),
+           "j_from_fixed");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_push(gasx_ctx* ctx, gasx_species* s, const gasx_fields* f)
+{
+    if (!ctx || !valid_species(s)) return GASX_E_ARG;
+    (void)f;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    CollArgs ca{};
+    return run_particles(ctx, kPush, s, nullptr, nullptr, ca);
+}
+
+extern "C" gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, int64_t n, float* se,
+                                           float* si, float* sx)
+{
+    if (!ctx || n < 0 || (n > 0 && (!e_kev || !se || !si || !sx))) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(launch_cross_sections(ctx->d_tab, e_kev, n, se, si, sx, ctx->stream), "cross_sections");
+    return GASX_OK;
+}
+
+// Appends what the last particle kernel over `src` staged.  Ions must be
+// appended before the electrons' own secondaries (the electron count fixes
+// the number of staged chunks).
+static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion)
+{
+    float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
+    const int32_t max_chunks = (int32_t)(ctx->stage_slots / kChunk);
+    HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? ctx->istage : ctx->stage,
+                         ion ? ctx->istage_cnt : ctx->stage_cnt, src->count, ctx->scan_tmp,
+                         ctx->base_tmp + (ion ? 2 : 0), max_chunks,
+                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->stream),
+           "append");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species* ions,
+                                    const gasx_gas* g, double dt_coll, uint64_t seed, uint32_t step,
+                                    uint32_t run)
+{
+    if (!ctx || !valid_species(e) || !g || (ions && !valid_species(ions))) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    gasx_status st = ensure_staging(ctx, e->capacity, ions != nullptr);
+    if (st != GASX_OK) return st;
+    CollArgs ca;
+    st = make_coll(ctx, g, dt_coll, seed, step, run, &ca);
+    if (st != GASX_OK) return st;
+    uint32_t flags = kCollide | kStageSec | (ions ? kStageIon : 0u);
+    st = run_particles(ctx, flags, e, nullptr, nullptr, ca);
+    if (st != GASX_OK) return st;
+    if (ions) {   // ions first: they read the same per-chunk ranks as the electrons
+        st = append_from_stage(ctx, e, ions, true);
+        if (st != GASX_OK) return st;
+    }
+    return append_from_stage(ctx, e, e, false);
+}
+
+extern "C" gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, double* pmax)
+{
+    if (!ctx || !g || !pmax) return GASX_E_ARG;
+    CollArgs ca;
+    gasx_status st = make_coll(ctx, g, dt_coll, 0, 0, 0, &ca);
+    if (st != GASX_OK) return st;
+    *pmax = ca.pmax;
+    return GASX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused loop shared by gasx_em_step (several pushed species, no collisions)
+// and gasx_run (electrons with collisions, optional appended ions).
+// ---------------------------------------------------------------------------
+struct LoopOpts {
+    int n_steps = 0;
+    bool collide = false, secondaries = false, ions = false;
+    int sort_every = 0;
+    uint64_t seed = 0;
+    uint32_t run_id = 0, step0 = 0;
+};
+
+static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_species* ions,
+                              gasx_fields* f, const gasx_gas* g, const LoopOpts& o)
+{
+    const gasx_params& p = ctx->p;
+    const size_t nh = (size_t)p.nr * p.nz, ner = (size_t)p.nr * (p.nz + 1), nez = (size_t)(p.nr + 1) * p.nz;
+    if (o.n_steps <= 0) return GASX_OK;
+    CollArgs ca{};
+    if (o.collide) {
+        gasx_status st = make_coll(ctx, g, p.ddt, o.seed, o.step0, o.run_id, &ca);
+        if (st != GASX_OK) return st;
+    }
+    if (o.collide) {
+        gasx_status st = ensure_staging(ctx, sp[0].capacity, o.ions);
+        if (st != GASX_OK) return st;
+    }
+    HIPCHK(launch_fill_u64(ctx->q[0][0], (int64_t)nh, ctx->stream), "zero q");
+    HIPCHK(launch_fill_u64(ctx->q[0][1], (int64_t)nh, ctx->stream), "zero q");
+    // prologue: EM2 half drift + EM7 deposit of every species (electricmagneticfield6.m:141-144)
+    for (int k = 0; k < nsp; ++k) {
+        gasx_status st = run_particles(ctx, kDrift | kDeposit, &sp[k], ctx->q[0][0], ctx->q[0][1], ca);
+        if (st != GASX_OK) return st;
+    }
+    for (int s = 0; s < o.n_steps; ++s) {
+        const int cur = s & 1, nxt = cur ^ 1;
+        const bool last = s == o.n_steps - 1;
+        gasx_status st;
+        if ((s % 2) == 0)
+            st = fdtd_cycle(ctx, f->er, f->ez, f->hphi, ctx->er_b, ctx->ez_b, ctx->h_b,
+                            ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
+                            last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
+        else
+            st = fdtd_cycle(ctx, ctx->er_b, ctx->ez_b, ctx->h_b, f->er, f->ez, f->hphi,
+                            ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
+                            last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
+        if (st != GASX_OK) return st;
+        ca.step = o.step0 + (uint32_t)s;
+        for (int k = 0; k < nsp; ++k) {
+            uint32_t flags = kPush;
+            if (k == 0 && o.collide) {
+                flags |= kCollide;
+                if (o.secondaries) flags |= kStageSec;
+                if (o.ions) flags |= kStageIon;
+            }
+            if (!last) flags |= kDrift | kDeposit;
+            st = run_particles(ctx, flags, &sp[k], ctx->q[nxt][0], ctx->q[nxt][1], ca);
+            if (st != GASX_OK) return st;
+            if (k == 0 && o.collide) {
+                if (o.ions) {
+                    st = append_from_stage(ctx, &sp[0], ions, true);
+                    if (st != GASX_OK) return st;
+                }
+                if (o.secondaries) {
+                    st = append_from_stage(ctx, &sp[0], &sp[0], false);
+                    if (st != GASX_OK) return st;
+                }
+            }
+        }
+        if (o.sort_every > 0 && !last && ((s + 1) % o.sort_every) == 0) {
+            for (int k = 0; k < nsp; ++k) {
+                st = gasx_sort(ctx, &sp[k]);
+                if (st != GASX_OK) return st;
+            }
+        }
+    }
+    if (o.n_steps % 2 == 1) {
+        HIPCHK(hipMemcpyAsync(f->er, ctx->er_b, ner * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+        HIPCHK(hipMemcpyAsync(f->ez, ctx->ez_b, nez * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+        HIPCHK(hipMemcpyAsync(f->hphi, ctx->h_b, nh * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream), "copy");
+    }
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_em_step(gasx_ctx* ctx, gasx_species* sp, int n_species, gasx_fields* f,
+                                    int n_push)
+{
+    if (!ctx || !sp || n_species < 1 || n_species > 8 || !valid_fields(f) || n_push < 0)
+        return GASX_E_ARG;
+    for (int k = 0; k < n_species; ++k)
+        if (!valid_species(&sp[k])) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    LoopOpts o;
+    o.n_steps = n_push;
+    return fused_loop(ctx, sp, n_species, nullptr, f, nullptr, o);
+}
+
+extern "C" gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, gasx_fields* f,
+                                const gasx_gas* g, const gasx_run_opts* ro)
+{
+    if (!ctx || !valid_species(e) || !valid_fields(f) || !ro || ro->n_steps < 0 ||
+        (ions && !valid_species(ions)) || (ro->collide && !g) || ro->sort_every < 0)
+        return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    LoopOpts o;
+    o.n_steps = ro->n_steps;
+    o.collide = ro->collide != 0;
+    o.secondaries = o.collide && ro->secondaries != 0;
+    o.ions = o.collide && ions != nullptr;
+    o.sort_every = ro->sort_every;
+    o.seed = ro->seed;
+    o.run_id = ro->run_id;
+    o.step0 = ro->step0;
+    return fused_loop(ctx, e, 1, ions, f, g, o);
+}
+
+extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
+{
+    if (!ctx || !valid_species(s)) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t need = 0;
+    hipError_t e = launch_sort(ctx->geom, s, nullptr, 0, &need, ctx->stream);
+    if (e != hipSuccess && e != hipErrorInvalidValue) return hip_fail(ctx, e, "sort size");
+    if (need > ctx->sort_bytes) {
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+        if (ctx->sort_tmp) hipFree(ctx->sort_tmp);
+        ctx->sort_tmp = nullptr;
+        HIPCHK(hipMalloc(&ctx->sort_tmp, need), "alloc sort");
+        ctx->sort_bytes = need;
+    }
+    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream), "sort");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_tallies(gasx_ctx* ctx, const gasx_fields* f, const gasx_species* sp,
+                                    int n_species, double out[4])
+{
+    if (!ctx || !valid_fields(f) || !out || n_species < 0 || (n_species > 0 && !sp)) return GASX_E_ARG;
+    for (int k = 0; k < n_species; ++k)
+        if (!valid_species(&sp[k])) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(launch_tallies(ctx->p, f->er, f->ez, f->hphi, sp, n_species, ctx->d_tally,
+                          ctx->d_tally + 4 * 256, ctx->stream),
+           "tallies");
+    HIPCHK(hipMemcpyAsync(out, ctx->d_tally + 4 * 256, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream),
+           "copy tallies");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_get_counters(gasx_ctx* ctx, gasx_counters* out, int reset)
+{
+    if (!ctx || !out) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    unsigned long long h[kNCounters];
+    HIPCHK(hipMemcpyAsync(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    out->n_elastic = (int64_t)h[kCElastic];
+    out->n_ionize = (int64_t)h[kCIonize];
+    out->n_excite = (int64_t)h[kCExcite];
+    out->n_dropped_sec = (int64_t)h[kCDropSec];
+    out->n_dropped_ion = (int64_t)h[kCDropIon];
+    out->n_bound_violations = (int64_t)h[kCViol];
+    out->n_absorbed = (int64_t)h[kCAbsorbed];
+    out->n_saturated = (int64_t)h[kCSat];
+    if (reset) {
+        HIPCHK(hipMemsetAsync(ctx->d_counters, 0, sizeof(h), ctx->stream), "zero");
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    }
+    if (out->n_dropped_sec > 0 || out->n_dropped_ion > 0)
+        return fail(ctx, GASX_E_CAPACITY, "appends exceeded capacity");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_get_count(gasx_ctx* ctx, const gasx_species* s, int64_t* n)
+{
+    if (!ctx || !s || !s->count || !n) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    int32_t v = 0;
+    HIPCHK(hipMemcpyAsync(&v, s->count, sizeof(v), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    *n = v;
+    return GASX_OK;
+}

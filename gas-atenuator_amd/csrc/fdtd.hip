@@ -1,0 +1,273 @@
+// One field cycle per launch (electricmagneticfield6.m:146-151):
+//   H -= dtf/2 curl E ; n_sub x { H += dtf curl E ; E += dtf (curl H - J)/eps } ; H += dtf/2 curl E
+// with the EM3_1.cu:9-15 / EM3.cu:14-42 update rules.
+//
+// Temporal blocking: a workgroup owns a ti x tk tile of Hphi cells and
+// carries a halo of `halo` cells on every side in LDS.  Each half update
+// invalidates one more ring of the halo (H reads E at i+1,k+1; E reads H at
+// i-1,k-1), so a halo of n_sub+3 keeps the interior exact for the whole
+// cycle and the cycle needs no grid-wide synchronisation.  State is FP64
+// (read from *_in, written to *_out: overlapping halos make in-place
+// updates racy); FP32 copies for the particle gather are written alongside.
+// The current comes either from the int64 fixed-point accumulators of the
+// deposit (J = junit * Q / V(i)) or from FP64 arrays.
+#include "gasx_internal.h"
+
+namespace gasx {
+
+template <int BS, int MAXP>
+__global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdRun* runs, int fixed_j)
+{
+    extern __shared__ double lds[];
+    const int tiles_per_run = a.ntr * a.ntk;
+    const int run = blockIdx.x / tiles_per_run;
+    const int t = blockIdx.x - run * tiles_per_run;
+    const FdtdRun R = runs ? runs[run] : r0;
+    const int nr = a.nr, nz = a.nz;
+    const int ti_idx = t % a.ntr, tk_idx = t / a.ntr;
+    const int i0 = ti_idx * a.ti, i1 = min(nr, i0 + a.ti);
+    const int k0 = tk_idx * a.tk, k1 = min(nz, k0 + a.tk);
+    const int ri0 = max(0, i0 - a.halo), ri1 = min(nr, i1 + a.halo);
+    const int rk0 = max(0, k0 - a.halo), rk1 = min(nz, k1 + a.halo);
+    const int W = ri1 - ri0, Hh = rk1 - rk0;
+    // LDS: H  W x Hh ; Er W x (Hh+1) ; Ez (W+1) x Hh
+    double* sH = lds;
+    double* sEr = sH + W * Hh;
+    double* sEz = sEr + W * (Hh + 1);
+    const int ldz = W + 1;
+    const int npos = (W + 1) * (Hh + 1);   // positions (i,k), i in [ri0,ri1], k in [rk0,rk1]
+
+    // ---- load --------------------------------------------------------------
+    for (int p = threadIdx.x; p < npos; p += BS) {
+        const int li = p % (W + 1), lk = p / (W + 1);
+        const int gi = ri0 + li, gk = rk0 + lk;
+        if (li < W && lk < Hh) sH[li + lk * W] = R.h_in[gi + gk * nr];
+        if (li < W) sEr[li + lk * W] = R.er_in[gi + gk * nr];
+        if (lk < Hh) sEz[li + lk * ldz] = R.ez_in[gi + gk * (nr + 1)];
+    }
+
+    // ---- per-position sources and coefficients (registers) -----------------
+    auto jr_at = [&](int i, int k) -> double {
+        if (fixed_j) {
+            const double V = a.vcoef * (double)((i + 1) * (i + 1) - i * i);
+            return a.junit * (double)(long long)R.qr[i + k * nr] / V;
+        }
+        return R.jr[i + k * nr];
+    };
+    auto jz_at = [&](int i, int k) -> double {
+        if (fixed_j) {
+            const double V = a.vcoef * (double)((i + 1) * (i + 1) - i * i);
+            return a.junit * (double)(long long)R.qz[i + k * nr] / V;
+        }
+        return R.jz[i + k * nr];
+    };
+    double srcR[MAXP], srcZ[MAXP], cA[MAXP], cB[MAXP];
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = threadIdx.x + q * BS;
+        srcR[q] = 0.0; srcZ[q] = 0.0; cA[q] = 0.0; cB[q] = 0.0;
+        if (p < npos) {
+            const int li = p % (W + 1), lk = p / (W + 1);
+            const int gi = ri0 + li, gk = rk0 + lk;
+            if (li < W && gk >= 1 && gk <= nz - 1)   // EM3.cu:29 (jr at k-1 and k)
+                srcR[q] = a.ce * (jr_at(gi, gk - 1) + jr_at(gi, gk)) / 2;
+            if (lk < Hh) {
+                if (gi == 0)                           // EM3.cu:35
+                    srcZ[q] = ((double)a.axis_sign * a.ce) * jz_at(0, gk);
+                else if (gi <= nr - 1) {               // EM3.cu:39
+                    srcZ[q] = a.ce * (jz_at(gi - 1, gk) + jz_at(gi, gk)) / 2;
+                    const double rs = (a.ez_metric ? (gi + 1) : gi) * a.dr;
+                    cA[q] = a.ce * (1 / (2 * rs) + 1 / a.dr);
+                    cB[q] = a.ce * (1 / (2 * rs) - 1 / a.dr);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- H update (EM3_1.cu:15), scaled by s in {-1/2, 1, 1/2} -------------
+    auto h_update = [&](double s) {
+        const double cr = s * a.ch_r, cz = s * a.ch_z;
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                if (li < W && lk < Hh) {
+                    const double dez = sEz[(li + 1) + lk * ldz] - sEz[li + lk * ldz];
+                    const double der = sEr[li + (lk + 1) * W] - sEr[li + lk * W];
+                    sH[li + lk * W] = sH[li + lk * W] + (cr * dez - cz * der);
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    // ---- E update (EM3.cu:14-42), Jacobi in the boundary copies -------------
+    auto e_update = [&]() {
+        double nEr[MAXP], nEz[MAXP];
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            nEr[q] = 0.0; nEz[q] = 0.0;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                const int gi = ri0 + li, gk = rk0 + lk;
+                if (li < W) {
+                    double v = sEr[li + lk * W];
+                    if (gk == 0) v = sEr[li + 1 * W];                     // :22
+                    else if (gk == nz) v = sEr[li + (lk - 1) * W];        // :26
+                    else if (lk > 0 && lk < Hh)                            // :29
+                        v = v - srcR[q] - a.cez * (sH[li + lk * W] - sH[li + (lk - 1) * W]);
+                    nEr[q] = v;
+                }
+                if (lk < Hh) {
+                    double v = sEz[li + lk * ldz];
+                    if (gi == 0)                                           // :35
+                        v = v + srcZ[q] + a.c4 * sH[0 + lk * W];
+                    else if (gi <= nr - 1 && li > 0 && li < W)            // :39
+                        v = v - srcZ[q] + cA[q] * sH[li + lk * W] + cB[q] * sH[(li - 1) + lk * W];
+                    nEz[q] = v;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                if (li < W) sEr[li + lk * W] = nEr[q];
+                if (lk < Hh) sEz[li + lk * ldz] = nEz[q];
+            }
+        }
+        __syncthreads();
+    };
+
+    h_update(-0.5);
+    for (int s = 0; s < a.n_sub; ++s) {
+        h_update(1.0);
+        e_update();
+    }
+    h_update(0.5);
+
+    // ---- write the tile interior ------------------------------------------
+    for (int p = threadIdx.x; p < npos; p += BS) {
+        const int li = p % (W + 1), lk = p / (W + 1);
+        const int gi = ri0 + li, gk = rk0 + lk;
+        const bool in_i = gi >= i0 && gi < i1;
+        const bool in_k = gk >= k0 && gk < k1;
+        if (li < W && lk < Hh && in_i && in_k) {
+            const double v = sH[li + lk * W];
+            R.h_out[gi + gk * nr] = v;
+            R.h32[gi + gk * nr] = (float)v;
+            if (R.qr_clear) {
+                R.qr_clear[gi + gk * nr] = 0ull;
+                R.qz_clear[gi + gk * nr] = 0ull;
+            }
+        }
+        if (li < W && in_i && (in_k || (gk == nz && k1 == nz))) {
+            const double v = sEr[li + lk * W];
+            R.er_out[gi + gk * nr] = v;
+            R.er32[gi + gk * nr] = (float)v;
+        }
+        if (lk < Hh && in_k && (in_i || (gi == nr && i1 == nr))) {
+            const double v = sEz[li + lk * ldz];
+            R.ez_out[gi + gk * (nr + 1)] = v;
+            R.ez32[gi + gk * (nr + 1)] = (float)v;
+        }
+    }
+}
+
+size_t fdtd_lds_bytes(const FdtdArgs& a)
+{
+    const size_t W = (size_t)a.ti + 2 * a.halo, Hh = (size_t)a.tk + 2 * a.halo;
+    return (W * Hh + W * (Hh + 1) + (W + 1) * Hh) * sizeof(double);
+}
+
+template <int BS, int MAXP>
+static hipError_t launch_fdtd_t(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev,
+                                int nruns, int fixed_j, hipStream_t s)
+{
+    const size_t lds = fdtd_lds_bytes(a);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fdtd<BS, MAXP>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const int blocks = nruns * a.ntr * a.ntk;
+    hipLaunchKernelGGL((k_fdtd<BS, MAXP>), dim3(blocks), dim3(BS), lds, s, a, r0, runs_dev, fixed_j);
+    return hipGetLastError();
+}
+
+hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
+                       int fixed_j, hipStream_t s)
+{
+    const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
+    const int npos = (W + 1) * (Hh + 1);
+    if (fdtd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
+    if (npos <= 256 * 4) return launch_fdtd_t<256, 4>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (npos <= 512 * 4) return launch_fdtd_t<512, 4>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (npos <= 512 * 8) return launch_fdtd_t<512, 8>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (npos <= 1024 * 8) return launch_fdtd_t<1024, 8>(a, r0, runs_dev, nruns, fixed_j, s);
+    return hipErrorInvalidValue;
+}
+
+// ---- fixed point -> A/m^2 ------------------------------------------------------
+__global__ void k_j_from_fixed(FdtdArgs a, const unsigned long long* qr,
+                               const unsigned long long* qz, double* jr, double* jz)
+{
+    const int n = a.nr * a.nz;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        const int i = c % a.nr;
+        const double V = a.vcoef * (double)((i + 1) * (i + 1) - i * i);
+        jr[c] = a.junit * (double)(long long)qr[c] / V;
+        jz[c] = a.junit * (double)(long long)qz[c] / V;
+    }
+}
+
+hipError_t launch_j_from_fixed(const FdtdArgs& a, const unsigned long long* qr,
+                               const unsigned long long* qz, double* jr, double* jz, hipStream_t s)
+{
+    const int n = a.nr * a.nz;
+    int blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_j_from_fixed, dim3(blocks), dim3(256), 0, s, a, qr, qz, jr, jz);
+    return hipGetLastError();
+}
+
+__global__ void k_to_f32(const double* src, float* dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (float)src[i];
+}
+
+hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace gasx

@@ -1,0 +1,145 @@
+// Internal declarations shared by the HIP translation units of libgasx.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "gasx.h"
+
+namespace gasx {
+
+constexpr int kNAng = 181;       // DCS angles 0..180 deg (ElasticCS_Nitrogen.csv rows 2..182)
+constexpr int kMaxTab = 64;
+constexpr int kMaxExc = 16;
+constexpr int kChunk = 256;      // particles per workgroup chunk (== particle kernel block size)
+
+// Device copy of the prepared N2 tables (electron_collision.m:12-41,
+// find_CS0.m:14-56).  All energies in eV, cross sections in m^2.
+struct DevTables {
+    int32_t n_el, n_sel, n_ion, n_exc, n_dcs, pad_;
+    float el_e[kMaxTab], el_s[kMaxTab];    // TotalElasticCS_Nitrogen (E>=50 eV), sigma*a0^2*2
+    float sel_e[kMaxTab], sel_s[kMaxTab];  // TotalElasticCS_smalleV (E<50 eV), sigma*1e-20
+    float ion_e[kMaxTab], ion_s[kMaxTab];  // TotalIonCS, sigma*1e-20
+    float exc_w[kMaxExc];                  // W_i (eV)
+    float exc_q[kMaxExc];                  // q0*A_i/W_i^2*1e-4  (sigma_exc prefactor, find_CS0.m:50)
+    float exc_a[kMaxExc];                  // A_i/W_i^2          (E_dep weights, get_ImpExc2.m:24)
+    float exc_om[kMaxExc], exc_nu[kMaxExc], exc_ga[kMaxExc];
+    float dcs_loge[kMaxTab];               // log(E keV) of the merged DCS energies
+    float dcs_logv[kMaxTab * kNAng];       // log DCS, [e*181 + angle]
+};
+
+// Host-side prepared tables (FP64) + the device image.
+struct HostTables {
+    int n_el = 0, n_sel = 0, n_ion = 0, n_exc = 0, n_dcs = 0;
+    double el_e[kMaxTab], el_s[kMaxTab], sel_e[kMaxTab], sel_s[kMaxTab], ion_e[kMaxTab], ion_s[kMaxTab];
+    double exc_w[kMaxExc], exc_a[kMaxExc], exc_om[kMaxExc], exc_nu[kMaxExc], exc_ga[kMaxExc];
+    double dcs_e[kMaxTab], dcs_loge[kMaxTab], dcs_tcs[kMaxTab], dcs_logv[kMaxTab * kNAng];
+};
+
+int prepare_tables(const gasx_raw_tables* raw, HostTables* h, DevTables* d);
+// max over 15.6 eV..20 keV of v(E)*sigma_T(E) (m^3/s), FP64, for the rejection bound
+double max_rate_coefficient(const HostTables& h, const gasx_params& p);
+
+// ---------------------------------------------------------------------------
+// Kernel argument blocks
+// ---------------------------------------------------------------------------
+struct Geom {
+    int32_t nr, nz;
+    float dr, dz, inv_dr, inv_dz, rp, lp, phip, cphip, sphip;
+    float dt;          // push step
+    float c, inv_c2, mu0;
+};
+
+enum : uint32_t {
+    kPush = 1u,        // EM4.cu: gather + push + half drift + BC
+    kCollide = 2u,     // electron_transport.cu
+    kDrift = 4u,       // EM2.cu: half drift + BC
+    kDeposit = 8u,     // EM7.cu
+    kStageSec = 16u,   // stage ionization secondaries for append
+    kStageIon = 32u,   // stage N2+ ions for append
+};
+
+struct CollArgs {
+    const DevTables* tab;
+    const float* dens;
+    const float* temp;
+    int32_t gnr, gnz;
+    float inv_gdr, inv_gdz, t_scalar;
+    float dt_coll;
+    float pmax;            // rejection bound on p_T (>= every candidate's p_T), <= 1
+    uint32_t key0, key1, step, run;
+    float ev_per_mc2_em;   // me_em c^2 / e_em (eV): energy of the push's electron
+    float mc2_mc_ev;       // me_mc c^2 / e_mc (eV): E_e of electron_transport.cu:52
+    float ion_vth_k;       // sqrt(k_b / m_ion)
+    int32_t exc_form;
+    float rot_eps;
+};
+
+struct PartArgs {
+    float *x, *y, *z, *vx, *vy, *vz;
+    int32_t* count;
+    int32_t cap;
+    float q_over_m;        // qnum*e_em/mass
+    float kq;              // qnum * 2^32 / c (deposit fixed-point scale)
+    const float *er, *ez, *h;   // FP32 gather copies
+    unsigned long long *qr, *qz;  // int64 fixed-point current (two's complement)
+    // staging for appends (SoA, indexed chunk*kChunk + rank)
+    float* sec[6];
+    int32_t* sec_cnt;
+    float* ion[6];
+    int32_t* ion_cnt;
+    unsigned long long* counters;  // gasx_counters layout
+};
+
+enum CounterIdx { kCElastic = 0, kCIonize, kCExcite, kCDropSec, kCDropIon, kCViol, kCAbsorbed, kCSat, kNCounters };
+
+struct FdtdArgs {
+    int32_t nr, nz;
+    int32_t ti, tk, halo;        // tile interior size and halo
+    int32_t ntr, ntk;            // tiles per run in r and z
+    int32_t n_sub;
+    double dtf;                  // field substep
+    double ch_r, ch_z;           // dtf/(mu dr), dtf/(mu dz)
+    double ce, cez, c4;          // dtf/eps, dtf/(eps dz), 4 dtf/(eps dr)
+    double dr;
+    int32_t ez_metric, axis_sign;
+    double junit;                // FN*e_em*c*2^-32 (fixed-point -> A/m^2 times V(i))
+    double vcoef;                // PHIp/2*dr^2*dz  (V(i) = vcoef*(2i+1))
+};
+
+struct FdtdRun {
+    const double *er_in, *ez_in, *h_in;
+    double *er_out, *ez_out, *h_out;
+    float *er32, *ez32, *h32;
+    const unsigned long long *qr, *qz;     // fixed-point current (or null)
+    const double *jr, *jz;                 // FP64 current (or null)
+    unsigned long long *qr_clear, *qz_clear;  // accumulators to zero for the next deposit (or null)
+};
+
+// ---------------------------------------------------------------------------
+// Launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+// runs_dev == nullptr: single run described by r0
+hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
+                       int fixed_j, hipStream_t s);
+size_t fdtd_lds_bytes(const FdtdArgs& a);
+hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
+                            int max_blocks, hipStream_t s);
+hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
+                         float* const stage[6], const int32_t* stage_cnt,
+                         const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
+                         int32_t max_chunks, unsigned long long* dropped, hipStream_t s);
+hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
+                                 float* si, float* sx, hipStream_t s);
+hipError_t launch_j_from_fixed(const FdtdArgs& a, const unsigned long long* qr,
+                               const unsigned long long* qz, double* jr, double* jz, hipStream_t s);
+hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
+hipError_t launch_tallies(const gasx_params& p, const double* er, const double* ez, const double* h,
+                          const gasx_species* sp, int nsp, double* partial, double* out,
+                          hipStream_t s);
+hipError_t launch_sort(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes,
+                       size_t* need_bytes, hipStream_t st);
+
+}  // namespace gasx

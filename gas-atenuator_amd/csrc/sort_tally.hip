@@ -1,0 +1,227 @@
+// Cell sort (new, no reference counterpart) and the FP64 energy tallies of
+// electricmagneticfield5.m:167-177.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "device_common.h"
+
+namespace gasx {
+
+// ---- cell sort ----------------------------------------------------------------
+// key = iz*nr + ir of the Hphi cell holding the particle; absorbed slots get
+// key nr*nz and sort last.  Stable radix sort keeps slot order within a cell.
+__global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
+                            uint32_t* keys, uint32_t* vals, int32_t* n_live)
+{
+    int live = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float zz = z[i];
+        uint32_t key;
+        if (is_dead(zz)) {
+            key = (uint32_t)(g.nr * g.nz);
+        } else {
+            float r = sqrtf(x[i] * x[i] + y[i] * y[i]);
+            int ir = (int)floorf(r * g.inv_dr), iz = (int)floorf(zz * g.inv_dz);
+            ir = ir < 0 ? 0 : (ir >= g.nr ? g.nr - 1 : ir);
+            iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
+            key = (uint32_t)(iz * g.nr + ir);
+            ++live;
+        }
+        keys[i] = key;
+        vals[i] = (uint32_t)i;
+    }
+    unsigned long long m = __ballot(live != 0);
+    (void)m;
+    // reduce live counts: wave sum then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) live += __shfl_down(live, o, 64);
+    if ((threadIdx.x & 63) == 0 && live) atomicAdd(n_live, live);
+}
+
+__global__ void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,
+                          const float* s4, const float* s5, float* d0, float* d1, float* d2,
+                          float* d3, float* d4, float* d5, const uint32_t* perm, const int32_t* n_live)
+{
+    const int n = *n_live;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = perm[i];
+        d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
+        d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
+    }
+}
+
+__global__ void k_copy_back(float* d0, float* d1, float* d2, float* d3, float* d4, float* d5,
+                            const float* s0, const float* s1, const float* s2, const float* s3,
+                            const float* s4, const float* s5, const int32_t* n_live, int32_t* count)
+{
+    const int n = *n_live;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        d0[i] = s0[i]; d1[i] = s1[i]; d2[i] = s2[i];
+        d3[i] = s3[i]; d4[i] = s4[i]; d5[i] = s5[i];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
+}
+
+// scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][n_live][rocprim tmp]
+hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
+                       size_t* need_bytes, hipStream_t st)
+{
+    int32_t n = 0;
+    hipError_t e = hipMemcpyAsync(&n, sp->count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const size_t cap = (size_t)sp->capacity;
+    const size_t a = (cap * 4 + 255) & ~(size_t)255;
+    uint32_t nkeys = (uint32_t)(g.nr * g.nz);
+    int bits = 1;
+    while ((1u << bits) <= nkeys && bits < 32) ++bits;
+    size_t tmp_bytes = 0;
+    e = rocprim::radix_sort_pairs((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)(cap > 0 ? cap : 1),
+                                  0, bits, st);
+    if (e != hipSuccess) return e;
+    const size_t need = 4 * a + 6 * a + 256 + ((tmp_bytes + 255) & ~(size_t)255);
+    if (need_bytes) *need_bytes = need;
+    if (!scratch || scratch_bytes < need) return hipErrorInvalidValue;
+    if (n <= 0) return hipSuccess;
+    char* base = (char*)scratch;
+    uint32_t* k_in = (uint32_t*)(base);
+    uint32_t* k_out = (uint32_t*)(base + a);
+    uint32_t* v_in = (uint32_t*)(base + 2 * a);
+    uint32_t* v_out = (uint32_t*)(base + 3 * a);
+    float* tmp[6];
+    for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * a);
+    int32_t* n_live = (int32_t*)(base + 10 * a);
+    void* rtmp = (void*)(base + 10 * a + 256);
+    e = hipMemsetAsync(n_live, 0, sizeof(int32_t), st);
+    if (e != hipSuccess) return e;
+    int blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, n, k_in,
+                       v_in, n_live);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    size_t tb = tmp_bytes;
+    e = rocprim::radix_sort_pairs(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_permute, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
+                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], v_out, n_live);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_copy_back, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
+                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], n_live,
+                       sp->count);
+    return hipGetLastError();
+}
+
+// ---- FP64 tallies (electricmagneticfield5.m:167-177) ---------------------------
+// W_E = PHIp*eps/2 * (trapz trapz Er^2 r + trapz trapz Ez^2 r), W_H likewise
+// with mu, on the grid coordinates of each array (electricmagneticfield6.m:25-61).
+constexpr int kTallyBlocks = 256;
+
+__device__ __forceinline__ double trap_w(int i, int n) { return (i == 0 || i == n - 1) ? 0.5 : 1.0; }
+
+__global__ __launch_bounds__(256) void k_tally_fields(int nr, int nz, double dr, double dz,
+                                                      const double* er, const double* ez,
+                                                      const double* h, double* partial)
+{
+    double se = 0.0, sh = 0.0;
+    const int ner = nr * (nz + 1), nez = (nr + 1) * nz, nh = nr * nz;
+    const int stride = gridDim.x * blockDim.x;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ner; c += stride) {
+        int i = c % nr, k = c / nr;
+        double r = (i + 0.5) * dr;
+        se += trap_w(i, nr) * trap_w(k, nz + 1) * er[c] * er[c] * r;
+    }
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nez; c += stride) {
+        int i = c % (nr + 1), k = c / (nr + 1);
+        double r = i * dr;
+        se += trap_w(i, nr + 1) * trap_w(k, nz) * ez[c] * ez[c] * r;
+    }
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nh; c += stride) {
+        int i = c % nr, k = c / nr;
+        double r = (i + 0.5) * dr;
+        sh += trap_w(i, nr) * trap_w(k, nz) * h[c] * h[c] * r;
+    }
+    __shared__ double s_e[256], s_h[256];
+    s_e[threadIdx.x] = se * dr * dz;
+    s_h[threadIdx.x] = sh * dr * dz;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_e[threadIdx.x] += s_e[threadIdx.x + o];
+            s_h[threadIdx.x] += s_h[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s_e[0];
+        partial[2 * blockIdx.x + 1] = s_h[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tally_ke(const float* vx, const float* vy, const float* vz,
+                                                  const float* z, const int32_t* count, double mc2,
+                                                  double inv_c2, double* partial)
+{
+    const int n = *count;
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (is_dead(z[i])) continue;
+        double a = vx[i], b = vy[i], c = vz[i];
+        double b2 = (a * a + b * b + c * c) * inv_c2;
+        double g2 = 1.0 / (1.0 - b2);
+        double gm = sqrt(g2);
+        s += mc2 * (b2 * g2 / (1.0 + gm));
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] += sm[0];
+}
+
+__global__ void k_tally_final(const double* pf, const double* pk, double phip, double eps0,
+                              double mu0, double fn, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double se = 0.0, sh = 0.0, sk = 0.0;
+    for (int b = 0; b < kTallyBlocks; ++b) {
+        se += pf[2 * b];
+        sh += pf[2 * b + 1];
+        sk += pk[b];
+    }
+    out[0] = phip * 0.5 * eps0 * se;
+    out[1] = phip * 0.5 * mu0 * sh;
+    out[2] = fn * sk;
+    out[3] = out[0] + out[1] + out[2];
+}
+
+__global__ void k_zero_d(double* p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+hipError_t launch_tallies(const gasx_params& p, const double* er, const double* ez, const double* h,
+                          const gasx_species* sp, int nsp, double* partial, double* out,
+                          hipStream_t s)
+{
+    // partial: 2*kTallyBlocks field partials + kTallyBlocks KE partials
+    double* pf = partial;
+    double* pk = partial + 2 * kTallyBlocks;
+    hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(256), 0, s, pk, kTallyBlocks);
+    hipLaunchKernelGGL(k_tally_fields, dim3(kTallyBlocks), dim3(256), 0, s, p.nr, p.nz, p.dr, p.dz,
+                       er, ez, h, pf);
+    for (int k = 0; k < nsp; ++k) {
+        const double c = p.c;
+        hipLaunchKernelGGL(k_tally_ke, dim3(kTallyBlocks), dim3(256), 0, s, sp[k].vx, sp[k].vy,
+                           sp[k].vz, sp[k].z, sp[k].count, sp[k].mass * c * c, 1.0 / (c * c), pk);
+    }
+    hipLaunchKernelGGL(k_tally_final, dim3(1), dim3(64), 0, s, pf, pk, p.phip, p.eps0, p.mu0, p.fn,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace gasx

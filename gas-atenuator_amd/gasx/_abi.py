@@ -1,0 +1,153 @@
+"""ctypes mirror of include/gasx.h and the loader of libgasx.so.
+
+The loader raises when the HIP library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libgasx.so"
+
+# gasx_status
+OK, E_ARG, E_CAPACITY, E_HIP, E_NOMEM, E_STATE, E_NUMERIC = range(7)
+
+
+class GasxError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gasx status {status}: {msg}")
+        self.status = status
+
+
+class CapacityError(GasxError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nz", C.c_int32),
+        ("dr", C.c_double), ("dz", C.c_double),
+        ("phip", C.c_double), ("ddt", C.c_double),
+        ("n_sub", C.c_int32), ("fn", C.c_double),
+        ("c", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+        ("e_em", C.c_double), ("me_em", C.c_double),
+        ("e_mc", C.c_double), ("me_mc", C.c_double),
+        ("m_ion", C.c_double), ("k_b", C.c_double),
+        ("ez_metric", C.c_int32), ("axis_j_sign", C.c_int32), ("exc_form", C.c_int32),
+        ("rot_eps", C.c_double),
+    ]
+
+
+class RawTables(C.Structure):
+    _fields_ = [
+        ("t_el", C.POINTER(C.c_double)), ("n_el", C.c_int32),
+        ("t_sel", C.POINTER(C.c_double)), ("n_sel", C.c_int32),
+        ("t_sel1", C.POINTER(C.c_double)), ("n_sel1", C.c_int32),
+        ("t_del", C.POINTER(C.c_double)), ("w_del", C.c_int32),
+        ("t_sdel", C.POINTER(C.c_double)), ("w_sdel", C.c_int32),
+        ("t_ion", C.POINTER(C.c_double)), ("n_ion", C.c_int32),
+        ("t_exc", C.POINTER(C.c_double)), ("n_exc", C.c_int32),
+    ]
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+        ("count", C.c_void_p), ("capacity", C.c_int64),
+        ("qnum", C.c_float), ("mass", C.c_double),
+    ]
+
+
+class Fields(C.Structure):
+    _fields_ = [("er", C.c_void_p), ("ez", C.c_void_p), ("hphi", C.c_void_p)]
+
+
+class Gas(C.Structure):
+    _fields_ = [
+        ("density", C.c_void_p), ("temperature", C.c_void_p),
+        ("gnr", C.c_int32), ("gnz", C.c_int32),
+        ("gdr", C.c_double), ("gdz", C.c_double),
+        ("t_scalar", C.c_double), ("n_max", C.c_double),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("n_elastic", C.c_int64), ("n_ionize", C.c_int64), ("n_excite", C.c_int64),
+        ("n_dropped_sec", C.c_int64), ("n_dropped_ion", C.c_int64),
+        ("n_bound_violations", C.c_int64), ("n_absorbed", C.c_int64), ("n_saturated", C.c_int64),
+    ]
+
+
+class RunOpts(C.Structure):
+    _fields_ = [
+        ("n_steps", C.c_int32), ("collide", C.c_int32), ("secondaries", C.c_int32),
+        ("sort_every", C.c_int32), ("seed", C.c_uint64), ("run_id", C.c_uint32),
+        ("step0", C.c_uint32), ("use_graph", C.c_int32),
+    ]
+
+
+# exported symbols and their prototypes (every declaration of include/gasx.h)
+_P = C.c_void_p
+PROTOTYPES = {
+    "gasx_abi_version": (C.c_int, []),
+    "gasx_status_string": (C.c_char_p, [C.c_int]),
+    "gasx_last_error": (C.c_char_p, [_P]),
+    "gasx_ctx_create": (C.c_int, [C.c_int, C.POINTER(Params), C.POINTER(RawTables), C.POINTER(_P)]),
+    "gasx_ctx_destroy": (C.c_int, [_P]),
+    "gasx_ctx_set_stream": (C.c_int, [_P, _P]),
+    "gasx_synchronize": (C.c_int, [_P]),
+    "gasx_fdtd": (C.c_int, [_P, C.POINTER(Fields), _P, _P, C.c_int]),
+    "gasx_half_drift": (C.c_int, [_P, C.POINTER(Species)]),
+    "gasx_deposit": (C.c_int, [_P, C.POINTER(Species), _P, _P]),
+    "gasx_current_from_fixed": (C.c_int, [_P, _P, _P, _P, _P]),
+    "gasx_push": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Fields)]),
+    "gasx_fields_sync": (C.c_int, [_P, C.POINTER(Fields)]),
+    "gasx_em_step": (C.c_int, [_P, C.POINTER(Species), C.c_int, C.POINTER(Fields), C.c_int]),
+    "gasx_cross_sections": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P]),
+    "gasx_collide": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Gas),
+                               C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "gasx_run": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Fields),
+                           C.POINTER(Gas), C.POINTER(RunOpts)]),
+    "gasx_sort": (C.c_int, [_P, C.POINTER(Species)]),
+    "gasx_tallies": (C.c_int, [_P, C.POINTER(Fields), C.POINTER(Species), C.c_int,
+                               C.POINTER(C.c_double)]),
+    "gasx_get_counters": (C.c_int, [_P, C.POINTER(Counters), C.c_int]),
+    "gasx_get_count": (C.c_int, [_P, C.POINTER(Species), C.POINTER(C.c_int64)]),
+    "gasx_collision_bound": (C.c_int, [_P, C.POINTER(Gas), C.c_double, C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def load(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libgasx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(
+            f"{p} is missing: build the HIP library first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.gasx_abi_version() != 1:
+        raise RuntimeError("libgasx ABI version mismatch")
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(lib, ctx, status: int) -> None:
+    if status == OK:
+        return
+    msg = (lib.gasx_last_error(ctx) or b"").decode() if ctx else ""
+    msg = msg or lib.gasx_status_string(status).decode()
+    if status == E_CAPACITY:
+        raise CapacityError(status, msg)
+    raise GasxError(status, msg)

@@ -1,0 +1,116 @@
+"""Synthetic workloads of the electron PIC step.
+
+The reference seeds electrons from its photoionization source
+(INCIDENCE1.m:51-158); the workloads here replace it with a Gaussian X-ray
+spot: (x, y) from a 2-D Gaussian of width sigma_r (resampled if r >= Rp),
+phi ~ U(0, PHIp), z ~ U(0, Lp) (INCIDENCE1.m:62 samples z uniformly per z
+cell), isotropic directions, and kinetic energies from a 70/30 mix of
+N(400, 20) eV (K-shell photoelectrons) and N(30, 10) eV (valence), resampled
+if E <= 1 eV.  Speeds use the push's electron constants so that the kernel's
+energy of the seeded velocity equals the drawn energy.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .core import C_LIGHT, E_EM, ME_EM
+
+
+@dataclass
+class Workload:
+    name: str
+    nr: int
+    nz: int
+    n_electrons: int
+    n_steps: int
+    density: float                  # m^-3 (uniform) or on-axis value for profiles
+    capacity_factor: float = 1.0
+    secondaries: bool = False
+    sort_every: int = 0
+    dr: float = 1e-5
+    dz: float = 1e-5
+    phip: float = 0.1
+    ddt: float = 1e-14
+    n_sub: int = 10
+    sigma_r: float = 50e-6
+    density_edge: float | None = None   # linear profile to this value at r = Rp
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def rp(self) -> float:
+        return self.nr * self.dr
+
+    @property
+    def lp(self) -> float:
+        return self.nz * self.dz
+
+    @property
+    def capacity(self) -> int:
+        return int(math.ceil(self.n_electrons * self.capacity_factor))
+
+
+# The five configurations of the measurement plan.
+WORKLOADS = {
+    "plain_cpu_64x128": Workload("plain_cpu_64x128", 64, 128, 100_000, 200, 2.5e25),
+    "plain_gpu_256x512": Workload("plain_gpu_256x512", 256, 512, 4_000_000, 1000, 2.5e25,
+                                  capacity_factor=2.0, secondaries=True, sort_every=10),
+    "batched_128x256": Workload("batched_128x256", 128, 256, 0, 500, 2.5e25,
+                                extra={"runs": 64, "n_lo": 1e4, "n_hi": 2e6,
+                                       "dens_lo": 2.5e24, "dens_hi": 2.5e26}),
+    "adjoint_128x256": Workload("adjoint_128x256", 128, 256, 500_000, 200, 2.5e25,
+                                extra={"nodes": 32, "checkpoint": 20}),
+    "avalanche_1024x2048": Workload("avalanche_1024x2048", 1024, 2048, 30_000_000, 5000, 5e25,
+                                    capacity_factor=3.0, secondaries=True, sort_every=5,
+                                    density_edge=5e24),
+}
+
+
+def speed_from_ev(e_ev: np.ndarray) -> np.ndarray:
+    g1 = e_ev * E_EM / (ME_EM * C_LIGHT * C_LIGHT)
+    return C_LIGHT * np.sqrt(g1 * (g1 + 2.0)) / (1.0 + g1)
+
+
+def seed_electrons(n: int, rp: float, lp: float, phip: float, sigma_r: float = 50e-6,
+                   seed: int = 0, dtype=np.float32):
+    """Gaussian-spot electrons; returns x, y, z, vx, vy, vz (host arrays)."""
+    rng = np.random.default_rng(seed)
+    # radius from a 2-D Gaussian (Rayleigh), resampled if r >= Rp
+    r = sigma_r * np.sqrt(-2.0 * np.log1p(-rng.random(n)))
+    bad = r >= rp
+    while bad.any():
+        r[bad] = sigma_r * np.sqrt(-2.0 * np.log1p(-rng.random(int(bad.sum()))))
+        bad = r >= rp
+    phi = rng.random(n) * phip
+    z = rng.random(n) * lp
+    # energies: 70% N(400,20) eV, 30% N(30,10) eV, resampled if <= 1 eV
+    k_shell = rng.random(n) < 0.7
+    e = np.where(k_shell, rng.normal(400.0, 20.0, n), rng.normal(30.0, 10.0, n))
+    bad = e <= 1.0
+    while bad.any():
+        kb = rng.random(int(bad.sum())) < 0.7
+        e[bad] = np.where(kb, rng.normal(400.0, 20.0, kb.size), rng.normal(30.0, 10.0, kb.size))
+        bad = e <= 1.0
+    v = speed_from_ev(e)
+    cth = rng.uniform(-1.0, 1.0, n)
+    sth = np.sqrt(1.0 - cth * cth)
+    psi = rng.random(n) * 2.0 * math.pi
+    x = r * np.cos(phi)
+    y = r * np.sin(phi)
+    vx = v * sth * np.cos(psi)
+    vy = v * sth * np.sin(psi)
+    vz = v * cth
+    return tuple(a.astype(dtype) for a in (x, y, z, vx, vy, vz))
+
+
+def density_grid(w: Workload, gnr: int | None = None):
+    """Gas density on (gnz=1, gnr) radial cells: uniform, or linear in r from
+    `density` on axis to `density_edge` at Rp."""
+    if w.density_edge is None:
+        return np.full((1, 1), w.density, dtype=np.float32), w.rp, w.lp
+    gnr = gnr or w.nr
+    rc = (np.arange(gnr) + 0.5) * (w.rp / gnr)
+    d = w.density + (w.density_edge - w.density) * rc / w.rp
+    return d.reshape(1, gnr).astype(np.float32), w.rp / gnr, w.lp

@@ -1,0 +1,205 @@
+/*
+ * gasx.h -- C ABI of the MI355X electron PIC step (libgasx.so).
+ *
+ * Drop-in boundary for the reference's MATLAB -> CUDA launch surface
+ * (parallel.gpu.CUDAKernel + feval, electricmagneticfield6.m:77-156 and
+ * electron_collision.m:4-58).  Each entry point below names the reference
+ * kernel or function it replaces.  All calls are extern "C", return a
+ * gasx_status, never throw, and take caller-owned device buffers with
+ * explicit sizes.  A gasx_ctx is bound to one device and one HIP stream; it
+ * is thread-compatible, not thread-safe.  Calls are asynchronous on the
+ * context's stream unless documented otherwise.
+ *
+ * Grid conventions (reference semantics, 0-based, r fastest):
+ *   Hphi  nr x nz       index i + k*nr        (EM3_1.cu:12)
+ *   Er    nr x (nz+1)   index i + k*nr        (EM3.cu:18)
+ *   Ez    (nr+1) x nz   index i + k*(nr+1)    (EM3.cu:19)
+ *   Jr,Jz nr x nz       on the Hphi grid      (EM7.cu:56-57)
+ * Rp = nr*dr, Lp = nz*dz; particles live in the wedge 0 <= phi <= PHIp.
+ *
+ * Particles are FP32 structure-of-arrays (x, y, z, vx, vy, vz) in the wedge's
+ * Cartesian frame.  A slot whose z is NaN holds an absorbed particle (the
+ * reference's parDelete = 1, EM2.cu:31-33); gasx_sort removes them.
+ * Field state is FP64 (the reference's double arrays); the particle push
+ * gathers from FP32 copies the library refreshes after every field cycle.
+ */
+#ifndef GASX_H
+#define GASX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GASX_ABI_VERSION 1
+
+typedef enum gasx_status {
+    GASX_OK = 0,
+    GASX_E_ARG = 1,        /* bad argument, shape or size */
+    GASX_E_CAPACITY = 2,   /* appends exceeded capacity; the excess was dropped and counted */
+    GASX_E_HIP = 3,        /* HIP runtime error (message in gasx_last_error) */
+    GASX_E_NOMEM = 4,      /* device allocation failed */
+    GASX_E_STATE = 5,      /* call sequence error */
+    GASX_E_NUMERIC = 6     /* non-finite state or fixed-point range exceeded */
+} gasx_status;
+
+typedef struct gasx_ctx gasx_ctx;
+
+/* Physical and numerical parameters.  Constants are kept per reference file
+ * because the reference spells them differently in different files. */
+typedef struct gasx_params {
+    int32_t nr, nz;       /* Hphi cells in r and z */
+    double dr, dz;        /* cell size (m) */
+    double phip;          /* wedge angle PHIp (rad), main3.m:9 */
+    double ddt;           /* push step (s): electricmagneticfield6.m:135 ddt = ratio*dt */
+    int32_t n_sub;        /* field substeps per push: electricmagneticfield6.m:147 (2*ratio) */
+    double fn;            /* macro-particle weight FN */
+    double c;             /* 3e8 */
+    double eps0, mu0;     /* electricmagneticfield6.m:9-10 */
+    double e_em, me_em;   /* 1.6e-19, 9.1096e-31: electricmagneticfield6.m:5,11 */
+    double e_mc, me_mc;   /* 1.6022e-19, 9.109e-31: electron_transport.cu:47,49 */
+    double m_ion, k_b;    /* 4.65e-26, 1.38064853e-23: electron_transport.cu:50-51 */
+    int32_t ez_metric;    /* 0: 1/(2r) at r = i*dr; 1: r = (i+1)*dr as EM3.cu:39 */
+    int32_t axis_j_sign;  /* +1 as EM3.cu:35; -1 alternative */
+    int32_t exc_form;     /* 0: (1-(W/T)^g)^nu, find_CS0.m:52; 1: ((1-W/T)^g)^nu, electron_transport.cu:215 */
+    double rot_eps;       /* rotate.m:5 degeneracy threshold on vx^2+vy^2 (1e-5) */
+} gasx_params;
+
+/* Raw N2 tables exactly as the reference's CSV files hold them (row-major;
+ * MATLAB callers pass the transpose of their column-major matrices).  The
+ * library performs the merges of electron_collision.m:12-41. */
+typedef struct gasx_raw_tables {
+    const double* t_el;   int32_t n_el;    /* 2 x n_el   TotalElasticCS_Nitrogen.csv (keV ; a0^2) */
+    const double* t_sel;  int32_t n_sel;   /* n_sel x 2  TotalElasticCS_smalleV_Nitrogen.csv (eV ; 1e-20 m^2) */
+    const double* t_sel1; int32_t n_sel1;  /* n_sel1 x 2 TotalElasticCS_smalleV_Nitrogen_1.csv */
+    const double* t_del;  int32_t w_del;   /* 182 x w_del ElasticCS_Nitrogen.csv */
+    const double* t_sdel; int32_t w_sdel;  /* 182 x w_sdel ElasticDCS_smalleV_Nitrogen.csv */
+    const double* t_ion;  int32_t n_ion;   /* n_ion x 2  TotalIonCS_Nitrogen.csv (eV ; 1e-20 m^2) */
+    const double* t_exc;  int32_t n_exc;   /* n_exc x 5  ParameterExcCS_Nitrogen.csv (j,A,omega,v,miu) */
+} gasx_raw_tables;
+
+/* One particle population (caller-owned device memory). */
+typedef struct gasx_species {
+    float *x, *y, *z, *vx, *vy, *vz;
+    int32_t* count;       /* device int32: slots in use (live + absorbed) */
+    int64_t capacity;     /* slots allocated */
+    float qnum;           /* charge in units of e_em: -1 electron, +1 N2+, +2 N2++ (electricmagneticfield6.m:68) */
+    double mass;          /* kg (electricmagneticfield6.m:69) */
+} gasx_species;
+
+/* Field state (caller-owned device memory, FP64, reference layout). */
+typedef struct gasx_fields {
+    double *er, *ez, *hphi;
+} gasx_fields;
+
+/* Neutral gas seen by the collision step: density (and temperature) on an
+ * r-z grid; electron_transport.cu:67-76 looks them up per electron. */
+typedef struct gasx_gas {
+    const float* density;      /* device, gnr*gnz, index iz*gnr + ir (m^-3) */
+    const float* temperature;  /* device, same layout, or NULL to use t_scalar (K) */
+    int32_t gnr, gnz;
+    double gdr, gdz;
+    double t_scalar;
+    double n_max;              /* >= max density; the rejection bound uses it (exactness needs an upper bound) */
+} gasx_gas;
+
+typedef struct gasx_counters {
+    int64_t n_elastic, n_ionize, n_excite;
+    int64_t n_dropped_sec;     /* secondaries dropped at capacity (GASX_E_CAPACITY) */
+    int64_t n_dropped_ion;
+    int64_t n_bound_violations;/* candidate with p_T above the rejection bound (must stay 0) */
+    int64_t n_absorbed;        /* particles absorbed at r > Rp */
+    int64_t n_saturated;       /* collisions with p_T > 1 (BASELINE D15) */
+} gasx_counters;
+
+/* Options of the fused production loop. */
+typedef struct gasx_run_opts {
+    int32_t n_steps;
+    int32_t collide;            /* 1: one collision check per push with dt_coll = ddt */
+    int32_t secondaries;        /* 1: append ionization secondaries (electron_collision.m:68-75) */
+    int32_t sort_every;         /* cell sort every k steps (0 = never) */
+    uint64_t seed;              /* Philox key */
+    uint32_t run_id;            /* Philox counter word 2 */
+    uint32_t step0;             /* Philox counter word 1 of the first step */
+    int32_t use_graph;          /* 1: capture the step sequence in a hipGraph and replay */
+} gasx_run_opts;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the per-call CUDAKernel construction of electricmagneticfield6.m:77-111
+ * and the per-call table reads of electron_transport_field3.m:8-37. */
+gasx_status gasx_ctx_create(int device, const gasx_params* p, const gasx_raw_tables* t, gasx_ctx** out);
+gasx_status gasx_ctx_destroy(gasx_ctx* ctx);
+gasx_status gasx_ctx_set_stream(gasx_ctx* ctx, void* hip_stream);
+gasx_status gasx_synchronize(gasx_ctx* ctx);
+const char* gasx_status_string(gasx_status s);
+const char* gasx_last_error(const gasx_ctx* ctx);
+int gasx_abi_version(void);
+
+/* ---- FDTD ------------------------------------------------------------------
+ * Replaces EM3_1.cu / EM3.cu as sequenced by electricmagneticfield6.m:146-151:
+ * n_push field cycles (H back half a substep, n_sub leapfrog substeps, H
+ * forward half a substep) with the current jr, jz (A/m^2, FP64 device, nr*nz)
+ * frozen over each cycle.  Also refreshes the FP32 gather copies. */
+gasx_status gasx_fdtd(gasx_ctx* ctx, gasx_fields* f, const double* jr, const double* jz, int n_push);
+
+/* ---- particle kernels -----------------------------------------------------
+ * EM2.cu: half drift with the wedge / z boundary conditions. */
+gasx_status gasx_half_drift(gasx_ctx* ctx, gasx_species* s);
+/* EM7.cu: deposit into int64 fixed-point accumulators qr, qz (device, nr*nz;
+ * the caller zeroes them).  Physical current = FN*e_em*c*2^-32 * q / V(i),
+ * V(i) = PHIp/2*((i+1)^2-i^2)*dr^2*dz (EM7.cu:174-176).  The integer sum is
+ * independent of particle order, so the result is bit-reproducible. */
+gasx_status gasx_deposit(gasx_ctx* ctx, const gasx_species* s, int64_t* qr, int64_t* qz);
+/* Converts fixed-point accumulators to A/m^2 (FP64). */
+gasx_status gasx_current_from_fixed(gasx_ctx* ctx, const int64_t* qr, const int64_t* qz, double* jr, double* jz);
+/* EM4.cu: gather E, H at the particle, relativistic push, half drift + BC.
+ * Uses the FP32 field copies refreshed by the last gasx_fdtd / gasx_em_step. */
+gasx_status gasx_push(gasx_ctx* ctx, gasx_species* s, const gasx_fields* f);
+/* Refreshes the FP32 gather copies from f (needed only after the caller wrote f). */
+gasx_status gasx_fields_sync(gasx_ctx* ctx, const gasx_fields* f);
+/* Replaces electricmagneticfield6.m:138-166: n_push pushes of EM2 -> EM7 ->
+ * field cycle -> EM4 for up to 3 species sharing one current. */
+gasx_status gasx_em_step(gasx_ctx* ctx, gasx_species* sp, int n_species, gasx_fields* f, int n_push);
+
+/* ---- collisions -------------------------------------------------------------
+ * Replaces find_CS0.m:1-61 (sigma in m^2; NaN outside the tables as interp1). */
+gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, int64_t n,
+                                float* sig_el, float* sig_ion, float* sig_exc);
+/* Replaces electron_transport.cu + electron_collision.m:58-75: one collision
+ * check of duration dt_coll for every live electron; ionization secondaries
+ * are appended to e (slot order of their parents) and, when ions != NULL,
+ * N2+ ions with Maxwellian velocities are appended to ions.  Random numbers
+ * come from Philox4x32-10 with key = seed and counter = (slot, step, run, block). */
+gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, const gasx_gas* g,
+                         double dt_coll, uint64_t seed, uint32_t step, uint32_t run);
+
+/* ---- production loop --------------------------------------------------------
+ * electron_transport_field3.m:61-116 with one push per collision step, fused:
+ * per step one particle kernel (gather, push, collide, append, next half drift,
+ * deposit) and one field kernel.  Secondaries are appended in parent slot order.
+ * ions may be NULL. */
+gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, gasx_fields* f,
+                     const gasx_gas* g, const gasx_run_opts* o);
+
+/* Cell sort by (iz, ir) of the particle's Hphi cell; absorbed slots are dropped
+ * and count is reduced (the reference's logical-index removal,
+ * electricmagneticfield6.m:176-178). */
+gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s);
+
+/* FP64 energy tallies of electricmagneticfield5.m:167-177:
+ * out = {W_E, W_H, KE (= FN * sum of kinetic energies, J), total}. */
+gasx_status gasx_tallies(gasx_ctx* ctx, const gasx_fields* f, const gasx_species* sp, int n_species,
+                         double out[4]);
+/* Event counters accumulated by gasx_collide / gasx_run (synchronizes). */
+gasx_status gasx_get_counters(gasx_ctx* ctx, gasx_counters* out, int reset);
+/* Host copy of a device count (synchronizes). */
+gasx_status gasx_get_count(gasx_ctx* ctx, const gasx_species* s, int64_t* n);
+/* Collision rejection bound in use for gas g with step dt_coll (for tests). */
+gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, double* pmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GASX_H */

@@ -1,0 +1,739 @@
+/*
+ * oracle.c -- FP64 CPU restatement of the reference electron PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for libgasx.so.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Each function restates the reference file:line named in its comment.
+ * Where the reference's MATLAB and CUDA versions disagree, the choice is a
+ * flag in orc_params (ez_metric, axis_j_sign, exc_form, rot_eps); defaults
+ * are set by the Python side.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Tables                                                              */
+/* ------------------------------------------------------------------ */
+
+/* MATLAB interp1(x, y, xq) 'linear': NaN outside [x(1), x(end)]
+ * (find_CS0.m:21,25,33 rely on this NaN to suppress collisions). */
+double orc_interp1(const double* x, const double* y, int n, double xq)
+{
+    if (!(xq >= x[0]) || !(xq <= x[n - 1])) return NAN;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) / 2;
+        if (x[mid] <= xq) lo = mid; else hi = mid;
+    }
+    if (xq == x[lo]) return y[lo];
+    if (xq == x[hi]) return y[hi];
+    double s = (xq - x[lo]) / (x[hi] - x[lo]);
+    return y[lo] + s * (y[hi] - y[lo]);
+}
+
+/* Table preparation.
+ *  find_CS0.m:16-33  : elastic TCS (keV, a0^2) , small-energy TCS (eV, 1e-20 m^2),
+ *                      ionization TCS (eV, 1e-20 m^2)
+ *  electron_collision.m:12-33 : merged TCS / log-DCS used for angle sampling
+ *  electron_collision.m:36-41 : excitation fit parameters (columns j,A,omega,v,miu)
+ */
+int orc_tables_build(orc_tables* t,
+                     const double* t_el, int n_el,
+                     const double* t_sel, int n_sel,
+                     const double* t_sel1, int n_sel1,
+                     const double* t_del, int w_del,
+                     const double* t_sdel, int w_sdel,
+                     const double* t_ion, int n_ion,
+                     const double* t_exc, int n_exc)
+{
+    const double sqr_a0 = 2.8002852e-21;  /* electron_collision.m:12 */
+    memset(t, 0, sizeof(*t));
+    if (n_el > ORC_MAXT || n_sel > ORC_MAXT || n_ion > ORC_MAXT || n_exc > 16) return 1;
+    int n_small = n_sel1 - 1;                      /* electron_collision.m:17 drops the last row */
+    int n_dcs = (w_sdel - 1) + (w_del - 1);        /* electron_collision.m:23-25 */
+    if (n_small + n_el != n_dcs || n_dcs > ORC_MAXT) return 2;
+
+    t->n_el = n_el;
+    for (int i = 0; i < n_el; ++i) { t->el_e[i] = t_el[i]; t->el_s[i] = t_el[n_el + i]; }
+    t->n_sel = n_sel;
+    for (int i = 0; i < n_sel; ++i) { t->sel_e[i] = t_sel[2 * i]; t->sel_s[i] = t_sel[2 * i + 1]; }
+    t->n_ion = n_ion;
+    for (int i = 0; i < n_ion; ++i) { t->ion_e[i] = t_ion[2 * i]; t->ion_s[i] = t_ion[2 * i + 1]; }
+    t->n_exc = n_exc;
+    for (int i = 0; i < n_exc; ++i) {
+        t->exc_w[i] = t_exc[5 * i + 0];
+        t->exc_a[i] = t_exc[5 * i + 1];
+        t->exc_om[i] = t_exc[5 * i + 2];
+        t->exc_nu[i] = t_exc[5 * i + 3];   /* column "v"   -> exponent nu */
+        t->exc_ga[i] = t_exc[5 * i + 4];   /* column "miu" -> exponent gamma */
+    }
+
+    /* merged TCS: [t_sel_1(1:end-1) converted to keV and a0^2 per atom ; t_el'] */
+    t->n_dcs = n_dcs;
+    for (int i = 0; i < n_small; ++i)
+        t->dcs_tcs[i] = t_sel1[2 * i + 1] * 1e-20 / sqr_a0 / 2.0;   /* electron_collision.m:15 */
+    for (int i = 0; i < n_el; ++i) t->dcs_tcs[n_small + i] = t->el_s[i];
+    /* merged DCS: [t_sdel , t_del(:,2:end)]; row 0 energies, rows 1..181 angles 0..180 */
+    for (int j = 0; j < n_dcs; ++j) {
+        const double* tab; int w, col;
+        if (j < w_sdel - 1) { tab = t_sdel; w = w_sdel; col = j + 1; }
+        else { tab = t_del; w = w_del; col = j - (w_sdel - 1) + 1; }
+        t->dcs_e[j] = tab[col];
+        t->dcs_loge[j] = log(tab[col]);                              /* xInterp, :32 */
+        for (int k = 0; k < ORC_NANG; ++k)
+            t->dcs_logv[j * ORC_NANG + k] = log(tab[(k + 1) * w + col]); /* V_interp, :30 */
+    }
+    return 0;
+}
+
+/* find_CS0.m:14-56: sigma_el, sigma_ion, sigma_exc for energies in keV. */
+static void cs_one(const orc_tables* t, double e, double* se, double* si, double* sx)
+{
+    const double sqr_a0 = 2.8002852e-21;          /* find_CS0.m:17 */
+    if (e >= 0.05)
+        *se = orc_interp1(t->el_e, t->el_s, t->n_el, e) * sqr_a0 * 2.0;   /* :21 */
+    else
+        *se = orc_interp1(t->sel_e, t->sel_s, t->n_sel, e * 1000.0) * 1e-20; /* :25 */
+    double T = e * 1000.0;
+    *si = orc_interp1(t->ion_e, t->ion_s, t->n_ion, T) * 1e-20;       /* :33 */
+    const double q0 = 6.542e-14;                  /* :42 */
+    double s = 0.0;
+    for (int i = 0; i < t->n_exc; ++i) {          /* :47-55 */
+        double a = q0 * t->exc_a[i] / (t->exc_w[i] * t->exc_w[i]);
+        double b = pow(t->exc_w[i] / T, t->exc_om[i]);
+        double c = pow(1.0 - pow(t->exc_w[i] / T, t->exc_ga[i]), t->exc_nu[i]);
+        s += a * b * c;
+    }
+    *sx = s * 1e-4;                               /* :56 */
+}
+
+void orc_cross_sections(const orc_tables* t, const double* e_kev, int64_t n,
+                        double* sig_el, double* sig_ion, double* sig_exc)
+{
+    for (int64_t i = 0; i < n; ++i) cs_one(t, e_kev[i], &sig_el[i], &sig_ion[i], &sig_exc[i]);
+}
+
+/* MATLAB sind() on integer degrees: exact zeros at 0 and 180. */
+static double sind_int(int k)
+{
+    if (k == 0 || k == 180) return 0.0;
+    if (k == 90) return 1.0;
+    return sin(k * M_PI / 180.0);
+}
+
+/* Elastic polar angle, get_ElasticSc_sub1.m:4-19 with the merged tables of
+ * electron_collision.m:22-33.  griddata at the node angles 0..180 reduces to
+ * linear interpolation of log-DCS in log E along each angle row.  The CDF is
+ * cumtrapz over theta in radians and theta = interp1(CDF, theta, U*CDF_end). */
+double orc_elastic_theta(const orc_tables* t, double e_kev, double u)
+{
+    double x = log(e_kev);
+    int n = t->n_dcs, i = 0;
+    while (i < n - 2 && x > t->dcs_loge[i + 1]) ++i;
+    double w = (x - t->dcs_loge[i]) / (t->dcs_loge[i + 1] - t->dcs_loge[i]);
+    const double* v0 = &t->dcs_logv[i * ORC_NANG];
+    const double* v1 = &t->dcs_logv[(i + 1) * ORC_NANG];
+    double cdf[ORC_NANG];
+    double dth = M_PI / 180.0, prev = 0.0;
+    cdf[0] = 0.0;
+    for (int k = 0; k < ORC_NANG; ++k) {
+        double a = (1.0 - w) * v0[k] + w * v1[k];
+        double pdf = 2.0 * M_PI * sind_int(k) * exp(a) * 2.0;   /* TCS_A cancels in U*CDF_end */
+        if (k > 0) cdf[k] = cdf[k - 1] + 0.5 * (prev + pdf) * dth;
+        prev = pdf;
+    }
+    double target = u * cdf[ORC_NANG - 1];
+    int k = 0;
+    while (k < ORC_NANG - 2 && cdf[k + 1] < target) ++k;
+    double s = (target - cdf[k]) / (cdf[k + 1] - cdf[k]);
+    return (k + s) * dth;
+}
+
+/* Green-Sawada secondary energy and binary-encounter angles,
+ * get_ImpIonized.m:13-63 (== electron_transport.cu:141-171). */
+void orc_ionize(double T, double rn, double e_e,
+                double* k0o, double* ep, double* es, double* th_p, double* th_s)
+{
+    const double I = 15.6, T_a = 1000.0, T_b = 2 * I, t_b = I, T_s = 4.17, t_s = 13.8;
+    double T0 = T_s - T_a / (T + T_b);
+    double t = t_s * T / (T + t_b);
+    double Tm = (T - I) / 2.0;
+    double k1 = atan((Tm - T0) / t);
+    double k2 = atan(T0 / t);
+    double k0 = T0 + t * tan(rn * (k1 + k2) - k2);
+    if (k0 < 0.0) k0 = 0.0;   /* rounding guard: k0 > 0 for rn > 0 in exact arithmetic */
+    if (k0 > Tm) k0 = Tm;
+    double E1 = k0, E2 = T - E1 - I;
+    if (E2 > E1) { *ep = E2; *es = E1; } else { *ep = E1; *es = E2; }
+    double sp = sqrt((k0 / T) / ((1.0 - k0 / T) * T / (2.0 * e_e) + 1.0));
+    double ss = sqrt((1.0 - k0 / T) / (1.0 + k0 / (2.0 * e_e)));
+    *th_p = asin(sp);
+    *th_s = asin(ss);
+    *k0o = k0;
+}
+
+/* Mean excitation loss: get_ImpExc2.m:19-32 (form 0) or the powf variant of
+ * electron_transport.cu:212-219 (form 1).  q0 cancels in the ratio. */
+double orc_exc_edep(const orc_tables* t, double T, int form)
+{
+    double k1 = 0.0, k2 = 0.0;
+    for (int i = 0; i < t->n_exc; ++i) {
+        double W = t->exc_w[i];
+        double a = t->exc_a[i] / (W * W);
+        double b = pow(W / T, t->exc_om[i]);
+        double c = form == 0 ? pow(1.0 - pow(W / T, t->exc_ga[i]), t->exc_nu[i])
+                             : pow(pow(1.0 - W / T, t->exc_ga[i]), t->exc_nu[i]);
+        k1 += a * b * c * W;
+        k2 += a * b * c;
+    }
+    return k1 / k2;
+}
+
+/* rotate.m:4-13; the degeneracy test v^2 - vz^2 is evaluated as vx^2+vy^2. */
+void orc_rotate(double* vx, double* vy, double* vz, double th, double ph, double eps)
+{
+    double x = *vx, y = *vy, zz = *vz;
+    double vp2 = x * x + y * y;
+    double v = sqrt(vp2 + zz * zz);
+    double st = sin(th), ct = cos(th), sp = sin(ph), cp = cos(ph);
+    if (vp2 > eps) {
+        double vp = sqrt(vp2);
+        *vx = x * ct + st / vp * (x * zz * cp - v * y * sp);
+        *vy = y * ct + st / vp * (y * zz * cp + v * x * sp);
+        *vz = zz * ct - vp * st * cp;
+    } else {
+        *vx = v * st * cp;
+        *vy = v * st * sp;
+        *vz = v * ct;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Philox4x32-10, Salmon, Moraes, Dror & Shaw, "Parallel random numbers: */
+/* as easy as 1, 2, 3", SC'11.                                          */
+/* ------------------------------------------------------------------ */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* u in (0,1): (2k+1) * 2^-24 with k the top 23 bits; exact in FP32 and FP64. */
+double orc_u01(uint32_t x)
+{
+    return (double)((x >> 8) | 1u) * (1.0 / 16777216.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* FDTD                                                                */
+/* ------------------------------------------------------------------ */
+
+/* EM3_1.cu:9-15 : Hphi += tau/mu * (dEz/dr - dEr/dz). */
+void orc_fdtd_h(double* h, const double* er, const double* ez, int nr, int nz,
+                double mu, double dr, double dz, double tau)
+{
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nr; ++i) {
+            double dez = ez[(i + 1) + k * (nr + 1)] - ez[i + k * (nr + 1)];
+            double der = er[i + (k + 1) * nr] - er[i + k * nr];
+            h[i + k * nr] = h[i + k * nr] + (tau / (mu * dr) * dez - tau / (mu * dz) * der);
+        }
+}
+
+/* EM3.cu:14-42 : Jacobi E update (Er0/Ez0 are the pre-update copies,
+ * electricmagneticfield6.m:149 passes the same arrays as inputs and outputs). */
+void orc_fdtd_e(double* er, double* ez, const double* h, const double* jr, const double* jz,
+                int nr, int nz, double eps, double dr, double dz, double tau,
+                int ez_metric, int axis_j_sign)
+{
+    size_t ner = (size_t)nr * (nz + 1), nez = (size_t)(nr + 1) * nz;
+    double* er0 = (double*)malloc(ner * sizeof(double));
+    double* ez0 = (double*)malloc(nez * sizeof(double));
+    memcpy(er0, er, ner * sizeof(double));
+    memcpy(ez0, ez, nez * sizeof(double));
+    for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i < nr; ++i) {
+            int n = i + k * nr;
+            if (k == 0) er[n] = er0[n + nr];                  /* :22 */
+            else if (k == nz) er[n] = er0[n - nr];            /* :26 */
+            else er[n] = er0[n] - tau / eps * (jr[n - nr] + jr[n]) / 2
+                         - tau / (eps * dz) * (h[n] - h[n - nr]);   /* :29 */
+        }
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nr; ++i) {                        /* Ez(nr,k) is never written */
+            int nzi = i + k * (nr + 1), n = i + k * nr;
+            if (i == 0)
+                ez[nzi] = ez0[nzi] + axis_j_sign * tau / eps * jz[n] + 4 * tau / (eps * dr) * h[n]; /* :35 */
+            else {
+                double rs = (ez_metric ? (i + 1) : i) * dr;                                  /* :39 (D1) */
+                ez[nzi] = ez0[nzi] - tau / eps * (jz[n - 1] + jz[n]) / 2
+                          + tau / eps * (1 / (2 * rs) + 1 / dr) * h[n]
+                          + tau / eps * (1 / (2 * rs) - 1 / dr) * h[n - 1];
+            }
+        }
+    free(er0);
+    free(ez0);
+}
+
+/* electricmagneticfield6.m:146-151: H back half a substep, n_sub leapfrog
+ * substeps with J frozen, H forward half a substep.  Substep = ddt/n_sub. */
+void orc_fdtd_push(const orc_params* p, double* er, double* ez, double* h,
+                   const double* jr, const double* jz, int n_push)
+{
+    double dtf = p->ddt / p->n_sub;
+    for (int s = 0; s < n_push; ++s) {
+        orc_fdtd_h(h, er, ez, p->nr, p->nz, p->mu0, p->dr, p->dz, -dtf / 2);
+        for (int i = 0; i < p->n_sub; ++i) {
+            orc_fdtd_h(h, er, ez, p->nr, p->nz, p->mu0, p->dr, p->dz, dtf);
+            orc_fdtd_e(er, ez, h, jr, jz, p->nr, p->nz, p->eps0, p->dr, p->dz, dtf,
+                       p->ez_metric, p->axis_j_sign);
+        }
+        orc_fdtd_h(h, er, ez, p->nr, p->nz, p->mu0, p->dr, p->dz, dtf / 2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Particles                                                           */
+/* ------------------------------------------------------------------ */
+
+/* wedge wrap with velocity rotation + z reflection: EM2.cu:31-56 / EM4.cu:152-177 */
+static void wedge_bc(const orc_params* p, double* x, double* y, double* z,
+                     double* vx, double* vy, double* vz, double* r, double* phi, int32_t* del)
+{
+    double Rp = p->nr * p->dr, Lp = p->nz * p->dz, PHIp = p->phip;
+    *r = sqrt(*x * *x + *y * *y);
+    *phi = atan(*y / *x);
+    if (*r > Rp) *del = 1;
+    double cP = cos(PHIp), sP = sin(PHIp);
+    while (*phi < 0) {
+        *phi += PHIp;
+        double vx1 = *vx * cP - *vy * sP;
+        *vy = *vx * sP + *vy * cP;
+        *vx = vx1;
+    }
+    while (*phi > PHIp) {
+        *phi -= PHIp;
+        double vx1 = *vx * cP + *vy * sP;
+        *vy = -*vx * sP + *vy * cP;
+        *vx = vx1;
+    }
+    *x = *r * cos(*phi);
+    *y = *r * sin(*phi);
+    if (*z > Lp) { *z = 2 * Lp - *z; *vz = -*vz; }
+    if (*z < 0) { *z = -*z; *vz = -*vz; }
+}
+
+/* EM2.cu:23-57: half drift, wedge/z boundary conditions, v_r. */
+void orc_em2(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+             double* r, double* phi, double* vr, int32_t* del, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        if (del[i] == 1) continue;
+        x[i] += 0.5 * vx[i] * p->ddt;
+        y[i] += 0.5 * vy[i] * p->ddt;
+        z[i] += 0.5 * vz[i] * p->ddt;
+        wedge_bc(p, &x[i], &y[i], &z[i], &vx[i], &vy[i], &vz[i], &r[i], &phi[i], &del[i]);
+        vr[i] = vx[i] * cos(phi[i]) + vy[i] * sin(phi[i]);
+    }
+}
+
+/* EM7.cu:51-202: clamped area weighting onto the Hphi grid,
+ * J += FN*b*q*v / V(i),  V(i) = PHIp/2*((i+1)^2-i^2) dr^2 dz. */
+void orc_em7(const orc_params* p, const double* r, const double* z, const double* q,
+             const double* vr, const double* vz, double* jr, double* jz,
+             const int32_t* del, int64_t n)
+{
+    int nr = p->nr, nz = p->nz;
+    double dr = p->dr, dz = p->dz;
+    for (int64_t m = 0; m < n; ++m) {
+        if (del[m] == 1) continue;       /* EM7.cu:163-173 zero weights */
+        double r0 = r[m], z0 = z[m];
+        int ar = (int)floor(r0 / dr - 0.5), az = (int)floor(z0 / dz - 0.5);
+        int ci[2], ck[2]; double wr[2], wz[2]; int ncr, ncz;
+        if (ar < 0) { ncr = 1; ci[0] = ar + 1; wr[0] = 1.0; }
+        else if (ar >= nr - 1) { ncr = 1; ci[0] = ar; wr[0] = 1.0; }
+        else { ncr = 2; ci[0] = ar; ci[1] = ar + 1;
+               wr[0] = ((ar + 1.5) * dr - r0) / dr; wr[1] = (r0 - (ar + 0.5) * dr) / dr; }
+        if (az < 0) { ncz = 1; ck[0] = az + 1; wz[0] = 1.0; }
+        else if (az >= nz - 1) { ncz = 1; ck[0] = az; wz[0] = 1.0; }
+        else { ncz = 2; ck[0] = az; ck[1] = az + 1;
+               wz[0] = ((az + 1.5) * dz - z0) / dz; wz[1] = (z0 - (az + 0.5) * dz) / dz; }
+        for (int a = 0; a < ncr; ++a) {
+            int i = ci[a];
+            double V = p->phip / 2 * ((((i + 1) * dr) * ((i + 1) * dr) - (i * dr) * (i * dr)) * dz);
+            for (int b = 0; b < ncz; ++b) {
+                double w = wr[a] * wz[b];
+                jr[i + ck[b] * nr] += p->fn * w * q[m] * vr[m] / V;
+                jz[i + ck[b] * nr] += p->fn * w * q[m] * vz[m] / V;
+            }
+        }
+    }
+}
+
+static int ifloor(double v) { return (int)floor(v); }
+
+/* EM4.cu:36-177: bilinear gather with the EM4 edge rules, Lorentz force with
+ * v^n in the magnetic term, relativistic momentum update, half drift + BC. */
+void orc_em4(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+             double* r, double* phi, const double* vr, const double* er, const double* ez,
+             const double* h, const double* q, const double* m, double* ke, int32_t* del, int64_t n)
+{
+    int R = p->nr, Z = p->nz;
+    double dr = p->dr, dz = p->dz, c = p->c, mu = p->mu0, dt = p->ddt;
+    for (int64_t i = 0; i < n; ++i) {
+        if (del[i] == 1) continue;
+        double r0 = r[i], z0 = z[i];
+        int ar, az;
+        double E_r, E_z, Hp;
+        /* Er (EM4.cu:41-60); indices past the array end carry zero weight */
+        ar = ifloor(r0 / dr - 0.5); az = ifloor(z0 / dz);
+        {
+            int kz1 = az + 1 > Z ? Z : az + 1;
+            double wz0 = (az * dz + dz - z0), wz1 = (z0 - az * dz);
+            if (ar < 0) E_r = (er[(ar + 1) + az * R] * wz0 + er[(ar + 1) + kz1 * R] * wz1) / dz;
+            else if (ar >= R - 1) E_r = (er[ar + az * R] * wz0 + er[ar + kz1 * R] * wz1) / dz;
+            else E_r = (er[ar + az * R] * ((ar + 1.5) * dr - r0) * ((az + 1) * dz - z0)
+                        + er[(ar + 1) + az * R] * (r0 - (ar + 0.5) * dr) * ((az + 1) * dz - z0)
+                        + er[ar + kz1 * R] * ((ar + 1.5) * dr - r0) * (z0 - az * dz)
+                        + er[(ar + 1) + kz1 * R] * (r0 - (ar + 0.5) * dr) * (z0 - az * dz)) / (dr * dz);
+        }
+        /* Ez (EM4.cu:61-80) */
+        ar = ifloor(r0 / dr); az = ifloor(z0 / dz - 0.5);
+        {
+            int ir1 = ar + 1 > R ? R : ar + 1;
+            double wr0 = (ar * dr + dr - r0), wr1 = (r0 - ar * dr);
+            if (az < 0) E_z = (ez[ar + (az + 1) * (R + 1)] * wr0 + ez[ir1 + (az + 1) * (R + 1)] * wr1) / dr;
+            else if (az >= Z - 1) E_z = (ez[ar + az * (R + 1)] * wr0 + ez[ir1 + az * (R + 1)] * wr1) / dr;
+            else E_z = (ez[ar + az * (R + 1)] * ((ar + 1) * dr - r0) * ((az + 1.5) * dz - z0)
+                        + ez[ir1 + az * (R + 1)] * (r0 - ar * dr) * ((az + 1.5) * dz - z0)
+                        + ez[ar + (az + 1) * (R + 1)] * ((ar + 1) * dr - r0) * (z0 - (az + 0.5) * dz)
+                        + ez[ir1 + (az + 1) * (R + 1)] * (r0 - ar * dr) * (z0 - (az + 0.5) * dz)) / (dr * dz);
+        }
+        /* Hphi (EM4.cu:82-122) */
+        ar = ifloor(r0 / dr - 0.5); az = ifloor(z0 / dz - 0.5);
+        if (ar < 0) {
+            if (az < 0) Hp = h[(ar + 1) + (az + 1) * R];
+            else if (az >= Z - 1) Hp = h[(ar + 1) + az * R];
+            else Hp = (h[(ar + 1) + az * R] * ((az + 1.5) * dz - z0) + h[(ar + 1) + (az + 1) * R] * (z0 - (az + 0.5) * dz)) / dz;
+        } else if (ar >= R - 1) {
+            if (az < 0) Hp = h[ar + (az + 1) * R];
+            else if (az >= Z - 1) Hp = h[ar + az * R];
+            else Hp = (h[ar + az * R] * ((az + 1.5) * dz - z0) + h[ar + (az + 1) * R] * (z0 - (az + 0.5) * dz)) / dz;
+        } else if (az < 0) {
+            Hp = (h[ar + (az + 1) * R] * ((ar + 1.5) * dr - r0) + h[(ar + 1) + (az + 1) * R] * (r0 - (ar + 0.5) * dr)) / dr;
+        } else if (az >= Z - 1) {
+            Hp = (h[ar + az * R] * ((ar + 1.5) * dr - r0) + h[(ar + 1) + az * R] * (r0 - (ar + 0.5) * dr)) / dr;
+        } else {
+            Hp = (h[ar + az * R] * ((ar + 1.5) * dr - r0) * ((az + 1.5) * dz - z0)
+                  + h[(ar + 1) + az * R] * (r0 - (ar + 0.5) * dr) * ((az + 1.5) * dz - z0)
+                  + h[ar + (az + 1) * R] * ((ar + 1.5) * dr - r0) * (z0 - (az + 0.5) * dz)
+                  + h[(ar + 1) + (az + 1) * R] * (r0 - (ar + 0.5) * dr) * (z0 - (az + 0.5) * dz)) / (dr * dz);
+        }
+        /* force, EM4.cu:126-129 */
+        double Fr = q[i] * (E_r + (-vz[i] * Hp * mu));
+        double Fz = q[i] * (E_z + (vr[i] * Hp * mu));
+        double Fx = Fr * cos(phi[i]), Fy = Fr * sin(phi[i]);
+        /* EM4.cu:134-144 */
+        double g = 1 / sqrt(1 - (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]) / (c * c));
+        double ux = g * vx[i] + Fx / m[i] * dt;
+        double uy = g * vy[i] + Fy / m[i] * dt;
+        double uz = g * vz[i] + Fz / m[i] * dt;
+        g = sqrt(1 + (ux * ux + uy * uy + uz * uz) / (c * c));
+        if (ke) ke[i] = (g - 1) * m[i] * c * c;
+        vx[i] = ux / g; vy[i] = uy / g; vz[i] = uz / g;
+        /* EM4.cu:146-177 */
+        x[i] += 0.5 * dt * vx[i];
+        y[i] += 0.5 * dt * vy[i];
+        z[i] += 0.5 * dt * vz[i];
+        wedge_bc(p, &x[i], &y[i], &z[i], &vx[i], &vy[i], &vz[i], &r[i], &phi[i], &del[i]);
+    }
+}
+
+/* electricmagneticfield6.m:138-156 for n_push pushes. */
+void orc_em_step(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                 const double* q, const double* m, double* ke, int32_t* del, int64_t n,
+                 double* er, double* ez, double* h, double* jr, double* jz, int n_push)
+{
+    double* r = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double* phi = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double* vr = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    size_t ncell = (size_t)p->nr * p->nz;
+    for (int s = 0; s < n_push; ++s) {
+        orc_em2(p, x, y, z, vx, vy, vz, r, phi, vr, del, n);
+        memset(jr, 0, ncell * sizeof(double));
+        memset(jz, 0, ncell * sizeof(double));
+        orc_em7(p, r, z, q, vr, vz, jr, jz, del, n);
+        orc_fdtd_push(p, er, ez, h, jr, jz, 1);
+        orc_em4(p, x, y, z, vx, vy, vz, r, phi, vr, er, ez, h, q, m, ke, del, n);
+    }
+    free(r); free(phi); free(vr);
+}
+
+/* Fixed-point emulation of the HIP deposit.  The HIP kernel computes, in
+ * FP32 with contraction off, the clamped area weights of EM7.cu:56-162 in
+ * normalized coordinates and accumulates llrint(w * v * qnum * 2^32 / c)
+ * into int64 cells; the physical current is FN*e*c*2^-32 * Q / V(i). */
+void orc_deposit_fx32(const orc_params* p, const float* x, const float* y, const float* z,
+                      const float* vx, const float* vy, const float* vz, int64_t n, float qnum,
+                      int64_t* jr, int64_t* jz)
+{
+    const int nr = p->nr, nz = p->nz;
+    const float inv_dr = (float)(1.0 / p->dr), inv_dz = (float)(1.0 / p->dz);
+    const float kq = (float)(4294967296.0 / p->c) * qnum;
+    for (int64_t m = 0; m < n; ++m) {
+        float xx = x[m], yy = y[m], zz = z[m];
+        if (!(zz == zz)) continue;                     /* absorbed particle */
+        float r2 = xx * xx;
+        r2 = r2 + yy * yy;
+        float r = sqrtf(r2);
+        float vr;
+        if (r > 0.0f) { float t = vx[m] * xx; t = t + vy[m] * yy; vr = t / r; }
+        else vr = vx[m];
+        float qr = vr * kq, qz = vz[m] * kq;
+        float sr = r * inv_dr; sr = sr - 0.5f;
+        float sz = zz * inv_dz; sz = sz - 0.5f;
+        int ar = (int)floorf(sr), az = (int)floorf(sz);
+        float fr = sr - (float)ar, fz = sz - (float)az;
+        int i0, i1, k0, k1; float wr0, wr1, wz0, wz1;
+        if (ar < 0) { i0 = 0; i1 = 0; wr0 = 1.0f; wr1 = 0.0f; }
+        else if (ar >= nr - 1) { i0 = nr - 1; i1 = nr - 1; wr0 = 1.0f; wr1 = 0.0f; }
+        else { i0 = ar; i1 = ar + 1; wr0 = 1.0f - fr; wr1 = fr; }
+        if (az < 0) { k0 = 0; k1 = 0; wz0 = 1.0f; wz1 = 0.0f; }
+        else if (az >= nz - 1) { k0 = nz - 1; k1 = nz - 1; wz0 = 1.0f; wz1 = 0.0f; }
+        else { k0 = az; k1 = az + 1; wz0 = 1.0f - fz; wz1 = fz; }
+        int cells[4] = { i0 + k0 * nr, i1 + k0 * nr, i0 + k1 * nr, i1 + k1 * nr };
+        float w[4] = { wr0 * wz0, wr1 * wz0, wr0 * wz1, wr1 * wz1 };
+        for (int a = 0; a < 4; ++a) {
+            float tr = w[a] * qr, tz = w[a] * qz;
+            jr[cells[a]] += llrintf(tr);
+            jz[cells[a]] += llrintf(tz);
+        }
+    }
+}
+
+void orc_j_from_fixed(const orc_params* p, const int64_t* qr, const int64_t* qz, double* jr, double* jz)
+{
+    double unit = p->fn * p->e_em * p->c / 4294967296.0;
+    double vcoef = p->phip / 2 * p->dr * p->dr * p->dz;
+    for (int k = 0; k < p->nz; ++k)
+        for (int i = 0; i < p->nr; ++i) {
+            double V = vcoef * (double)((i + 1) * (i + 1) - i * i);
+            jr[i + k * p->nr] = unit * (double)qr[i + k * p->nr] / V;
+            jz[i + k * p->nr] = unit * (double)qz[i + k * p->nr] / V;
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Collisions                                                          */
+/* ------------------------------------------------------------------ */
+
+/* speed from kinetic energy in eV: findV.m:5-8 in the cancellation-free form
+ * beta = sqrt(g1*(g1+2))/(1+g1), g1 = E*e/(m c^2). */
+static double speed_from_ev(double E, double e, double m, double c)
+{
+    double g1 = E * e / (m * c * c);
+    return c * sqrt(g1 * (g1 + 2.0)) / (1.0 + g1);
+}
+
+/* kinetic energy in eV from speed: EM4.cu:138-140 with /(e) of
+ * electricmagneticfield6.m:172, written as m c^2 * beta^2 g^2/(1+g). */
+static double ev_from_speed(double v2, double e, double m, double c)
+{
+    double b2 = v2 / (c * c);
+    double g2 = 1.0 / (1.0 - b2);
+    double g = sqrt(g2);
+    return m * c * c * (b2 * g2 / (1.0 + g)) / e;
+}
+
+static void gas_lookup(const orc_gas* g, double r, double z, double* n, double* T)
+{
+    int ir = (int)floor(r / g->gdr), iz = (int)floor(z / g->gdz);
+    if (ir < 0) ir = 0;
+    if (ir > g->gnr - 1) ir = g->gnr - 1;
+    if (iz < 0) iz = 0;
+    if (iz > g->gnz - 1) iz = g->gnz - 1;
+    *n = g->dens[iz * g->gnr + ir];
+    *T = g->temp ? g->temp[iz * g->gnr + ir] : g->t_scalar;
+}
+
+/* electron_transport.cu:64-227 with the MATLAB choices (find_CS0.m cross
+ * sections, NaN gating, get_ElasticSc_sub1.m angles, get_ImpExc2.m loss).
+ * Random numbers: Philox4x32-10, key = seed, counter = (slot, step, run, block). */
+void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
+                 double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                 const int32_t* del, int64_t n, double dt_coll,
+                 uint64_t seed, uint32_t step, uint32_t run,
+                 int8_t* ev, double* sec_v, double* ion_v)
+{
+    const double I = 15.6;
+    const double c = p->c;
+    const double e_e = p->me_mc * c * c / p->e_mc;      /* electron_transport.cu:52 */
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    for (int64_t m = 0; m < n; ++m) {
+        ev[m] = 0;
+        if (del[m] == 1) continue;
+        double v2 = vx[m] * vx[m] + vy[m] * vy[m] + vz[m] * vz[m];
+        double ve = sqrt(v2);
+        double T = ev_from_speed(v2, p->e_em, p->me_em, c);     /* eV */
+        uint32_t ctr[4] = { (uint32_t)m, step, run, 0 }, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        double u1 = orc_u01(o[0]), u2 = orc_u01(o[1]), u3 = orc_u01(o[2]), u4 = orc_u01(o[3]);
+        if (!(T >= 15.6 && T <= 20000.0)) continue;    /* interp1 NaN gating */
+        double se, si, sx;
+        cs_one(t, T / 1000.0, &se, &si, &sx);
+        double r = sqrt(x[m] * x[m] + y[m] * y[m]);
+        double dens, Tg;
+        gas_lookup(g, r, z[m], &dens, &Tg);
+        double st = se + si + sx;
+        double pt = ve * dt_coll * dens * st;         /* electron_transport.cu:74-77 */
+        if (!(u1 < pt)) continue;                      /* :85 */
+        if (u2 * st < se) {                            /* :86 elastic */
+            double th = orc_elastic_theta(t, T / 1000.0, u3);
+            double ph = 2 * M_PI * u4;
+            orc_rotate(&vx[m], &vy[m], &vz[m], th, ph, p->rot_eps);
+            ev[m] = 1;
+        } else if (u2 * st < se + si && T > I) {      /* :139 ionization */
+            double k0, ep, es, thp, ths;
+            orc_ionize(T, u3, e_e, &k0, &ep, &es, &thp, &ths);
+            double php = 2 * M_PI * u4, phs = php - M_PI;
+            double vp = speed_from_ev(ep, p->e_mc, p->me_mc, c);
+            double ratio = vp / ve;
+            vx[m] *= ratio; vy[m] *= ratio; vz[m] *= ratio;
+            orc_rotate(&vx[m], &vy[m], &vz[m], thp, php, p->rot_eps);
+            double vs = speed_from_ev(es, p->e_mc, p->me_mc, c);
+            sec_v[3 * m + 0] = vs * sin(ths) * cos(phs);   /* :198-200 */
+            sec_v[3 * m + 1] = vs * sin(ths) * sin(phs);
+            sec_v[3 * m + 2] = vs * cos(ths);
+            if (ion_v) {                                    /* :202-204 */
+                uint32_t ctr2[4] = { (uint32_t)m, step, run, 1 }, o2[4];
+                orc_philox4x32_10(ctr2, key, o2);
+                double a1 = orc_u01(o2[0]), a2 = orc_u01(o2[1]), a3 = orc_u01(o2[2]), a4 = orc_u01(o2[3]);
+                double rr1 = sqrt(-2 * log(a1)), rr2 = sqrt(-2 * log(a3));
+                double sg = sqrt(p->k_b * Tg / p->m_ion);
+                ion_v[3 * m + 0] = sg * rr1 * cos(2 * M_PI * a2);
+                ion_v[3 * m + 1] = sg * rr1 * sin(2 * M_PI * a2);
+                ion_v[3 * m + 2] = sg * rr2 * cos(2 * M_PI * a4);
+            }
+            ev[m] = 2;
+        } else {                                        /* :206 excitation */
+            double edep = orc_exc_edep(t, T, p->exc_form);
+            double vp = speed_from_ev(T - edep, p->e_mc, p->me_mc, c);
+            double ratio = vp / ve;
+            vx[m] *= ratio; vy[m] *= ratio; vz[m] *= ratio;
+            ev[m] = 3;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Tallies: electricmagneticfield5.m:167-177                            */
+/* ------------------------------------------------------------------ */
+static double trapz2(const double* f, int ni, int nk, int ld, double r0, double dr, double dz, int rweight)
+{
+    /* trapz over i (r coordinate r0+i*dr) then over k (spacing dz) of f*r */
+    double tot = 0.0;
+    for (int k = 0; k < nk; ++k) {
+        double row = 0.0;
+        for (int i = 0; i + 1 < ni; ++i) {
+            double ra = r0 + i * dr, rb = r0 + (i + 1) * dr;
+            double fa = f[i + k * ld] * (rweight ? ra : 1), fb = f[i + 1 + k * ld] * (rweight ? rb : 1);
+            row += 0.5 * (fa + fb) * dr;
+        }
+        double wk = (k == 0 || k == nk - 1) ? 0.5 : 1.0;
+        if (nk == 1) wk = 0.0;
+        tot += wk * row * dz;
+    }
+    return tot;
+}
+
+void orc_tallies(const orc_params* p, const double* er, const double* ez, const double* h,
+                 const double* vx, const double* vy, const double* vz, const double* m,
+                 const int32_t* del, int64_t n, double out[4])
+{
+    int R = p->nr, Z = p->nz;
+    size_t ner = (size_t)R * (Z + 1), nez = (size_t)(R + 1) * Z, nh = (size_t)R * Z;
+    double* s = (double*)malloc((ner > nez ? (ner > nh ? ner : nh) : (nez > nh ? nez : nh)) * sizeof(double));
+    for (size_t i = 0; i < ner; ++i) s[i] = er[i] * er[i];
+    double wer = trapz2(s, R, Z + 1, R, p->dr / 2, p->dr, p->dz, 1);
+    for (size_t i = 0; i < nez; ++i) s[i] = ez[i] * ez[i];
+    double wez = trapz2(s, R + 1, Z, R + 1, 0.0, p->dr, p->dz, 1);
+    for (size_t i = 0; i < nh; ++i) s[i] = h[i] * h[i];
+    double wh = trapz2(s, R, Z, R, p->dr / 2, p->dr, p->dz, 1);
+    free(s);
+    double ke = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (del[i] == 1) continue;
+        double v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+        ke += ev_from_speed(v2, 1.0, m[i], p->c);     /* joules */
+    }
+    out[0] = p->phip * 0.5 * p->eps0 * (wer + wez);
+    out[1] = p->phip * 0.5 * p->mu0 * wh;
+    out[2] = p->fn * ke;
+    out[3] = out[0] + out[1] + out[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* Transport loop: per step EM push (field6 sequence, 1 push) then a    */
+/* collision step with dt_coll = ddt (electron_transport_field3.m:61-109 */
+/* with one push per collision step); secondaries appended in slot order */
+/* (electron_collision.m:68-75).  Absorbed particles keep their slot.   */
+/* ------------------------------------------------------------------ */
+int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
+            double* x, double* y, double* z, double* vx, double* vy, double* vz, int32_t* del,
+            int64_t* n_io, int64_t cap, double* er, double* ez, double* h,
+            int n_steps, int collide, int secondaries, uint64_t seed, uint32_t run,
+            uint32_t step0, orc_counts* counts)
+{
+    int64_t n = *n_io;
+    size_t ncell = (size_t)p->nr * p->nz;
+    double* jr = (double*)calloc(ncell, sizeof(double));
+    double* jz = (double*)calloc(ncell, sizeof(double));
+    double* q = (double*)malloc((size_t)cap * sizeof(double));
+    double* m = (double*)malloc((size_t)cap * sizeof(double));
+    int8_t* ev = (int8_t*)malloc((size_t)cap);
+    double* sv = (double*)malloc((size_t)cap * 3 * sizeof(double));
+    if (!jr || !jz || !q || !m || !ev || !sv) return 1;
+    for (int64_t i = 0; i < cap; ++i) { q[i] = -p->e_em; m[i] = p->me_em; }
+    memset(counts, 0, sizeof(*counts));
+    for (int s = 0; s < n_steps; ++s) {
+        orc_em_step(p, x, y, z, vx, vy, vz, q, m, NULL, del, n, er, ez, h, jr, jz, 1);
+        if (!collide) continue;
+        orc_collide(p, t, g, x, y, z, vx, vy, vz, del, n, p->ddt, seed, step0 + (uint32_t)s, run,
+                    ev, sv, NULL);
+        int64_t n0 = n;
+        for (int64_t i = 0; i < n0; ++i) {
+            if (ev[i] == 1) counts->n_elastic++;
+            else if (ev[i] == 3) counts->n_excite++;
+            else if (ev[i] == 2) {
+                counts->n_ionize++;
+                if (!secondaries) continue;
+                if (n >= cap) { counts->n_dropped++; continue; }
+                x[n] = x[i]; y[n] = y[i]; z[n] = z[i];
+                vx[n] = sv[3 * i]; vy[n] = sv[3 * i + 1]; vz[n] = sv[3 * i + 2];
+                del[n] = 0;
+                ++n;
+            }
+        }
+    }
+    *n_io = n;
+    free(jr); free(jz); free(q); free(m); free(ev); free(sv);
+    return 0;
+}

@@ -1,0 +1,143 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the reference electron PIC step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * product path (libgasx.so): only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it.  Nothing in the product links,
+ * imports or falls back to it.
+ *
+ * Every function cites the reference file:line it restates.  Reference
+ * files are MATLAB (.m) and CUDA (.cu) sources of ShanmeiHE/GAS-ATENUATOR.
+ *
+ * Array conventions (reference semantics, 0-based, r fastest):
+ *   Hphi  nr x nz        index i + k*nr        (EM3_1.cu:12)
+ *   Er    nr x (nz+1)    index i + k*nr        (EM3.cu:18)
+ *   Ez    (nr+1) x nz    index i + k*(nr+1)    (EM3.cu:19)
+ *   Jr,Jz nr x nz        on the Hphi grid      (EM7.cu:56-57)
+ */
+#ifndef GASX_ORACLE_H
+#define GASX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NANG 181
+#define ORC_MAXT 64
+
+typedef struct {
+    int32_t nr, nz;
+    double dr, dz;
+    double phip;        /* wedge angle PHIp */
+    double ddt;         /* push step (electricmagneticfield6.m:135, ddt = ratio*dt) */
+    int32_t n_sub;      /* field substeps per push (electricmagneticfield6.m:147, 2*ratio) */
+    double fn;          /* macro weight FN */
+    double c, eps0, mu0;      /* electricmagneticfield6.m:8-10 */
+    double e_em, me_em;       /* electricmagneticfield6.m:5,11 */
+    double e_mc, me_mc;       /* electron_transport.cu:47,49 */
+    double m_ion, k_b;        /* electron_transport.cu:50-51 */
+    int32_t ez_metric;        /* 0: 1/(2r) with r = i*dr; 1: r=(i+1)*dr as EM3.cu:39 */
+    int32_t axis_j_sign;      /* +1 as EM3.cu:35, -1 alternative */
+    int32_t exc_form;         /* 0: (1-(W/T)^g)^nu (find_CS0.m:52); 1: ((1-W/T)^g)^nu (electron_transport.cu:215) */
+    double rot_eps;           /* rotate.m:5 threshold on vx^2+vy^2 */
+} orc_params;
+
+typedef struct {
+    int32_t n_el;  double el_e[ORC_MAXT], el_s[ORC_MAXT];    /* keV ; a0^2 per atom */
+    int32_t n_sel; double sel_e[ORC_MAXT], sel_s[ORC_MAXT];  /* eV ; 1e-20 m^2 */
+    int32_t n_ion; double ion_e[ORC_MAXT], ion_s[ORC_MAXT];  /* eV ; 1e-20 m^2 */
+    int32_t n_exc; double exc_w[16], exc_a[16], exc_om[16], exc_nu[16], exc_ga[16];
+    int32_t n_dcs;                       /* merged DCS energies (36) */
+    double dcs_e[ORC_MAXT];              /* keV */
+    double dcs_loge[ORC_MAXT];           /* log(E keV) = xInterp */
+    double dcs_tcs[ORC_MAXT];            /* yInterp, a0^2 per atom */
+    double dcs_logv[ORC_MAXT * ORC_NANG];/* V_interp, [e*181 + angle] */
+} orc_tables;
+
+/* ---- tables (electron_collision.m:12-41, find_CS0.m:14-56) ---- */
+int orc_tables_build(orc_tables* t,
+                     const double* t_el, int n_el,      /* 2 x n_el row-major */
+                     const double* t_sel, int n_sel,    /* n_sel x 2 */
+                     const double* t_sel1, int n_sel1,  /* n_sel1 x 2 */
+                     const double* t_del, int w_del,    /* 182 x w_del */
+                     const double* t_sdel, int w_sdel,  /* 182 x w_sdel */
+                     const double* t_ion, int n_ion,    /* n_ion x 2 */
+                     const double* t_exc, int n_exc);   /* n_exc x 5 */
+double orc_interp1(const double* x, const double* y, int n, double xq);
+void orc_cross_sections(const orc_tables* t, const double* e_kev, int64_t n,
+                        double* sig_el, double* sig_ion, double* sig_exc);
+double orc_elastic_theta(const orc_tables* t, double e_kev, double u);
+void orc_ionize(double t_ev, double rn, double e_e,
+                double* k0, double* ep, double* es, double* th_p, double* th_s);
+double orc_exc_edep(const orc_tables* t, double t_ev, int form);
+void orc_rotate(double* vx, double* vy, double* vz, double th, double ph, double eps);
+
+/* ---- Philox4x32-10 (Salmon et al., SC'11) ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_u01(uint32_t x);
+
+/* ---- FDTD (EM3_1.cu:2-17, EM3.cu:2-43, electricmagneticfield6.m:146-151) ---- */
+void orc_fdtd_h(double* h, const double* er, const double* ez, int nr, int nz,
+                double mu, double dr, double dz, double tau);
+void orc_fdtd_e(double* er, double* ez, const double* h, const double* jr, const double* jz,
+                int nr, int nz, double eps, double dr, double dz, double tau,
+                int ez_metric, int axis_j_sign);
+void orc_fdtd_push(const orc_params* p, double* er, double* ez, double* h,
+                   const double* jr, const double* jz, int n_push);
+
+/* ---- particles (EM2.cu, EM4.cu, EM7.cu) ---- */
+void orc_em2(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+             double* r, double* phi, double* vr, int32_t* del, int64_t n);
+void orc_em7(const orc_params* p, const double* r, const double* z, const double* q,
+             const double* vr, const double* vz, double* jr, double* jz,
+             const int32_t* del, int64_t n);
+void orc_em4(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+             double* r, double* phi, const double* vr, const double* er, const double* ez,
+             const double* h, const double* q, const double* m, double* ke, int32_t* del, int64_t n);
+/* one field6 push sequence (electricmagneticfield6.m:138-156) on a mixed-species set */
+void orc_em_step(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                 const double* q, const double* m, double* ke, int32_t* del, int64_t n,
+                 double* er, double* ez, double* h, double* jr, double* jz, int n_push);
+
+/* fixed-point emulation of the HIP deposit (bit-exact target) */
+void orc_deposit_fx32(const orc_params* p, const float* x, const float* y, const float* z,
+                      const float* vx, const float* vy, const float* vz, int64_t n, float qnum,
+                      int64_t* jr, int64_t* jz);
+void orc_j_from_fixed(const orc_params* p, const int64_t* qr, const int64_t* qz, double* jr, double* jz);
+
+/* ---- collisions (electron_transport.cu:64-227 with find_CS0.m / MATLAB choices) ---- */
+typedef struct {
+    const double* dens;  /* gas density n[iz*gnr + ir] (m^-3) */
+    const double* temp;  /* gas temperature (K), may be NULL -> t_scalar */
+    int32_t gnr, gnz; double gdr, gdz; double t_scalar;
+} orc_gas;
+
+/* event codes written to ev[]: 0 none, 1 elastic, 2 ionization, 3 excitation */
+void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
+                 double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                 const int32_t* del, int64_t n, double dt_coll,
+                 uint64_t seed, uint32_t step, uint32_t run,
+                 int8_t* ev, double* sec_v /* 3*n: secondary velocity */,
+                 double* ion_v /* 3*n or NULL */);
+
+/* ---- tallies (electricmagneticfield5.m:167-177) ---- */
+void orc_tallies(const orc_params* p, const double* er, const double* ez, const double* h,
+                 const double* vx, const double* vy, const double* vz, const double* m,
+                 const int32_t* del, int64_t n, double out[4]);
+
+/* ---- the full transport loop used as the CPU baseline ---- */
+typedef struct {
+    int64_t n_elastic, n_ionize, n_excite, n_dropped;
+} orc_counts;
+/* arrays have capacity cap; *n is updated with appended secondaries */
+int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
+            double* x, double* y, double* z, double* vx, double* vy, double* vz, int32_t* del,
+            int64_t* n, int64_t cap, double* er, double* ez, double* h,
+            int n_steps, int collide, int secondaries, uint64_t seed, uint32_t run,
+            uint32_t step0, orc_counts* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

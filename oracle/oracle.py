@@ -1,0 +1,256 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP product path.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+NANG = 181
+MAXT = 64
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nz", C.c_int32), ("dr", C.c_double), ("dz", C.c_double),
+        ("phip", C.c_double), ("ddt", C.c_double), ("n_sub", C.c_int32), ("fn", C.c_double),
+        ("c", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+        ("e_em", C.c_double), ("me_em", C.c_double), ("e_mc", C.c_double), ("me_mc", C.c_double),
+        ("m_ion", C.c_double), ("k_b", C.c_double),
+        ("ez_metric", C.c_int32), ("axis_j_sign", C.c_int32), ("exc_form", C.c_int32),
+        ("rot_eps", C.c_double),
+    ]
+
+
+class OrcTables(C.Structure):
+    _fields_ = [
+        ("n_el", C.c_int32), ("el_e", C.c_double * MAXT), ("el_s", C.c_double * MAXT),
+        ("n_sel", C.c_int32), ("sel_e", C.c_double * MAXT), ("sel_s", C.c_double * MAXT),
+        ("n_ion", C.c_int32), ("ion_e", C.c_double * MAXT), ("ion_s", C.c_double * MAXT),
+        ("n_exc", C.c_int32), ("exc_w", C.c_double * 16), ("exc_a", C.c_double * 16),
+        ("exc_om", C.c_double * 16), ("exc_nu", C.c_double * 16), ("exc_ga", C.c_double * 16),
+        ("n_dcs", C.c_int32), ("dcs_e", C.c_double * MAXT), ("dcs_loge", C.c_double * MAXT),
+        ("dcs_tcs", C.c_double * MAXT), ("dcs_logv", C.c_double * (MAXT * NANG)),
+    ]
+
+
+class OrcGas(C.Structure):
+    _fields_ = [("dens", _D), ("temp", _D), ("gnr", C.c_int32), ("gnz", C.c_int32),
+                ("gdr", C.c_double), ("gdz", C.c_double), ("t_scalar", C.c_double)]
+
+
+class OrcCounts(C.Structure):
+    _fields_ = [("n_elastic", C.c_int64), ("n_ionize", C.c_int64), ("n_excite", C.c_int64),
+                ("n_dropped", C.c_int64)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB_PATH))
+        L.orc_interp1.restype = C.c_double
+        L.orc_interp1.argtypes = [_D, _D, C.c_int, C.c_double]
+        L.orc_elastic_theta.restype = C.c_double
+        L.orc_elastic_theta.argtypes = [C.POINTER(OrcTables), C.c_double, C.c_double]
+        L.orc_exc_edep.restype = C.c_double
+        L.orc_exc_edep.argtypes = [C.POINTER(OrcTables), C.c_double, C.c_int]
+        L.orc_u01.restype = C.c_double
+        L.orc_u01.argtypes = [C.c_uint32]
+        L.orc_ionize.restype = None
+        L.orc_ionize.argtypes = [C.c_double, C.c_double, C.c_double] + [_D] * 5
+        L.orc_rotate.restype = None
+        L.orc_rotate.argtypes = [_D, _D, _D, C.c_double, C.c_double, C.c_double]
+        L.orc_collide.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcTables), C.POINTER(OrcGas),
+                                  _P, _P, _P, _P, _P, _P, _P, C.c_int64, C.c_double,
+                                  C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P]
+        L.orc_run.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcTables), C.POINTER(OrcGas),
+                              _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64), C.c_int64,
+                              _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint32,
+                              C.c_uint32, C.POINTER(OrcCounts)]
+        L.orc_fdtd_push.argtypes = [C.POINTER(OrcParams), _P, _P, _P, _P, _P, C.c_int]
+        L.orc_fdtd_h.argtypes = [_P, _P, _P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_fdtd_e.argtypes = [_P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_em2.argtypes = [C.POINTER(OrcParams)] + [_P] * 10 + [C.c_int64]
+        L.orc_em4.argtypes = [C.POINTER(OrcParams)] + [_P] * 16 + [C.c_int64]
+        L.orc_em7.argtypes = [C.POINTER(OrcParams)] + [_P] * 8 + [C.c_int64]
+        L.orc_em_step.argtypes = [C.POINTER(OrcParams)] + [_P] * 10 + [C.c_int64] + [_P] * 5 + [C.c_int]
+        L.orc_deposit_fx32.argtypes = [C.POINTER(OrcParams)] + [_P] * 6 + [C.c_int64, C.c_float, _P, _P]
+        L.orc_j_from_fixed.argtypes = [C.POINTER(OrcParams), _P, _P, _P, _P]
+        L.orc_tallies.argtypes = [C.POINTER(OrcParams)] + [_P] * 8 + [C.c_int64, _P]
+        L.orc_cross_sections.argtypes = [C.POINTER(OrcTables), _P, C.c_int64, _P, _P, _P]
+        L.orc_tables_build.argtypes = [C.POINTER(OrcTables), _P, C.c_int, _P, C.c_int, _P, C.c_int,
+                                       _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        _lib = L
+    return _lib
+
+
+def a(x, dtype=np.float64):
+    """contiguous array + its pointer"""
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def ptr(x: np.ndarray):
+    return x.ctypes.data_as(C.c_void_p)
+
+
+def params_from(p) -> OrcParams:
+    """Copy a gasx Params (or any object with the same fields) into OrcParams."""
+    return OrcParams(*[getattr(p, n) for n, _ in OrcParams._fields_])
+
+
+def build_tables(raw: dict) -> OrcTables:
+    t = OrcTables()
+    r = {k: a(v) for k, v in raw.items()}
+    st = lib().orc_tables_build(C.byref(t), ptr(r["t_el"]), r["t_el"].shape[1], ptr(r["t_sel"]),
+                                r["t_sel"].shape[0], ptr(r["t_sel1"]), r["t_sel1"].shape[0],
+                                ptr(r["t_del"]), r["t_del"].shape[1], ptr(r["t_sdel"]),
+                                r["t_sdel"].shape[1], ptr(r["t_ion"]), r["t_ion"].shape[0],
+                                ptr(r["t_exc"]), r["t_exc"].shape[0])
+    if st != 0:
+        raise ValueError(f"orc_tables_build failed ({st})")
+    return t
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def cross_sections(t: OrcTables, e_kev):
+    e = a(np.atleast_1d(e_kev))
+    se, si, sx = (np.empty_like(e) for _ in range(3))
+    lib().orc_cross_sections(C.byref(t), ptr(e), e.size, ptr(se), ptr(si), ptr(sx))
+    return se, si, sx
+
+
+def fdtd_push(p: OrcParams, er, ez, h, jr, jz, n_push=1):
+    er, ez, h = a(er).copy(), a(ez).copy(), a(h).copy()
+    jr, jz = a(jr), a(jz)
+    lib().orc_fdtd_push(C.byref(p), ptr(er), ptr(ez), ptr(h), ptr(jr), ptr(jz), n_push)
+    return er, ez, h
+
+
+def deposit_fx32(p: OrcParams, state32: np.ndarray, qnum=-1.0):
+    s = [a(state32[i], np.float32) for i in range(6)]
+    n = s[0].size
+    qr = np.zeros(p.nr * p.nz, dtype=np.int64)
+    qz = np.zeros_like(qr)
+    lib().orc_deposit_fx32(C.byref(p), *[ptr(v) for v in s], n, qnum, ptr(qr), ptr(qz))
+    return qr, qz
+
+
+def j_from_fixed(p: OrcParams, qr, qz):
+    qr, qz = a(qr, np.int64), a(qz, np.int64)
+    jr = np.empty(qr.size)
+    jz = np.empty(qr.size)
+    lib().orc_j_from_fixed(C.byref(p), ptr(qr), ptr(qz), ptr(jr), ptr(jz))
+    return jr, jz
+
+
+class State:
+    """FP64 particle state in the oracle's (reference) representation."""
+
+    def __init__(self, arr6, cap=None, q=None, m=None):
+        arr6 = np.asarray(arr6, dtype=np.float64)
+        n = arr6.shape[1]
+        cap = cap or n
+        self.v = np.zeros((6, cap))
+        self.v[:, :n] = arr6
+        self.n = n
+        self.dele = np.zeros(cap, dtype=np.int32)
+        self.dele[:n][np.isnan(arr6[2])] = 1
+        self.v[2, :n][np.isnan(arr6[2])] = 0.0
+        self.q = np.full(cap, -1.6e-19) if q is None else a(q)
+        self.m = np.full(cap, 9.1096e-31) if m is None else a(m)
+
+    def rows(self):
+        return [self.v[i] for i in range(6)]
+
+
+def em2(p, st: State):
+    r, phi, vr = (np.zeros(st.v.shape[1]) for _ in range(3))
+    lib().orc_em2(C.byref(p), *[ptr(x) for x in st.rows()], ptr(r), ptr(phi), ptr(vr), ptr(st.dele), st.n)
+    return r, phi, vr
+
+
+def em4(p, st: State, r, phi, vr, er, ez, h):
+    ke = np.zeros(st.v.shape[1])
+    er, ez, h = a(er), a(ez), a(h)
+    lib().orc_em4(C.byref(p), *[ptr(x) for x in st.rows()], ptr(r), ptr(phi), ptr(vr), ptr(er),
+                  ptr(ez), ptr(h), ptr(st.q), ptr(st.m), ptr(ke), ptr(st.dele), st.n)
+    return ke
+
+
+def em7(p, st: State, r, vr):
+    jr = np.zeros(p.nr * p.nz)
+    jz = np.zeros(p.nr * p.nz)
+    lib().orc_em7(C.byref(p), ptr(r), ptr(st.v[2]), ptr(st.q), ptr(vr), ptr(st.v[5]), ptr(jr),
+                  ptr(jz), ptr(st.dele), st.n)
+    return jr, jz
+
+
+def em_step(p, st: State, er, ez, h, n_push):
+    er, ez, h = a(er).copy(), a(ez).copy(), a(h).copy()
+    jr = np.zeros(p.nr * p.nz)
+    jz = np.zeros(p.nr * p.nz)
+    ke = np.zeros(st.v.shape[1])
+    lib().orc_em_step(C.byref(p), *[ptr(x) for x in st.rows()], ptr(st.q), ptr(st.m), ptr(ke),
+                      ptr(st.dele), st.n, ptr(er), ptr(ez), ptr(h), ptr(jr), ptr(jz), n_push)
+    return er, ez, h, ke
+
+
+def make_gas(dens, gdr, gdz, t_scalar=300.0):
+    d = a(np.atleast_2d(dens))
+    g = OrcGas(d.ctypes.data_as(_D), None, d.shape[1], d.shape[0], gdr, gdz, t_scalar)
+    g._keep = d
+    return g
+
+
+def collide(p, t, g, st: State, dt_coll, seed, step, run=0, ions=False):
+    ev = np.zeros(st.v.shape[1], dtype=np.int8)
+    sv = np.zeros(3 * st.v.shape[1])
+    iv = np.zeros(3 * st.v.shape[1]) if ions else None
+    lib().orc_collide(C.byref(p), C.byref(t), C.byref(g), *[ptr(x) for x in st.rows()],
+                      ptr(st.dele), st.n, dt_coll, seed, step, run, ptr(ev), ptr(sv),
+                      ptr(iv) if ions else None)
+    return ev[: st.n], sv.reshape(-1, 3)[: st.n], (iv.reshape(-1, 3)[: st.n] if ions else None)
+
+
+def run(p, t, g, st: State, er, ez, h, n_steps, collide=True, secondaries=False, seed=0, run_id=0,
+        step0=0):
+    er, ez, h = a(er).copy(), a(ez).copy(), a(h).copy()
+    n = C.c_int64(st.n)
+    cnt = OrcCounts()
+    rc = lib().orc_run(C.byref(p), C.byref(t), C.byref(g), *[ptr(x) for x in st.rows()],
+                       ptr(st.dele), C.byref(n), st.v.shape[1], ptr(er), ptr(ez), ptr(h), n_steps,
+                       int(collide), int(secondaries), seed, run_id, step0, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("orc_run failed")
+    st.n = n.value
+    return er, ez, h, cnt
+
+
+def tallies(p, er, ez, h, st: State):
+    out = np.zeros(4)
+    lib().orc_tallies(C.byref(p), ptr(a(er)), ptr(a(ez)), ptr(a(h)), ptr(st.v[3]), ptr(st.v[4]),
+                      ptr(st.v[5]), ptr(st.m), ptr(st.dele), st.n, ptr(out))
+    return out

@@ -1,0 +1,389 @@
+"""Pins of the FP64 oracle against known answers, closed forms and invariants.
+
+The reference ships no tests or golden vectors, so each oracle function is
+pinned here by something it does not share code with: published test
+vectors (Philox), the reference's own data tables (cross sections), closed
+forms (Green-Sawada CDF, binary-encounter limit, uniform-field momentum gain,
+Bessel cavity frequency, trapezoid integral of a linear function) and
+invariants (|v| under rotation, current sum rule, wedge-wrap invariance).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate, special
+
+import oracle
+
+A0SQ = 2.8002852e-21
+
+
+def params(nr=16, nz=16, **kw):
+    import gasx
+    return oracle.params_from(gasx.make_params(nr, nz, **kw))
+
+
+# ---- Philox4x32-10: Random123 known-answer vectors ----------------------------
+@pytest.mark.parametrize("ctr,key,out", [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+])
+def test_philox_kat(ctr, key, out):
+    assert oracle.philox(ctr, key) == out
+
+
+def test_u01_open_interval_and_fp32_exact():
+    L = oracle.lib()
+    for x in (0, 1, 255, 256, 0xffffffff, 0x80000000):
+        u = L.orc_u01(x)
+        assert 0.0 < u < 1.0
+        assert float(np.float32(u)) == u
+
+
+# ---- cross sections at table nodes (data files of the reference) ------------
+def test_cross_section_known_answers(otab):
+    e = np.array([0.4, 0.0156, 0.03, 0.0155, 20.1, 0.0009, 0.001, 20.0])
+    se, si, sx = oracle.cross_sections(otab, e)
+    assert si[0] == pytest.approx(1.632e-20, rel=1e-12)       # TotalIonCS_Nitrogen.csv:12
+    assert si[1] == pytest.approx(1e-23, rel=1e-12)           # :1
+    assert se[0] == pytest.approx(3.935 * 2 * A0SQ, rel=1e-12)  # TotalElasticCS_Nitrogen.csv
+    assert se[2] == pytest.approx(10.67e-20, rel=1e-12)       # TotalElasticCS_smalleV_Nitrogen.csv:25
+    assert np.isnan(si[3]) and not np.isnan(se[3])            # below the ionization threshold
+    assert np.isnan(se[4]) and np.isnan(si[4])                # above 20 keV
+    assert np.isnan(se[5]) and se[6] == pytest.approx(9.74e-20)   # small-energy table starts at 1 eV
+    assert se[7] == pytest.approx(0.125 * 2 * A0SQ) and si[7] == pytest.approx(0.073e-20)
+
+
+def test_interp1_linear_between_nodes(otab):
+    # 450 eV lies halfway between the 400 and 500 eV ionization nodes
+    _, si, _ = oracle.cross_sections(otab, np.array([0.45]))
+    assert si[0] == pytest.approx(0.5 * (1.632 + 1.428) * 1e-20, rel=1e-12)
+
+
+def test_table_merge(raw, otab):
+    """electron_collision.m:15-33: 7 small-energy + 29 large-energy columns."""
+    assert otab.n_dcs == 36
+    e = np.array(otab.dcs_e[:36])
+    assert np.allclose(e[:7], [0.005, 0.01, 0.015, 0.02, 0.025, 0.03, 0.04])
+    assert np.allclose(e[7:], raw["t_el"][0])
+    # small-energy TCS converted to a0^2 per atom; the 50 eV row is dropped
+    assert otab.dcs_tcs[0] == pytest.approx(12.74e-20 / A0SQ / 2)
+    assert otab.dcs_tcs[6] == pytest.approx(9.68e-20 / A0SQ / 2)
+    assert otab.dcs_tcs[7] == pytest.approx(15.5)
+    lv = np.array(otab.dcs_logv[:36 * 181]).reshape(36, 181)
+    assert np.allclose(lv[0], np.log(raw["t_sdel"][1:, 1]))
+    assert np.allclose(lv[35], np.log(raw["t_del"][1:, 29]))
+    assert np.allclose(lv[7], np.log(raw["t_del"][1:, 1]))
+
+
+# ---- excitation --------------------------------------------------------------
+def test_excitation_loss_limits(otab, raw):
+    L = oracle.lib()
+    W = raw["t_exc"][:, 0]
+    for T in (15.6, 50.0, 400.0, 5000.0, 20000.0):
+        ed = L.orc_exc_edep(otab, T, 0)
+        assert W.min() <= ed <= W.max()
+    # T -> infinity: only the omega = 0.75 terms survive and (1-(W/T)^g)^nu -> 1,
+    # so E_dep -> sum A W^-0.25 / sum A W^-1.25 over those levels
+    t = raw["t_exc"]
+    m = t[:, 2] == 0.75
+    lim = np.sum(t[m, 1] * t[m, 0] ** -0.25) / np.sum(t[m, 1] * t[m, 0] ** -1.25)
+    assert L.orc_exc_edep(otab, 1e40, 0) == pytest.approx(lim, rel=1e-8)
+
+
+def test_excitation_cross_section_high_energy_slope(otab):
+    _, _, sx = oracle.cross_sections(otab, np.array([10.0, 20.0]))
+    slope = math.log(sx[1] / sx[0]) / math.log(2.0)
+    assert -0.9 < slope < -0.7   # dominated by the omega = 0.75 levels
+
+
+# ---- elastic angle -----------------------------------------------------------
+def _mean_cos_from_table(raw, col_e_kev):
+    d = raw["t_del"]
+    j = int(np.nonzero(np.isclose(d[0, 1:], col_e_kev))[0][0]) + 1
+    th = np.deg2rad(d[1:, 0])
+    dcs = d[1:, j]
+    f = lambda t: np.interp(t, th, dcs) * np.sin(t)
+    num = integrate.quad(lambda t: f(t) * np.cos(t), 0, np.pi, limit=400, points=th[1:-1][::5])[0]
+    den = integrate.quad(f, 0, np.pi, limit=400, points=th[1:-1][::5])[0]
+    return num / den
+
+
+@pytest.mark.parametrize("e_kev", [0.1, 0.4, 2.0])
+def test_elastic_angle_matches_table_distribution(otab, raw, e_kev):
+    L = oracle.lib()
+    u = (np.arange(20000) + 0.5) / 20000
+    th = np.array([L.orc_elastic_theta(otab, e_kev, x) for x in u])
+    assert np.all(np.diff(th) >= 0) and th[0] >= 0 and th[-1] <= np.pi
+    assert np.mean(np.cos(th)) == pytest.approx(_mean_cos_from_table(raw, e_kev), abs=2e-3)
+
+
+def test_elastic_angle_between_nodes(otab, raw):
+    L = oracle.lib()
+    u = (np.arange(4000) + 0.5) / 4000
+    m = lambda e: np.mean(np.cos([L.orc_elastic_theta(otab, e, x) for x in u]))
+    lo, mid, hi = m(0.3), m(0.35), m(0.4)
+    assert min(lo, hi) <= mid <= max(lo, hi)
+
+
+# ---- ionization -----------------------------------------------------------------
+@pytest.mark.parametrize("T", [20.0, 100.0, 400.0, 5000.0])
+def test_green_sawada_inverse_cdf(T):
+    L = oracle.lib()
+    I, T0 = 15.6, 4.17 - 1000.0 / (T + 31.2)
+    G = 13.8 * T / (T + 15.6)
+    Tm = (T - I) / 2
+    cdf = lambda E: (math.atan((E - T0) / G) + math.atan(T0 / G)) / (math.atan((Tm - T0) / G) + math.atan(T0 / G))
+    import ctypes as C
+    out = [C.c_double() for _ in range(5)]
+    ee = 9.109e-31 * 9e16 / 1.6022e-19
+    for u in (0.01, 0.3, 0.5, 0.77, 0.99):
+        L.orc_ionize(T, u, ee, *[C.byref(o) for o in out])
+        k0, ep, es, thp, ths = (o.value for o in out)
+        assert cdf(k0) == pytest.approx(u, abs=1e-12)
+        assert ep + es + I == pytest.approx(T, rel=1e-12)      # energy balance
+        assert es == pytest.approx(k0) and ep >= es
+        # binary encounter: close to 90 degrees between the outgoing electrons
+        assert thp + ths == pytest.approx(math.pi / 2, abs=0.02 * math.sqrt(T / 1000) + 1e-3)
+
+
+def test_rotate_preserves_speed_and_angle():
+    import ctypes as C
+    L = oracle.lib()
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        v = rng.standard_normal(3) * 1e7
+        th, ph = rng.random() * np.pi, rng.random() * 2 * np.pi
+        a = [C.c_double(x) for x in v]
+        L.orc_rotate(*[C.byref(x) for x in a], th, ph, 1e-5)
+        w = np.array([x.value for x in a])
+        assert np.linalg.norm(w) == pytest.approx(np.linalg.norm(v), rel=1e-12)
+        assert np.dot(v, w) / np.dot(v, v) == pytest.approx(math.cos(th), abs=1e-12)
+    # degenerate branch: velocity along z
+    a = [C.c_double(0.0), C.c_double(0.0), C.c_double(5.0)]
+    L.orc_rotate(*[C.byref(x) for x in a], 0.3, 1.0, 1e-5)
+    assert a[2].value == pytest.approx(5 * math.cos(0.3))
+
+
+# ---- FDTD ----------------------------------------------------------------------
+def _cavity_frequency(ez_metric):
+    nr, nz = 64, 4
+    p = params(nr, nz, dr=1e-5, dz=1e-5, ddt=1e-14, n_sub=10, ez_metric=ez_metric)
+    rp = nr * p.dr
+    r_nodes = np.arange(nr + 1) * p.dr
+    ez = np.tile(special.j0(2.404825557695773 * r_nodes / rp), (nz, 1))
+    ez[:, -1] = 0.0
+    er = np.zeros((nz + 1, nr))
+    h = np.zeros((nz, nr))
+    j0 = np.zeros(nr * nz)
+    samples = []
+    for _ in range(800):
+        er, ez, h = oracle.fdtd_push(p, er, ez, h, j0, j0, 1)
+        ez = ez.reshape(nz, nr + 1)
+        er = er.reshape(nz + 1, nr)
+        h = h.reshape(nz, nr)
+        samples.append(ez[1, 0])
+    s = np.array(samples)
+    zc = np.nonzero(np.diff(np.sign(s)) != 0)[0]
+    # linear interpolation of the crossings
+    tz = zc + s[zc] / (s[zc] - s[zc + 1])
+    period = 2 * np.mean(np.diff(tz)) * p.ddt
+    return 2 * math.pi / period, 2.404825557695773 * 3e8 / rp
+
+
+def test_fdtd_tm010_cavity_frequency():
+    """PEC cylinder: omega = c * j01 / Rp (z-uniform TM010 mode)."""
+    w, w_exact = _cavity_frequency(0)
+    assert w == pytest.approx(w_exact, rel=2e-3)
+
+
+def test_fdtd_ez_metric_variant_is_off_resonance():
+    """The EM3.cu:39 metric r=(i+1)dr detunes the mode (documented divergence D1)."""
+    w, w_exact = _cavity_frequency(1)
+    assert abs(w / w_exact - 1) > 5e-3
+
+
+def test_fdtd_vacuum_energy_bounded():
+    nr, nz = 24, 32
+    p = params(nr, nz)
+    rng = np.random.default_rng(0)
+    er = rng.standard_normal((nz + 1, nr))
+    ez = rng.standard_normal((nz, nr + 1))
+    ez[:, -1] = 0
+    h = rng.standard_normal((nz, nr)) / 377.0
+    st = oracle.State(np.zeros((6, 1)))
+    st.dele[:] = 1
+    e0 = oracle.tallies(p, er, ez, h, st)[3]
+    er2, ez2, h2 = oracle.fdtd_push(p, er, ez, h, np.zeros(nr * nz), np.zeros(nr * nz), 200)
+    e1 = oracle.tallies(p, er2, ez2, h2, st)[3]
+    assert np.all(np.isfinite(er2)) and e1 < 3 * e0
+
+
+# ---- push ---------------------------------------------------------------------------
+def _uniform_fields(nr, nz, Er=0.0, Ez=0.0, H=0.0):
+    return (np.full((nz + 1) * nr, Er), np.full(nz * (nr + 1), Ez), np.full(nz * nr, H))
+
+
+def test_push_uniform_ez_momentum_gain():
+    """dp/dt = qE: gamma*v_z grows by qE dt/m per push, exactly in exact arithmetic."""
+    nr, nz = 16, 4000
+    p = params(nr, nz, dz=1e-5)
+    er, ez, h = _uniform_fields(nr, nz, Ez=-2e7)
+    st = oracle.State(np.array([[3e-5], [1e-6], [0.02], [1e5], [2e5], [1e6]]))
+    u0 = st.v[3:6, 0] / math.sqrt(1 - np.sum(st.v[3:6, 0] ** 2) / 9e16)
+    n = 50
+    for _ in range(n):
+        r, phi, vr = oracle.em2(p, st)
+        oracle.em4(p, st, r, phi, vr, er, ez, h)
+    v = st.v[3:6, 0]
+    u = v / math.sqrt(1 - np.sum(v ** 2) / 9e16)
+    du = -1.6e-19 * -2e7 * p.ddt / 9.1096e-31 * n
+    assert u[2] - u0[2] == pytest.approx(du, rel=1e-9)
+    assert np.hypot(u[0], u[1]) == pytest.approx(np.hypot(u0[0], u0[1]), rel=1e-9)
+
+
+def test_push_free_flight_and_reflection():
+    nr, nz = 16, 16
+    p = params(nr, nz)
+    er, ez, h = _uniform_fields(nr, nz)
+    lp = nz * p.dz
+    st = oracle.State(np.array([[5e-5], [1e-6], [lp - 1e-9], [0.0], [0.0], [1e7]]))
+    r, phi, vr = oracle.em2(p, st)
+    oracle.em4(p, st, r, phi, vr, er, ez, h)
+    assert st.v[5, 0] == pytest.approx(-1e7)                 # reflected at z = Lp
+    assert st.v[2, 0] == pytest.approx(lp - 1e-9 - (1e7 * p.ddt - 1e-9) + 0, abs=1e-12) or st.v[2, 0] < lp
+    assert np.linalg.norm(st.v[3:6, 0]) == pytest.approx(1e7, rel=1e-12)
+
+
+def test_wedge_wrap_is_rotation_invariant():
+    """A particle and its copy rotated by PHIp end with the same r, v_r, v_phi, v_z."""
+    nr, nz = 16, 16
+    p = params(nr, nz)
+    rot = lambda a, t: np.array([a[0] * math.cos(t) - a[1] * math.sin(t), a[0] * math.sin(t) + a[1] * math.cos(t)])
+    x = np.array([6e-5, 0.2e-6])
+    v = np.array([1e6, -3e7])
+    z, vz = 5e-5, 2e6
+    outs = []
+    for t in (0.0, p.phip):
+        xr, vrr = rot(x, t), rot(v, t)
+        st = oracle.State(np.array([[xr[0]], [xr[1]], [z], [vrr[0]], [vrr[1]], [vz]]))
+        oracle.em2(p, st)
+        X, V = st.v[0:2, 0], st.v[3:5, 0]
+        rr = np.linalg.norm(X)
+        outs.append((rr, (X @ V) / rr, (X[0] * V[1] - X[1] * V[0]) / rr, st.v[5, 0]))
+        assert 0 <= math.atan2(X[1], X[0]) <= p.phip + 1e-12
+    assert np.allclose(outs[0], outs[1], rtol=1e-9)
+
+
+# ---- deposit ------------------------------------------------------------------------
+def test_em7_current_sum_rule():
+    """sum J V = FN * sum q v (the clamped area weights sum to one)."""
+    nr, nz = 20, 30
+    p = params(nr, nz, fn=2.0)
+    rng = np.random.default_rng(1)
+    n = 5000
+    r = rng.random(n) * nr * p.dr
+    ph = rng.random(n) * p.phip
+    st = oracle.State(np.array([r * np.cos(ph), r * np.sin(ph), rng.random(n) * nz * p.dz,
+                                rng.standard_normal(n) * 1e6, rng.standard_normal(n) * 1e6,
+                                rng.standard_normal(n) * 1e6]))
+    rr, phi, vr = oracle.em2(p, st)
+    jr, jz = oracle.em7(p, st, rr, vr)
+    i = np.arange(nr)
+    V = np.tile(p.phip / 2 * ((i + 1) ** 2 - i ** 2) * p.dr ** 2 * p.dz, nz)
+    live = st.dele[:n] == 0
+    assert np.sum(jr * V) == pytest.approx(p.fn * np.sum(st.q[:n][live] * vr[:n][live]), rel=1e-10)
+    assert np.sum(jz * V) == pytest.approx(p.fn * np.sum(st.q[:n][live] * st.v[5, :n][live]), rel=1e-10)
+
+
+def test_em7_cell_centre_hits_one_cell():
+    nr, nz = 10, 10
+    p = params(nr, nz)
+    r0, z0 = 3.5 * p.dr, 6.5 * p.dz
+    st = oracle.State(np.array([[r0], [0.0], [z0], [2e6], [0.0], [1e6]]))
+    rr, phi, vr = np.array([r0]), np.array([0.0]), np.array([2e6])
+    jr, jz = oracle.em7(p, st, rr, vr)
+    assert np.count_nonzero(jr) == 1 and jr.reshape(nz, nr)[6, 3] != 0
+
+
+def test_fixed_point_deposit_tracks_fp64():
+    nr, nz = 20, 30
+    p = params(nr, nz, fn=1.0)
+    rng = np.random.default_rng(2)
+    n = 4000
+    r = rng.random(n) * nr * p.dr * 0.999
+    ph = rng.random(n) * p.phip
+    s32 = np.array([r * np.cos(ph), r * np.sin(ph), rng.random(n) * nz * p.dz,
+                    rng.standard_normal(n) * 1e6, rng.standard_normal(n) * 1e6,
+                    rng.standard_normal(n) * 1e6]).astype(np.float32)
+    qr, qz = oracle.deposit_fx32(p, s32, -1.0)
+    jr, jz = oracle.j_from_fixed(p, qr, qz)
+    st = oracle.State(s32)
+    rr = np.hypot(st.v[0], st.v[1])
+    vr = (st.v[3] * st.v[0] + st.v[4] * st.v[1]) / rr
+    ojr, ojz = oracle.em7(p, st, rr, vr)
+    # bound: FP32 positions resolve r/dr and z/dz to max(nr, nz) * 2^-24 cells,
+    # plus a few roundings of v_r and the products, relative to the cell's sum
+    # of |contributions|
+    sa = oracle.State(np.abs(s32))
+    sa.q[:] = 1.6e-19
+    ajr, ajz = oracle.em7(p, sa, rr, np.abs(vr))
+    tol = (8 + 2 * max(nr, nz)) * 2.0 ** -24
+    assert np.all(np.abs(jr - ojr) <= tol * ajr + 1e-300)
+    assert np.all(np.abs(jz - ojz) <= tol * ajz + 1e-300)
+    # order independence of the integer sum
+    perm = rng.permutation(n)
+    qr2, qz2 = oracle.deposit_fx32(p, s32[:, perm], -1.0)
+    assert np.array_equal(qr, qr2) and np.array_equal(qz, qz2)
+
+
+# ---- tallies -----------------------------------------------------------------------
+def test_tallies_trapz_closed_form():
+    nr, nz = 12, 9
+    p = params(nr, nz, fn=3.0)
+    E0 = 5.0
+    er = np.full((nz + 1) * nr, E0)
+    ez = np.zeros(nz * (nr + 1))
+    h = np.zeros(nz * nr)
+    st = oracle.State(np.array([[1e-5], [0.0], [1e-5], [3e6], [0.0], [4e6]]))
+    out = oracle.tallies(p, er, ez, h, st)
+    r0, r1 = p.dr / 2, nr * p.dr - p.dr / 2
+    we = p.phip * 0.5 * p.eps0 * E0 ** 2 * (r1 ** 2 - r0 ** 2) / 2 * (nz * p.dz)
+    assert out[0] == pytest.approx(we, rel=1e-12)
+    g = 1 / math.sqrt(1 - 25e12 / 9e16)
+    assert out[2] == pytest.approx(3.0 * (g - 1) * 9.1096e-31 * 9e16, rel=1e-9)
+
+
+# ---- collisions: decision logic against the tables -----------------------------------
+def test_collision_branch_frequencies(otab):
+    """Event fractions follow sigma_el : sigma_ion : sigma_exc (binomial bounds)."""
+    import gasx
+    p = oracle.params_from(gasx.make_params(20, 20))
+    n = 40000
+    from gasx.workloads import speed_from_ev
+    v = speed_from_ev(np.full(n, 400.0))
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    st = oracle.State(np.vstack([np.full(n, 5e-5), np.full(n, 1e-6), np.full(n, 1e-4), d * v]))
+    g = oracle.make_gas(np.full((1, 1), 1e25), 20 * p.dr, 20 * p.dz)
+    ev, sv, _ = oracle.collide(p, otab, g, st, 1e-13, seed=3, step=0)
+    se, si, sx = oracle.cross_sections(otab, np.array([0.4]))
+    st_ = se[0] + si[0] + sx[0]
+    pt = v[0] * 1e-13 * 1e25 * st_
+    ncol = np.count_nonzero(ev)
+    assert abs(ncol - n * pt) < 5 * math.sqrt(n * pt * (1 - pt))
+    for code, s in ((1, se[0]), (2, si[0]), (3, sx[0])):
+        k = np.count_nonzero(ev == code)
+        f = s / st_
+        assert abs(k - ncol * f) < 5 * math.sqrt(ncol * f * (1 - f))
+    # excitation leaves T - E_dep, ionization conserves T = Ep + Es + I
+    from gasx.workloads import speed_from_ev as sp
+    ex = np.nonzero(ev == 3)[0][:5]
+    edep = oracle.lib().orc_exc_edep(otab, 400.0, 0)
+    # speeds after excitation come from the collision constants (electron_transport.cu)
+    g1 = (400.0 - edep) * 1.6022e-19 / (9.109e-31 * 9e16)
+    vexp = 3e8 * math.sqrt(g1 * (g1 + 2)) / (1 + g1)
+    assert np.allclose(np.linalg.norm(st.v[3:6, ex], axis=0), vexp, rtol=1e-12)
