@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Electron PIC step benchmark on one MI355X.
+
+Workload (default): the plain GPU step at scale -- axisymmetric r-z grid
+256 x 512 (dr = dz = 10 um, PHIp = 0.1), 4e6 electrons from a Gaussian X-ray
+spot (sigma_r = 50 um) with the 70/30 400 +- 20 eV / 30 +- 10 eV energy mix,
+uniform N2 at 2.5e25 m^-3, one collision check per push (dt = 1e-14 s,
+10 field substeps), ionization secondaries on (capacity 2x), cell sort every
+10 steps, FP32 particles, FP64 field solver.
+
+A step = one push of every electron + its collision check + the field cycle.
+value = particle-steps/s over the K timed steps (inputs resident in HBM).
+roofline: the fused particle kernel, 48 algorithmic bytes per particle-step
+(FP32 x, y, z, vx, vy, vz read + write) over its average launch duration
+measured with HIP events on the library's stream; peak 8 TB/s.
+cpu_baseline: the FP64 C oracle (one thread) on a bounded sample of the same
+workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "gas-atenuator_amd"))
+sys.path.insert(0, str(REPO / "oracle"))
+
+METRIC = "particle-steps/s + %HBM roofline; batched vs plain; adjoint/forward time"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE_STEP = 48
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="plain_gpu_256x512")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    return ap.parse_args()
+
+
+def cpu_baseline(w, host, n_particles: int, n_steps: int, seed: int):
+    """FP64 oracle on the first n_particles of the same seeded electrons."""
+    import numpy as np
+
+    import gasx
+    import oracle
+
+    p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
+    op = oracle.params_from(p)
+    ot = oracle.build_tables(gasx.raw_tables())
+    n = min(n_particles, host[0].size)
+    st = np.stack([a[:n] for a in host]).astype(np.float64)
+    o = oracle.State(st, cap=2 * n)
+    from gasx.workloads import density_grid
+    dens, gdr, gdz = density_grid(w)
+    og = oracle.make_gas(dens.astype(np.float64), gdr, gdz)
+    er = np.zeros((w.nz + 1) * w.nr)
+    ez = np.zeros(w.nz * (w.nr + 1))
+    h = np.zeros(w.nz * w.nr)
+    t0 = time.perf_counter()
+    _, _, _, cnt = oracle.run(op, ot, og, o, er, ez, h, n_steps, collide=True,
+                              secondaries=w.secondaries, seed=seed)
+    dt = time.perf_counter() - t0
+    # pushes performed: live particles each step (absorption is rare; count live at the end)
+    live_end = int(np.sum(o.dele[: o.n] == 0))
+    psteps = 0.5 * (n + live_end) * n_steps
+    return {
+        "value": psteps / dt,
+        "unit": "particle-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 C oracle, 1 thread: first {n} of the seeded electrons on the "
+                   f"{w.nr}x{w.nz} grid, {n_steps} steps with collisions"
+                   f"{' and secondaries' if w.secondaries else ''} ({dt:.1f} s)"),
+    }
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "the multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gasx
+    from gasx.workloads import WORKLOADS, density_grid, seed_electrons
+
+    w = WORKLOADS[args.workload]
+    if w.n_electrons <= 0:
+        raise SystemExit(f"workload {w.name} is not a single-run bench workload")
+    p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
+    ctx = gasx.Context(p, device=0)
+    host = seed_electrons(w.n_electrons, w.rp, w.lp, w.phip, w.sigma_r, seed=args.seed)
+    host_t = [torch.from_numpy(a).pin_memory() for a in host]
+    e = gasx.ParticleSet(w.capacity)
+    f = gasx.FieldState(w.nr, w.nz)
+    dens, gdr, gdz = density_grid(w)
+    gas = gasx.GasGrid(dens, gdr, gdz, temperature=300.0)
+
+    def reset():
+        for row, t in enumerate(host_t):
+            e.data[row, : w.n_electrons].copy_(t, non_blocking=True)
+        e.count_t.fill_(w.n_electrons)
+        for t in (f.er, f.ez, f.hphi):
+            t.zero_()
+
+    run_kw = dict(collide=True, secondaries=w.secondaries, sort_every=w.sort_every, seed=args.seed + 1)
+    reset()
+    ctx.run(e, f, gas, args.warmup, time_kernels=True, **run_kw)
+    torch.cuda.synchronize()
+    reset()
+    ctx.counters(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(e, f, gas, args.steps, time_kernels=True, **run_kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = ctx.counters()
+    tm = ctx.timing()
+    psteps = c.n_pushed
+    value = psteps / wall
+
+    n_launch = max(1, tm["n_particle_launches"])
+    avg_ms = tm["particle_ms"] / n_launch
+    bytes_per_launch = BYTES_PER_PARTICLE_STEP * psteps / n_launch
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = REPO / "profiles" / f"traffic_{w.name}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name, "grid": f"{w.nr}x{w.nz}", "electrons": w.n_electrons,
+            "capacity": w.capacity, "density_m3": w.density, "dt_s": w.ddt, "field_substeps": w.n_sub,
+            "secondaries": w.secondaries, "sort_every": w.sort_every,
+            "field_solver": "fp64", "particles": "fp32 SoA",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_particles (fused push+collide+drift+deposit)",
+            "avg_launch_ms": avg_ms,
+            "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
+        },
+        "breakdown_ms": {k: tm[k] for k in ("particle_ms", "fdtd_ms", "append_ms", "sort_ms")},
+        "events": {"elastic": c.n_elastic, "ionize": c.n_ionize, "excite": c.n_excite,
+                   "absorbed": c.n_absorbed, "dropped_secondaries": c.n_dropped_sec,
+                   "bound_violations": c.n_bound_violations, "final_slots": e.n},
+        "fdtd_cell_updates_per_s": w.nr * w.nz * w.n_sub * args.steps / (tm["fdtd_ms"] * 1e-3)
+        if tm["fdtd_ms"] > 0 else None,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, host, args.cpu_particles, args.cpu_steps, args.seed + 1)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

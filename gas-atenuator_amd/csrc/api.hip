@@ -6,6 +6,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "gasx_internal.h"
 
@@ -38,8 +40,52 @@ struct gasx_ctx {
     size_t sort_bytes = 0;
     double* d_tally = nullptr;
     int cu_count = 256;
+    double max_rate = 0.0;   // max over the collision window of v(E) sigma_T(E) (m^3/s)
     std::string err;
+    // kernel timing (gasx_run_opts.time_kernels): event pairs per launch kind
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    std::vector<std::pair<size_t, size_t>> tpairs[4];   // particle, fdtd, append, sort
 };
+
+namespace {
+enum TimedKind { kTParticle = 0, kTFdtd, kTAppend, kTSort };
+
+size_t next_event(gasx_ctx* c)
+{
+    if (c->ev_used == c->events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        c->events.push_back(e);
+    }
+    return c->ev_used++;
+}
+
+// Records an event pair around the enclosed launches when timing is on.
+struct TimedScope {
+    gasx_ctx* c;
+    int kind;
+    size_t a = (size_t)-1;
+    TimedScope(gasx_ctx* c_, int k) : c(c_), kind(k)
+    {
+        if (c->timing) {
+            a = next_event(c);
+            if (a != (size_t)-1) (void)hipEventRecord(c->events[a], c->stream);
+        }
+    }
+    ~TimedScope()
+    {
+        if (c->timing && a != (size_t)-1) {
+            size_t b = next_event(c);
+            if (b != (size_t)-1) {
+                (void)hipEventRecord(c->events[b], c->stream);
+                c->tpairs[kind].push_back({a, b});
+            }
+        }
+    }
+};
+}  // namespace
 
 namespace {
 
@@ -152,31 +198,59 @@ int gasx::prepare_tables(const gasx_raw_tables* raw, HostTables* h, DevTables* d
 
     // device image: energies in eV, sigma in m^2
     std::memset(d, 0, sizeof(*d));
-    d->n_el = h->n_el; d->n_sel = h->n_sel; d->n_ion = h->n_ion; d->n_exc = h->n_exc; d->n_dcs = n_dcs;
+    SmallTabs& s = d->s;
+    s.n_el = h->n_el; s.n_sel = h->n_sel; s.n_ion = h->n_ion; s.n_exc = h->n_exc; s.n_dcs = n_dcs;
     for (int i = 0; i < h->n_el; ++i) {
-        d->el_e[i] = (float)(h->el_e[i] * 1000.0);
-        d->el_s[i] = (float)(h->el_s[i] * sqr_a0 * 2.0);     // find_CS0.m:21
+        s.el_e[i] = (float)(h->el_e[i] * 1000.0);
+        s.el_s[i] = (float)(h->el_s[i] * sqr_a0 * 2.0);     // find_CS0.m:21
     }
     for (int i = 0; i < h->n_sel; ++i) {
-        d->sel_e[i] = (float)h->sel_e[i];
-        d->sel_s[i] = (float)(h->sel_s[i] * 1e-20);           // find_CS0.m:25
+        s.sel_e[i] = (float)h->sel_e[i];
+        s.sel_s[i] = (float)(h->sel_s[i] * 1e-20);           // find_CS0.m:25
     }
     for (int i = 0; i < h->n_ion; ++i) {
-        d->ion_e[i] = (float)h->ion_e[i];
-        d->ion_s[i] = (float)(h->ion_s[i] * 1e-20);           // find_CS0.m:33
+        s.ion_e[i] = (float)h->ion_e[i];
+        s.ion_s[i] = (float)(h->ion_s[i] * 1e-20);           // find_CS0.m:33
     }
-    const double q0 = 6.542e-14;                              // find_CS0.m:42
+    const double q0 = 6.542e-14;                             // find_CS0.m:42
     for (int i = 0; i < h->n_exc; ++i) {
-        d->exc_w[i] = (float)h->exc_w[i];
-        d->exc_q[i] = (float)(q0 * h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]) * 1e-4);
-        d->exc_a[i] = (float)(h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]));
-        d->exc_om[i] = (float)h->exc_om[i];
-        d->exc_nu[i] = (float)h->exc_nu[i];
-        d->exc_ga[i] = (float)h->exc_ga[i];
+        s.exc_w[i] = (float)h->exc_w[i];
+        s.exc_q[i] = (float)(q0 * h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]) * 1e-4);
+        s.exc_a[i] = (float)(h->exc_a[i] / (h->exc_w[i] * h->exc_w[i]));
+        s.exc_om[i] = (float)h->exc_om[i];
+        s.exc_nu[i] = (float)h->exc_nu[i];
+        s.exc_ga[i] = (float)h->exc_ga[i];
+        s.exc_l2w[i] = (float)std::log2(h->exc_w[i]);
+    }
+    double sind[kNAng];
+    for (int k = 0; k < kNAng; ++k) {
+        sind[k] = (k == 0 || k == 180) ? 0.0 : (k == 90 ? 1.0 : std::sin(k * 3.14159265358979323846 / 180.0));
+        s.sind[k] = (float)sind[k];
     }
     for (int j = 0; j < n_dcs; ++j) {
-        d->dcs_loge[j] = (float)h->dcs_loge[j];
-        for (int k = 0; k < kNAng; ++k) d->dcs_logv[j * kNAng + k] = (float)h->dcs_logv[j * kNAng + k];
+        s.dcs_loge[j] = (float)h->dcs_loge[j];
+        double p[kNAng], z = 0.0;
+        for (int k = 0; k < kNAng; ++k) {
+            d->dcs_logv[j * kNAng + k] = (float)h->dcs_logv[j * kNAng + k];
+            p[k] = sind[k] * std::exp(h->dcs_logv[j * kNAng + k]);
+            d->dcs_p[j * kNAng + k] = (float)p[k];
+        }
+        for (int k = 0; k + 1 < kNAng; ++k) z += p[k] + p[k + 1];
+        double acc = 0.0;
+        float* C = &d->dcs_cdf[j * kNAng];
+        C[0] = 0.0f;
+        for (int k = 0; k + 1 < kNAng; ++k) {
+            acc += p[k] + p[k + 1];
+            C[k + 1] = (k + 2 == kNAng) ? 1.0f : (float)(acc / z);
+        }
+        s.dcs_z[j] = (float)z;
+        // guide: largest k in [0, 179] with C[k] <= g / kGuide (on the FP32 values)
+        for (int g = 0; g <= kGuide; ++g) {
+            const float b = (float)g / (float)kGuide;
+            int k = 0;
+            while (k < kNAng - 2 && C[k + 1] <= b) ++k;
+            s.guide[j][g] = (uint8_t)k;
+        }
     }
     return 0;
 }
@@ -295,6 +369,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         return GASX_E_ARG;
     }
     fill_geometry(ctx);
+    ctx->max_rate = max_rate_coefficient(ctx->ht, ctx->p);
     gasx_status st = GASX_OK;
     auto chk = [&](hipError_t e, const char* w) {
         if (e != hipSuccess && st == GASX_OK) st = hip_fail(ctx, e, w);
@@ -318,13 +393,14 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
             for (int k = 0; k < 2; ++k) chk(dalloc(&ctx->q[b][k], nh), "alloc q");
         chk(dalloc(&ctx->jr_tmp, nh), "alloc jr");
         chk(dalloc(&ctx->jz_tmp, nh), "alloc jz");
-        chk(dalloc(&ctx->d_counters, kNCounters), "alloc counters");
+        chk(dalloc(&ctx->d_counters, (size_t)kNCounters * kCounterSlots), "alloc counters");
         chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
         chk(dalloc(&ctx->base_tmp, 4), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
         if (st == GASX_OK) {
             chk(hipMemcpy(ctx->d_tab, &dt, sizeof(dt), hipMemcpyHostToDevice), "upload tables");
-            chk(hipMemset(ctx->d_counters, 0, kNCounters * sizeof(unsigned long long)), "zero counters");
+            chk(hipMemset(ctx->d_counters, 0, (size_t)kNCounters * kCounterSlots * sizeof(unsigned long long)),
+                "zero counters");
             chk(hipMemset(ctx->er32, 0, ner * sizeof(float)), "zero er32");
             chk(hipMemset(ctx->ez32, 0, nez * sizeof(float)), "zero ez32");
             chk(hipMemset(ctx->h32, 0, nh * sizeof(float)), "zero h32");
@@ -354,6 +430,7 @@ extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
         if (ctx->stage[c]) hipFree(ctx->stage[c]);
         if (ctx->istage[c]) hipFree(ctx->istage[c]);
     }
+    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return GASX_OK;
@@ -438,7 +515,7 @@ static gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, u
     ca->inv_gdr = (float)(1.0 / g->gdr); ca->inv_gdz = (float)(1.0 / g->gdz);
     ca->t_scalar = (float)g->t_scalar;
     ca->dt_coll = (float)dt_coll;
-    double pm = dt_coll * g->n_max * max_rate_coefficient(ctx->ht, p) * 1.02 + 1e-7;
+    double pm = dt_coll * g->n_max * ctx->max_rate * 1.02 + 1e-7;
     ca->pmax = (float)std::min(1.0, pm);
     ca->key0 = (uint32_t)seed; ca->key1 = (uint32_t)(seed >> 32);
     ca->step = step; ca->run = run;
@@ -453,7 +530,7 @@ static gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, u
 static int particle_blocks(gasx_ctx* ctx, int64_t cap)
 {
     const int64_t chunks = (cap + kChunk - 1) / kChunk;
-    const int64_t persistent = (int64_t)ctx->cu_count * 16;
+    const int64_t persistent = std::min<int64_t>((int64_t)ctx->cu_count * 16, kCounterSlots);
     return (int)std::max<int64_t>(1, std::min(chunks, persistent));
 }
 
@@ -470,6 +547,7 @@ static gasx_status fdtd_cycle(gasx_ctx* ctx, const double* er_in, const double* 
     r.er32 = ctx->er32; r.ez32 = ctx->ez32; r.h32 = ctx->h32;
     r.qr = qr; r.qz = qz; r.jr = jr; r.jz = jz;
     r.qr_clear = qr_clear; r.qz_clear = qz_clear;
+    TimedScope ts(ctx, kTFdtd);
     HIPCHK(launch_fdtd(ctx->fa, r, nullptr, 1, qr != nullptr, ctx->stream), "fdtd");
     return GASX_OK;
 }
@@ -519,6 +597,7 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
 {
     PartArgs pa = make_part(ctx, s);
     pa.qr = qr; pa.qz = qz;
+    TimedScope ts(ctx, kTParticle);
     HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity), ctx->stream),
            "particles");
     return GASX_OK;
@@ -572,14 +651,18 @@ extern "C" gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, in
 // Appends what the last particle kernel over `src` staged.  Ions must be
 // appended before the electrons' own secondaries (the electron count fixes
 // the number of staged chunks).
-static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion)
+static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion,
+                                     unsigned long long* qr = nullptr, unsigned long long* qz = nullptr)
 {
     float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
     const int32_t max_chunks = (int32_t)(ctx->stage_slots / kChunk);
+    const float kq = (float)(4294967296.0 / ctx->p.c) * dst->qnum;
+    TimedScope ts(ctx, kTAppend);
     HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? ctx->istage : ctx->stage,
                          ion ? ctx->istage_cnt : ctx->stage_cnt, src->count, ctx->scan_tmp,
                          ctx->base_tmp + (ion ? 2 : 0), max_chunks,
-                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->stream),
+                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz,
+                         ctx->stream),
            "append");
     return GASX_OK;
 }
@@ -642,6 +725,13 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         gasx_status st = ensure_staging(ctx, sp[0].capacity, o.ions);
         if (st != GASX_OK) return st;
     }
+    // cell-sorted slots keep each chunk's current within one LDS tile
+    if (o.sort_every > 0) {
+        for (int k = 0; k < nsp; ++k) {
+            gasx_status st = gasx_sort(ctx, &sp[k]);
+            if (st != GASX_OK) return st;
+        }
+    }
     HIPCHK(launch_fill_u64(ctx->q[0][0], (int64_t)nh, ctx->stream), "zero q");
     HIPCHK(launch_fill_u64(ctx->q[0][1], (int64_t)nh, ctx->stream), "zero q");
     // prologue: EM2 half drift + EM7 deposit of every species (electricmagneticfield6.m:141-144)
@@ -679,7 +769,8 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                     if (st != GASX_OK) return st;
                 }
                 if (o.secondaries) {
-                    st = append_from_stage(ctx, &sp[0], &sp[0], false);
+                    st = append_from_stage(ctx, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
+                                           last ? nullptr : ctx->q[nxt][1]);
                     if (st != GASX_OK) return st;
                 }
             }
@@ -728,7 +819,14 @@ extern "C" gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* io
     o.seed = ro->seed;
     o.run_id = ro->run_id;
     o.step0 = ro->step0;
-    return fused_loop(ctx, e, 1, ions, f, g, o);
+    if (ro->time_kernels) {
+        ctx->timing = true;
+        ctx->ev_used = 0;
+        for (auto& v : ctx->tpairs) v.clear();
+    }
+    gasx_status st = fused_loop(ctx, e, 1, ions, f, g, o);
+    ctx->timing = false;
+    return st;
 }
 
 extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
@@ -745,7 +843,31 @@ extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
         HIPCHK(hipMalloc(&ctx->sort_tmp, need), "alloc sort");
         ctx->sort_bytes = need;
     }
+    TimedScope ts(ctx, kTSort);
     HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream), "sort");
+    return GASX_OK;
+}
+
+extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)
+{
+    if (!ctx || !out) return GASX_E_ARG;
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    double ms[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 4; ++k)
+        for (auto& pr : ctx->tpairs[k]) {
+            float t = 0.f;
+            HIPCHK(hipEventElapsedTime(&t, ctx->events[pr.first], ctx->events[pr.second]), "event time");
+            ms[k] += t;
+        }
+    out->particle_ms = ms[kTParticle];
+    out->fdtd_ms = ms[kTFdtd];
+    out->append_ms = ms[kTAppend];
+    out->sort_ms = ms[kTSort];
+    out->n_particle_launches = (int64_t)ctx->tpairs[kTParticle].size();
+    out->n_fdtd_launches = (int64_t)ctx->tpairs[kTFdtd].size();
+    out->n_append_launches = (int64_t)ctx->tpairs[kTAppend].size();
+    out->n_sorts = (int64_t)ctx->tpairs[kTSort].size();
     return GASX_OK;
 }
 
@@ -770,9 +892,14 @@ extern "C" gasx_status gasx_get_counters(gasx_ctx* ctx, gasx_counters* out, int 
 {
     if (!ctx || !out) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    unsigned long long h[kNCounters];
-    HIPCHK(hipMemcpyAsync(h, ctx->d_counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream), "copy");
+    std::vector<unsigned long long> slots((size_t)kNCounters * kCounterSlots);
+    HIPCHK(hipMemcpyAsync(slots.data(), ctx->d_counters, slots.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, ctx->stream),
+           "copy");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    unsigned long long h[kNCounters] = {};
+    for (size_t b = 0; b < (size_t)kCounterSlots; ++b)
+        for (int k = 0; k < kNCounters; ++k) h[k] += slots[b * kNCounters + k];
     out->n_elastic = (int64_t)h[kCElastic];
     out->n_ionize = (int64_t)h[kCIonize];
     out->n_excite = (int64_t)h[kCExcite];
@@ -781,8 +908,10 @@ extern "C" gasx_status gasx_get_counters(gasx_ctx* ctx, gasx_counters* out, int 
     out->n_bound_violations = (int64_t)h[kCViol];
     out->n_absorbed = (int64_t)h[kCAbsorbed];
     out->n_saturated = (int64_t)h[kCSat];
+    out->n_pushed = (int64_t)h[kCPushed];
     if (reset) {
-        HIPCHK(hipMemsetAsync(ctx->d_counters, 0, sizeof(h), ctx->stream), "zero");
+        HIPCHK(hipMemsetAsync(ctx->d_counters, 0, slots.size() * sizeof(unsigned long long), ctx->stream),
+               "zero");
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     }
     if (out->n_dropped_sec > 0 || out->n_dropped_ion > 0)

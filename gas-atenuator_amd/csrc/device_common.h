@@ -39,41 +39,71 @@ __device__ __forceinline__ float u01(uint32_t x)
 // Marks an absorbed slot (reference parDelete = 1).
 __device__ __forceinline__ bool is_dead(float z) { return !(z == z); }
 
+// Hardware square root / reciprocal (about 1 ulp).  Used everywhere except
+// the fixed-point deposit, whose arithmetic must be IEEE-exact.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// x <= 0 (a particle that crossed the axis): the reference's literal
+// phi = atan(y/x) (not the polar angle there), shift by multiples of PHIp,
+// x = r cos phi, y = r sin phi, velocity rotated by the shift.  Rare.
+__device__ __forceinline__ void wedge_wrap_axis(const Geom& g, float r2, float& x, float& y,
+                                             float& vx, float& vy)
+{
+    const float r = fsqrt(r2);
+    float phi = atanf(y / x);
+    float rot;
+    if (phi < 0.0f) {
+        const float k = ceilf(-phi / g.phip);
+        phi = phi + k * g.phip;
+        rot = k * g.phip;
+    } else if (phi > g.phip) {
+        const float k = ceilf((phi - g.phip) / g.phip);
+        phi = phi - k * g.phip;
+        rot = -k * g.phip;
+    } else {
+        rot = 0.0f;
+    }
+    if (!(phi == phi)) { phi = 0.0f; rot = 0.0f; }   // x = y = 0: on the axis
+    float sr, cr, sp, cp;
+    __sincosf(rot, &sr, &cr);                        // |rot| <= pi/2 + PHIp
+    __sincosf(phi, &sp, &cp);                        // phi in [0, PHIp]
+    const float vx1 = vx * cr - vy * sr;
+    const float vy1 = vx * sr + vy * cr;
+    vx = vx1; vy = vy1;
+    x = r * cp; y = r * sp;
+}
+
 // Wedge wrap with velocity rotation and z reflection: EM2.cu:29-56 /
-// EM4.cu:150-177.  phi = atan(y/x) (the reference's quadrant convention); a
-// particle outside [0, PHIp] is moved by whole multiples of PHIp, its velocity
-// rotated by the same angle.  Returns false when r > Rp (absorbed).
+// EM4.cu:150-177.  The reference takes phi = atan(y/x) and, while phi < 0 or
+// phi > PHIp, shifts phi by +-PHIp and rotates the velocity by the same
+// angle; x = r cos phi, y = r sin phi then rotates the position by the same
+// total angle.  For x > 0 (atan is the polar angle) this is done here as
+// repeated rotations by +-PHIp with precomputed cos/sin (at most 16).  For
+// x <= 0, where atan(y/x) is not the polar angle, the reference's formula is
+// followed literally.  Returns false when r > Rp (absorbed).
 __device__ __forceinline__ bool wedge_bc(const Geom& g, float& x, float& y, float& z,
                                          float& vx, float& vy, float& vz)
 {
-    float r2 = x * x;
-    r2 = r2 + y * y;
-    float r = sqrtf(r2);
-    if (r > g.rp) return false;
-    // inside test without atan: x > 0, y >= 0, y*cos(PHIp) <= x*sin(PHIp)
-    bool inside = (x > 0.0f) && (y >= 0.0f) && (y * g.cphip <= x * g.sphip);
-    if (!inside) {
-        float phi = atanf(y / x);
-        float rot;
-        if (phi < 0.0f) {
-            float k = ceilf(-phi / g.phip);
-            phi = phi + k * g.phip;
-            rot = k * g.phip;
-        } else if (phi > g.phip) {
-            float k = ceilf((phi - g.phip) / g.phip);
-            phi = phi - k * g.phip;
-            rot = -k * g.phip;
-        } else {
-            rot = 0.0f;
+    const float r2 = x * x + y * y;
+    if (r2 > g.rp * g.rp) return false;
+    if (x > 0.0f) {
+        int guard = 0;
+#pragma unroll 1
+        while (y < 0.0f && guard++ < 32) {              // phi < 0: rotate by +PHIp
+            const float x1 = x * g.cphip - y * g.sphip, y1 = x * g.sphip + y * g.cphip;
+            const float u1 = vx * g.cphip - vy * g.sphip, w1 = vx * g.sphip + vy * g.cphip;
+            x = x1; y = y1; vx = u1; vy = w1;
         }
-        if (!(phi == phi)) { phi = 0.0f; rot = 0.0f; }   // x = y = 0: on the axis
-        float sr, cr, sp, cp;
-        sincosf(rot, &sr, &cr);
-        sincosf(phi, &sp, &cp);
-        float vx1 = vx * cr - vy * sr;
-        float vy1 = vx * sr + vy * cr;
-        vx = vx1; vy = vy1;
-        x = r * cp; y = r * sp;
+#pragma unroll 1
+        while (y * g.cphip > x * g.sphip && guard++ < 32) {   // phi > PHIp: rotate by -PHIp
+            const float x1 = x * g.cphip + y * g.sphip, y1 = -x * g.sphip + y * g.cphip;
+            const float u1 = vx * g.cphip + vy * g.sphip, w1 = -vx * g.sphip + vy * g.cphip;
+            x = x1; y = y1; vx = u1; vy = w1;
+        }
+    } else {
+        wedge_wrap_axis(g, r2, x, y, vx, vy);
     }
     if (z > g.lp) { z = 2.0f * g.lp - z; vz = -vz; }
     if (z < 0.0f) { z = -z; vz = -vz; }
@@ -86,23 +116,27 @@ __device__ __forceinline__ bool wedge_bc(const Geom& g, float& x, float& y, floa
 __device__ __forceinline__ void half_axis(float s, int n, int& i0, int& i1, float& w0, float& w1)
 {
     // s = x/d - 0.5
-    float fl = floorf(s);
-    int a = (int)fl;
-    float f = s - fl;
-    if (a < 0) { i0 = 0; i1 = 0; w0 = 1.0f; w1 = 0.0f; }
-    else if (a >= n - 1) { i0 = n - 1; i1 = n - 1; w0 = 1.0f; w1 = 0.0f; }
-    else { i0 = a; i1 = a + 1; w0 = 1.0f - f; w1 = f; }
+    const float fl = floorf(s);
+    const int a = (int)fl;
+    const float f = s - fl;
+    const bool lo = a < 0, hi = a >= n - 1;
+    i0 = lo ? 0 : (hi ? n - 1 : a);
+    i1 = lo ? 0 : (hi ? n - 1 : a + 1);
+    w0 = (lo || hi) ? 1.0f : 1.0f - f;
+    w1 = (lo || hi) ? 0.0f : f;
 }
 
 // Linear weights on a node axis (nodes at i*d, i in [0,n]): EM4.cu:41-80.
 __device__ __forceinline__ void node_axis(float s, int n, int& i0, int& i1, float& w0, float& w1)
 {
-    float fl = floorf(s);
+    const float fl = floorf(s);
     int a = (int)fl;
     float f = s - fl;
-    if (a < 0) { a = 0; f = 0.0f; }
-    if (a >= n) { a = n; f = 0.0f; }
-    i0 = a; i1 = a + 1 > n ? n : a + 1;
+    const bool out = a < 0 || a >= n;
+    f = out ? 0.0f : f;
+    a = a < 0 ? 0 : (a > n ? n : a);
+    i0 = a;
+    i1 = a + 1 > n ? n : a + 1;
     w0 = 1.0f - f; w1 = f;
 }
 
@@ -117,22 +151,21 @@ __device__ __forceinline__ float interp_in(const float* x, const float* y, int n
         int mid = (lo + hi) >> 1;
         if (x[mid] <= xq) lo = mid; else hi = mid;
     }
-    float s = (xq - x[lo]) / (x[hi] - x[lo]);
+    const float s = (xq - x[lo]) * frcp(x[hi] - x[lo]);
     return y[lo] + s * (y[hi] - y[lo]);
 }
 
 // find_CS0.m:14-56 for an energy T (eV) inside [15.6, 20000].
-__device__ __forceinline__ void cross_sections(const DevTables* t, float T, float& se, float& si, float& sx)
+__device__ __forceinline__ void cross_sections(const SmallTabs* t, float T, float& se, float& si, float& sx)
 {
     if (T >= 50.0f) se = interp_in(t->el_e, t->el_s, t->n_el, T);
     else se = interp_in(t->sel_e, t->sel_s, t->n_sel, T);
     si = interp_in(t->ion_e, t->ion_s, t->n_ion, T);
     float s = 0.0f;
-    float l2T = __log2f(T);
+    const float l2T = __log2f(T);
     for (int i = 0; i < t->n_exc; ++i) {
-        float W = t->exc_w[i];
-        float l2r = __log2f(W) - l2T;                     // log2(W/T)
-        float b = exp2f(t->exc_om[i] * l2r);              // (W/T)^omega
+        const float l2r = t->exc_l2w[i] - l2T;            // log2(W/T)
+        const float b = exp2f(t->exc_om[i] * l2r);        // (W/T)^omega
         float c = 1.0f - exp2f(t->exc_ga[i] * l2r);       // 1 - (W/T)^gamma
         c = exp2f(t->exc_nu[i] * __log2f(c));              // ^nu
         s += t->exc_q[i] * b * c;
@@ -143,32 +176,33 @@ __device__ __forceinline__ void cross_sections(const DevTables* t, float T, floa
 // Speed from kinetic energy E (eV), findV.m:5-8 in cancellation-free form.
 __device__ __forceinline__ float speed_from_ev(float E, float mc2_ev, float c)
 {
-    float g1 = E / mc2_ev;
-    return c * sqrtf(g1 * (g1 + 2.0f)) / (1.0f + g1);
+    const float g1 = E * frcp(mc2_ev);
+    return c * fsqrt(g1 * (g1 + 2.0f)) * frcp(1.0f + g1);
 }
 
 // Kinetic energy (eV) of speed^2 v2: EM4.cu:138-140, /e of
 // electricmagneticfield6.m:172, as m c^2 beta^2 gamma^2 / (1+gamma).
 __device__ __forceinline__ float ev_from_speed2(float v2, float inv_c2, float mc2_ev)
 {
-    float b2 = v2 * inv_c2;
-    float g2 = 1.0f / (1.0f - b2);
-    float gm = sqrtf(g2);
-    return mc2_ev * (b2 * g2 / (1.0f + gm));
+    const float b2 = v2 * inv_c2;
+    const float g2 = frcp(1.0f - b2);
+    const float gm = fsqrt(g2);
+    return mc2_ev * (b2 * g2 * frcp(1.0f + gm));
 }
 
 // rotate.m:4-13 with v^2 - vz^2 evaluated as vx^2 + vy^2.
 __device__ __forceinline__ void rotate_v(float& vx, float& vy, float& vz, float th, float ph, float eps)
 {
-    float x = vx, y = vy, zz = vz;
-    float vp2 = x * x + y * y;
-    float v = sqrtf(vp2 + zz * zz);
+    const float x = vx, y = vy, zz = vz;
+    const float vp2 = x * x + y * y;
+    const float v = fsqrt(vp2 + zz * zz);
     float st, ct, sp, cp;
-    sincosf(th, &st, &ct);
-    sincosf(ph, &sp, &cp);
+    __sincosf(th, &st, &ct);   // th in [0, pi], ph in [0, 2pi): no range reduction needed
+    __sincosf(ph, &sp, &cp);
     if (vp2 > eps) {
-        float vp = sqrtf(vp2);
-        float k = st / vp;
+        const float ivp = frsq(vp2);
+        const float vp = vp2 * ivp;
+        const float k = st * ivp;
         vx = x * ct + k * (x * zz * cp - v * y * sp);
         vy = y * ct + k * (y * zz * cp + v * x * sp);
         vz = zz * ct - vp * st * cp;
@@ -181,13 +215,13 @@ __device__ __forceinline__ void rotate_v(float& vx, float& vy, float& vz, float 
 
 // Mean excitation loss, get_ImpExc2.m:19-32 (form 0) or
 // electron_transport.cu:212-219 (form 1).
-__device__ __forceinline__ float exc_edep(const DevTables* t, float T, int form)
+__device__ __forceinline__ float exc_edep(const SmallTabs* t, float T, int form)
 {
     float k1 = 0.0f, k2 = 0.0f;
-    float l2T = __log2f(T);
+    const float l2T = __log2f(T);
     for (int i = 0; i < t->n_exc; ++i) {
-        float W = t->exc_w[i];
-        float l2r = __log2f(W) - l2T;
+        const float W = t->exc_w[i];
+        const float l2r = t->exc_l2w[i] - l2T;
         float b = exp2f(t->exc_om[i] * l2r);
         float c;
         if (form == 0) {

@@ -12,11 +12,20 @@ namespace gasx {
 constexpr int kNAng = 181;       // DCS angles 0..180 deg (ElasticCS_Nitrogen.csv rows 2..182)
 constexpr int kMaxTab = 64;
 constexpr int kMaxExc = 16;
-constexpr int kChunk = 256;      // particles per workgroup chunk (== particle kernel block size)
+#ifndef GASX_KPER
+#define GASX_KPER 4
+#endif
+constexpr int kBlock = 256;      // particle kernel workgroup size
+constexpr int kPer = GASX_KPER;  // particles per thread per chunk
+constexpr int kChunk = kBlock * kPer;   // slots per workgroup chunk (and per staging chunk)
+constexpr int kCounterSlots = 8192;  // per-block event counter slots (>= particle kernel grid)
+
+constexpr int kGuide = 32;       // guide entries per energy node of the elastic-angle CDF
 
 // Device copy of the prepared N2 tables (electron_collision.m:12-41,
-// find_CS0.m:14-56).  All energies in eV, cross sections in m^2.
-struct DevTables {
+// find_CS0.m:14-56).  All energies in eV, cross sections in m^2.  The small
+// part is copied into LDS by the collision kernels.
+struct SmallTabs {
     int32_t n_el, n_sel, n_ion, n_exc, n_dcs, pad_;
     float el_e[kMaxTab], el_s[kMaxTab];    // TotalElasticCS_Nitrogen (E>=50 eV), sigma*a0^2*2
     float sel_e[kMaxTab], sel_s[kMaxTab];  // TotalElasticCS_smalleV (E<50 eV), sigma*1e-20
@@ -25,8 +34,22 @@ struct DevTables {
     float exc_q[kMaxExc];                  // q0*A_i/W_i^2*1e-4  (sigma_exc prefactor, find_CS0.m:50)
     float exc_a[kMaxExc];                  // A_i/W_i^2          (E_dep weights, get_ImpExc2.m:24)
     float exc_om[kMaxExc], exc_nu[kMaxExc], exc_ga[kMaxExc];
+    float exc_l2w[kMaxExc];                // log2 W_i
     float dcs_loge[kMaxTab];               // log(E keV) of the merged DCS energies
+    float dcs_z[kMaxTab];                  // total interval mass Z of each node (see below)
+    float sind[184];                       // sin of integer degrees, exact 0 at 0 and 180
+    // guide[j][g]: largest k with cdf_j[k] <= g/kGuide (start of the CDF scan)
+    uint8_t guide[kMaxTab][kGuide + 1];
+};
+
+struct DevTables {
+    SmallTabs s;
     float dcs_logv[kMaxTab * kNAng];       // log DCS, [e*181 + angle]
+    // elastic-angle sampler: p_k = sind(k) DCS_k per energy node and the
+    // normalized cumulative mass of the 1-degree intervals (m_k = p_k + p_{k+1};
+    // cdf[0] = 0 ... cdf[180] = 1)
+    float dcs_p[kMaxTab * kNAng];
+    float dcs_cdf[kMaxTab * kNAng];
 };
 
 // Host-side prepared tables (FP64) + the device image.
@@ -92,7 +115,10 @@ struct PartArgs {
     unsigned long long* counters;  // gasx_counters layout
 };
 
-enum CounterIdx { kCElastic = 0, kCIonize, kCExcite, kCDropSec, kCDropIon, kCViol, kCAbsorbed, kCSat, kNCounters };
+enum CounterIdx {
+    kCElastic = 0, kCIonize, kCExcite, kCDropSec, kCDropIon, kCViol, kCAbsorbed, kCSat, kCPushed,
+    kNCounters
+};
 
 struct FdtdArgs {
     int32_t nr, nz;
@@ -129,7 +155,8 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
-                         int32_t max_chunks, unsigned long long* dropped, hipStream_t s);
+                         int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
+                         unsigned long long* qr, unsigned long long* qz, hipStream_t s);
 hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx, hipStream_t s);
 hipError_t launch_j_from_fixed(const FdtdArgs& a, const unsigned long long* qr,

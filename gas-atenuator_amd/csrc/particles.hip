@@ -8,20 +8,32 @@
 //                             in slot order so the final order is deterministic
 //
 // A workgroup owns chunks of kChunk consecutive slots (grid-stride over
-// chunks).  The slot count is read once at kernel start; appends land in a
-// later kernel (launch_append), so the count is stable inside this kernel.
+// chunks).  Per chunk:
+//   A  every lane: load, push, boundary conditions, first Philox block;
+//      lanes with u1 < pmax (the exact rejection bound on p_T) are candidates
+//   B  candidates are compacted into an LDS queue and processed by the first
+//      lanes of the block (cross sections, decision, elastic / ionization /
+//      excitation), so the collision work is dense instead of divergent
+//   C  every lane: read back, next half drift, deposit through a block-local
+//      LDS tile, store, stage secondaries and ions
+// The slot count is read once at kernel start; appends land in a later
+// kernel (launch_append), so the count is stable inside this kernel.
 #include "device_common.h"
 
 namespace gasx {
 
 // ---- deposit (EM7.cu:56-202 in fixed point) ---------------------------------
-// Weights are computed in FP32 with contraction off (the library is built with
-// -ffp-contract=off) so that the integer contributions are reproducible on the
-// CPU; the int64 sum is then order independent.
-__device__ __forceinline__ void deposit_one(const Geom& g, float x, float y, float z, float vx,
-                                            float vy, float vz, float kq,
-                                            unsigned long long* qr, unsigned long long* qz)
+// The contributions are computed in FP32 with contraction off so that they
+// are reproducible on the CPU; their int64 sum is order independent.
+struct DepCells {
+    int i0, i1, k0, k1;
+    long long qr[4], qz[4];   // cells (i0,k0) (i1,k0) (i0,k1) (i1,k1)
+};
+
+__device__ __forceinline__ void deposit_cells(const Geom& g, float x, float y, float z, float vx,
+                                              float vy, float vz, float kq, DepCells& d)
 {
+#pragma clang fp contract(off)
     float r2 = x * x;
     r2 = r2 + y * y;
     float r = sqrtf(r2);
@@ -38,95 +50,214 @@ __device__ __forceinline__ void deposit_one(const Geom& g, float x, float y, flo
     sr = sr - 0.5f;
     float sz = z * g.inv_dz;
     sz = sz - 0.5f;
-    int i0, i1, k0, k1;
     float wr0, wr1, wz0, wz1;
-    half_axis(sr, g.nr, i0, i1, wr0, wr1);
-    half_axis(sz, g.nz, k0, k1, wz0, wz1);
-    const int c00 = i0 + k0 * g.nr, c10 = i1 + k0 * g.nr, c01 = i0 + k1 * g.nr, c11 = i1 + k1 * g.nr;
-    const float w00 = wr0 * wz0, w10 = wr1 * wz0, w01 = wr0 * wz1, w11 = wr1 * wz1;
-    atomicAdd(&qr[c00], (unsigned long long)(long long)rintf(w00 * qrv));
-    atomicAdd(&qz[c00], (unsigned long long)(long long)rintf(w00 * qzv));
-    if (w10 != 0.0f) {
-        atomicAdd(&qr[c10], (unsigned long long)(long long)rintf(w10 * qrv));
-        atomicAdd(&qz[c10], (unsigned long long)(long long)rintf(w10 * qzv));
+    half_axis(sr, g.nr, d.i0, d.i1, wr0, wr1);
+    half_axis(sz, g.nz, d.k0, d.k1, wz0, wz1);
+    const float w[4] = {wr0 * wz0, wr1 * wz0, wr0 * wz1, wr1 * wz1};
+    // |w q v / c| 2^32 < 2^31 whenever |q v| < c/2 (always for the electrons
+    // here): one float->int32 conversion instead of the longer int64 one
+    const bool small = fabsf(qrv) < 2147483648.0f && fabsf(qzv) < 2147483648.0f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const float tr = rintf(w[a] * qrv), tz = rintf(w[a] * qzv);
+        d.qr[a] = small ? (long long)(int)tr : (long long)tr;
+        d.qz[a] = small ? (long long)(int)tz : (long long)tz;
     }
-    if (w01 != 0.0f) {
-        atomicAdd(&qr[c01], (unsigned long long)(long long)rintf(w01 * qrv));
-        atomicAdd(&qz[c01], (unsigned long long)(long long)rintf(w01 * qzv));
+}
+
+// A box that contains the particle's deposit stencil: the stencil cells
+// computed with fast arithmetic, widened by one cell (the exact arithmetic of
+// deposit_cells can land one cell over at a boundary).  tile_add still checks
+// every cell against the box.
+__device__ __forceinline__ void deposit_box(const Geom& g, float x, float y, float z, int& i0,
+                                            int& i1, int& k0, int& k1)
+{
+    const float r = fsqrt(x * x + y * y);
+    const int a = (int)floorf(r * g.inv_dr - 0.5f), b = (int)floorf(z * g.inv_dz - 0.5f);
+    i0 = max(min(a - 1, g.nr - 1), 0); i1 = max(min(a + 2, g.nr - 1), 0);
+    k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
+}
+
+__device__ __forceinline__ int dep_cell(const DepCells& d, int a, int nr)
+{
+    return ((a & 1) ? d.i1 : d.i0) + ((a & 2) ? d.k1 : d.k0) * nr;
+}
+
+__device__ __forceinline__ void deposit_global(const DepCells& d, int nr, unsigned long long* qr,
+                                               unsigned long long* qz)
+{
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int c = dep_cell(d, a, nr);
+        if (d.qr[a]) atomicAdd(&qr[c], (unsigned long long)d.qr[a]);
+        if (d.qz[a]) atomicAdd(&qz[c], (unsigned long long)d.qz[a]);
     }
-    if (w11 != 0.0f) {
-        atomicAdd(&qr[c11], (unsigned long long)(long long)rintf(w11 * qrv));
-        atomicAdd(&qz[c11], (unsigned long long)(long long)rintf(w11 * qzv));
+}
+
+constexpr int kTile = 512;    // cells of the block-local current tile (2 x 4 KB of LDS)
+
+// Block-local current tile.  When a chunk's deposit stencils fit in a 2-D box
+// of at most kTile cells (cell-sorted particles), contributions are summed in
+// LDS and flushed with one global atomic per non-zero cell; otherwise every
+// contribution goes to global memory directly.
+struct TileBox {
+    int ilo, klo, wi, span;   // span > kTile: direct global atomics; span == 0: nothing to do
+};
+
+// Block-wide bounding box of the stencils [ilo,ihi] x [klo,khi] every thread
+// holds for its particles (ilo > ihi: none).  Every thread must call it.
+__device__ __forceinline__ TileBox tile_box(int ilo, int ihi, int klo, int khi, int* s_red)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        ilo = min(ilo, __shfl_xor(ilo, o, 64));
+        ihi = max(ihi, __shfl_xor(ihi, o, 64));
+        klo = min(klo, __shfl_xor(klo, o, 64));
+        khi = max(khi, __shfl_xor(khi, o, 64));
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        s_red[wave] = ilo; s_red[nw + wave] = ihi;
+        s_red[2 * nw + wave] = klo; s_red[3 * nw + wave] = khi;
+    }
+    __syncthreads();
+    ilo = 0x7fffffff; ihi = -1; klo = 0x7fffffff; khi = -1;
+    for (int w = 0; w < nw; ++w) {
+        ilo = min(ilo, s_red[w]); ihi = max(ihi, s_red[nw + w]);
+        klo = min(klo, s_red[2 * nw + w]); khi = max(khi, s_red[3 * nw + w]);
+    }
+    __syncthreads();
+    TileBox b;
+    if (ihi < ilo) { b.ilo = b.klo = b.wi = b.span = 0; return b; }
+    b.ilo = ilo; b.klo = klo; b.wi = ihi - ilo + 1;
+    b.span = b.wi * (khi - klo + 1);
+    return b;
+}
+
+__device__ __forceinline__ void tile_zero(const TileBox& b, unsigned long long* s_tr,
+                                          unsigned long long* s_tz)
+{
+    for (int t = threadIdx.x; t < b.span; t += blockDim.x) { s_tr[t] = 0ull; s_tz[t] = 0ull; }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void tile_add(const TileBox& b, const DepCells& d, int nr,
+                                         unsigned long long* s_tr, unsigned long long* s_tz,
+                                         unsigned long long* qr, unsigned long long* qz)
+{
+    if (b.span > kTile) { deposit_global(d, nr, qr, qz); return; }
+    const int nk = b.span / b.wi;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = ((a & 1) ? d.i1 : d.i0) - b.ilo, k = ((a & 2) ? d.k1 : d.k0) - b.klo;
+        const unsigned long long vr = (unsigned long long)d.qr[a], vz = (unsigned long long)d.qz[a];
+        if ((unsigned)i < (unsigned)b.wi && (unsigned)k < (unsigned)nk) {
+            const int t = i + k * b.wi;
+            if (vr) atomicAdd(&s_tr[t], vr);
+            if (vz) atomicAdd(&s_tz[t], vz);
+        } else {                                 // outside the box: straight to global
+            const int c = (i + b.ilo) + (k + b.klo) * nr;
+            if (vr) atomicAdd(&qr[c], vr);
+            if (vz) atomicAdd(&qz[c], vz);
+        }
+    }
+}
+
+__device__ __forceinline__ void tile_flush(const TileBox& b, int nr, unsigned long long* s_tr,
+                                           unsigned long long* s_tz, unsigned long long* qr,
+                                           unsigned long long* qz)
+{
+    __syncthreads();
+    for (int t = threadIdx.x; t < b.span; t += blockDim.x) {
+        const unsigned long long a = s_tr[t], c = s_tz[t];
+        const int cell = (b.ilo + t % b.wi) + (b.klo + t / b.wi) * nr;
+        if (a) atomicAdd(&qr[cell], a);
+        if (c) atomicAdd(&qz[cell], c);
+    }
+    __syncthreads();                         // the tile is reused by the next chunk
 }
 
 // ---- gather + push (EM4.cu:36-148) ------------------------------------------
 __device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, float x, float y,
                                          float z, float& vx, float& vy, float& vz)
 {
-    float r2 = x * x;
-    r2 = r2 + y * y;
-    float r = sqrtf(r2);
-    float cph = 1.0f, sph = 0.0f;
-    if (r > 0.0f) { cph = x / r; sph = y / r; }
-    float vr = vx * cph + vy * sph;
-    const float srh = r * g.inv_dr - 0.5f, srn = r * g.inv_dr;
-    const float szh = z * g.inv_dz - 0.5f, szn = z * g.inv_dz;
+    const float r2 = x * x + y * y;
+    float cph = 1.0f, sph = 0.0f, r = 0.0f;
+    if (r2 > 0.0f) {
+        const float ir = frsq(r2);
+        r = r2 * ir;
+        cph = x * ir; sph = y * ir;
+    }
+    const float vr = vx * cph + vy * sph;
+    const float srn = r * g.inv_dr, szn = z * g.inv_dz;
+    const float srh = srn - 0.5f, szh = szn - 0.5f;
     int i0, i1, k0, k1;
     float a0, a1, b0, b1;
+    const int nr = g.nr;
     // Er: cell-centred in r, nodes in z (EM4.cu:41-60)
     half_axis(srh, g.nr, i0, i1, a0, a1);
     node_axis(szn, g.nz, k0, k1, b0, b1);
-    const int nr = g.nr;
-    float Er = b0 * (a0 * pa.er[i0 + k0 * nr] + a1 * pa.er[i1 + k0 * nr]) +
-               b1 * (a0 * pa.er[i0 + k1 * nr] + a1 * pa.er[i1 + k1 * nr]);
+    const float Er = b0 * (a0 * pa.er[i0 + k0 * nr] + a1 * pa.er[i1 + k0 * nr]) +
+                     b1 * (a0 * pa.er[i0 + k1 * nr] + a1 * pa.er[i1 + k1 * nr]);
+    // Hphi: cell-centred in both (EM4.cu:82-122); shares the r weights of Er
+    int kh0, kh1;
+    float c0, c1;
+    half_axis(szh, g.nz, kh0, kh1, c0, c1);
+    const float H = c0 * (a0 * pa.h[i0 + kh0 * nr] + a1 * pa.h[i1 + kh0 * nr]) +
+                    c1 * (a0 * pa.h[i0 + kh1 * nr] + a1 * pa.h[i1 + kh1 * nr]);
     // Ez: nodes in r, cell-centred in z (EM4.cu:61-80)
     node_axis(srn, g.nr, i0, i1, a0, a1);
-    half_axis(szh, g.nz, k0, k1, b0, b1);
     const int ld = nr + 1;
-    float Ez = b0 * (a0 * pa.ez[i0 + k0 * ld] + a1 * pa.ez[i1 + k0 * ld]) +
-               b1 * (a0 * pa.ez[i0 + k1 * ld] + a1 * pa.ez[i1 + k1 * ld]);
-    // Hphi: cell-centred in both (EM4.cu:82-122)
-    half_axis(srh, g.nr, i0, i1, a0, a1);
-    float H = b0 * (a0 * pa.h[i0 + k0 * nr] + a1 * pa.h[i1 + k0 * nr]) +
-              b1 * (a0 * pa.h[i0 + k1 * nr] + a1 * pa.h[i1 + k1 * nr]);
+    const float Ez = c0 * (a0 * pa.ez[i0 + kh0 * ld] + a1 * pa.ez[i1 + kh0 * ld]) +
+                     c1 * (a0 * pa.ez[i0 + kh1 * ld] + a1 * pa.ez[i1 + kh1 * ld]);
     // EM4.cu:126-144: magnetic term uses v^n; explicit (not Boris) update
-    float fr = pa.q_over_m * (Er - vz * g.mu0 * H);
-    float fz = pa.q_over_m * (Ez + vr * g.mu0 * H);
-    float ax = fr * cph, ay = fr * sph;
-    float v2 = vx * vx + vy * vy + vz * vz;
-    float gm = 1.0f / sqrtf(1.0f - v2 * g.inv_c2);
-    float ux = gm * vx + ax * g.dt, uy = gm * vy + ay * g.dt, uz = gm * vz + fz * g.dt;
-    float gp = sqrtf(1.0f + (ux * ux + uy * uy + uz * uz) * g.inv_c2);
-    vx = ux / gp; vy = uy / gp; vz = uz / gp;
+    const float fr = pa.q_over_m * (Er - vz * g.mu0 * H);
+    const float fz = pa.q_over_m * (Ez + vr * g.mu0 * H);
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    const float gm = rsqrtf(1.0f - v2 * g.inv_c2);
+    const float ux = gm * vx + (fr * cph) * g.dt;
+    const float uy = gm * vy + (fr * sph) * g.dt;
+    const float uz = gm * vz + fz * g.dt;
+    const float igp = rsqrtf(1.0f + (ux * ux + uy * uy + uz * uz) * g.inv_c2);
+    vx = ux * igp; vy = uy * igp; vz = uz * igp;
 }
 
-// ---- elastic angle: get_ElasticSc_sub1.m:4-19 ---------------------------------
-// log-DCS linear in log E at each node angle (griddata on the grid lines),
-// trapezoid CDF over theta, inverse linear interpolation of U*CDF_end.
-// Constant factors (2*pi*2/TCS) cancel in the normalised inverse.
-__device__ float elastic_theta(const DevTables* t, const float* sind, float e_kev, float u)
+__device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, float& z, float vx,
+                                           float vy, float vz)
 {
-    float lx = logf(e_kev);
-    int n = t->n_dcs, i = 0;
-    while (i < n - 2 && lx > t->dcs_loge[i + 1]) ++i;
-    float w = (lx - t->dcs_loge[i]) / (t->dcs_loge[i + 1] - t->dcs_loge[i]);
+    const float h = 0.5f * g.dt;
+    x = x + h * vx;
+    y = y + h * vy;
+    z = z + h * vz;
+}
+
+// ---- elastic angle ------------------------------------------------------------
+// Reference semantics (get_ElasticSc_sub1.m:4-19 on the merged tables): log DCS
+// linear in log E at each node angle, trapezoid CDF over theta, inverse linear
+// interpolation -- i.e. interval k of 1 deg is chosen with probability
+// proportional to m_k(w) = p_k(w) + p_{k+1}(w), p_k(w) = sind(k) exp((1-w) V_i[k]
+// + w V_{i+1}[k]), and theta is uniform inside it.  Sampled exactly by
+// rejection: exp is convex, so m_k(w) <= (1-w) m_k(i) + w m_k(i+1); propose
+// node i or i+1 with weights (1-w) Z_i, w Z_{i+1}, draw k from that node's
+// precomputed interval CDF (binary search), accept with m_k(w) / M_k.
+__device__ float elastic_theta_exact(const DevTables* t, const SmallTabs* st, int i,
+                                                 float w, float u)
+{
     const float* v0 = &t->dcs_logv[i * kNAng];
     const float* v1 = &t->dcs_logv[(i + 1) * kNAng];
     float tot = 0.0f, prev = 0.0f;
+#pragma unroll 1
     for (int k = 0; k < kNAng; ++k) {
-        float pdf = sind[k] * __expf((1.0f - w) * v0[k] + w * v1[k]);
-        if (k > 0) tot += 0.5f * (prev + pdf);
-        prev = pdf;
+        const float p = st->sind[k] * __expf((1.0f - w) * v0[k] + w * v1[k]);
+        if (k > 0) tot += prev + p;
+        prev = p;
     }
-    float target = u * tot;
-    float cdf = 0.0f;
-    float p0 = sind[0] * __expf((1.0f - w) * v0[0] + w * v1[0]);
+    const float target = u * tot;
+    float cdf = 0.0f, p0 = 0.0f, s = 1.0f;
     int k = 0;
-    float s = 1.0f;
+#pragma unroll 1
     for (k = 0; k < kNAng - 1; ++k) {
-        float p1 = sind[k + 1] * __expf((1.0f - w) * v0[k + 1] + w * v1[k + 1]);
-        float nxt = cdf + 0.5f * (p0 + p1);
+        const float p1 = st->sind[k + 1] * __expf((1.0f - w) * v0[k + 1] + w * v1[k + 1]);
+        const float nxt = cdf + (p0 + p1);
         if (nxt >= target || k == kNAng - 2) {
             s = (nxt > cdf) ? (target - cdf) / (nxt - cdf) : 0.0f;
             break;
@@ -138,105 +269,130 @@ __device__ float elastic_theta(const DevTables* t, const float* sind, float e_ke
     return ((float)k + s) * (3.14159265358979f / 180.0f);
 }
 
+__device__ float elastic_theta(const DevTables* t, const SmallTabs* st, const CollArgs& ca,
+                               uint32_t slot, float e_kev)
+{
+    const float lx = logf(e_kev);
+    int i = 0, hi = st->n_dcs - 1;                 // bin: dcs_loge[i] <= lx (clamped to n-2)
+    while (hi - i > 1) {
+        const int mid = (i + hi) >> 1;
+        if (st->dcs_loge[mid] < lx) i = mid; else hi = mid;
+    }
+    const float w = (lx - st->dcs_loge[i]) / (st->dcs_loge[i + 1] - st->dcs_loge[i]);
+    const float wa = (1.0f - w) * st->dcs_z[i], wb = w * st->dcs_z[i + 1];
+    const float* v0 = &t->dcs_logv[i * kNAng];
+    const float* v1 = &t->dcs_logv[(i + 1) * kNAng];
+    const float* p0 = &t->dcs_p[i * kNAng];
+    const float* p1 = &t->dcs_p[(i + 1) * kNAng];
+#pragma unroll 1
+    for (uint32_t it = 0; it < 64; ++it) {
+        const U4 R = philox4x32_10(slot, ca.step, ca.run, 2u + it, ca.key0, ca.key1);
+        const float a = u01(R.a), b = u01(R.b), c = u01(R.c), d = u01(R.d);
+        const int j = (a * (wa + wb) < wa) ? i : i + 1;
+        const float* C = &t->dcs_cdf[j * kNAng];
+        // largest k with C[k] <= b lies in [guide[g], guide[g+1]] (binary search)
+        const int gi = (int)(b * (float)kGuide);
+        int k = st->guide[j][gi], khi = st->guide[j][gi + 1] + 1;
+        if (khi > kNAng - 1) khi = kNAng - 1;
+        while (khi - k > 1) {
+            const int mid = (k + khi) >> 1;
+            if (C[mid] <= b) k = mid; else khi = mid;
+        }
+        const float M = (1.0f - w) * (p0[k] + p0[k + 1]) + w * (p1[k] + p1[k + 1]);
+        const float m = st->sind[k] * __expf((1.0f - w) * v0[k] + w * v1[k]) +
+                        st->sind[k + 1] * __expf((1.0f - w) * v0[k + 1] + w * v1[k + 1]);
+        if (c * M < m) return ((float)k + d) * (3.14159265358979f / 180.0f);
+    }
+    const U4 R = philox4x32_10(slot, ca.step, ca.run, 66u, ca.key0, ca.key1);
+    return elastic_theta_exact(t, st, i, w, u01(R.a));
+}
+
+// ---- collision of one candidate (electron_transport.cu:64-227) ---------------
+enum : int { kEvMask = 3, kEvViol = 4, kEvSat = 8, kEvSec = 16, kEvIon = 32 };
+
 struct CollOut {
-    int ev;          // 0 none, 1 elastic, 2 ionization, 3 excitation
-    bool sec;        // secondary electron produced
+    int ev;
     float svx, svy, svz;
-    bool ion;
     float ivx, ivy, ivz;
-    bool viol, sat;
 };
 
-// electron_transport.cu:64-227 with the find_CS0.m cross sections and the
-// interp1 NaN gating (collide only if 15.6 eV <= T <= 20 keV).
-__device__ __forceinline__ void collide_one(const Geom& g, const CollArgs& ca, const float* sind,
-                                            uint32_t slot, float x, float y, float z,
-                                            float& vx, float& vy, float& vz, bool want_ion,
-                                            CollOut& o)
+// find_CS0.m cross sections, interp1 NaN gating (collide only for
+// 15.6 eV <= T <= 20 keV), linear p_T, branch choice by u2, and the elastic /
+// ionization / excitation kinematics of electron_transport.cu with the
+// MATLAB choices.
+__device__ void collide_candidate(const Geom& g, const CollArgs& ca, const SmallTabs* stab,
+                                  uint32_t slot, float x, float y, float z, float& vx, float& vy,
+                                  float& vz, float u1, float u2, float u3, float u4, bool want_ion,
+                                  CollOut& o)
 {
-    o.ev = 0; o.sec = false; o.ion = false; o.viol = false; o.sat = false;
-    U4 R = philox4x32_10(slot, ca.step, ca.run, 0u, ca.key0, ca.key1);
-    float u1 = u01(R.a);
-    if (!(u1 < ca.pmax)) return;                 // exact rejection: p_T <= pmax for every candidate
-    float v2 = vx * vx + vy * vy + vz * vz;
-    float T = ev_from_speed2(v2, g.inv_c2, ca.ev_per_mc2_em);
+    o.ev = 0;
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    const float T = ev_from_speed2(v2, g.inv_c2, ca.ev_per_mc2_em);
     if (!(T >= 15.6f && T <= 20000.0f)) return;
     const DevTables* t = ca.tab;
     float se, si, sx;
-    cross_sections(t, T, se, si, sx);
-    float r = sqrtf(x * x + y * y);
+    cross_sections(stab, T, se, si, sx);
+    const float r = fsqrt(x * x + y * y);
     int ir = (int)floorf(r * ca.inv_gdr), iz = (int)floorf(z * ca.inv_gdz);
     ir = ir < 0 ? 0 : (ir > ca.gnr - 1 ? ca.gnr - 1 : ir);
     iz = iz < 0 ? 0 : (iz > ca.gnz - 1 ? ca.gnz - 1 : iz);
-    float dens = ca.dens[iz * ca.gnr + ir];
-    float st = se + si + sx;
-    float ve = sqrtf(v2);
-    float pt = ve * ca.dt_coll * dens * st;      // electron_transport.cu:74-77
-    o.viol = pt > ca.pmax && ca.pmax < 1.0f;
-    if (!(u1 < pt)) return;                      // :85
-    o.sat = pt > 1.0f;
-    float u2 = u01(R.b), u3 = u01(R.c), u4 = u01(R.d);
+    const float dens = ca.dens[iz * ca.gnr + ir];
+    const float st = se + si + sx;
+    const float ve = fsqrt(v2);
+    const float pt = ve * ca.dt_coll * dens * st;      // electron_transport.cu:74-77
+    if (pt > ca.pmax && ca.pmax < 1.0f) o.ev |= kEvViol;
+    if (!(u1 < pt)) return;                             // :85
+    if (pt > 1.0f) o.ev |= kEvSat;
     const float twopi = 6.28318530717959f;
-    if (u2 * st < se) {                          // elastic (:86-137)
-        float th = elastic_theta(t, sind, T * 1e-3f, u3);
+    if (u2 * st < se) {                                 // elastic (:86-137)
+        const float th = elastic_theta(t, stab, ca, slot, T * 1e-3f);
         rotate_v(vx, vy, vz, th, twopi * u4, ca.rot_eps);
-        o.ev = 1;
-    } else if (u2 * st < se + si && T > 15.6f) { // ionization (:139-205)
+        o.ev |= 1;
+    } else if (u2 * st < se + si && T > 15.6f) {        // ionization (:139-205)
         const float I = 15.6f;
-        float T0 = 4.17f - 1000.0f / (T + 2.0f * I);
-        float tt = 13.8f * T / (T + I);
-        float Tm = (T - I) * 0.5f;
-        float k1 = atanf((Tm - T0) / tt);
-        float k2 = atanf(T0 / tt);
+        const float T0 = 4.17f - 1000.0f / (T + 2.0f * I);
+        const float tt = 13.8f * T / (T + I);
+        const float Tm = (T - I) * 0.5f;
+        const float k1 = atanf((Tm - T0) / tt);
+        const float k2 = atanf(T0 / tt);
         float k0 = T0 + tt * tanf(u3 * (k1 + k2) - k2);
         k0 = fminf(fmaxf(k0, 0.0f), Tm);
-        float E1 = k0, E2 = T - E1 - I;
-        float Ep = E2 > E1 ? E2 : E1;
-        float Es = E2 > E1 ? E1 : E2;
-        float Ee = ca.mc2_mc_ev;
-        float kt = k0 / T;
-        float sp = sqrtf(kt / ((1.0f - kt) * T / (2.0f * Ee) + 1.0f));
-        float ss = sqrtf((1.0f - kt) / (1.0f + k0 / (2.0f * Ee)));
-        float thp = asinf(fminf(sp, 1.0f)), ths = asinf(fminf(ss, 1.0f));
-        float php = twopi * u4, phs = php - 3.14159265358979f;
-        float vp = speed_from_ev(Ep, Ee, g.c);
-        float ratio = vp / ve;
+        const float E1 = k0, E2 = T - E1 - I;
+        const float Ep = E2 > E1 ? E2 : E1;
+        const float Es = E2 > E1 ? E1 : E2;
+        const float Ee = ca.mc2_mc_ev;
+        const float kt = k0 / T;
+        const float sp = sqrtf(kt / ((1.0f - kt) * T / (2.0f * Ee) + 1.0f));
+        const float ss = sqrtf((1.0f - kt) / (1.0f + k0 / (2.0f * Ee)));
+        const float thp = asinf(fminf(sp, 1.0f)), ths = asinf(fminf(ss, 1.0f));
+        const float php = twopi * u4, phs = php - 3.14159265358979f;
+        const float ratio = speed_from_ev(Ep, Ee, g.c) / ve;
         vx *= ratio; vy *= ratio; vz *= ratio;
         rotate_v(vx, vy, vz, thp, php, ca.rot_eps);
-        float vs = speed_from_ev(Es, Ee, g.c);
+        const float vs = speed_from_ev(Es, Ee, g.c);
         float sts, cts, sps, cps;
-        sincosf(ths, &sts, &cts);
-        sincosf(phs, &sps, &cps);
-        o.sec = true;
+        __sincosf(ths, &sts, &cts);   // |angles| <= pi: no range reduction needed
+        __sincosf(phs, &sps, &cps);
+        o.ev |= 2 | kEvSec;
         o.svx = vs * sts * cps; o.svy = vs * sts * sps; o.svz = vs * cts;   // :198-200
-        if (want_ion) {                           // :202-204, Box-Muller normals
-            U4 Q = philox4x32_10(slot, ca.step, ca.run, 1u, ca.key0, ca.key1);
-            float a1 = u01(Q.a), a2 = u01(Q.b), a3 = u01(Q.c), a4 = u01(Q.d);
-            float Tg = ca.temp ? ca.temp[iz * ca.gnr + ir] : ca.t_scalar;
-            float sg = ca.ion_vth_k * sqrtf(Tg);
-            float rr1 = sqrtf(-2.0f * logf(a1)), rr2 = sqrtf(-2.0f * logf(a3));
+        if (want_ion) {                                  // :202-204, Box-Muller normals
+            const U4 Q = philox4x32_10(slot, ca.step, ca.run, 1u, ca.key0, ca.key1);
+            const float a1 = u01(Q.a), a2 = u01(Q.b), a3 = u01(Q.c), a4 = u01(Q.d);
+            const float Tg = ca.temp ? ca.temp[iz * ca.gnr + ir] : ca.t_scalar;
+            const float sg = ca.ion_vth_k * sqrtf(Tg);
+            const float rr1 = sqrtf(-2.0f * logf(a1)), rr2 = sqrtf(-2.0f * logf(a3));
             float s1, c1, s2, c2;
-            sincosf(twopi * a2, &s1, &c1);
-            sincosf(twopi * a4, &s2, &c2);
-            o.ion = true;
+            __sincosf(twopi * a2, &s1, &c1);
+            __sincosf(twopi * a4, &s2, &c2);
+            o.ev |= kEvIon;
             o.ivx = sg * rr1 * c1; o.ivy = sg * rr1 * s1; o.ivz = sg * rr2 * c2;
         }
-        o.ev = 2;
-    } else {                                     // excitation (:206-226)
-        float edep = exc_edep(t, T, ca.exc_form);
-        float vp = speed_from_ev(T - edep, ca.mc2_mc_ev, g.c);
-        float ratio = vp / ve;
+    } else {                                             // excitation (:206-226)
+        const float edep = exc_edep(stab, T, ca.exc_form);
+        const float ratio = speed_from_ev(T - edep, ca.mc2_mc_ev, g.c) / ve;
         vx *= ratio; vy *= ratio; vz *= ratio;
-        o.ev = 3;
+        o.ev |= 3;
     }
-}
-
-__device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, float& z, float vx,
-                                           float vy, float vz)
-{
-    x = x + 0.5f * g.dt * vx;
-    y = y + 0.5f * g.dt * vy;
-    z = z + 0.5f * g.dt * vz;
 }
 
 // Exclusive rank of `pred` among the workgroup's lanes (lane order) and the
@@ -244,14 +400,14 @@ __device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, fl
 __device__ __forceinline__ int block_rank(bool pred, int* s_wave, int& total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    unsigned long long m = __ballot(pred);
-    unsigned long long lt = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    int within = __popcll(lt);
+    const unsigned long long m = __ballot(pred);
+    const unsigned long long lt = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
+    const int within = __popcll(lt);
     if (lane == 0) s_wave[wave] = __popcll(m);
     __syncthreads();
     int off = 0, tot = 0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-        int c = s_wave[w];
+        const int c = s_wave[w];
         if (w < wave) off += c;
         tot += c;
     }
@@ -260,107 +416,288 @@ __device__ __forceinline__ int block_rank(bool pred, int* s_wave, int& total)
     return off + within;
 }
 
-__device__ __forceinline__ void wave_count(bool pred, unsigned long long* ctr)
+constexpr int kQW = 11;     // LDS queue words per candidate
+constexpr int kQCap = 512;  // candidates per dense round
+
+// Exclusive prefix over the block of per-thread counts (thread order) and the
+// block total.  Every thread must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 {
-    unsigned long long m = __ballot(pred);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (unsigned long long)__popcll(m));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        const int c = s_wave[w];
+        if (w < wave) off += c;
+        tot += c;
+    }
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
+}
+
+// Ranks of kPer predicates in slot order (sub-chunk p major, lane minor) and
+// the total.  Every thread must call it.
+__device__ __forceinline__ int block_rank_slots(const bool (&pred)[kPer], int (&rank)[kPer],
+                                                int* s_w4)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const unsigned long long lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+        const unsigned long long m = __ballot(pred[p]);
+        rank[p] = __popcll(m & lt);
+        if (lane == 0) s_w4[p * nw + wave] = __popcll(m);
+    }
+    __syncthreads();
+    int base = 0;
+#pragma unroll
+    for (int p = 0; p < kPer; ++p) {
+        int off = 0, tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int c = s_w4[p * nw + w];
+            if (w < wave) off += c;
+            tot += c;
+        }
+        rank[p] += base + off;
+        base += tot;
+    }
+    __syncthreads();
+    return base;
 }
 
 template <uint32_t F>
-__global__ __launch_bounds__(kChunk) void k_particles(Geom g, PartArgs pa, CollArgs ca)
+__global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa, CollArgs ca)
 {
-    __shared__ int s_wave[kChunk / 64];
-    __shared__ float s_sind[kNAng];
-    if (F & kCollide) {
-        for (int k = threadIdx.x; k < kNAng; k += blockDim.x) {
-            float s = (k == 0 || k == 180) ? 0.0f : (k == 90 ? 1.0f : sinf(k * (3.14159265358979f / 180.0f)));
-            s_sind[k] = s;
-        }
+    constexpr bool kDep = (F & kDeposit) != 0;
+    constexpr bool kCol = (F & kCollide) != 0;
+    __shared__ int s_wave[kBlock / 64];
+    __shared__ int s_red[4 * kPer * (kBlock / 64)];
+    __shared__ unsigned long long s_tr[kDep ? kTile : 1], s_tz[kDep ? kTile : 1];
+    __shared__ float s_q[kCol ? kQW : 1][kCol ? kQCap : 1];
+    static_assert(sizeof(SmallTabs) % 4 == 0, "SmallTabs is copied in 32-bit words");
+    __shared__ uint32_t s_tab_raw[kCol ? sizeof(SmallTabs) / 4 : 1];
+    SmallTabs* s_tab = reinterpret_cast<SmallTabs*>(s_tab_raw);
+    if (kCol) {   // small collision tables into LDS (find_CS0 tables, excitation fit, guides)
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&ca.tab->s);
+        for (int i = threadIdx.x; i < (int)(sizeof(SmallTabs) / 4); i += blockDim.x) s_tab_raw[i] = src[i];
         __syncthreads();
     }
+
     const int n = *pa.count;
     const int nchunks = (n + kChunk - 1) / kChunk;
+    unsigned c_el = 0, c_ion = 0, c_exc = 0, c_viol = 0, c_sat = 0, c_abs = 0, c_push = 0;
     for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-        const int slot = chunk * kChunk + (int)threadIdx.x;
-        const bool valid = slot < n;
-        float x = 0.f, y = 0.f, z = 0.f, vx = 0.f, vy = 0.f, vz = 0.f;
-        bool alive = false;
-        if (valid) {
-            z = pa.z[slot];
-            alive = !is_dead(z);
-            if (alive) {
-                x = pa.x[slot]; y = pa.y[slot];
-                vx = pa.vx[slot]; vy = pa.vy[slot]; vz = pa.vz[slot];
+        // slot of sub-chunk p: chunk*kChunk + p*kBlock + lane (coalesced per p)
+        float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
+        bool alive[kPer], was_alive[kPer], absorbed[kPer], cand[kPer];
+        int ev[kPer];
+        float svx[kPer], svy[kPer], svz[kPer];
+        float ivx[kPer], ivy[kPer], ivz[kPer];
+        float u1[kPer], u2[kPer], u3[kPer], u4[kPer];
+#pragma unroll
+        for (int p = 0; p < kPer; ++p) {
+            const int slot = chunk * kChunk + p * kBlock + (int)threadIdx.x;
+            x[p] = y[p] = z[p] = vx[p] = vy[p] = vz[p] = 0.f;
+            alive[p] = false;
+            if (slot < n) {
+                z[p] = pa.z[slot];
+                alive[p] = !is_dead(z[p]);
+                if (alive[p]) {
+                    x[p] = pa.x[slot]; y[p] = pa.y[slot];
+                    vx[p] = pa.vx[slot]; vy[p] = pa.vy[slot]; vz[p] = pa.vz[slot];
+                }
+            }
+            was_alive[p] = alive[p];
+            absorbed[p] = false;
+            ev[p] = 0;
+            cand[p] = false;
+            svx[p] = svy[p] = svz[p] = ivx[p] = ivy[p] = ivz[p] = 0.f;
+            // ---- A: push + first Philox block ------------------------------
+            if ((F & kPush) && alive[p]) {
+                push_one(g, pa, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) { alive[p] = false; absorbed[p] = true; }
+            }
+            u1[p] = 1.f; u2[p] = u3[p] = u4[p] = 0.f;
+            if (kCol && alive[p]) {
+                const U4 R = philox4x32_10((uint32_t)slot, ca.step, ca.run, 0u, ca.key0, ca.key1);
+                u1[p] = u01(R.a); u2[p] = u01(R.b); u3[p] = u01(R.c); u4[p] = u01(R.d);
+                cand[p] = u1[p] < ca.pmax;     // exact: p_T <= pmax for every particle
             }
         }
-        const bool was_alive = alive;
-        bool absorbed = false;
-        if ((F & kPush) && alive) {
-            push_one(g, pa, x, y, z, vx, vy, vz);
-            half_drift(g, x, y, z, vx, vy, vz);
-            if (!wedge_bc(g, x, y, z, vx, vy, vz)) { alive = false; absorbed = true; }
-        }
-        CollOut co;
-        co.ev = 0; co.sec = false; co.ion = false; co.viol = false; co.sat = false;
-        if ((F & kCollide) && alive) {
-            collide_one(g, ca, s_sind, (uint32_t)slot, x, y, z, vx, vy, vz, (F & kStageIon) != 0, co);
-        }
-        const float cx = x, cy = y, cz = z;   // collision site
-        bool has_sec = (F & kStageSec) && co.sec;
-        float sx = x, sy = y, sz = z, svx = co.svx, svy = co.svy, svz = co.svz;
-        bool sec_absorbed = false;
-        if (F & kDrift) {
-            if (alive) {
-                half_drift(g, x, y, z, vx, vy, vz);
-                if (!wedge_bc(g, x, y, z, vx, vy, vz)) { alive = false; absorbed = true; }
+        // ---- B: collisions of the candidates, densely, in rounds of kQCap -------
+        if (kCol) {
+            int mine = 0;
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) mine += cand[p];
+            int total;
+            const int off = block_excl_scan(mine, s_wave, total);
+            for (int base = 0; base < total; base += kQCap) {
+                int pos = off;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (cand[p]) {
+                        const int q = pos - base;
+                        if (q >= 0 && q < kQCap) {
+                            s_q[0][q] = x[p]; s_q[1][q] = y[p]; s_q[2][q] = z[p];
+                            s_q[3][q] = vx[p]; s_q[4][q] = vy[p]; s_q[5][q] = vz[p];
+                            s_q[6][q] = u1[p]; s_q[7][q] = u2[p]; s_q[8][q] = u3[p]; s_q[9][q] = u4[p];
+                            s_q[10][q] = __int_as_float(chunk * kChunk + p * kBlock + (int)threadIdx.x);
+                        }
+                        ++pos;
+                    }
+                }
+                __syncthreads();
+                const int nq = min(kQCap, total - base);
+                for (int q = threadIdx.x; q < nq; q += blockDim.x) {
+                    float qvx = s_q[3][q], qvy = s_q[4][q], qvz = s_q[5][q];
+                    CollOut co;
+                    collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(s_q[10][q]), s_q[0][q],
+                                      s_q[1][q], s_q[2][q], qvx, qvy, qvz, s_q[6][q], s_q[7][q],
+                                      s_q[8][q], s_q[9][q], (F & kStageIon) != 0, co);
+                    s_q[3][q] = qvx; s_q[4][q] = qvy; s_q[5][q] = qvz;
+                    s_q[10][q] = __int_as_float(co.ev);
+                    if (co.ev & kEvSec) { s_q[6][q] = co.svx; s_q[7][q] = co.svy; s_q[8][q] = co.svz; }
+                    if (co.ev & kEvIon) { s_q[0][q] = co.ivx; s_q[1][q] = co.ivy; s_q[2][q] = co.ivz; }
+                }
+                __syncthreads();
+                pos = off;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (cand[p]) {
+                        const int q = pos - base;
+                        if (q >= 0 && q < kQCap) {
+                            vx[p] = s_q[3][q]; vy[p] = s_q[4][q]; vz[p] = s_q[5][q];
+                            ev[p] = __float_as_int(s_q[10][q]);
+                            if (ev[p] & kEvSec) { svx[p] = s_q[6][q]; svy[p] = s_q[7][q]; svz[p] = s_q[8][q]; }
+                            if (ev[p] & kEvIon) { ivx[p] = s_q[0][q]; ivy[p] = s_q[1][q]; ivz[p] = s_q[2][q]; }
+                        }
+                        ++pos;
+                    }
+                }
+                __syncthreads();                 // s_q is rewritten by the next round / chunk
             }
-            if (has_sec) {
-                half_drift(g, sx, sy, sz, svx, svy, svz);
-                if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; sec_absorbed = true; }
+        }
+        // ---- C: next half drift, store ------------------------------------------
+        bool has_sec[kPer], has_ion[kPer];
+        float sx[kPer], sy[kPer], sz[kPer];   // secondary position (drifted with it)
+        float cx[kPer], cy[kPer], cz[kPer];   // collision site (ions are placed there)
+        int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
+#pragma unroll
+        for (int p = 0; p < kPer; ++p) {
+            const int slot = chunk * kChunk + p * kBlock + (int)threadIdx.x;
+            has_sec[p] = (F & kStageSec) && (ev[p] & kEvSec);
+            has_ion[p] = (F & kStageIon) && (ev[p] & kEvIon);
+            cx[p] = sx[p] = x[p]; cy[p] = sy[p] = y[p]; cz[p] = sz[p] = z[p];
+            if (F & kDrift) {
+                if (alive[p]) {
+                    half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                    if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) { alive[p] = false; absorbed[p] = true; }
+                }
+                if (has_sec[p]) {
+                    half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
+                    if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) { has_sec[p] = false; ++c_abs; }
+                }
+            }
+            if (slot < n && was_alive[p]) {
+                if (alive[p]) {
+                    pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
+                    pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
+                } else {
+                    pa.z[slot] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+                }
+            }
+            if (kDep && alive[p]) {
+                int i0, i1, k0, k1;
+                deposit_box(g, x[p], y[p], z[p], i0, i1, k0, k1);
+                ilo = min(ilo, i0); ihi = max(ihi, i1); klo = min(klo, k0); khi = max(khi, k1);
             }
         }
-        if (F & kDeposit) {
-            if (alive) deposit_one(g, x, y, z, vx, vy, vz, pa.kq, pa.qr, pa.qz);
-            if (has_sec) deposit_one(g, sx, sy, sz, svx, svy, svz, pa.kq, pa.qr, pa.qz);
-        }
-        if (valid && was_alive) {
-            if (alive) {
-                pa.x[slot] = x; pa.y[slot] = y; pa.z[slot] = z;
-                pa.vx[slot] = vx; pa.vy[slot] = vy; pa.vz[slot] = vz;
-            } else {
-                pa.z[slot] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+        // ---- deposit through the block-local tile ---------------------------------
+        // secondaries deposit in the append kernel, once their slot is known
+        // (a secondary dropped at capacity must not carry current)
+        if (kDep) {
+            const TileBox bx = tile_box(ilo, ihi, klo, khi, s_red);
+            const bool use_tile = bx.span > 0 && bx.span <= kTile;
+            if (use_tile) tile_zero(bx, s_tr, s_tz);
+            if (bx.span > 0) {
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (alive[p]) {
+                        DepCells dc;
+                        deposit_cells(g, x[p], y[p], z[p], vx[p], vy[p], vz[p], pa.kq, dc);
+                        tile_add(bx, dc, g.nr, s_tr, s_tz, pa.qr, pa.qz);
+                    }
+                }
             }
+            if (use_tile) tile_flush(bx, g.nr, s_tr, s_tz, pa.qr, pa.qz);
         }
+        // ---- staging in slot order -------------------------------------------------
         if (F & kStageSec) {
-            int tot;
-            int rank = block_rank(has_sec, s_wave, tot);
-            if (has_sec) {
-                const int dst = chunk * kChunk + rank;
-                pa.sec[0][dst] = sx; pa.sec[1][dst] = sy; pa.sec[2][dst] = sz;
-                pa.sec[3][dst] = svx; pa.sec[4][dst] = svy; pa.sec[5][dst] = svz;
+            int rank[kPer];
+            const int tot = block_rank_slots(has_sec, rank, s_red);
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) {
+                if (has_sec[p]) {
+                    const int dst = chunk * kChunk + rank[p];
+                    pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
+                    pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
+                }
             }
             if (threadIdx.x == 0) pa.sec_cnt[chunk] = tot;
         }
         if (F & kStageIon) {
-            int tot;
-            int rank = block_rank(co.ion, s_wave, tot);
-            if (co.ion) {
-                const int dst = chunk * kChunk + rank;
-                // ion at the ionization site (electron_collision.m:63)
-                pa.ion[0][dst] = cx; pa.ion[1][dst] = cy; pa.ion[2][dst] = cz;
-                pa.ion[3][dst] = co.ivx; pa.ion[4][dst] = co.ivy; pa.ion[5][dst] = co.ivz;
+            int rank[kPer];
+            const int tot = block_rank_slots(has_ion, rank, s_red);
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) {
+                if (has_ion[p]) {
+                    const int dst = chunk * kChunk + rank[p];
+                    // ion at the ionization site (electron_collision.m:63)
+                    pa.ion[0][dst] = cx[p]; pa.ion[1][dst] = cy[p]; pa.ion[2][dst] = cz[p];
+                    pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
+                }
             }
             if (threadIdx.x == 0) pa.ion_cnt[chunk] = tot;
         }
-        if (pa.counters) {
-            if (F & kCollide) {
-                wave_count(co.ev == 1, &pa.counters[kCElastic]);
-                wave_count(co.ev == 2, &pa.counters[kCIonize]);
-                wave_count(co.ev == 3, &pa.counters[kCExcite]);
-                wave_count(co.viol, &pa.counters[kCViol]);
-                wave_count(co.sat, &pa.counters[kCSat]);
+#pragma unroll
+        for (int p = 0; p < kPer; ++p) {
+            if (kCol) {
+                const int typ = ev[p] & kEvMask;
+                c_el += typ == 1;
+                c_ion += typ == 2;
+                c_exc += typ == 3;
+                c_viol += (ev[p] & kEvViol) != 0;
+                c_sat += (ev[p] & kEvSat) != 0;
             }
-            wave_count(absorbed || sec_absorbed, &pa.counters[kCAbsorbed]);
+            c_abs += (unsigned)absorbed[p];
+            if (F & kPush) c_push += was_alive[p];
+        }
+    }
+    // one non-atomic update of this block's counter slot (launches on one
+    // stream never overlap, and a slot belongs to one block index)
+    if (pa.counters) {
+        const unsigned v[kNCounters] = {c_el, c_ion, c_exc, 0u, 0u, c_viol, c_abs, c_sat, c_push};
+        __shared__ unsigned s_cnt[kNCounters][kBlock / 64];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int k = 0; k < kNCounters; ++k) {
+            unsigned t = v[k];
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+            if (lane == 0) s_cnt[k][wave] = t;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < kNCounters) {
+            unsigned long long s = 0;
+            for (int w = 0; w < kBlock / 64; ++w) s += s_cnt[threadIdx.x][w];
+            if (s) pa.counters[(size_t)blockIdx.x * kNCounters + threadIdx.x] += s;
         }
     }
 }
@@ -389,7 +726,7 @@ __global__ __launch_bounds__(1024) void k_append_scan(const int32_t* src_count, 
     __syncthreads();
     // Hillis-Steele inclusive scan over the 1024 partials
     for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        int v = (int)threadIdx.x >= o ? s_part[threadIdx.x - o] : 0;
+        const int v = (int)threadIdx.x >= o ? s_part[threadIdx.x - o] : 0;
         __syncthreads();
         s_part[threadIdx.x] += v;
         __syncthreads();
@@ -410,24 +747,33 @@ __global__ __launch_bounds__(1024) void k_append_scan(const int32_t* src_count, 
     }
 }
 
-__global__ __launch_bounds__(kChunk) void k_append_copy(float* d0, float* d1, float* d2, float* d3,
+__global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, float* d2, float* d3,
                                                         float* d4, float* d5, int32_t cap,
                                                         const float* s0, const float* s1,
                                                         const float* s2, const float* s3,
                                                         const float* s4, const float* s5,
                                                         const int32_t* stage_cnt,
-                                                        const int32_t* offs, const int32_t* tmp2)
+                                                        const int32_t* offs, const int32_t* tmp2,
+                                                        Geom g, float kq, unsigned long long* qr,
+                                                        unsigned long long* qz)
 {
     const int base = tmp2[0];
     const int nch = tmp2[1];
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int cnt = stage_cnt[c];
-        if ((int)threadIdx.x < cnt) {
-            const int dst = base + offs[c] + (int)threadIdx.x;
-            const int src = c * kChunk + (int)threadIdx.x;
+        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const int dst = base + offs[c] + j;
+            const int src = c * kChunk + j;
             if (dst < cap) {
-                d0[dst] = s0[src]; d1[dst] = s1[src]; d2[dst] = s2[src];
-                d3[dst] = s3[src]; d4[dst] = s4[src]; d5[dst] = s5[src];
+                const float x = s0[src], y = s1[src], z = s2[src];
+                const float vx = s3[src], vy = s4[src], vz = s5[src];
+                d0[dst] = x; d1[dst] = y; d2[dst] = z;
+                d3[dst] = vx; d4[dst] = vy; d5[dst] = vz;
+                if (qr) {                                  // EM7 of the new slot
+                    DepCells dc;
+                    deposit_cells(g, x, y, z, vx, vy, vz, kq, dc);
+                    deposit_global(dc, g.nr, qr, qz);
+                }
             }
         }
     }
@@ -436,28 +782,30 @@ __global__ __launch_bounds__(kChunk) void k_append_copy(float* d0, float* d1, fl
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
-                         int32_t max_chunks, unsigned long long* dropped, hipStream_t s)
+                         int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
+                         unsigned long long* qr, unsigned long long* qz, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append_scan, dim3(1), dim3(1024), 0, s, src_count, dst_count, cap,
                        stage_cnt, scan_tmp, tmp2, max_chunks, dropped);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    int blocks = max_chunks < 4096 ? (max_chunks > 0 ? max_chunks : 1) : 4096;
-    hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kChunk), 0, s, dst[0], dst[1], dst[2],
+    const int blocks = max_chunks < 4096 ? (max_chunks > 0 ? max_chunks : 1) : 4096;
+    hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, scan_tmp, tmp2);
+                       stage[4], stage[5], stage_cnt, scan_tmp, tmp2, g, kq, qr, qz);
     return hipGetLastError();
 }
 
 // ---- cross sections (find_CS0.m) --------------------------------------------
-__global__ void k_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
+__global__ void k_cross_sections(const DevTables* dt, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx)
 {
+    const SmallTabs* t = &dt->s;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float T = e_kev[i] * 1000.0f;
+        const float T = e_kev[i] * 1000.0f;
         const float nanv = __int_as_float(0x7fc00000);
-        float a, b, c;
+        float a, b;
         // elastic: find_CS0.m:16-25 (NaN outside 1 eV .. 20 keV)
         if (e_kev[i] >= 0.05f)
             a = (T >= t->el_e[0] && T <= t->el_e[t->n_el - 1]) ? interp_in(t->el_e, t->el_s, t->n_el, T) : nanv;
@@ -466,14 +814,12 @@ __global__ void k_cross_sections(const DevTables* t, const float* e_kev, int64_t
         b = (T >= t->ion_e[0] && T <= t->ion_e[t->n_ion - 1]) ? interp_in(t->ion_e, t->ion_s, t->n_ion, T) : nanv;
         float s = 0.0f;
         for (int j = 0; j < t->n_exc; ++j) {
-            float W = t->exc_w[j];
-            float wr = W / T;
-            float bb = powf(wr, t->exc_om[j]);
-            float cc = powf(1.0f - powf(wr, t->exc_ga[j]), t->exc_nu[j]);
+            const float wr = t->exc_w[j] / T;
+            const float bb = powf(wr, t->exc_om[j]);
+            const float cc = powf(1.0f - powf(wr, t->exc_ga[j]), t->exc_nu[j]);
             s += t->exc_q[j] * bb * cc;
         }
-        c = s;
-        se[i] = a; si[i] = b; sx[i] = c;
+        se[i] = a; si[i] = b; sx[i] = s;
     }
 }
 
@@ -493,37 +839,31 @@ template <uint32_t F>
 static hipError_t launch_t(const Geom& g, const PartArgs& pa, const CollArgs& ca, int blocks,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_particles<F>, dim3(blocks), dim3(kChunk), 0, s, g, pa, ca);
+    hipLaunchKernelGGL(k_particles<F>, dim3(blocks), dim3(kBlock), 0, s, g, pa, ca);
     return hipGetLastError();
 }
 
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s)
 {
-    int blocks = max_blocks > 0 ? max_blocks : 1;
+    const int blocks = max_blocks > 0 ? max_blocks : 1;
+    constexpr uint32_t P = kPush, Co = kCollide, D = kDrift, De = kDeposit, S = kStageSec, I = kStageIon;
     switch (flags) {
-    case kDrift: return launch_t<kDrift>(g, pa, ca, blocks, s);
-    case kDeposit: return launch_t<kDeposit>(g, pa, ca, blocks, s);
-    case kDrift | kDeposit: return launch_t<kDrift | kDeposit>(g, pa, ca, blocks, s);
-    case kPush: return launch_t<kPush>(g, pa, ca, blocks, s);
-    case kPush | kDrift | kDeposit: return launch_t<kPush | kDrift | kDeposit>(g, pa, ca, blocks, s);
-    case kCollide: return launch_t<kCollide>(g, pa, ca, blocks, s);
-    case kCollide | kStageSec: return launch_t<kCollide | kStageSec>(g, pa, ca, blocks, s);
-    case kCollide | kStageSec | kStageIon:
-        return launch_t<kCollide | kStageSec | kStageIon>(g, pa, ca, blocks, s);
-    case kPush | kCollide: return launch_t<kPush | kCollide>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kStageSec: return launch_t<kPush | kCollide | kStageSec>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kStageSec | kStageIon:
-        return launch_t<kPush | kCollide | kStageSec | kStageIon>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kStageIon: return launch_t<kPush | kCollide | kStageIon>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kDrift | kDeposit | kStageIon:
-        return launch_t<kPush | kCollide | kDrift | kDeposit | kStageIon>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kDrift | kDeposit:
-        return launch_t<kPush | kCollide | kDrift | kDeposit>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kDrift | kDeposit | kStageSec:
-        return launch_t<kPush | kCollide | kDrift | kDeposit | kStageSec>(g, pa, ca, blocks, s);
-    case kPush | kCollide | kDrift | kDeposit | kStageSec | kStageIon:
-        return launch_t<kPush | kCollide | kDrift | kDeposit | kStageSec | kStageIon>(g, pa, ca, blocks, s);
+    case D: return launch_t<D>(g, pa, ca, blocks, s);
+    case De: return launch_t<De>(g, pa, ca, blocks, s);
+    case D | De: return launch_t<D | De>(g, pa, ca, blocks, s);
+    case P: return launch_t<P>(g, pa, ca, blocks, s);
+    case P | D | De: return launch_t<P | D | De>(g, pa, ca, blocks, s);
+    case Co | S: return launch_t<Co | S>(g, pa, ca, blocks, s);
+    case Co | S | I: return launch_t<Co | S | I>(g, pa, ca, blocks, s);
+    case P | Co: return launch_t<P | Co>(g, pa, ca, blocks, s);
+    case P | Co | S: return launch_t<P | Co | S>(g, pa, ca, blocks, s);
+    case P | Co | I: return launch_t<P | Co | I>(g, pa, ca, blocks, s);
+    case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s);
+    case P | Co | D | De: return launch_t<P | Co | D | De>(g, pa, ca, blocks, s);
+    case P | Co | D | De | S: return launch_t<P | Co | D | De | S>(g, pa, ca, blocks, s);
+    case P | Co | D | De | I: return launch_t<P | Co | D | De | I>(g, pa, ca, blocks, s);
+    case P | Co | D | De | S | I: return launch_t<P | Co | D | De | S | I>(g, pa, ca, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -12,6 +12,7 @@ namespace gasx {
 __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
                             uint32_t* keys, uint32_t* vals, int32_t* n_live)
 {
+#pragma clang fp contract(off)
     int live = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float zz = z[i];
@@ -29,11 +30,16 @@ __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float*
         keys[i] = key;
         vals[i] = (uint32_t)i;
     }
-    unsigned long long m = __ballot(live != 0);
-    (void)m;
-    // reduce live counts: wave sum then one atomic per wave
+    // live count: wave sums, block sum, one atomic per block
     for (int o = 32; o > 0; o >>= 1) live += __shfl_down(live, o, 64);
-    if ((threadIdx.x & 63) == 0 && live) atomicAdd(n_live, live);
+    __shared__ int s_live[16];
+    if ((threadIdx.x & 63) == 0) s_live[threadIdx.x >> 6] = live;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_live[w];
+        if (t) atomicAdd(n_live, t);
+    }
 }
 
 __global__ void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,

@@ -78,6 +78,15 @@ class Counters(C.Structure):
         ("n_elastic", C.c_int64), ("n_ionize", C.c_int64), ("n_excite", C.c_int64),
         ("n_dropped_sec", C.c_int64), ("n_dropped_ion", C.c_int64),
         ("n_bound_violations", C.c_int64), ("n_absorbed", C.c_int64), ("n_saturated", C.c_int64),
+        ("n_pushed", C.c_int64),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("particle_ms", C.c_double), ("fdtd_ms", C.c_double), ("append_ms", C.c_double),
+        ("sort_ms", C.c_double), ("n_particle_launches", C.c_int64), ("n_fdtd_launches", C.c_int64),
+        ("n_append_launches", C.c_int64), ("n_sorts", C.c_int64),
     ]
 
 
@@ -85,7 +94,7 @@ class RunOpts(C.Structure):
     _fields_ = [
         ("n_steps", C.c_int32), ("collide", C.c_int32), ("secondaries", C.c_int32),
         ("sort_every", C.c_int32), ("seed", C.c_uint64), ("run_id", C.c_uint32),
-        ("step0", C.c_uint32), ("use_graph", C.c_int32),
+        ("step0", C.c_uint32), ("use_graph", C.c_int32), ("time_kernels", C.c_int32),
     ]
 
 
@@ -117,6 +126,7 @@ PROTOTYPES = {
     "gasx_get_counters": (C.c_int, [_P, C.POINTER(Counters), C.c_int]),
     "gasx_get_count": (C.c_int, [_P, C.POINTER(Species), C.POINTER(C.c_int64)]),
     "gasx_collision_bound": (C.c_int, [_P, C.POINTER(Gas), C.c_double, C.POINTER(C.c_double)]),
+    "gasx_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
 }
 
 _lib = None

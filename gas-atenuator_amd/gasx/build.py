@@ -2,8 +2,9 @@
 
 ``build_library()`` always recompiles: each ``csrc/*.hip`` is compiled to an
 object with hipcc for gfx950 and linked into ``gasx/libgasx.so``.  All HIP
-code is built with packed FP32 instructions disabled and with FP contraction
-off, so that the fixed-point deposit weights are reproducible on the CPU.
+code is built with packed FP32 instructions disabled; FP contraction is on
+except where the source turns it off (the fixed-point deposit weights, which
+must be reproducible on the CPU, and the sort keys).
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ ARCH = os.environ.get("GASX_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-fPIC",
-    "-ffp-contract=off",
+    "-ffp-contract=fast-honor-pragmas",   # the deposit arithmetic turns contraction off locally
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-Wno-unused-value",   # hipFree/hipStreamDestroy results on cleanup paths are ignored on purpose

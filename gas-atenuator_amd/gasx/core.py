@@ -153,6 +153,7 @@ class CounterValues:
     n_bound_violations: int
     n_absorbed: int
     n_saturated: int
+    n_pushed: int
 
 
 class Context:
@@ -249,14 +250,21 @@ class Context:
     # ---- production loop ---------------------------------------------------------
     def run(self, e: ParticleSet, f: FieldState, gas: GasGrid | None, n_steps: int,
             collide: bool = True, secondaries: bool = False, sort_every: int = 0, seed: int = 0,
-            run_id: int = 0, step0: int = 0, ions: ParticleSet | None = None, use_graph: bool = False):
+            run_id: int = 0, step0: int = 0, ions: ParticleSet | None = None, use_graph: bool = False,
+            time_kernels: bool = False):
         s = e.struct()
         fs = f.struct()
         g = gas.struct() if gas is not None else None
         si = ions.struct() if ions is not None else None
-        o = RunOpts(n_steps, int(collide), int(secondaries), sort_every, seed, run_id, step0, int(use_graph))
+        o = RunOpts(n_steps, int(collide), int(secondaries), sort_every, seed, run_id, step0,
+                    int(use_graph), int(time_kernels))
         self._check(self.lib.gasx_run(self.ctx, C.byref(s), C.byref(si) if si is not None else None,
                                       C.byref(fs), C.byref(g) if g is not None else None, C.byref(o)))
+
+    def timing(self) -> dict:
+        t = _abi.Timing()
+        self._check(self.lib.gasx_get_timing(self.ctx, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in _abi.Timing._fields_}
 
     def sort(self, ps: ParticleSet):
         s = ps.struct()

@@ -111,8 +111,19 @@ typedef struct gasx_counters {
     int64_t n_dropped_ion;
     int64_t n_bound_violations;/* candidate with p_T above the rejection bound (must stay 0) */
     int64_t n_absorbed;        /* particles absorbed at r > Rp */
-    int64_t n_saturated;       /* collisions with p_T > 1 (BASELINE D15) */
+    int64_t n_saturated;       /* collisions with p_T > 1 (linear p_T of electron_transport.cu:74-85 saturates) */
+    int64_t n_pushed;          /* particle pushes performed (particle-steps) */
 } gasx_counters;
+
+/* Kernel timing of the last gasx_run with time_kernels = 1 (HIP events on
+ * the context's stream around every launch). */
+typedef struct gasx_timing {
+    double particle_ms;        /* sum over fused particle-kernel launches */
+    double fdtd_ms;            /* sum over field-kernel launches */
+    double append_ms;          /* sum over secondary/ion append launches */
+    double sort_ms;            /* sum over cell sorts */
+    int64_t n_particle_launches, n_fdtd_launches, n_append_launches, n_sorts;
+} gasx_timing;
 
 /* Options of the fused production loop. */
 typedef struct gasx_run_opts {
@@ -124,6 +135,7 @@ typedef struct gasx_run_opts {
     uint32_t run_id;            /* Philox counter word 2 */
     uint32_t step0;             /* Philox counter word 1 of the first step */
     int32_t use_graph;          /* 1: capture the step sequence in a hipGraph and replay */
+    int32_t time_kernels;       /* 1: record HIP events around launches (gasx_get_timing) */
 } gasx_run_opts;
 
 /* ---- context ------------------------------------------------------------ */
@@ -198,6 +210,8 @@ gasx_status gasx_get_counters(gasx_ctx* ctx, gasx_counters* out, int reset);
 gasx_status gasx_get_count(gasx_ctx* ctx, const gasx_species* s, int64_t* n);
 /* Collision rejection bound in use for gas g with step dt_coll (for tests). */
 gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, double* pmax);
+/* Kernel times of the last timed gasx_run (synchronizes). */
+gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out);
 
 #ifdef __cplusplus
 }

@@ -156,6 +156,60 @@ double orc_elastic_theta(const orc_tables* t, double e_kev, double u)
     return (k + s) * dth;
 }
 
+/* The distribution of orc_elastic_theta, sampled by rejection instead of by
+ * inversion (the HIP kernel's method).  The inverse-CDF picks interval k with
+ * probability m_k(w) = p_k(w) + p_{k+1}(w), p_k(w) = sind(k) exp((1-w)V_i[k] +
+ * w V_{i+1}[k]), and theta uniformly inside it.  By convexity of exp,
+ * m_k(w) <= (1-w) m_k(i) + w m_k(i+1) =: M_k, so: pick node i or i+1 with
+ * weights (1-w) Z_i and w Z_{i+1} (Z = sum of m), draw k from that node's
+ * interval distribution, accept with probability m_k(w)/M_k, and place
+ * theta uniformly in the interval.  Uniforms come from Philox blocks 2, 3, ...
+ * of the particle's counter. */
+static void node_masses(const orc_tables* t, int j, double* m, double* Z)
+{
+    double prev = 0.0, z = 0.0;
+    for (int k = 0; k < ORC_NANG; ++k) {
+        double p = sind_int(k) * exp(t->dcs_logv[j * ORC_NANG + k]);
+        if (k > 0) { m[k - 1] = prev + p; z += prev + p; }
+        prev = p;
+    }
+    *Z = z;
+}
+
+double orc_elastic_theta_rej(const orc_tables* t, double e_kev, uint32_t slot, uint32_t step,
+                             uint32_t run, uint64_t seed)
+{
+    double x = log(e_kev);
+    int n = t->n_dcs, i = 0;
+    while (i < n - 2 && x > t->dcs_loge[i + 1]) ++i;
+    double w = (x - t->dcs_loge[i]) / (t->dcs_loge[i + 1] - t->dcs_loge[i]);
+    double m0[ORC_NANG], m1[ORC_NANG], Z0, Z1;
+    node_masses(t, i, m0, &Z0);
+    node_masses(t, i + 1, m1, &Z1);
+    const double* v0 = &t->dcs_logv[i * ORC_NANG];
+    const double* v1 = &t->dcs_logv[(i + 1) * ORC_NANG];
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    for (uint32_t it = 0; it < 64; ++it) {
+        uint32_t ctr[4] = { slot, step, run, 2u + it }, o[4];
+        orc_philox4x32_10(ctr, key, o);
+        double a = orc_u01(o[0]), b = orc_u01(o[1]), c = orc_u01(o[2]), d = orc_u01(o[3]);
+        double wa = (1.0 - w) * Z0, wb = w * Z1;
+        const double* mm = (a * (wa + wb) < wa) ? m0 : m1;
+        double Z = (mm == m0) ? Z0 : Z1;
+        double target = b * Z, acc = 0.0;
+        int k = 0;
+        while (k < ORC_NANG - 2 && acc + mm[k] <= target) { acc += mm[k]; ++k; }
+        double M = (1.0 - w) * m0[k] + w * m1[k];
+        double pk = sind_int(k) * exp((1.0 - w) * v0[k] + w * v1[k]);
+        double pk1 = sind_int(k + 1) * exp((1.0 - w) * v0[k + 1] + w * v1[k + 1]);
+        if (c * M < pk + pk1) return (k + d) * (M_PI / 180.0);
+    }
+    /* not reached in practice (acceptance >= ~0.9); keep the exact sampler */
+    uint32_t ctr[4] = { slot, step, run, 66u }, o[4];
+    orc_philox4x32_10(ctr, key, o);
+    return orc_elastic_theta(t, e_kev, orc_u01(o[0]));
+}
+
 /* Green-Sawada secondary energy and binary-encounter angles,
  * get_ImpIonized.m:13-63 (== electron_transport.cu:141-171). */
 void orc_ionize(double T, double rn, double e_e,
@@ -580,7 +634,7 @@ static void gas_lookup(const orc_gas* g, double r, double z, double* n, double* 
 void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
                  double* x, double* y, double* z, double* vx, double* vy, double* vz,
                  const int32_t* del, int64_t n, double dt_coll,
-                 uint64_t seed, uint32_t step, uint32_t run,
+                 uint64_t seed, uint32_t step, uint32_t run, int elastic_mode,
                  int8_t* ev, double* sec_v, double* ion_v)
 {
     const double I = 15.6;
@@ -606,7 +660,8 @@ void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
         double pt = ve * dt_coll * dens * st;         /* electron_transport.cu:74-77 */
         if (!(u1 < pt)) continue;                      /* :85 */
         if (u2 * st < se) {                            /* :86 elastic */
-            double th = orc_elastic_theta(t, T / 1000.0, u3);
+            double th = elastic_mode == 0 ? orc_elastic_theta(t, T / 1000.0, u3)
+                                          : orc_elastic_theta_rej(t, T / 1000.0, (uint32_t)m, step, run, seed);
             double ph = 2 * M_PI * u4;
             orc_rotate(&vx[m], &vy[m], &vz[m], th, ph, p->rot_eps);
             ev[m] = 1;
@@ -700,7 +755,7 @@ int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
             double* x, double* y, double* z, double* vx, double* vy, double* vz, int32_t* del,
             int64_t* n_io, int64_t cap, double* er, double* ez, double* h,
             int n_steps, int collide, int secondaries, uint64_t seed, uint32_t run,
-            uint32_t step0, orc_counts* counts)
+            uint32_t step0, int elastic_mode, orc_counts* counts)
 {
     int64_t n = *n_io;
     size_t ncell = (size_t)p->nr * p->nz;
@@ -717,7 +772,7 @@ int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
         orc_em_step(p, x, y, z, vx, vy, vz, q, m, NULL, del, n, er, ez, h, jr, jz, 1);
         if (!collide) continue;
         orc_collide(p, t, g, x, y, z, vx, vy, vz, del, n, p->ddt, seed, step0 + (uint32_t)s, run,
-                    ev, sv, NULL);
+                    elastic_mode, ev, sv, NULL);
         int64_t n0 = n;
         for (int64_t i = 0; i < n0; ++i) {
             if (ev[i] == 1) counts->n_elastic++;

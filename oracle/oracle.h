@@ -68,6 +68,8 @@ double orc_interp1(const double* x, const double* y, int n, double xq);
 void orc_cross_sections(const orc_tables* t, const double* e_kev, int64_t n,
                         double* sig_el, double* sig_ion, double* sig_exc);
 double orc_elastic_theta(const orc_tables* t, double e_kev, double u);
+double orc_elastic_theta_rej(const orc_tables* t, double e_kev, uint32_t slot, uint32_t step,
+                             uint32_t run, uint64_t seed);
 void orc_ionize(double t_ev, double rn, double e_e,
                 double* k0, double* ep, double* es, double* th_p, double* th_s);
 double orc_exc_edep(const orc_tables* t, double t_ev, int form);
@@ -118,6 +120,7 @@ void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
                  double* x, double* y, double* z, double* vx, double* vy, double* vz,
                  const int32_t* del, int64_t n, double dt_coll,
                  uint64_t seed, uint32_t step, uint32_t run,
+                 int elastic_mode /* 0: inverse CDF (reference), 1: rejection (HIP) */,
                  int8_t* ev, double* sec_v /* 3*n: secondary velocity */,
                  double* ion_v /* 3*n or NULL */);
 
@@ -135,7 +138,7 @@ int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
             double* x, double* y, double* z, double* vx, double* vy, double* vz, int32_t* del,
             int64_t* n, int64_t cap, double* er, double* ez, double* h,
             int n_steps, int collide, int secondaries, uint64_t seed, uint32_t run,
-            uint32_t step0, orc_counts* counts);
+            uint32_t step0, int elastic_mode, orc_counts* counts);
 
 #ifdef __cplusplus
 }

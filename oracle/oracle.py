@@ -76,11 +76,14 @@ def lib() -> C.CDLL:
         L.orc_rotate.argtypes = [_D, _D, _D, C.c_double, C.c_double, C.c_double]
         L.orc_collide.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcTables), C.POINTER(OrcGas),
                                   _P, _P, _P, _P, _P, _P, _P, C.c_int64, C.c_double,
-                                  C.c_uint64, C.c_uint32, C.c_uint32, _P, _P, _P]
+                                  C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P]
         L.orc_run.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcTables), C.POINTER(OrcGas),
                               _P, _P, _P, _P, _P, _P, _P, C.POINTER(C.c_int64), C.c_int64,
                               _P, _P, _P, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_uint32,
-                              C.c_uint32, C.POINTER(OrcCounts)]
+                              C.c_uint32, C.c_int, C.POINTER(OrcCounts)]
+        L.orc_elastic_theta_rej.restype = C.c_double
+        L.orc_elastic_theta_rej.argtypes = [C.POINTER(OrcTables), C.c_double, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_uint64]
         L.orc_fdtd_push.argtypes = [C.POINTER(OrcParams), _P, _P, _P, _P, _P, C.c_int]
         L.orc_fdtd_h.argtypes = [_P, _P, _P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_fdtd_e.argtypes = [_P, _P, _P, _P, _P, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -225,24 +228,28 @@ def make_gas(dens, gdr, gdz, t_scalar=300.0):
     return g
 
 
-def collide(p, t, g, st: State, dt_coll, seed, step, run=0, ions=False):
+ELASTIC_INVERSE_CDF, ELASTIC_REJECTION = 0, 1
+
+
+def collide(p, t, g, st: State, dt_coll, seed, step, run=0, ions=False, elastic_mode=ELASTIC_REJECTION):
     ev = np.zeros(st.v.shape[1], dtype=np.int8)
     sv = np.zeros(3 * st.v.shape[1])
     iv = np.zeros(3 * st.v.shape[1]) if ions else None
     lib().orc_collide(C.byref(p), C.byref(t), C.byref(g), *[ptr(x) for x in st.rows()],
-                      ptr(st.dele), st.n, dt_coll, seed, step, run, ptr(ev), ptr(sv),
+                      ptr(st.dele), st.n, dt_coll, seed, step, run, elastic_mode, ptr(ev), ptr(sv),
                       ptr(iv) if ions else None)
     return ev[: st.n], sv.reshape(-1, 3)[: st.n], (iv.reshape(-1, 3)[: st.n] if ions else None)
 
 
 def run(p, t, g, st: State, er, ez, h, n_steps, collide=True, secondaries=False, seed=0, run_id=0,
-        step0=0):
+        step0=0, elastic_mode=ELASTIC_REJECTION):
     er, ez, h = a(er).copy(), a(ez).copy(), a(h).copy()
     n = C.c_int64(st.n)
     cnt = OrcCounts()
     rc = lib().orc_run(C.byref(p), C.byref(t), C.byref(g), *[ptr(x) for x in st.rows()],
                        ptr(st.dele), C.byref(n), st.v.shape[1], ptr(er), ptr(ez), ptr(h), n_steps,
-                       int(collide), int(secondaries), seed, run_id, step0, C.byref(cnt))
+                       int(collide), int(secondaries), seed, run_id, step0, elastic_mode,
+                       C.byref(cnt))
     if rc != 0:
         raise RuntimeError("orc_run failed")
     st.n = n.value

@@ -35,6 +35,33 @@ def gpu():
     return torch.device("cuda:0")
 
 
+def deposit_error_bound(op, st, vel):
+    """Per-cell bound on |J(FP32 fixed point) - J(FP64 EM7)|.
+
+    FP32 positions resolve r/dr and z/dz to an absolute max(nr, nz) * 2^-24,
+    so every area weight of a particle is off by at most a few times that
+    (absolute, not relative to the weight); v_r and the products add a few
+    2^-24 of |v|.  The bound sums FN*e*|v|/V(i) over the particles whose 3x3
+    cell neighbourhood contains the cell."""
+    import numpy as np
+    nr, nz = op.nr, op.nz
+    ok = ~np.isnan(st[2])
+    r = np.sqrt(st[0][ok].astype(np.float64) ** 2 + st[1][ok].astype(np.float64) ** 2)
+    ir = np.clip(np.floor(r / op.dr - 0.5), -1, nr - 1).astype(int) + 1
+    iz = np.clip(np.floor(st[2][ok] / op.dz - 0.5), -1, nz - 1).astype(int) + 1
+    h = np.zeros((nz + 2, nr + 2))
+    np.add.at(h, (iz, ir), np.abs(vel[ok]))
+    box = np.zeros_like(h)
+    for dk in (-1, 0, 1):
+        for di in (-1, 0, 1):
+            box += np.roll(np.roll(h, dk, 0), di, 1)
+    box = box[1:nz + 1, 1:nr + 1]
+    i = np.arange(nr)
+    V = op.phip / 2 * ((i + 1) ** 2 - i ** 2) * op.dr ** 2 * op.dz
+    tol = (16 + 4 * max(nr, nz)) * 2.0 ** -24
+    return (tol * op.fn * 1.6e-19 * box / V).ravel()
+
+
 def log(msg: str):
     """Name the case before GPU work (flushed, so a fault still shows it)."""
     print(f"[case] {msg}", flush=True)

@@ -128,6 +128,39 @@ def test_elastic_angle_between_nodes(otab, raw):
     assert min(lo, hi) <= mid <= max(lo, hi)
 
 
+@pytest.mark.parametrize("e_kev", [0.4, 0.35, 0.02, 3.3])
+def test_elastic_rejection_sampler_matches_inverse_cdf(otab, e_kev):
+    """The kernel's rejection sampler draws the reference (inverse-CDF)
+    distribution: chi-square over the 1-degree intervals and uniformity
+    inside them."""
+    from scipy import stats
+    L = oracle.lib()
+    nu = 60000
+    u = (np.arange(nu) + 0.5) / nu
+    th_ref = np.array([L.orc_elastic_theta(otab, e_kev, x) for x in u])
+    p_ref = np.bincount(np.minimum((th_ref * 180 / np.pi).astype(int), 179), minlength=180) / nu
+    ns = 40000
+    th = np.array([L.orc_elastic_theta_rej(otab, e_kev, s, 5, 1, 12345) for s in range(ns)])
+    deg = th * 180 / np.pi
+    k = np.minimum(deg.astype(int), 179)
+    obs = np.bincount(k, minlength=180)
+    exp_ = p_ref * ns
+    # merge sparse backward bins so that every expected count is >= 20
+    order = np.argsort(exp_)
+    groups, cur, acc = [], [], 0.0
+    for i in order:
+        cur.append(i); acc += exp_[i]
+        if acc >= 20:
+            groups.append(cur); cur, acc = [], 0.0
+    if cur:
+        groups[-1].extend(cur)
+    o = np.array([obs[g].sum() for g in groups], float)
+    e = np.array([exp_[g].sum() for g in groups])
+    e *= o.sum() / e.sum()
+    assert stats.chisquare(o, e).pvalue > 1e-3
+    assert stats.kstest(deg - k, "uniform").pvalue > 1e-3
+
+
 # ---- ionization -----------------------------------------------------------------
 @pytest.mark.parametrize("T", [20.0, 100.0, 400.0, 5000.0])
 def test_green_sawada_inverse_cdf(T):
@@ -324,15 +357,16 @@ def test_fixed_point_deposit_tracks_fp64():
     rr = np.hypot(st.v[0], st.v[1])
     vr = (st.v[3] * st.v[0] + st.v[4] * st.v[1]) / rr
     ojr, ojz = oracle.em7(p, st, rr, vr)
-    # bound: FP32 positions resolve r/dr and z/dz to max(nr, nz) * 2^-24 cells,
-    # plus a few roundings of v_r and the products, relative to the cell's sum
-    # of |contributions|
-    sa = oracle.State(np.abs(s32))
-    sa.q[:] = 1.6e-19
-    ajr, ajz = oracle.em7(p, sa, rr, np.abs(vr))
-    tol = (8 + 2 * max(nr, nz)) * 2.0 ** -24
-    assert np.all(np.abs(jr - ojr) <= tol * ajr + 1e-300)
-    assert np.all(np.abs(jz - ojz) <= tol * ajz + 1e-300)
+    from conftest import deposit_error_bound
+    bound = deposit_error_bound(p, s32, np.sqrt(np.sum(s32[3:6].astype(np.float64) ** 2, axis=0)))
+    assert np.all(np.abs(jr - ojr) <= bound) and np.all(np.abs(jz - ojz) <= bound)
+    # and the bound is tight enough to catch a plausible mistake: a deposit
+    # that drops the z half-cell offset is far outside it
+    sh = s32.copy()
+    sh[2] += np.float32(0.5 * p.dz)
+    qb, _ = oracle.deposit_fx32(p, sh, -1.0)
+    jb, _ = oracle.j_from_fixed(p, qb, qb)
+    assert not np.all(np.abs(jb - ojr) <= bound)
     # order independence of the integer sum
     perm = rng.permutation(n)
     qr2, qz2 = oracle.deposit_fx32(p, s32[:, perm], -1.0)
