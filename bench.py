@@ -43,8 +43,23 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (5 times) and print its size: "
+                         "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
 
 
 def cpu_baseline(w, host, n_particles: int, n_steps: int, seed: int):
@@ -91,6 +106,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}); "
               "the multi-GPU sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
 
     import numpy as np
     import torch
@@ -137,13 +155,15 @@ def main():
     avg_ms = tm["particle_ms"] / n_launch
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * psteps / n_launch
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE passes
+    # (calibrated against a plain copy), stored per particle-step and scaled
+    # to this run's particle-steps per launch.
     traffic = None
-    tfile = REPO / "profiles" / f"traffic_{w.name}.json"
+    tfile = REPO / "profiles" / f"k_particles_{w.name}.traffic.txt"
     if tfile.exists():
-        try:
-            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+        for ln in tfile.read_text().splitlines():
+            if ln.startswith("hbm_bytes_per_particle_step:"):
+                traffic = float(ln.split(":", 1)[1]) * psteps / n_launch
     line = {
         "metric": METRIC,
         "value": value,
