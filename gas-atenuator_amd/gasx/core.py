@@ -32,9 +32,10 @@ K_B = 1.38064853e-23                # electron_transport.cu:51
 def make_params(nr: int, nz: int, dr: float = 1e-5, dz: float = 1e-5, phip: float = 0.1,
                 ddt: float = 1e-14, n_sub: int = 10, fn: float = 1.0, ez_metric: int = 0,
                 axis_j_sign: int = 1, exc_form: int = 0, rot_eps: float = 1e-5) -> Params:
-    """Parameters of the r-z PIC step; defaults are the reference choices
-    (SURVEY D1-D7 defaults: ez_metric=0 -> 1/(2r) at r = i*dr, axis current
-    sign +1 as EM3.cu:35, excitation form of find_CS0.m, rotate.m threshold)."""
+    """Parameters of the r-z PIC step.  Where the reference's files disagree
+    the defaults are: ez_metric=0 -> 1/(2r) at r = i*dr (electricmagneticfield3.m:187
+    rather than EM3.cu:39), axis current sign +1 as EM3.cu:35, the excitation
+    form of find_CS0.m:52, and the rotate.m:5 degeneracy threshold 1e-5."""
     return Params(nr, nz, dr, dz, phip, ddt, n_sub, fn, C_LIGHT, EPS0, MU0, E_EM, ME_EM,
                   E_MC, ME_MC, M_ION, K_B, ez_metric, axis_j_sign, exc_form, rot_eps)
 
