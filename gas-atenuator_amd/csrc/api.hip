@@ -9,50 +9,12 @@
 #include <utility>
 #include <vector>
 
-#include "gasx_internal.h"
+#include "context.h"
 
 using namespace gasx;
+using namespace gasx_host;
 
-struct gasx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    gasx_params p{};
-    Geom geom{};
-    FdtdArgs fa{};
-    HostTables ht{};
-    DevTables* d_tab = nullptr;
-    float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies
-    double *er_b = nullptr, *ez_b = nullptr, *h_b = nullptr;      // FP64 ping-pong partner
-    unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buf][r/z]
-    double *jr_tmp = nullptr, *jz_tmp = nullptr;
-    unsigned long long* d_counters = nullptr;
-    // append staging
-    int64_t stage_slots = 0;
-    float* stage[6] = {};
-    int32_t* stage_cnt = nullptr;
-    float* istage[6] = {};
-    int32_t* istage_cnt = nullptr;
-    int32_t* scan_tmp = nullptr;
-    int32_t* base_tmp = nullptr;
-    // sort scratch
-    void* sort_tmp = nullptr;
-    size_t sort_bytes = 0;
-    double* d_tally = nullptr;
-    int cu_count = 256;
-    double max_rate = 0.0;   // max over the collision window of v(E) sigma_T(E) (m^3/s)
-    std::string err;
-    // kernel timing (gasx_run_opts.time_kernels): event pairs per launch kind
-    bool timing = false;
-    std::vector<hipEvent_t> events;
-    size_t ev_used = 0;
-    std::vector<std::pair<size_t, size_t>> tpairs[4];   // particle, fdtd, append, sort
-};
-
-namespace {
-enum TimedKind { kTParticle = 0, kTFdtd, kTAppend, kTSort };
-
-size_t next_event(gasx_ctx* c)
+size_t gasx_host::next_event(gasx_ctx* c)
 {
     if (c->ev_used == c->events.size()) {
         hipEvent_t e;
@@ -62,57 +24,7 @@ size_t next_event(gasx_ctx* c)
     return c->ev_used++;
 }
 
-// Records an event pair around the enclosed launches when timing is on.
-struct TimedScope {
-    gasx_ctx* c;
-    int kind;
-    size_t a = (size_t)-1;
-    TimedScope(gasx_ctx* c_, int k) : c(c_), kind(k)
-    {
-        if (c->timing) {
-            a = next_event(c);
-            if (a != (size_t)-1) (void)hipEventRecord(c->events[a], c->stream);
-        }
-    }
-    ~TimedScope()
-    {
-        if (c->timing && a != (size_t)-1) {
-            size_t b = next_event(c);
-            if (b != (size_t)-1) {
-                (void)hipEventRecord(c->events[b], c->stream);
-                c->tpairs[kind].push_back({a, b});
-            }
-        }
-    }
-};
-}  // namespace
-
 namespace {
-
-gasx_status fail(gasx_ctx* c, gasx_status s, const std::string& msg)
-{
-    if (c) c->err = msg;
-    return s;
-}
-
-gasx_status hip_fail(gasx_ctx* c, hipError_t e, const char* where)
-{
-    std::string m = std::string(where) + ": " + hipGetErrorString(e);
-    if (c) c->err = m;
-    return e == hipErrorOutOfMemory ? GASX_E_NOMEM : GASX_E_HIP;
-}
-
-#define HIPCHK(expr, where)                                   \
-    do {                                                      \
-        hipError_t e_ = (expr);                               \
-        if (e_ != hipSuccess) return hip_fail(ctx, e_, where); \
-    } while (0)
-
-template <typename T>
-hipError_t dalloc(T** p, size_t n)
-{
-    return hipMalloc((void**)p, (n > 0 ? n : 1) * sizeof(T));
-}
 
 double interp1_nan(const double* x, const double* y, int n, double xq)
 {
@@ -375,6 +287,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         if (e != hipSuccess && st == GASX_OK) st = hip_fail(ctx, e, w);
     };
     chk(hipSetDevice(device), "hipSetDevice");
+    if (st == GASX_OK) chk(fdtd_prepare(), "field kernel LDS limit");
     if (st == GASX_OK) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
@@ -431,6 +344,7 @@ extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
         if (ctx->istage[c]) hipFree(ctx->istage[c]);
     }
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    batch_release(ctx);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return GASX_OK;
@@ -499,8 +413,8 @@ static PartArgs make_part(gasx_ctx* ctx, const gasx_species* s)
     return pa;
 }
 
-static gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, uint64_t seed,
-                             uint32_t step, uint32_t run, CollArgs* ca)
+gasx_status gasx_host::make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, uint64_t seed,
+                                 uint32_t step, uint32_t run, CollArgs* ca)
 {
     *ca = CollArgs{};
     if (!g) return GASX_OK;
@@ -527,11 +441,15 @@ static gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, u
     return GASX_OK;
 }
 
+int gasx_host::particle_grid(const gasx_ctx* ctx, int64_t chunks, int per_cu)
+{
+    const int64_t persistent = std::min<int64_t>((int64_t)ctx->cu_count * per_cu, kCounterSlots);
+    return (int)std::max<int64_t>(1, std::min(chunks, persistent));
+}
+
 static int particle_blocks(gasx_ctx* ctx, int64_t cap)
 {
-    const int64_t chunks = (cap + kChunk - 1) / kChunk;
-    const int64_t persistent = std::min<int64_t>((int64_t)ctx->cu_count * 16, kCounterSlots);
-    return (int)std::max<int64_t>(1, std::min(chunks, persistent));
+    return particle_grid(ctx, (cap + kChunk - 1) / kChunk, 16);
 }
 
 // one field cycle: reads (in) writes (out) and refreshes the FP32 copies
@@ -853,8 +771,8 @@ extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)
     if (!ctx || !out) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-    double ms[4] = {0, 0, 0, 0};
-    for (int k = 0; k < 4; ++k)
+    double ms[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < 5; ++k)
         for (auto& pr : ctx->tpairs[k]) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ctx->events[pr.first], ctx->events[pr.second]), "event time");
@@ -868,6 +786,8 @@ extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)
     out->n_fdtd_launches = (int64_t)ctx->tpairs[kTFdtd].size();
     out->n_append_launches = (int64_t)ctx->tpairs[kTAppend].size();
     out->n_sorts = (int64_t)ctx->tpairs[kTSort].size();
+    out->graph_ms = ms[kTGraph];
+    out->n_graph_launches = (int64_t)ctx->tpairs[kTGraph].size();
     return GASX_OK;
 }
 

@@ -190,16 +190,23 @@ static hipError_t launch_fdtd_t(const FdtdArgs& a, const FdtdRun& r0, const Fdtd
                                 int nruns, int fixed_j, hipStream_t s)
 {
     const size_t lds = fdtd_lds_bytes(a);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fdtd<BS, MAXP>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
     const int blocks = nruns * a.ntr * a.ntk;
     hipLaunchKernelGGL((k_fdtd<BS, MAXP>), dim3(blocks), dim3(BS), lds, s, a, r0, runs_dev, fixed_j);
     return hipGetLastError();
+}
+
+// Raises the dynamic LDS limit of every field-kernel variant to the CU's
+// 160 KB (once per device, at context creation: not a stream operation, so it
+// must not wait for the first launch, which may be under graph capture).
+hipError_t fdtd_prepare()
+{
+    const void* fns[] = {(const void*)k_fdtd<256, 4>, (const void*)k_fdtd<512, 4>,
+                         (const void*)k_fdtd<512, 8>, (const void*)k_fdtd<1024, 8>};
+    for (const void* f : fns) {
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,

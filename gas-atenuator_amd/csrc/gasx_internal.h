@@ -81,6 +81,18 @@ enum : uint32_t {
     kDeposit = 8u,     // EM7.cu
     kStageSec = 16u,   // stage ionization secondaries for append
     kStageIon = 32u,   // stage N2+ ions for append
+    kBatch = 64u,      // chunks belong to the runs of a batched ensemble
+};
+
+constexpr int kMaxRuns = 1024;   // runs per batched ensemble
+
+// One run of a batched ensemble (device array).
+struct BatchRun {
+    int64_t base;      // first pooled slot of the run
+    int32_t cap;       // slots owned
+    float pmax;        // collision rejection bound of the run's gas
+    uint32_t run;      // Philox counter word 2
+    int32_t pad_;
 };
 
 struct CollArgs {
@@ -113,6 +125,12 @@ struct PartArgs {
     float* ion[6];
     int32_t* ion_cnt;
     unsigned long long* counters;  // gasx_counters layout
+    // batched ensemble (kBatch): run r's particles start at runs[r].base,
+    // its count is count[r], its fields, current and gas follow run-major
+    const BatchRun* runs;
+    int32_t nruns;
+    int32_t* chunk_start;          // nruns+1 chunk prefix of the launch (written by block 0)
+    unsigned long long* run_counters;  // nruns x kNCounters
 };
 
 enum CounterIdx {
@@ -157,6 +175,14 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
                          int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
                          unsigned long long* qr, unsigned long long* qz, hipStream_t s);
+// batched append: per-run scan over the chunks recorded in chunk_start, then copy
+hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,
+                               float* const stage[6], const int32_t* stage_cnt,
+                               const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
+                               int32_t total_chunks, unsigned long long* run_counters,
+                               unsigned long long* dropped, const Geom& g, float kq,
+                               unsigned long long* qr, unsigned long long* qz, hipStream_t s);
+hipError_t fdtd_prepare();
 hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx, hipStream_t s);
 hipError_t launch_j_from_fixed(const FdtdArgs& a, const unsigned long long* qr,

@@ -472,28 +472,116 @@ __device__ __forceinline__ int block_rank_slots(const bool (&pred)[kPer], int (&
     return base;
 }
 
+// Block totals of the per-thread event counts, added to the block's own slot
+// (plain add: a slot belongs to one block index and launches on one stream do
+// not overlap) and, when dst_run is set, atomically to a shared per-run slot.
+// Every thread must call it.
+__device__ __forceinline__ void block_counts_add(const unsigned (&v)[kNCounters],
+                                                 unsigned long long* dst_block,
+                                                 unsigned long long* dst_run)
+{
+    __shared__ unsigned s_cnt[kNCounters][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kNCounters; ++k) {
+        unsigned t = v[k];
+        for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+        if (lane == 0) s_cnt[k][wave] = t;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < kNCounters) {
+        unsigned long long s = 0;
+        for (int w = 0; w < kBlock / 64; ++w) s += s_cnt[threadIdx.x][w];
+        if (s) {
+            dst_block[threadIdx.x] += s;
+            if (dst_run) atomicAdd(&dst_run[threadIdx.x], s);
+        }
+    }
+    __syncthreads();
+}
+
 template <uint32_t F>
-__global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa, CollArgs ca)
+__global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, CollArgs ca0)
 {
     constexpr bool kDep = (F & kDeposit) != 0;
     constexpr bool kCol = (F & kCollide) != 0;
+    constexpr bool kBat = (F & kBatch) != 0;
     __shared__ int s_wave[kBlock / 64];
     __shared__ int s_red[4 * kPer * (kBlock / 64)];
     __shared__ unsigned long long s_tr[kDep ? kTile : 1], s_tz[kDep ? kTile : 1];
     __shared__ float s_q[kCol ? kQW : 1][kCol ? kQCap : 1];
+    __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
     static_assert(sizeof(SmallTabs) % 4 == 0, "SmallTabs is copied in 32-bit words");
     __shared__ uint32_t s_tab_raw[kCol ? sizeof(SmallTabs) / 4 : 1];
     SmallTabs* s_tab = reinterpret_cast<SmallTabs*>(s_tab_raw);
     if (kCol) {   // small collision tables into LDS (find_CS0 tables, excitation fit, guides)
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(&ca.tab->s);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(&ca0.tab->s);
         for (int i = threadIdx.x; i < (int)(sizeof(SmallTabs) / 4); i += blockDim.x) s_tab_raw[i] = src[i];
         __syncthreads();
     }
 
-    const int n = *pa.count;
-    const int nchunks = (n + kChunk - 1) / kChunk;
+    int total_chunks;
+    if constexpr (kBat) {
+        // chunk prefix over the runs (every block computes it; block 0 records
+        // it for the append kernels, which must see the counts of this launch)
+        constexpr int kRPer = kMaxRuns / kBlock;
+        int loc[kRPer], sum = 0;
+#pragma unroll
+        for (int j = 0; j < kRPer; ++j) {
+            const int r = (int)threadIdx.x * kRPer + j;
+            loc[j] = r < pa0.nruns ? (pa0.count[r] + kChunk - 1) / kChunk : 0;
+            sum += loc[j];
+        }
+        int tot;
+        int off = block_excl_scan(sum, s_wave, tot);
+#pragma unroll
+        for (int j = 0; j < kRPer; ++j) {
+            const int r = (int)threadIdx.x * kRPer + j;
+            if (r < pa0.nruns) s_cs[r] = off;
+            off += loc[j];
+        }
+        if (threadIdx.x == 0) s_cs[pa0.nruns] = tot;
+        __syncthreads();
+        if (blockIdx.x == 0)
+            for (int r = threadIdx.x; r <= pa0.nruns; r += blockDim.x) pa0.chunk_start[r] = s_cs[r];
+        total_chunks = tot;
+    } else {
+        total_chunks = (*pa0.count + kChunk - 1) / kChunk;
+    }
+    const int nr_ = g.nr, nz_ = g.nz;
     unsigned c_el = 0, c_ion = 0, c_exc = 0, c_viol = 0, c_sat = 0, c_abs = 0, c_push = 0;
-    for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    for (int gchunk = blockIdx.x; gchunk < total_chunks; gchunk += gridDim.x) {
+        // chunk: index within the population (slots, Philox counters);
+        // gchunk: index of the staging area
+        PartArgs pa = pa0;
+        CollArgs ca = ca0;
+        int chunk = gchunk, n, run = 0;
+        if constexpr (kBat) {
+            int lo = 0, hi = pa0.nruns;            // s_cs[lo] <= gchunk < s_cs[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_cs[mid] <= gchunk) lo = mid; else hi = mid;
+            }
+            run = lo;
+            chunk = gchunk - s_cs[lo];
+            const BatchRun br = pa0.runs[run];
+            pa.x += br.base; pa.y += br.base; pa.z += br.base;
+            pa.vx += br.base; pa.vy += br.base; pa.vz += br.base;
+            n = pa0.count[run];
+            const size_t nh = (size_t)nr_ * nz_;
+            pa.er += (size_t)run * (nh + nr_);
+            pa.ez += (size_t)run * (nh + nz_);
+            pa.h += (size_t)run * nh;
+            if (pa.qr) { pa.qr += (size_t)run * nh; pa.qz += (size_t)run * nh; }
+            const size_t gc = (size_t)ca0.gnr * ca0.gnz;
+            if (ca.dens) ca.dens += (size_t)run * gc;
+            if (ca.temp) ca.temp += (size_t)run * gc;
+            ca.pmax = br.pmax;
+            ca.run = br.run;
+            c_el = c_ion = c_exc = c_viol = c_sat = c_abs = c_push = 0;
+        } else {
+            n = *pa0.count;
+        }
         // slot of sub-chunk p: chunk*kChunk + p*kBlock + lane (coalesced per p)
         float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
         bool alive[kPer], was_alive[kPer], absorbed[kPer], cand[kPer];
@@ -646,12 +734,12 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa, CollA
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
                 if (has_sec[p]) {
-                    const int dst = chunk * kChunk + rank[p];
+                    const int dst = gchunk * kChunk + rank[p];
                     pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
                     pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
                 }
             }
-            if (threadIdx.x == 0) pa.sec_cnt[chunk] = tot;
+            if (threadIdx.x == 0) pa.sec_cnt[gchunk] = tot;
         }
         if (F & kStageIon) {
             int rank[kPer];
@@ -659,13 +747,13 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa, CollA
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
                 if (has_ion[p]) {
-                    const int dst = chunk * kChunk + rank[p];
+                    const int dst = gchunk * kChunk + rank[p];
                     // ion at the ionization site (electron_collision.m:63)
                     pa.ion[0][dst] = cx[p]; pa.ion[1][dst] = cy[p]; pa.ion[2][dst] = cz[p];
                     pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
                 }
             }
-            if (threadIdx.x == 0) pa.ion_cnt[chunk] = tot;
+            if (threadIdx.x == 0) pa.ion_cnt[gchunk] = tot;
         }
 #pragma unroll
         for (int p = 0; p < kPer; ++p) {
@@ -680,25 +768,19 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa, CollA
             c_abs += (unsigned)absorbed[p];
             if (F & kPush) c_push += was_alive[p];
         }
+        if constexpr (kBat) {   // per run: one atomic per counter and chunk
+            if (pa0.counters) {
+                const unsigned v[kNCounters] = {c_el, c_ion, c_exc, 0u, 0u, c_viol, c_abs, c_sat, c_push};
+                block_counts_add(v, pa0.counters + (size_t)blockIdx.x * kNCounters,
+                                 pa0.run_counters + (size_t)run * kNCounters);
+            }
+        }
     }
     // one non-atomic update of this block's counter slot (launches on one
     // stream never overlap, and a slot belongs to one block index)
-    if (pa.counters) {
+    if (!kBat && pa0.counters) {
         const unsigned v[kNCounters] = {c_el, c_ion, c_exc, 0u, 0u, c_viol, c_abs, c_sat, c_push};
-        __shared__ unsigned s_cnt[kNCounters][kBlock / 64];
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-        for (int k = 0; k < kNCounters; ++k) {
-            unsigned t = v[k];
-            for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-            if (lane == 0) s_cnt[k][wave] = t;
-        }
-        __syncthreads();
-        if ((int)threadIdx.x < kNCounters) {
-            unsigned long long s = 0;
-            for (int w = 0; w < kBlock / 64; ++w) s += s_cnt[threadIdx.x][w];
-            if (s) pa.counters[(size_t)blockIdx.x * kNCounters + threadIdx.x] += s;
-        }
+        block_counts_add(v, pa0.counters + (size_t)blockIdx.x * kNCounters, nullptr);
     }
 }
 
@@ -796,6 +878,114 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
     return hipGetLastError();
 }
 
+// ---- batched append: workgroup r scans run r's staged chunks -----------------
+// chunk_start (written by the particle launch) delimits each run's chunks in
+// the staging area; tmp_run[2r] <- run r's old count.
+__global__ __launch_bounds__(1024) void k_append_scan_batch(int32_t* counts, const BatchRun* runs,
+                                                            const int32_t* stage_cnt,
+                                                            const int32_t* chunk_start, int32_t* offs,
+                                                            int32_t* tmp_run,
+                                                            unsigned long long* run_counters,
+                                                            unsigned long long* dropped)
+{
+    __shared__ int s_part[1024];
+    const int r = blockIdx.x;
+    const int c0 = chunk_start[r], nch = chunk_start[r + 1] - c0;
+    const int n = counts[r];
+    const int per = (nch + blockDim.x - 1) / blockDim.x;
+    const int beg = threadIdx.x * per;
+    const int end = min(beg + per, nch);
+    int local = 0;
+    for (int i = beg; i < end; ++i) local += stage_cnt[c0 + i];
+    s_part[threadIdx.x] = local;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        const int v = (int)threadIdx.x >= o ? s_part[threadIdx.x - o] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int acc = s_part[threadIdx.x] - local;
+    for (int i = beg; i < end; ++i) {
+        offs[c0 + i] = acc;
+        acc += stage_cnt[c0 + i];
+    }
+    if (threadIdx.x == blockDim.x - 1) {
+        const int total = s_part[threadIdx.x];
+        const int room = runs[r].cap - n;
+        const int app = total < room ? total : room;
+        tmp_run[2 * r] = n;
+        counts[r] = n + app;
+        if (total > app) {
+            atomicAdd(&run_counters[(size_t)r * kNCounters + kCDropSec], (unsigned long long)(total - app));
+            atomicAdd(dropped, (unsigned long long)(total - app));
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_append_copy_batch(float* d0, float* d1, float* d2, float* d3,
+                                                              float* d4, float* d5, const BatchRun* runs,
+                                                              int nruns, const float* s0, const float* s1,
+                                                              const float* s2, const float* s3,
+                                                              const float* s4, const float* s5,
+                                                              const int32_t* stage_cnt,
+                                                              const int32_t* chunk_start,
+                                                              const int32_t* offs, const int32_t* tmp_run,
+                                                              Geom g, float kq, unsigned long long* qr,
+                                                              unsigned long long* qz)
+{
+    __shared__ int s_cs[kMaxRuns + 1];
+    for (int r = threadIdx.x; r <= nruns; r += blockDim.x) s_cs[r] = chunk_start[r];
+    __syncthreads();
+    const int total = s_cs[nruns];
+    const size_t nh = (size_t)g.nr * g.nz;
+    for (int c = blockIdx.x; c < total; c += gridDim.x) {
+        const int cnt = stage_cnt[c];
+        if (cnt == 0) continue;
+        int lo = 0, hi = nruns;                    // s_cs[lo] <= c < s_cs[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_cs[mid] <= c) lo = mid; else hi = mid;
+        }
+        const BatchRun br = runs[lo];
+        const int first = tmp_run[2 * lo] + offs[c];
+        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const int local = first + j;
+            if (local < br.cap) {
+                const int64_t dst = br.base + local;
+                const int src = c * kChunk + j;
+                const float x = s0[src], y = s1[src], z = s2[src];
+                const float vx = s3[src], vy = s4[src], vz = s5[src];
+                d0[dst] = x; d1[dst] = y; d2[dst] = z;
+                d3[dst] = vx; d4[dst] = vy; d5[dst] = vz;
+                if (qr) {
+                    DepCells dc;
+                    deposit_cells(g, x, y, z, vx, vy, vz, kq, dc);
+                    deposit_global(dc, g.nr, qr + (size_t)lo * nh, qz + (size_t)lo * nh);
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,
+                               float* const stage[6], const int32_t* stage_cnt,
+                               const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
+                               int32_t total_chunks, unsigned long long* run_counters,
+                               unsigned long long* dropped, const Geom& g, float kq,
+                               unsigned long long* qr, unsigned long long* qz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_append_scan_batch, dim3(nruns), dim3(1024), 0, s, counts, runs, stage_cnt,
+                       chunk_start, offs, tmp_run, run_counters, dropped);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int blocks = total_chunks < 4096 ? (total_chunks > 0 ? total_chunks : 1) : 4096;
+    hipLaunchKernelGGL(k_append_copy_batch, dim3(blocks), dim3(kBlock), 0, s, pool[0], pool[1], pool[2],
+                       pool[3], pool[4], pool[5], runs, nruns, stage[0], stage[1], stage[2], stage[3],
+                       stage[4], stage[5], stage_cnt, chunk_start, offs, tmp_run, g, kq, qr, qz);
+    return hipGetLastError();
+}
+
 // ---- cross sections (find_CS0.m) --------------------------------------------
 __global__ void k_cross_sections(const DevTables* dt, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx)
@@ -847,7 +1037,8 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
                             int max_blocks, hipStream_t s)
 {
     const int blocks = max_blocks > 0 ? max_blocks : 1;
-    constexpr uint32_t P = kPush, Co = kCollide, D = kDrift, De = kDeposit, S = kStageSec, I = kStageIon;
+    constexpr uint32_t P = kPush, Co = kCollide, D = kDrift, De = kDeposit, S = kStageSec, I = kStageIon,
+                       B = kBatch;
     switch (flags) {
     case D: return launch_t<D>(g, pa, ca, blocks, s);
     case De: return launch_t<De>(g, pa, ca, blocks, s);
@@ -864,6 +1055,14 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
     case P | Co | D | De | S: return launch_t<P | Co | D | De | S>(g, pa, ca, blocks, s);
     case P | Co | D | De | I: return launch_t<P | Co | D | De | I>(g, pa, ca, blocks, s);
     case P | Co | D | De | S | I: return launch_t<P | Co | D | De | S | I>(g, pa, ca, blocks, s);
+    // batched ensemble
+    case B | D | De: return launch_t<B | D | De>(g, pa, ca, blocks, s);
+    case B | P: return launch_t<B | P>(g, pa, ca, blocks, s);
+    case B | P | D | De: return launch_t<B | P | D | De>(g, pa, ca, blocks, s);
+    case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s);
+    case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s);
+    case B | P | Co | D | De: return launch_t<B | P | Co | D | De>(g, pa, ca, blocks, s);
+    case B | P | Co | D | De | S: return launch_t<B | P | Co | D | De | S>(g, pa, ca, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -7,10 +7,10 @@ The compute runs in libgasx.so (HIP, gfx950).  Importing this package does
 not load the library; the first Context does, and raises if it is missing.
 """
 from .core import (C_LIGHT, E_EM, E_MC, EPS0, K_B, M_ION, ME_EM, ME_MC, MU0, Context,
-                   CounterValues, FieldState, GasGrid, ParticleSet, make_params)
+                   CounterValues, Ensemble, FieldState, GasGrid, ParticleSet, make_params)
 from .tables import raw_tables
 
 __all__ = [
-    "Context", "ParticleSet", "FieldState", "GasGrid", "CounterValues", "make_params",
+    "Context", "Ensemble", "ParticleSet", "FieldState", "GasGrid", "CounterValues", "make_params",
     "raw_tables", "C_LIGHT", "EPS0", "MU0", "E_EM", "ME_EM", "E_MC", "ME_MC", "M_ION", "K_B",
 ]

@@ -87,7 +87,26 @@ class Timing(C.Structure):
         ("particle_ms", C.c_double), ("fdtd_ms", C.c_double), ("append_ms", C.c_double),
         ("sort_ms", C.c_double), ("n_particle_launches", C.c_int64), ("n_fdtd_launches", C.c_int64),
         ("n_append_launches", C.c_int64), ("n_sorts", C.c_int64),
+        ("graph_ms", C.c_double), ("n_graph_launches", C.c_int64),
     ]
+
+
+class Batch(C.Structure):
+    _fields_ = [
+        ("nruns", C.c_int32),
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+        ("base", C.POINTER(C.c_int64)), ("capacity", C.POINTER(C.c_int64)),
+        ("counts", C.c_void_p),
+        ("er", C.c_void_p), ("ez", C.c_void_p), ("hphi", C.c_void_p),
+        ("density", C.c_void_p), ("n_max", C.POINTER(C.c_double)),
+        ("gnr", C.c_int32), ("gnz", C.c_int32),
+        ("gdr", C.c_double), ("gdz", C.c_double), ("t_scalar", C.c_double),
+    ]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("fdtd_ti", C.c_int32), ("fdtd_tk", C.c_int32), ("particle_blocks_per_cu", C.c_int32)]
 
 
 class RunOpts(C.Structure):
@@ -127,7 +146,16 @@ PROTOTYPES = {
     "gasx_get_count": (C.c_int, [_P, C.POINTER(Species), C.POINTER(C.c_int64)]),
     "gasx_collision_bound": (C.c_int, [_P, C.POINTER(Gas), C.c_double, C.POINTER(C.c_double)]),
     "gasx_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "gasx_batch_align": (C.c_int, []),
+    "gasx_batch_run": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(RunOpts)]),
+    "gasx_batch_autotune": (C.c_int, [_P, C.POINTER(Batch), C.POINTER(RunOpts), C.c_int,
+                                      C.POINTER(BatchTuning)]),
+    "gasx_batch_set_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "gasx_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
+    "gasx_batch_counters": (C.c_int, [_P, C.c_int, C.POINTER(Counters), C.c_int]),
 }
+
+ABI_VERSION = 2
 
 _lib = None
 
@@ -146,7 +174,7 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.gasx_abi_version() != 1:
+    if lib.gasx_abi_version() != ABI_VERSION:
         raise RuntimeError("libgasx ABI version mismatch")
     if path is None:
         _lib = lib

@@ -288,3 +288,101 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.gasx_synchronize(self.ctx))
+
+
+class Ensemble:
+    """Independent runs on the context's grid, advanced together by
+    gasx_batch_run: one particle launch and one field launch per step cover
+    every run.  Run r owns pooled slots [base[r], base[r] + capacity[r]), its
+    own fields, gas density (gnz x gnr) and Philox run id (run_id + r)."""
+
+    def __init__(self, ctx: Context, capacities, densities, gdr: float, gdz: float,
+                 t_scalar: float = 300.0, device="cuda"):
+        self.ctx = ctx
+        lib = ctx.lib
+        align = lib.gasx_batch_align()
+        self.nruns = len(capacities)
+        if not 1 <= self.nruns <= 1024:
+            raise ValueError("an ensemble holds 1..1024 runs")
+        self.capacity = np.asarray([int(c) for c in capacities], dtype=np.int64)
+        if np.any(self.capacity < 1):
+            raise ValueError("capacities must be positive")
+        padded = (self.capacity + align - 1) // align * align
+        self.base = np.concatenate([[0], np.cumsum(padded)[:-1]]).astype(np.int64)
+        total = int(padded.sum())
+        p = ctx.params
+        self.nr, self.nz = p.nr, p.nz
+        self.pool = torch.zeros((6, total), dtype=torch.float32, device=device)
+        self.counts = torch.zeros(self.nruns, dtype=torch.int32, device=device)
+        self.er = torch.zeros((self.nruns, p.nz + 1, p.nr), dtype=torch.float64, device=device)
+        self.ez = torch.zeros((self.nruns, p.nz, p.nr + 1), dtype=torch.float64, device=device)
+        self.hphi = torch.zeros((self.nruns, p.nz, p.nr), dtype=torch.float64, device=device)
+        d = [np.atleast_2d(np.asarray(x, dtype=np.float32)) for x in densities]
+        if len(d) != self.nruns or any(x.shape != d[0].shape for x in d):
+            raise ValueError("one density grid of a common shape per run")
+        self.density = torch.from_numpy(np.stack(d)).to(device).contiguous()
+        self.n_max = np.asarray([float(x.max()) for x in d], dtype=np.float64)
+        self.gnz, self.gnr = d[0].shape
+        self.gdr, self.gdz, self.t_scalar = float(gdr), float(gdz), float(t_scalar)
+
+    def load_run(self, r: int, x, y, z, vx, vy, vz):
+        n = len(x)
+        if n > self.capacity[r]:
+            raise ValueError("more particles than the run's capacity")
+        b = int(self.base[r])
+        for row, a in enumerate((x, y, z, vx, vy, vz)):
+            self.pool[row, b: b + n].copy_(torch.as_tensor(np.asarray(a), dtype=torch.float32))
+        self.counts[r] = n
+
+    def run_state(self, r: int) -> np.ndarray:
+        """(6, n) float32 host copy of run r (absorbed slots have z = NaN)."""
+        n = int(self.counts[r].item())
+        b = int(self.base[r])
+        return self.pool[:, b: b + n].cpu().numpy()
+
+    def struct(self) -> "_abi.Batch":
+        d = self.pool
+        self._keep = (np.ascontiguousarray(self.base), np.ascontiguousarray(self.capacity),
+                      np.ascontiguousarray(self.n_max))
+        P64, PD = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        return _abi.Batch(self.nruns, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(),
+                          d[3].data_ptr(), d[4].data_ptr(), d[5].data_ptr(),
+                          self._keep[0].ctypes.data_as(P64), self._keep[1].ctypes.data_as(P64),
+                          self.counts.data_ptr(), self.er.data_ptr(), self.ez.data_ptr(),
+                          self.hphi.data_ptr(), self.density.data_ptr(),
+                          self._keep[2].ctypes.data_as(PD), self.gnr, self.gnz, self.gdr, self.gdz,
+                          self.t_scalar)
+
+    @staticmethod
+    def _opts(n_steps, collide, secondaries, seed, run_id, step0, use_graph, time_kernels):
+        return RunOpts(n_steps, int(collide), int(secondaries), 0, seed, run_id, step0,
+                       int(use_graph), int(time_kernels))
+
+    def run(self, n_steps: int, collide: bool = True, secondaries: bool = False, seed: int = 0,
+            run_id: int = 0, step0: int = 0, use_graph: bool = False, time_kernels: bool = False):
+        b = self.struct()
+        o = self._opts(n_steps, collide, secondaries, seed, run_id, step0, use_graph, time_kernels)
+        self.ctx._check(self.ctx.lib.gasx_batch_run(self.ctx.ctx, C.byref(b), C.byref(o)))
+
+    def autotune(self, trial_steps: int = 5, collide: bool = True, secondaries: bool = False,
+                 seed: int = 0, run_id: int = 0) -> dict:
+        b = self.struct()
+        o = self._opts(trial_steps, collide, secondaries, seed, run_id, 0, False, False)
+        t = _abi.BatchTuning()
+        self.ctx._check(self.ctx.lib.gasx_batch_autotune(self.ctx.ctx, C.byref(b), C.byref(o),
+                                                         trial_steps, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in _abi.BatchTuning._fields_}
+
+    def set_tuning(self, fdtd_ti: int, fdtd_tk: int, particle_blocks_per_cu: int):
+        t = _abi.BatchTuning(fdtd_ti, fdtd_tk, particle_blocks_per_cu)
+        self.ctx._check(self.ctx.lib.gasx_batch_set_tuning(self.ctx.ctx, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = _abi.BatchTuning()
+        self.ctx._check(self.ctx.lib.gasx_batch_get_tuning(self.ctx.ctx, C.byref(t)))
+        return {n: getattr(t, n) for n, _ in _abi.BatchTuning._fields_}
+
+    def counters(self, reset: bool = False) -> list[CounterValues]:
+        arr = (Counters * self.nruns)()
+        self.ctx._check(self.ctx.lib.gasx_batch_counters(self.ctx.ctx, self.nruns, arr, int(reset)))
+        return [CounterValues(*[getattr(c, n) for n, _ in Counters._fields_]) for c in arr]

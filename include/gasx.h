@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 1
+#define GASX_ABI_VERSION 2
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -123,6 +123,8 @@ typedef struct gasx_timing {
     double append_ms;          /* sum over secondary/ion append launches */
     double sort_ms;            /* sum over cell sorts */
     int64_t n_particle_launches, n_fdtd_launches, n_append_launches, n_sorts;
+    double graph_ms;           /* sum over graph launches (gasx_batch_run with use_graph) */
+    int64_t n_graph_launches;
 } gasx_timing;
 
 /* Options of the fused production loop. */
@@ -194,6 +196,48 @@ gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, con
  * ions may be NULL. */
 gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, gasx_fields* f,
                      const gasx_gas* g, const gasx_run_opts* o);
+
+/* ---- batched ensemble -------------------------------------------------------
+ * Many independent runs of the production loop on the context's grid, advanced
+ * together: one particle launch and one field launch per step cover every run.
+ * The reference has no batched form; each run is exactly what gasx_run with
+ * the same seed, run id (opts.run_id + r) and gas would produce (sort_every
+ * must be 0; ions are not supported).  Particle slots of all runs live in one
+ * pooled SoA: run r owns [base[r], base[r] + capacity[r]), base[r] a multiple
+ * of gasx_batch_align(). */
+typedef struct gasx_batch {
+    int32_t nruns;                       /* 1 .. 1024 */
+    float *x, *y, *z, *vx, *vy, *vz;     /* pooled SoA, device */
+    const int64_t* base;                 /* host, nruns slot offsets */
+    const int64_t* capacity;             /* host, nruns */
+    int32_t* counts;                     /* device int32[nruns]: slots in use per run */
+    double *er, *ez, *hphi;              /* device: nruns field sets back to back, reference layout each */
+    const float* density;                /* device: nruns * gnr * gnz, index r*gnr*gnz + iz*gnr + ir (m^-3) */
+    const double* n_max;                 /* host, nruns: upper bound of each run's density */
+    int32_t gnr, gnz;
+    double gdr, gdz;
+    double t_scalar;                     /* gas temperature (K) */
+} gasx_batch;
+
+/* Launch shapes of the batched loop (picked by gasx_batch_autotune). */
+typedef struct gasx_batch_tuning {
+    int32_t fdtd_ti, fdtd_tk;            /* field tile in cells (r x z) */
+    int32_t particle_blocks_per_cu;      /* persistent particle workgroups per CU */
+} gasx_batch_tuning;
+
+int gasx_batch_align(void);
+/* Advances every run by o->n_steps steps (collide / secondaries / seed /
+ * run_id / step0 / use_graph as gasx_run; use_graph = 1 captures all steps in
+ * one hipGraph, instantiated once per batch shape and replayed). */
+gasx_status gasx_batch_run(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o);
+/* Times each candidate launch shape for trial_steps steps on a snapshot of the
+ * batch (restored afterwards), keeps the fastest in the context and returns it. */
+gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o, int trial_steps,
+                                gasx_batch_tuning* out);
+gasx_status gasx_batch_set_tuning(gasx_ctx* ctx, const gasx_batch_tuning* t);
+gasx_status gasx_batch_get_tuning(gasx_ctx* ctx, gasx_batch_tuning* t);
+/* Per-run event counters (nruns entries; synchronizes). */
+gasx_status gasx_batch_counters(gasx_ctx* ctx, int nruns, gasx_counters* out, int reset);
 
 /* Cell sort by (iz, ir) of the particle's Hphi cell; absorbed slots are dropped
  * and count is reduced (the reference's logical-index removal,
