@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--workload", default="plain_gpu_256x512")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -99,6 +100,89 @@ def cpu_baseline(w, host, n_particles: int, n_steps: int, seed: int):
     }
 
 
+def cell_sorted(host, w):
+    """Initial electrons ordered by their (iz, ir) cell (both legs below use it)."""
+    import numpy as np
+    x, y, z = (a.astype(np.float64) for a in host[:3])
+    ir = np.minimum((np.hypot(x, y) / w.dr).astype(np.int64), w.nr - 1)
+    iz = np.minimum((z / w.dz).astype(np.int64), w.nz - 1)
+    order = np.argsort(iz * w.nr + ir, kind="stable")
+    return tuple(a[order] for a in host)
+
+
+def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
+    """Batched ensemble (64 runs of 128x256, log-uniform density and electron
+    count, collisions on, secondaries off) in one captured graph with the
+    autotuned launch shapes, against the same runs one after another with
+    gasx_run.  Both legs start from the same cell-ordered electrons and produce
+    identical events run by run (checked from the per-run counters)."""
+    import numpy as np
+    import torch
+
+    import gasx
+    from gasx.workloads import WORKLOADS, batched_runs, seed_electrons
+
+    w = WORKLOADS["batched_128x256"]
+    n_steps = steps or w.n_steps
+    counts, dens = batched_runs(w, seed)
+    p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
+    ctx = gasx.Context(p, device=0)
+    runs = [cell_sorted(seed_electrons(int(n), w.rp, w.lp, w.phip, w.sigma_r, seed=seed + 100 + r), w)
+            for r, n in enumerate(counts)]
+    ens = gasx.Ensemble(ctx, [int(n) for n in counts], dens, w.rp, w.lp)
+    for r, h in enumerate(runs):
+        ens.load_run(r, *h)
+    init = (ens.pool.clone(), ens.counts.clone())
+
+    def reset():
+        ens.pool.copy_(init[0])
+        ens.counts.copy_(init[1])
+        for t in (ens.er, ens.ez, ens.hphi):
+            t.zero_()
+        ens.counters(reset=True)
+        torch.cuda.synchronize()
+
+    kw = dict(collide=True, secondaries=False, seed=seed + 7, run_id=0)
+    tuning = ens.autotune(trial_steps=5, **kw)
+    reset()
+    ens.run(n_steps, use_graph=True, **kw)            # capture + instantiate + first replay
+    reset()
+    t0 = time.perf_counter()
+    ens.run(n_steps, use_graph=True, **kw)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    per_run = ens.counters()
+    psteps_b = sum(c.n_pushed for c in per_run)
+    out = {"value": psteps_b / wall_b, "unit": "particle-steps/s", "runs": len(counts),
+           "steps": n_steps, "electrons": int(counts.sum()), "wall_s": wall_b, "tuning": tuning,
+           "graph": True, "grid": f"{w.nr}x{w.nz}"}
+    del ens, init
+    torch.cuda.empty_cache()
+    if not sequential:
+        return out
+    wall_s, psteps_s, same = 0.0, 0, True
+    gas_grids = [gasx.GasGrid(d, w.rp, w.lp, temperature=300.0) for d in dens]
+    for r, h in enumerate(runs):
+        e = gasx.ParticleSet(int(counts[r]))
+        e.load(*h)
+        f = gasx.FieldState(w.nr, w.nz)
+        ctx.counters(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(e, f, gas_grids[r], n_steps, sort_every=0, **{**kw, "run_id": r})
+        torch.cuda.synchronize()
+        wall_s += time.perf_counter() - t0
+        c = ctx.counters()
+        psteps_s += c.n_pushed
+        b = per_run[r]
+        same &= (c.n_elastic, c.n_ionize, c.n_excite, c.n_pushed) == \
+                (b.n_elastic, b.n_ionize, b.n_excite, b.n_pushed)
+    out.update({"sequential_value": psteps_s / wall_s, "sequential_wall_s": wall_s,
+                "speedup_vs_sequential": (psteps_b / wall_b) / (psteps_s / wall_s),
+                "identical_events_to_sequential": bool(same)})
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -117,6 +201,16 @@ def main():
     from gasx.workloads import WORKLOADS, density_grid, seed_electrons
 
     w = WORKLOADS[args.workload]
+    if w.name == "batched_128x256":
+        b = bench_batched(args.seed, steps=args.steps if args.steps != 1000 else None)
+        print(json.dumps({
+            "metric": METRIC, "value": b["value"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": b["steps"], "warmup": 1, "ms_per_step": b["wall_s"] * 1e3 / b["steps"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic", "config": {"workload": w.name, "runs": b["runs"], "grid": b["grid"],
+                                            "electrons": b["electrons"]},
+            "batched": b}))
+        return
     if w.n_electrons <= 0:
         raise SystemExit(f"workload {w.name} is not a single-run bench workload")
     p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
@@ -151,15 +245,21 @@ def main():
     psteps = c.n_pushed
     value = psteps / wall
 
+    # The particle pass of a step is two kernels: k_particles (push + collide,
+    # the dominant one, reported as the roofline kernel) and k_drift_deposit
+    # (half drift + current deposit).  Each reads and writes the 24-byte state.
     n_launch = max(1, tm["n_particle_launches"])
     avg_ms = tm["particle_ms"] / n_launch
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * psteps / n_launch
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    n_dd = max(1, tm["n_drift_launches"])
+    dd_ms = tm["drift_ms"] / n_dd
+    pass_ms = (tm["particle_ms"] + tm["drift_ms"]) / args.steps
     # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE passes
     # (calibrated against a plain copy), stored per particle-step and scaled
     # to this run's particle-steps per launch.
     traffic = None
-    tfile = REPO / "profiles" / f"k_particles_{w.name}.traffic.txt"
+    tfile = REPO / "profiles" / f"k_particles_push_collide_{w.name}.traffic.txt"
     if tfile.exists():
         for ln in tfile.read_text().splitlines():
             if ln.startswith("hbm_bytes_per_particle_step:"):
@@ -190,11 +290,19 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_particles (fused push+collide+drift+deposit)",
+            "kernel": "k_particles (gather + push + collision Monte Carlo + secondary staging)",
             "avg_launch_ms": avg_ms,
             "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
         },
-        "breakdown_ms": {k: tm[k] for k in ("particle_ms", "fdtd_ms", "append_ms", "sort_ms")},
+        "drift_deposit_kernel": {
+            "avg_launch_ms": dd_ms,
+            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps / n_dd / (dd_ms * 1e-3) / 1e9,
+        },
+        "particle_pass": {   # both kernels of a step against the step's 48 algorithmic bytes per particle
+            "ms_per_step": pass_ms,
+            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps / args.steps / (pass_ms * 1e-3) / 1e9,
+        },
+        "breakdown_ms": {k: tm[k] for k in ("particle_ms", "drift_ms", "fdtd_ms", "append_ms", "sort_ms")},
         "events": {"elastic": c.n_elastic, "ionize": c.n_ionize, "excite": c.n_excite,
                    "absorbed": c.n_absorbed, "dropped_secondaries": c.n_dropped_sec,
                    "bound_violations": c.n_bound_violations, "final_slots": e.n},
@@ -203,6 +311,10 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, host, args.cpu_particles, args.cpu_steps, args.seed + 1)
+    if not args.no_batched:
+        del e, f, host_t
+        torch.cuda.empty_cache()
+        line["batched"] = bench_batched(args.seed)
     print(json.dumps(line))
 
 

@@ -303,7 +303,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->ez_b, nez), "alloc ez_b");
         chk(dalloc(&ctx->h_b, nh), "alloc h_b");
         for (int b = 0; b < 2; ++b)
-            for (int k = 0; k < 2; ++k) chk(dalloc(&ctx->q[b][k], nh), "alloc q");
+            for (int k = 0; k < 2; ++k) chk(dalloc(&ctx->q[b][k], nh * kJCopies), "alloc q");
         chk(dalloc(&ctx->jr_tmp, nh), "alloc jr");
         chk(dalloc(&ctx->jz_tmp, nh), "alloc jz");
         chk(dalloc(&ctx->d_counters, (size_t)kNCounters * kCounterSlots), "alloc counters");
@@ -371,7 +371,7 @@ extern "C" gasx_status gasx_synchronize(gasx_ctx* ctx)
 // staging for appends, sized for `cap` slots
 static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
 {
-    const int64_t slots = ((cap + kChunk - 1) / kChunk) * kChunk;
+    const int64_t slots = ((cap + kSeg - 1) / kSeg) * kSeg;
     if (slots > ctx->stage_slots || (ions && !ctx->istage[0])) {
         const int64_t ns = std::max(slots, ctx->stage_slots);
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
@@ -389,9 +389,9 @@ static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
             HIPCHK(dalloc(&ctx->stage[c], ns), "alloc stage");
             HIPCHK(dalloc(&ctx->istage[c], ns), "alloc ion stage");
         }
-        HIPCHK(dalloc(&ctx->stage_cnt, ns / kChunk), "alloc stage cnt");
-        HIPCHK(dalloc(&ctx->istage_cnt, ns / kChunk), "alloc ion stage cnt");
-        HIPCHK(dalloc(&ctx->scan_tmp, ns / kChunk), "alloc scan");
+        HIPCHK(dalloc(&ctx->stage_cnt, ns / kSeg), "alloc stage cnt");
+        HIPCHK(dalloc(&ctx->istage_cnt, ns / kSeg), "alloc ion stage cnt");
+        HIPCHK(dalloc(&ctx->scan_tmp, ns / kSeg), "alloc scan");
         ctx->stage_slots = ns;
     }
     return GASX_OK;
@@ -441,15 +441,16 @@ gasx_status gasx_host::make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_col
     return GASX_OK;
 }
 
-int gasx_host::particle_grid(const gasx_ctx* ctx, int64_t chunks, int per_cu)
+int gasx_host::particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu)
 {
     const int64_t persistent = std::min<int64_t>((int64_t)ctx->cu_count * per_cu, kCounterSlots);
-    return (int)std::max<int64_t>(1, std::min(chunks, persistent));
+    const int64_t need = (segments + kBlock / kWaveSize - 1) / (kBlock / kWaveSize);
+    return (int)std::max<int64_t>(1, std::min(need, persistent));
 }
 
 static int particle_blocks(gasx_ctx* ctx, int64_t cap)
 {
-    return particle_grid(ctx, (cap + kChunk - 1) / kChunk, 16);
+    return particle_grid(ctx, (cap + kSeg - 1) / kSeg, 16);
 }
 
 // one field cycle: reads (in) writes (out) and refreshes the FP32 copies
@@ -510,14 +511,35 @@ extern "C" gasx_status gasx_fields_sync(gasx_ctx* ctx, const gasx_fields* f)
 // ---------------------------------------------------------------------------
 // Particle entry points
 // ---------------------------------------------------------------------------
+// jstride > 0: qr, qz are the context's kJCopies-fold accumulators
 static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_species* s,
-                                 unsigned long long* qr, unsigned long long* qz, const CollArgs& ca)
+                                 unsigned long long* qr, unsigned long long* qz, const CollArgs& ca,
+                                 size_t jstride = 0, int timed_kind = kTParticle)
 {
     PartArgs pa = make_part(ctx, s);
     pa.qr = qr; pa.qz = qz;
-    TimedScope ts(ctx, kTParticle);
+    pa.jstride = qr ? jstride : 0;
+    TimedScope ts(ctx, timed_kind);
     HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity), ctx->stream),
            "particles");
+    return GASX_OK;
+}
+
+// EM2 half drift and / or EM7 deposit (k_drift_deposit); jstride > 0: qr, qz
+// are the context's kJCopies-fold accumulators
+static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool drift, bool deposit,
+                                     unsigned long long* qr, unsigned long long* qz, size_t jstride = 0,
+                                     int timed_kind = kTParticle)
+{
+    PartArgs pa = make_part(ctx, s);
+    pa.qr = qr; pa.qz = qz;
+    pa.jstride = qr ? jstride : 0;
+    const int64_t chunks = (s->capacity + kChunk - 1) / kChunk;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, std::min<int64_t>(
+                                                                             (int64_t)ctx->cu_count * 16,
+                                                                             kCounterSlots)));
+    TimedScope ts(ctx, timed_kind);
+    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream), "drift/deposit");
     return GASX_OK;
 }
 
@@ -525,16 +547,14 @@ extern "C" gasx_status gasx_half_drift(gasx_ctx* ctx, gasx_species* s)
 {
     if (!ctx || !valid_species(s)) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    CollArgs ca{};
-    return run_particles(ctx, kDrift, s, nullptr, nullptr, ca);
+    return run_drift_deposit(ctx, s, true, false, nullptr, nullptr);
 }
 
 extern "C" gasx_status gasx_deposit(gasx_ctx* ctx, const gasx_species* s, int64_t* qr, int64_t* qz)
 {
     if (!ctx || !valid_species(s) || !qr || !qz) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    CollArgs ca{};
-    return run_particles(ctx, kDeposit, s, (unsigned long long*)qr, (unsigned long long*)qz, ca);
+    return run_drift_deposit(ctx, s, false, true, (unsigned long long*)qr, (unsigned long long*)qz);
 }
 
 extern "C" gasx_status gasx_current_from_fixed(gasx_ctx* ctx, const int64_t* qr, const int64_t* qz,
@@ -570,16 +590,17 @@ extern "C" gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, in
 // appended before the electrons' own secondaries (the electron count fixes
 // the number of staged chunks).
 static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion,
-                                     unsigned long long* qr = nullptr, unsigned long long* qz = nullptr)
+                                     unsigned long long* qr = nullptr, unsigned long long* qz = nullptr,
+                                     size_t jstride = 0)
 {
     float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
-    const int32_t max_chunks = (int32_t)(ctx->stage_slots / kChunk);
+    const int32_t max_chunks = (int32_t)(ctx->stage_slots / kSeg);
     const float kq = (float)(4294967296.0 / ctx->p.c) * dst->qnum;
     TimedScope ts(ctx, kTAppend);
     HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? ctx->istage : ctx->stage,
                          ion ? ctx->istage_cnt : ctx->stage_cnt, src->count, ctx->scan_tmp,
                          ctx->base_tmp + (ion ? 2 : 0), max_chunks,
-                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz,
+                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz, jstride,
                          ctx->stream),
            "append");
     return GASX_OK;
@@ -650,17 +671,24 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
             if (st != GASX_OK) return st;
         }
     }
-    HIPCHK(launch_fill_u64(ctx->q[0][0], (int64_t)nh, ctx->stream), "zero q");
-    HIPCHK(launch_fill_u64(ctx->q[0][1], (int64_t)nh, ctx->stream), "zero q");
+    // both current buffers, every copy (the field kernel clears copy 0 of the
+    // next buffer, the copy reduction zeroes the others after reading them)
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(launch_fill_u64(ctx->q[b][k], (int64_t)(nh * kJCopies), ctx->stream), "zero q");
     // prologue: EM2 half drift + EM7 deposit of every species (electricmagneticfield6.m:141-144)
     for (int k = 0; k < nsp; ++k) {
-        gasx_status st = run_particles(ctx, kDrift | kDeposit, &sp[k], ctx->q[0][0], ctx->q[0][1], ca);
+        gasx_status st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[0][0], ctx->q[0][1], nh, kTDrift);
         if (st != GASX_OK) return st;
     }
     for (int s = 0; s < o.n_steps; ++s) {
         const int cur = s & 1, nxt = cur ^ 1;
         const bool last = s == o.n_steps - 1;
         gasx_status st;
+        {
+            TimedScope ts(ctx, kTFdtd);
+            HIPCHK(launch_j_reduce(ctx->q[cur][0], ctx->q[cur][1], (int64_t)nh, nh, ctx->stream), "j reduce");
+        }
         if ((s % 2) == 0)
             st = fdtd_cycle(ctx, f->er, f->ez, f->hphi, ctx->er_b, ctx->ez_b, ctx->h_b,
                             ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
@@ -672,15 +700,22 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         if (st != GASX_OK) return st;
         ca.step = o.step0 + (uint32_t)s;
         for (int k = 0; k < nsp; ++k) {
+            // two launches: push (+ collide, staged secondaries already
+            // drifted), then drift + deposit of the existing slots; the
+            // collision phase's registers would otherwise limit the occupancy
+            // of the streaming half drift and deposit
             uint32_t flags = kPush;
             if (k == 0 && o.collide) {
                 flags |= kCollide;
-                if (o.secondaries) flags |= kStageSec;
+                if (o.secondaries) flags |= kStageSec | (last ? 0u : kDriftSec);
                 if (o.ions) flags |= kStageIon;
             }
-            if (!last) flags |= kDrift | kDeposit;
-            st = run_particles(ctx, flags, &sp[k], ctx->q[nxt][0], ctx->q[nxt][1], ca);
+            st = run_particles(ctx, flags, &sp[k], nullptr, nullptr, ca);
             if (st != GASX_OK) return st;
+            if (!last) {
+                st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
+                if (st != GASX_OK) return st;
+            }
             if (k == 0 && o.collide) {
                 if (o.ions) {
                     st = append_from_stage(ctx, &sp[0], ions, true);
@@ -688,7 +723,7 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                 }
                 if (o.secondaries) {
                     st = append_from_stage(ctx, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
-                                           last ? nullptr : ctx->q[nxt][1]);
+                                           last ? nullptr : ctx->q[nxt][1], nh);
                     if (st != GASX_OK) return st;
                 }
             }
@@ -771,8 +806,8 @@ extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)
     if (!ctx || !out) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
-    double ms[5] = {0, 0, 0, 0, 0};
-    for (int k = 0; k < 5; ++k)
+    double ms[kNTimed] = {};
+    for (int k = 0; k < kNTimed; ++k)
         for (auto& pr : ctx->tpairs[k]) {
             float t = 0.f;
             HIPCHK(hipEventElapsedTime(&t, ctx->events[pr.first], ctx->events[pr.second]), "event time");
@@ -788,6 +823,8 @@ extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)
     out->n_sorts = (int64_t)ctx->tpairs[kTSort].size();
     out->graph_ms = ms[kTGraph];
     out->n_graph_launches = (int64_t)ctx->tpairs[kTGraph].size();
+    out->drift_ms = ms[kTDrift];
+    out->n_drift_launches = (int64_t)ctx->tpairs[kTDrift].size();
     return GASX_OK;
 }
 

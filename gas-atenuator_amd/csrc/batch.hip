@@ -40,7 +40,7 @@ void free_stage_buffers(BatchState& b)
 {
     for (auto& s : b.stage) dfree(s);
     dfree(b.offs); dfree(b.stage_cnt);
-    b.stage_chunks = 0;
+    b.stage_segs = 0;
 }
 
 void drop_graph(BatchState& b)
@@ -60,7 +60,7 @@ Sizes sizes(const gasx_ctx* ctx)
     return Sizes{nr * nz, nr * (nz + 1), (nr + 1) * nz};
 }
 
-gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_chunks)
+gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs)
 {
     BatchState& b = ctx->batch;
     const Sizes z = sizes(ctx);
@@ -76,7 +76,7 @@ gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_chunks)
         HIPCHK(dalloc(&b.ez32, n * z.nez), "alloc batch ez32");
         HIPCHK(dalloc(&b.h32, n * z.nh), "alloc batch h32");
         for (auto& qq : b.q)
-            for (auto& q : qq) HIPCHK(dalloc(&q, n * z.nh), "alloc batch current");
+            for (auto& q : qq) HIPCHK(dalloc(&q, n * z.nh * kJCopies), "alloc batch current");
         HIPCHK(dalloc(&b.runs, n), "alloc batch runs");
         HIPCHK(dalloc(&b.fdtd_runs, 4 * n), "alloc batch field runs");
         HIPCHK(dalloc(&b.chunk_start, n + 1), "alloc batch chunks");
@@ -85,19 +85,19 @@ gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_chunks)
         HIPCHK(hipMemset(b.run_counters, 0, n * kNCounters * sizeof(unsigned long long)), "zero counters");
         b.nruns_cap = nruns;
     }
-    if (stage_chunks > b.stage_chunks) {
+    if (stage_segs > b.stage_segs) {
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
         drop_graph(b);
         free_stage_buffers(b);
-        for (auto& s : b.stage) HIPCHK(dalloc(&s, (size_t)stage_chunks * kChunk), "alloc batch stage");
-        HIPCHK(dalloc(&b.offs, (size_t)stage_chunks), "alloc batch offsets");
-        HIPCHK(dalloc(&b.stage_cnt, (size_t)stage_chunks), "alloc batch stage counts");
-        b.stage_chunks = stage_chunks;
+        for (auto& s : b.stage) HIPCHK(dalloc(&s, (size_t)stage_segs * kSeg), "alloc batch stage");
+        HIPCHK(dalloc(&b.offs, (size_t)stage_segs), "alloc batch offsets");
+        HIPCHK(dalloc(&b.stage_cnt, (size_t)stage_segs), "alloc batch stage counts");
+        b.stage_segs = stage_segs;
     }
     return GASX_OK;
 }
 
-gasx_status validate(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o, int64_t* stage_chunks,
+gasx_status validate(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o, int64_t* stage_segs,
                      int64_t* pool_slots)
 {
     if (!b || !o || b->nruns < 1 || b->nruns > kMaxRuns || !b->x || !b->y || !b->z || !b->vx || !b->vy ||
@@ -114,13 +114,13 @@ gasx_status validate(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o,
         if (b->base[r] < 0 || b->base[r] % kChunk != 0 || b->capacity[r] < 1 ||
             b->capacity[r] > (int64_t)INT32_MAX - kChunk || (o->collide && !(b->n_max[r] >= 0)))
             return fail(ctx, GASX_E_ARG, "run " + std::to_string(r) + ": bad base, capacity or n_max");
-        chunks += (b->capacity[r] + kChunk - 1) / kChunk;
+        chunks += (b->capacity[r] + kSeg - 1) / kSeg;
     }
     std::sort(idx.begin(), idx.end(), [&](int a, int c) { return b->base[a] < b->base[c]; });
     for (int k = 1; k < b->nruns; ++k)
         if (b->base[idx[k - 1]] + b->capacity[idx[k - 1]] > b->base[idx[k]])
             return fail(ctx, GASX_E_ARG, "batch runs overlap in the pool");
-    *stage_chunks = chunks;
+    *stage_segs = chunks;
     *pool_slots = b->base[idx.back()] + b->capacity[idx.back()];
     return GASX_OK;
 }
@@ -205,7 +205,7 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
 
 // Enqueues the whole step sequence (capturable: kernels and device copies only).
 gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o,
-                          const gasx_batch_tuning& tune, int64_t stage_chunks)
+                          const gasx_batch_tuning& tune, int64_t stage_segs)
 {
     BatchState& st = ctx->batch;
     const Sizes z = sizes(ctx);
@@ -235,30 +235,37 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
     pa.nruns = n;
     pa.chunk_start = st.chunk_start;
     pa.run_counters = st.run_counters;
-    const int grid = particle_grid(ctx, stage_chunks, tune.particle_blocks_per_cu);
+    // current layout: [copy][run][cell], copy stride = the allocated runs
+    const size_t jstride = (size_t)st.nruns_cap * z.nh;
+    pa.jstride = jstride;
+    const int grid = particle_grid(ctx, stage_segs, tune.particle_blocks_per_cu);
     float* pool[6] = {b->x, b->y, b->z, b->vx, b->vy, b->vz};
     const bool sec = o->collide && o->secondaries;
 
-    HIPCHK(launch_fill_u64(st.q[0][0], (int64_t)(n * z.nh), ctx->stream), "zero q");
-    HIPCHK(launch_fill_u64(st.q[0][1], (int64_t)(n * z.nh), ctx->stream), "zero q");
+    for (int bb = 0; bb < 2; ++bb)
+        for (int k = 0; k < 2; ++k)
+            HIPCHK(launch_fill_u64(st.q[bb][k], (int64_t)(jstride * kJCopies), ctx->stream), "zero q");
     {   // prologue: EM2 half drift + EM7 deposit of every run
         PartArgs p0 = pa;
         p0.qr = st.q[0][0]; p0.qz = st.q[0][1];
-        TimedScope ts(ctx, kTParticle);
-        HIPCHK(launch_particles(kBatch | kDrift | kDeposit, ctx->geom, p0, ca, grid, ctx->stream), "particles");
+        TimedScope ts(ctx, kTDrift);
+        HIPCHK(launch_drift_deposit(true, true, true, ctx->geom, p0, grid, ctx->stream), "drift + deposit");
     }
     for (int s = 0; s < o->n_steps; ++s) {
         const int cur = s & 1, nxt = cur ^ 1;
         const bool last = s == o->n_steps - 1;
         {
             TimedScope ts(ctx, kTFdtd);
+            HIPCHK(launch_j_reduce(st.q[cur][0], st.q[cur][1], (int64_t)(n * z.nh), jstride, ctx->stream),
+                   "j reduce");
             HIPCHK(launch_fdtd(fa, FdtdRun{}, st.fdtd_runs + (size_t)(cur * 2 + (last ? 1 : 0)) * n, n, 1,
                                ctx->stream),
                    "fdtd");
         }
+        // push (+ collide, secondaries staged already drifted), then drift +
+        // deposit of the existing slots, as the single-run loop does
         uint32_t flags = kBatch | kPush;
-        if (o->collide) flags |= kCollide | (sec ? kStageSec : 0u);
-        if (!last) flags |= kDrift | kDeposit;
+        if (o->collide) flags |= kCollide | (sec ? (kStageSec | (last ? 0u : kDriftSec)) : 0u);
         PartArgs ps = pa;
         ps.qr = last ? nullptr : st.q[nxt][0];
         ps.qz = last ? nullptr : st.q[nxt][1];
@@ -266,14 +273,20 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
         cs.step = o->step0 + (uint32_t)s;
         {
             TimedScope ts(ctx, kTParticle);
-            HIPCHK(launch_particles(flags, ctx->geom, ps, cs, grid, ctx->stream), "particles");
+            PartArgs pp = ps;
+            pp.qr = pp.qz = nullptr;
+            HIPCHK(launch_particles(flags, ctx->geom, pp, cs, grid, ctx->stream), "particles");
+        }
+        if (!last) {
+            TimedScope ts(ctx, kTDrift);
+            HIPCHK(launch_drift_deposit(true, true, true, ctx->geom, ps, grid, ctx->stream), "drift + deposit");
         }
         if (sec) {
             TimedScope ts(ctx, kTAppend);
             HIPCHK(launch_append_batch(pool, b->counts, st.runs, n, st.stage, st.stage_cnt, st.chunk_start,
-                                       st.offs, st.tmp_run, (int32_t)stage_chunks, st.run_counters,
+                                       st.offs, st.tmp_run, (int32_t)stage_segs, st.run_counters,
                                        &ctx->d_counters[kCDropSec], ctx->geom, pa.kq, ps.qr, ps.qz,
-                                       ctx->stream),
+                                       jstride, ctx->stream),
                    "append");
         }
     }
@@ -308,17 +321,17 @@ std::vector<unsigned char> graph_key(const gasx_batch* b, const gasx_run_opts* o
 
 gasx_status run_batch(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o, const gasx_batch_tuning& tune)
 {
-    int64_t stage_chunks = 0, pool = 0;
-    gasx_status s = validate(ctx, b, o, &stage_chunks, &pool);
+    int64_t stage_segs = 0, pool = 0;
+    gasx_status s = validate(ctx, b, o, &stage_segs, &pool);
     if (s != GASX_OK) return s;
     if (o->n_steps == 0) return GASX_OK;
-    s = ensure_batch(ctx, b->nruns, stage_chunks);
+    s = ensure_batch(ctx, b->nruns, stage_segs);
     if (s != GASX_OK) return s;
     BatchState& st = ctx->batch;
     if (!o->use_graph) {
         s = upload(ctx, b, o);
         if (s != GASX_OK) return s;
-        return enqueue_steps(ctx, b, o, tune, stage_chunks);
+        return enqueue_steps(ctx, b, o, tune, stage_segs);
     }
     std::vector<unsigned char> key = graph_key(b, o, tune);
     if (!st.exec || key != st.graph_key) {
@@ -337,7 +350,7 @@ gasx_status run_batch(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o, cons
         hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
         hipGraph_t graph = nullptr;
         if (e == hipSuccess) {
-            s = enqueue_steps(ctx, b, o, tune, stage_chunks);
+            s = enqueue_steps(ctx, b, o, tune, stage_segs);
             e = hipStreamEndCapture(ctx->stream, &graph);
         }
         ctx->stream = user;
@@ -410,10 +423,10 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
 {
     if (!ctx || !b || !o || trial_steps < 1) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    int64_t stage_chunks = 0, pool = 0;
-    gasx_status s = validate(ctx, b, o, &stage_chunks, &pool);
+    int64_t stage_segs = 0, pool = 0;
+    gasx_status s = validate(ctx, b, o, &stage_segs, &pool);
     if (s != GASX_OK) return s;
-    s = ensure_batch(ctx, b->nruns, stage_chunks);
+    s = ensure_batch(ctx, b->nruns, stage_segs);
     if (s != GASX_OK) return s;
     const Sizes z = sizes(ctx);
     const size_t n = (size_t)b->nruns;

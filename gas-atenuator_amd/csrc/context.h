@@ -11,16 +11,16 @@
 // Device state of a batched ensemble, sized by the largest batch seen.
 struct BatchState {
     int nruns_cap = 0;            // runs the buffers below hold
-    int64_t stage_chunks = 0;     // staging chunks allocated
+    int64_t stage_segs = 0;       // staging segments allocated
     double *er_b = nullptr, *ez_b = nullptr, *h_b = nullptr;      // FP64 ping-pong partners, run-major
     float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies, run-major
     unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // int64 current, run-major
     gasx::BatchRun* runs = nullptr;             // device descriptors
     gasx::FdtdRun* fdtd_runs = nullptr;         // 4 x nruns: [even|odd] x [clear|last]
-    int32_t* chunk_start = nullptr;             // nruns + 1
+    int32_t* chunk_start = nullptr;             // nruns + 1 segment prefix
     int32_t* tmp_run = nullptr;                 // 2 x nruns
-    int32_t* offs = nullptr;                    // per staging chunk
-    int32_t* stage_cnt = nullptr;               // per staging chunk
+    int32_t* offs = nullptr;                    // per staging segment
+    int32_t* stage_cnt = nullptr;               // per staging segment
     float* stage[6] = {};
     unsigned long long* run_counters = nullptr; // nruns x kNCounters
     // captured step sequence (use_graph), replayed while the key matches
@@ -62,7 +62,7 @@ struct gasx_ctx {
     bool timing = false;
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
-    std::vector<std::pair<size_t, size_t>> tpairs[5];   // particle, fdtd, append, sort, graph
+    std::vector<std::pair<size_t, size_t>> tpairs[6];   // particle, fdtd, append, sort, graph, drift
     // batched ensemble
     BatchState batch;
     gasx_batch_tuning tuning{16, 32, 16};
@@ -70,7 +70,7 @@ struct gasx_ctx {
 
 namespace gasx_host {
 
-enum TimedKind { kTParticle = 0, kTFdtd, kTAppend, kTSort, kTGraph };
+enum TimedKind { kTParticle = 0, kTFdtd, kTAppend, kTSort, kTGraph, kTDrift, kNTimed };
 
 size_t next_event(gasx_ctx* c);
 
@@ -121,8 +121,8 @@ hipError_t dalloc(T** p, size_t n)
 // context's maximum rate coefficient).
 gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, uint64_t seed, uint32_t step,
                       uint32_t run, gasx::CollArgs* ca);
-// Persistent particle-kernel grid for `chunks` chunks at `per_cu` workgroups per CU.
-int particle_grid(const gasx_ctx* ctx, int64_t chunks, int per_cu);
+// Persistent particle-kernel grid for `segments` wave segments at most `per_cu` workgroups per CU.
+int particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu);
 // Frees the batched-ensemble buffers and graph (batch.hip).
 void batch_release(gasx_ctx* ctx);
 

@@ -261,6 +261,33 @@ hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s
     return hipGetLastError();
 }
 
+__global__ void k_j_reduce(unsigned long long* qr, unsigned long long* qz, int64_t n, size_t stride)
+{
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        unsigned long long a = qr[c], b = qz[c];
+#pragma unroll
+        for (int k = 1; k < kJCopies; ++k) {
+            a += qr[k * stride + c];
+            b += qz[k * stride + c];
+            qr[k * stride + c] = 0ull;
+            qz[k * stride + c] = 0ull;
+        }
+        qr[c] = a;
+        qz[c] = b;
+    }
+}
+
+hipError_t launch_j_reduce(unsigned long long* qr, unsigned long long* qz, int64_t ncells, size_t stride,
+                           hipStream_t s)
+{
+    if (ncells <= 0) return hipSuccess;
+    int64_t blocks = (ncells + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_j_reduce, dim3((unsigned)blocks), dim3(256), 0, s, qr, qz, ncells, stride);
+    return hipGetLastError();
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;

@@ -16,9 +16,16 @@ constexpr int kMaxExc = 16;
 #define GASX_KPER 4
 #endif
 constexpr int kBlock = 256;      // particle kernel workgroup size
-constexpr int kPer = GASX_KPER;  // particles per thread per chunk
-constexpr int kChunk = kBlock * kPer;   // slots per workgroup chunk (and per staging chunk)
+constexpr int kWaveSize = 64;
+constexpr int kPer = GASX_KPER;  // consecutive slots per lane
+constexpr int kSeg = kWaveSize * kPer;  // slots per wave segment: the unit of work and of staging
+constexpr int kChunk = kBlock * kPer;   // alignment unit of pooled batch runs (a multiple of kSeg)
 constexpr int kCounterSlots = 8192;  // per-block event counter slots (>= particle kernel grid)
+// Copies of the fixed-point current accumulators: workgroup b adds into copy
+// b % kJCopies, which spreads the atomics on the hot cells of a dense spot
+// over as many addresses; the copies are summed (exact integer adds) before
+// the field cycle reads the current.
+constexpr int kJCopies = 8;
 
 constexpr int kGuide = 32;       // guide entries per energy node of the elastic-angle CDF
 
@@ -82,6 +89,7 @@ enum : uint32_t {
     kStageSec = 16u,   // stage ionization secondaries for append
     kStageIon = 32u,   // stage N2+ ions for append
     kBatch = 64u,      // chunks belong to the runs of a batched ensemble
+    kDriftSec = 128u,  // half drift + BC of staged secondaries (when kDrift runs in a later launch)
 };
 
 constexpr int kMaxRuns = 1024;   // runs per batched ensemble
@@ -119,7 +127,8 @@ struct PartArgs {
     float kq;              // qnum * 2^32 / c (deposit fixed-point scale)
     const float *er, *ez, *h;   // FP32 gather copies
     unsigned long long *qr, *qz;  // int64 fixed-point current (two's complement)
-    // staging for appends (SoA, indexed chunk*kChunk + rank)
+    size_t jstride;               // distance between current copies (0: one copy)
+    // staging for appends (SoA, indexed segment*kSeg + rank)
     float* sec[6];
     int32_t* sec_cnt;
     float* ion[6];
@@ -129,7 +138,7 @@ struct PartArgs {
     // its count is count[r], its fields, current and gas follow run-major
     const BatchRun* runs;
     int32_t nruns;
-    int32_t* chunk_start;          // nruns+1 chunk prefix of the launch (written by block 0)
+    int32_t* chunk_start;          // nruns+1 segment prefix of the launch (written by block 0)
     unsigned long long* run_counters;  // nruns x kNCounters
 };
 
@@ -168,20 +177,25 @@ struct FdtdRun {
 hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
                        int fixed_j, hipStream_t s);
 size_t fdtd_lds_bytes(const FdtdArgs& a);
+// push / collide / stage (kPush, kCollide, kStageSec, kStageIon, kDriftSec, kBatch)
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s);
+// EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk chunks
+hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
+                                int blocks, hipStream_t s);
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
                          int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, hipStream_t s);
+                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s);
 // batched append: per-run scan over the chunks recorded in chunk_start, then copy
 hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,
                                float* const stage[6], const int32_t* stage_cnt,
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
                                unsigned long long* dropped, const Geom& g, float kq,
-                               unsigned long long* qr, unsigned long long* qz, hipStream_t s);
+                               unsigned long long* qr, unsigned long long* qz, size_t jstride,
+                               hipStream_t s);
 hipError_t fdtd_prepare();
 hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx, hipStream_t s);
@@ -189,6 +203,10 @@ hipError_t launch_j_from_fixed(const FdtdArgs& a, const unsigned long long* qr,
                                const unsigned long long* qz, double* jr, double* jz, hipStream_t s);
 hipError_t launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, int64_t n, hipStream_t s);
+// q[c] (c = 0..ncells-1) += sum over copies k = 1..kJCopies-1 of q[k*stride + c],
+// for both components; the added copies are zeroed.
+hipError_t launch_j_reduce(unsigned long long* qr, unsigned long long* qz, int64_t ncells, size_t stride,
+                           hipStream_t s);
 hipError_t launch_tallies(const gasx_params& p, const double* er, const double* ez, const double* h,
                           const gasx_species* sp, int nsp, double* partial, double* out,
                           hipStream_t s);

@@ -4,18 +4,21 @@
 //   kCollide  electron_transport.cu:64-227 (+ find_CS0.m cross sections)
 //   kDrift    EM2.cu:23-57    next half drift + BC  (x^{n+3/2})
 //   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+3/2} with v^{n+1}
-//   kStageSec / kStageIon     electron_collision.m:63-75 append, staged per chunk
-//                             in slot order so the final order is deterministic
+//   kStageSec / kStageIon     electron_collision.m:63-75 append, staged per wave
+//                             segment in slot order so the final order is deterministic
 //
-// A workgroup owns chunks of kChunk consecutive slots (grid-stride over
-// chunks).  Per chunk:
+// Each wave owns segments of kSeg consecutive slots, kPer consecutive slots
+// per lane (vector loads and stores), and grid-strides over segments with no
+// workgroup barrier, so waves hide each other's latency.  Per segment:
 //   A  every lane: load, push, boundary conditions, first Philox block;
-//      lanes with u1 < pmax (the exact rejection bound on p_T) are candidates
-//   B  candidates are compacted into an LDS queue and processed by the first
-//      lanes of the block (cross sections, decision, elastic / ionization /
+//      slots with u1 < pmax (the exact rejection bound on p_T) are candidates
+//   B  the wave's candidates are compacted into its LDS queue and processed
+//      one per lane (cross sections, decision, elastic / ionization /
 //      excitation), so the collision work is dense instead of divergent
-//   C  every lane: read back, next half drift, deposit through a block-local
-//      LDS tile, store, stage secondaries and ions
+//   C  every lane: read back, next half drift, store; the fixed-point current
+//      is summed over runs of equal deposit stencils (per lane, then across
+//      the wave) and added with one global atomic per run, cell and component;
+//      secondaries and ions are staged
 // The slot count is read once at kernel start; appends land in a later
 // kernel (launch_append), so the count is stable inside this kernel.
 #include "device_common.h"
@@ -65,19 +68,6 @@ __device__ __forceinline__ void deposit_cells(const Geom& g, float x, float y, f
     }
 }
 
-// A box that contains the particle's deposit stencil: the stencil cells
-// computed with fast arithmetic, widened by one cell (the exact arithmetic of
-// deposit_cells can land one cell over at a boundary).  tile_add still checks
-// every cell against the box.
-__device__ __forceinline__ void deposit_box(const Geom& g, float x, float y, float z, int& i0,
-                                            int& i1, int& k0, int& k1)
-{
-    const float r = fsqrt(x * x + y * y);
-    const int a = (int)floorf(r * g.inv_dr - 0.5f), b = (int)floorf(z * g.inv_dz - 0.5f);
-    i0 = max(min(a - 1, g.nr - 1), 0); i1 = max(min(a + 2, g.nr - 1), 0);
-    k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
-}
-
 __device__ __forceinline__ int dep_cell(const DepCells& d, int a, int nr)
 {
     return ((a & 1) ? d.i1 : d.i0) + ((a & 2) ? d.k1 : d.k0) * nr;
@@ -92,88 +82,6 @@ __device__ __forceinline__ void deposit_global(const DepCells& d, int nr, unsign
         if (d.qr[a]) atomicAdd(&qr[c], (unsigned long long)d.qr[a]);
         if (d.qz[a]) atomicAdd(&qz[c], (unsigned long long)d.qz[a]);
     }
-}
-
-constexpr int kTile = 512;    // cells of the block-local current tile (2 x 4 KB of LDS)
-
-// Block-local current tile.  When a chunk's deposit stencils fit in a 2-D box
-// of at most kTile cells (cell-sorted particles), contributions are summed in
-// LDS and flushed with one global atomic per non-zero cell; otherwise every
-// contribution goes to global memory directly.
-struct TileBox {
-    int ilo, klo, wi, span;   // span > kTile: direct global atomics; span == 0: nothing to do
-};
-
-// Block-wide bounding box of the stencils [ilo,ihi] x [klo,khi] every thread
-// holds for its particles (ilo > ihi: none).  Every thread must call it.
-__device__ __forceinline__ TileBox tile_box(int ilo, int ihi, int klo, int khi, int* s_red)
-{
-    for (int o = 32; o > 0; o >>= 1) {
-        ilo = min(ilo, __shfl_xor(ilo, o, 64));
-        ihi = max(ihi, __shfl_xor(ihi, o, 64));
-        klo = min(klo, __shfl_xor(klo, o, 64));
-        khi = max(khi, __shfl_xor(khi, o, 64));
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    if (lane == 0) {
-        s_red[wave] = ilo; s_red[nw + wave] = ihi;
-        s_red[2 * nw + wave] = klo; s_red[3 * nw + wave] = khi;
-    }
-    __syncthreads();
-    ilo = 0x7fffffff; ihi = -1; klo = 0x7fffffff; khi = -1;
-    for (int w = 0; w < nw; ++w) {
-        ilo = min(ilo, s_red[w]); ihi = max(ihi, s_red[nw + w]);
-        klo = min(klo, s_red[2 * nw + w]); khi = max(khi, s_red[3 * nw + w]);
-    }
-    __syncthreads();
-    TileBox b;
-    if (ihi < ilo) { b.ilo = b.klo = b.wi = b.span = 0; return b; }
-    b.ilo = ilo; b.klo = klo; b.wi = ihi - ilo + 1;
-    b.span = b.wi * (khi - klo + 1);
-    return b;
-}
-
-__device__ __forceinline__ void tile_zero(const TileBox& b, unsigned long long* s_tr,
-                                          unsigned long long* s_tz)
-{
-    for (int t = threadIdx.x; t < b.span; t += blockDim.x) { s_tr[t] = 0ull; s_tz[t] = 0ull; }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void tile_add(const TileBox& b, const DepCells& d, int nr,
-                                         unsigned long long* s_tr, unsigned long long* s_tz,
-                                         unsigned long long* qr, unsigned long long* qz)
-{
-    if (b.span > kTile) { deposit_global(d, nr, qr, qz); return; }
-    const int nk = b.span / b.wi;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int i = ((a & 1) ? d.i1 : d.i0) - b.ilo, k = ((a & 2) ? d.k1 : d.k0) - b.klo;
-        const unsigned long long vr = (unsigned long long)d.qr[a], vz = (unsigned long long)d.qz[a];
-        if ((unsigned)i < (unsigned)b.wi && (unsigned)k < (unsigned)nk) {
-            const int t = i + k * b.wi;
-            if (vr) atomicAdd(&s_tr[t], vr);
-            if (vz) atomicAdd(&s_tz[t], vz);
-        } else {                                 // outside the box: straight to global
-            const int c = (i + b.ilo) + (k + b.klo) * nr;
-            if (vr) atomicAdd(&qr[c], vr);
-            if (vz) atomicAdd(&qz[c], vz);
-        }
-    }
-}
-
-__device__ __forceinline__ void tile_flush(const TileBox& b, int nr, unsigned long long* s_tr,
-                                           unsigned long long* s_tz, unsigned long long* qr,
-                                           unsigned long long* qz)
-{
-    __syncthreads();
-    for (int t = threadIdx.x; t < b.span; t += blockDim.x) {
-        const unsigned long long a = s_tr[t], c = s_tz[t];
-        const int cell = (b.ilo + t % b.wi) + (b.klo + t / b.wi) * nr;
-        if (a) atomicAdd(&qr[cell], a);
-        if (c) atomicAdd(&qz[cell], c);
-    }
-    __syncthreads();                         // the tile is reused by the next chunk
 }
 
 // ---- gather + push (EM4.cu:36-148) ------------------------------------------
@@ -395,35 +303,42 @@ __device__ void collide_candidate(const Geom& g, const CollArgs& ca, const Small
     }
 }
 
-// Exclusive rank of `pred` among the workgroup's lanes (lane order) and the
-// workgroup total.  Every thread of the block must call it.
-__device__ __forceinline__ int block_rank(bool pred, int* s_wave, int& total)
+// ---- wave-level helpers (no workgroup barriers) ------------------------------
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWaveSize - 1)); }
+
+// Orders this wave's LDS accesses around a cross-lane exchange.
+__device__ __forceinline__ void wave_sync()
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long m = __ballot(pred);
-    const unsigned long long lt = lane == 0 ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    const int within = __popcll(lt);
-    if (lane == 0) s_wave[wave] = __popcll(m);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
-        const int c = s_wave[w];
-        if (w < wave) off += c;
-        tot += c;
-    }
-    __syncthreads();
-    total = tot;
-    return off + within;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kQW = 11;     // LDS queue words per candidate
-constexpr int kQCap = 512;  // candidates per dense round
+// Exclusive prefix (lane order) and wave total of a per-lane count in
+// [0, 2^kBits): one ballot per bit.
+template <int kBits>
+__device__ __forceinline__ int wave_prefix_small(int v, int& total)
+{
+    const int lane = lane_id();
+    const unsigned long long lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < kBits; ++b) {
+        const unsigned long long m = __ballot((v >> b) & 1);
+        pre += __popcll(m & lt) << b;
+        tot += __popcll(m) << b;
+    }
+    total = tot;
+    return pre;
+}
+
+constexpr int kPerBits = kPer <= 1 ? 1 : (kPer <= 3 ? 2 : (kPer <= 7 ? 3 : 4));
 
 // Exclusive prefix over the block of per-thread counts (thread order) and the
-// block total.  Every thread must call it.
+// block total.  Every thread must call it (used once, before the main loop).
 __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
     int incl = v;
     for (int o = 1; o < 64; o <<= 1) {
         const int t = __shfl_up(incl, o, 64);
@@ -442,74 +357,139 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
     return off + incl - v;
 }
 
-// Ranks of kPer predicates in slot order (sub-chunk p major, lane minor) and
-// the total.  Every thread must call it.
-__device__ __forceinline__ int block_rank_slots(const bool (&pred)[kPer], int (&rank)[kPer],
-                                                int* s_w4)
+// ---- block-local current tile ---------------------------------------------------
+// A chunk's deposit stencils are summed in a 2-D LDS box of at most kTile
+// cells and flushed with one global atomic per non-zero cell and component;
+// a stencil cell outside the box (or a box too large) goes to global memory
+// directly.  Integer sums: the result does not depend on the grouping.
+constexpr int kTile = 512;    // cells (2 x 4 KB of LDS)
+
+struct TileBox {
+    int ilo, klo, wi, nk;     // nk == 0: no tile (nothing to deposit, or the box exceeds kTile)
+};
+
+// A box that contains the particle's deposit stencil: the stencil cells
+// computed with fast arithmetic, widened by one cell (the exact arithmetic of
+// deposit_cells can land one cell over at a boundary).
+__device__ __forceinline__ void deposit_box(const Geom& g, float x, float y, float z, int& i0, int& i1,
+                                            int& k0, int& k1)
 {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const unsigned long long lt = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-#pragma unroll
-    for (int p = 0; p < kPer; ++p) {
-        const unsigned long long m = __ballot(pred[p]);
-        rank[p] = __popcll(m & lt);
-        if (lane == 0) s_w4[p * nw + wave] = __popcll(m);
-    }
-    __syncthreads();
-    int base = 0;
-#pragma unroll
-    for (int p = 0; p < kPer; ++p) {
-        int off = 0, tot = 0;
-        for (int w = 0; w < nw; ++w) {
-            const int c = s_w4[p * nw + w];
-            if (w < wave) off += c;
-            tot += c;
-        }
-        rank[p] += base + off;
-        base += tot;
-    }
-    __syncthreads();
-    return base;
+    const float r = fsqrt(x * x + y * y);
+    const int a = (int)floorf(r * g.inv_dr - 0.5f), b = (int)floorf(z * g.inv_dz - 0.5f);
+    i0 = max(min(a - 1, g.nr - 1), 0); i1 = max(min(a + 2, g.nr - 1), 0);
+    k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
 }
 
-// Block totals of the per-thread event counts, added to the block's own slot
-// (plain add: a slot belongs to one block index and launches on one stream do
-// not overlap) and, when dst_run is set, atomically to a shared per-run slot.
-// Every thread must call it.
-__device__ __forceinline__ void block_counts_add(const unsigned (&v)[kNCounters],
-                                                 unsigned long long* dst_block,
-                                                 unsigned long long* dst_run)
+// Block-wide bounding box of [ilo,ihi] x [klo,khi] (ilo > ihi: none).  Every
+// thread must call it.
+__device__ __forceinline__ TileBox tile_box(int ilo, int ihi, int klo, int khi, int* s_red)
 {
-    __shared__ unsigned s_cnt[kNCounters][kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        ilo = min(ilo, __shfl_xor(ilo, o, 64));
+        ihi = max(ihi, __shfl_xor(ihi, o, 64));
+        klo = min(klo, __shfl_xor(klo, o, 64));
+        khi = max(khi, __shfl_xor(khi, o, 64));
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (lane == 0) {
+        s_red[wave] = ilo; s_red[nw + wave] = ihi;
+        s_red[2 * nw + wave] = klo; s_red[3 * nw + wave] = khi;
+    }
+    __syncthreads();
+    ilo = 0x7fffffff; ihi = -1; klo = 0x7fffffff; khi = -1;
+    for (int w = 0; w < nw; ++w) {
+        ilo = min(ilo, s_red[w]); ihi = max(ihi, s_red[nw + w]);
+        klo = min(klo, s_red[2 * nw + w]); khi = max(khi, s_red[3 * nw + w]);
+    }
+    __syncthreads();
+    TileBox b{0, 0, 0, 0};
+    if (ihi >= ilo && (ihi - ilo + 1) * (khi - klo + 1) <= kTile) {
+        b.ilo = ilo; b.klo = klo; b.wi = ihi - ilo + 1; b.nk = khi - klo + 1;
+    }
+    return b;
+}
+
+// The summed contributions of a run of particles with one stencil (first cell
+// (i0, k0), di / dk = 1 when the stencil spans two cells in r / z).
+struct DepRun {
+    int i0, k0, di, dk;       // i0 < 0: empty
+    long long q[8];           // qr of the 4 cells, then qz
+};
+
+__device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int nr, unsigned long long* s_tr,
+                                        unsigned long long* s_tz, unsigned long long* qr,
+                                        unsigned long long* qz)
+{
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int i = d.i0 + ((a & 1) ? d.di : 0), k = d.k0 + ((a & 2) ? d.dk : 0);
+        const unsigned long long vr = (unsigned long long)d.q[a], vz = (unsigned long long)d.q[4 + a];
+        const int ti = i - b.ilo, tk = k - b.klo;
+        if ((unsigned)ti < (unsigned)b.wi && (unsigned)tk < (unsigned)b.nk) {
+            const int t = ti + tk * b.wi;
+            if (vr) atomicAdd(&s_tr[t], vr);
+            if (vz) atomicAdd(&s_tz[t], vz);
+        } else {                                 // outside the box: straight to global
+            const int c = i + k * nr;
+            if (vr) atomicAdd(&qr[c], vr);
+            if (vz) atomicAdd(&qz[c], vz);
+        }
+    }
+}
+
+constexpr int kQW = 11;     // LDS queue words per candidate
+constexpr int kQCapW = 64;  // candidates per wave round (one per lane)
+
+// Event counts of a wave added to the block's slot and, for an ensemble, to
+// the run's slot (lane 0 adds; several waves share the block slot).
+__device__ __forceinline__ void wave_counts_add(unsigned (&v)[kNCounters], unsigned long long* dst_block,
+                                                unsigned long long* dst_run)
+{
 #pragma unroll
     for (int k = 0; k < kNCounters; ++k) {
         unsigned t = v[k];
         for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-        if (lane == 0) s_cnt[k][wave] = t;
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < kNCounters) {
-        unsigned long long s = 0;
-        for (int w = 0; w < kBlock / 64; ++w) s += s_cnt[threadIdx.x][w];
-        if (s) {
-            dst_block[threadIdx.x] += s;
-            if (dst_run) atomicAdd(&dst_run[threadIdx.x], s);
+        if (lane_id() == 0 && t) {
+            atomicAdd(&dst_block[k], (unsigned long long)t);
+            if (dst_run) atomicAdd(&dst_run[k], (unsigned long long)t);
         }
+        v[k] = 0;
     }
-    __syncthreads();
 }
 
+template <int N> struct VecF;
+template <> struct VecF<1> { using T = float; };
+template <> struct VecF<2> { using T = float2; };
+template <> struct VecF<4> { using T = float4; };
+
+__device__ __forceinline__ void vload(const float* p, float (&v)[kPer])
+{
+    using T = typename VecF<kPer>::T;
+    const T t = *reinterpret_cast<const T*>(p);
+    static_assert(sizeof(T) == kPer * sizeof(float), "vector width");
+    __builtin_memcpy(v, &t, sizeof(T));
+}
+
+__device__ __forceinline__ void vstore(float* p, const float (&v)[kPer])
+{
+    using T = typename VecF<kPer>::T;
+    T t;
+    __builtin_memcpy(&t, v, sizeof(T));
+    *reinterpret_cast<T*>(p) = t;
+}
+
+// Push and collision pass.  Waves work independently: wave segments of kSeg
+// consecutive slots (lane l owns slots seg*kSeg + l*kPer .. + kPer-1),
+// grid-stride over segments; no workgroup barrier inside the loop.
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, CollArgs ca0)
 {
-    constexpr bool kDep = (F & kDeposit) != 0;
+    static_assert((F & (kDrift | kDeposit)) == 0, "drift and deposit run in k_drift_deposit");
     constexpr bool kCol = (F & kCollide) != 0;
     constexpr bool kBat = (F & kBatch) != 0;
-    __shared__ int s_wave[kBlock / 64];
-    __shared__ int s_red[4 * kPer * (kBlock / 64)];
-    __shared__ unsigned long long s_tr[kDep ? kTile : 1], s_tz[kDep ? kTile : 1];
-    __shared__ float s_q[kCol ? kQW : 1][kCol ? kQCap : 1];
+    constexpr int kWaves = kBlock / kWaveSize;
+    __shared__ int s_wave[kWaves];
+    __shared__ float s_q[kCol ? kWaves : 1][kCol ? kQW : 1][kCol ? kQCapW : 1];
     __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
     static_assert(sizeof(SmallTabs) % 4 == 0, "SmallTabs is copied in 32-bit words");
     __shared__ uint32_t s_tab_raw[kCol ? sizeof(SmallTabs) / 4 : 1];
@@ -520,16 +500,16 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
         __syncthreads();
     }
 
-    int total_chunks;
+    int total_segs;
     if constexpr (kBat) {
-        // chunk prefix over the runs (every block computes it; block 0 records
+        // segment prefix over the runs (every block computes it; block 0 records
         // it for the append kernels, which must see the counts of this launch)
         constexpr int kRPer = kMaxRuns / kBlock;
         int loc[kRPer], sum = 0;
 #pragma unroll
         for (int j = 0; j < kRPer; ++j) {
             const int r = (int)threadIdx.x * kRPer + j;
-            loc[j] = r < pa0.nruns ? (pa0.count[r] + kChunk - 1) / kChunk : 0;
+            loc[j] = r < pa0.nruns ? (pa0.count[r] + kSeg - 1) / kSeg : 0;
             sum += loc[j];
         }
         int tot;
@@ -544,26 +524,34 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
         __syncthreads();
         if (blockIdx.x == 0)
             for (int r = threadIdx.x; r <= pa0.nruns; r += blockDim.x) pa0.chunk_start[r] = s_cs[r];
-        total_chunks = tot;
+        total_segs = tot;
     } else {
-        total_chunks = (*pa0.count + kChunk - 1) / kChunk;
+        total_segs = (*pa0.count + kSeg - 1) / kSeg;
     }
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
     const int nr_ = g.nr, nz_ = g.nz;
-    unsigned c_el = 0, c_ion = 0, c_exc = 0, c_viol = 0, c_sat = 0, c_abs = 0, c_push = 0;
-    for (int gchunk = blockIdx.x; gchunk < total_chunks; gchunk += gridDim.x) {
-        // chunk: index within the population (slots, Philox counters);
-        // gchunk: index of the staging area
+    constexpr uintptr_t kAlign = sizeof(typename VecF<kPer>::T) - 1;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(pa0.x) | reinterpret_cast<uintptr_t>(pa0.y) |
+                           reinterpret_cast<uintptr_t>(pa0.z) | reinterpret_cast<uintptr_t>(pa0.vx) |
+                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz)) &
+                          kAlign) == 0;
+    unsigned cnt[kNCounters] = {};
+    int cnt_run = -1;
+    const int n_waves = gridDim.x * kWaves;
+    for (int gseg = blockIdx.x * kWaves + wave; gseg < total_segs; gseg += n_waves) {
+        // seg: index within the population (slots, Philox counters);
+        // gseg: index of the staging area
         PartArgs pa = pa0;
         CollArgs ca = ca0;
-        int chunk = gchunk, n, run = 0;
+        int seg = gseg, n, run = 0;
         if constexpr (kBat) {
-            int lo = 0, hi = pa0.nruns;            // s_cs[lo] <= gchunk < s_cs[hi]
+            int lo = 0, hi = pa0.nruns;            // s_cs[lo] <= gseg < s_cs[hi]
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
-                if (s_cs[mid] <= gchunk) lo = mid; else hi = mid;
+                if (s_cs[mid] <= gseg) lo = mid; else hi = mid;
             }
             run = lo;
-            chunk = gchunk - s_cs[lo];
+            seg = gseg - s_cs[lo];
             const BatchRun br = pa0.runs[run];
             pa.x += br.base; pa.y += br.base; pa.z += br.base;
             pa.vx += br.base; pa.vy += br.base; pa.vz += br.base;
@@ -572,36 +560,47 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
             pa.er += (size_t)run * (nh + nr_);
             pa.ez += (size_t)run * (nh + nz_);
             pa.h += (size_t)run * nh;
-            if (pa.qr) { pa.qr += (size_t)run * nh; pa.qz += (size_t)run * nh; }
             const size_t gc = (size_t)ca0.gnr * ca0.gnz;
             if (ca.dens) ca.dens += (size_t)run * gc;
             if (ca.temp) ca.temp += (size_t)run * gc;
             ca.pmax = br.pmax;
             ca.run = br.run;
-            c_el = c_ion = c_exc = c_viol = c_sat = c_abs = c_push = 0;
+            if (run != cnt_run) {
+                if (cnt_run >= 0 && pa0.counters)
+                    wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
+                                    pa0.run_counters + (size_t)cnt_run * kNCounters);
+                cnt_run = run;
+            }
         } else {
             n = *pa0.count;
         }
-        // slot of sub-chunk p: chunk*kChunk + p*kBlock + lane (coalesced per p)
+        const int slot0 = seg * kSeg + lane * kPer;
+        const bool vec = aligned && slot0 + kPer <= n;
         float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
         bool alive[kPer], was_alive[kPer], absorbed[kPer], cand[kPer];
         int ev[kPer];
         float svx[kPer], svy[kPer], svz[kPer];
         float ivx[kPer], ivy[kPer], ivz[kPer];
         float u1[kPer], u2[kPer], u3[kPer], u4[kPer];
+        if (vec) {
+            vload(pa.x + slot0, x); vload(pa.y + slot0, y); vload(pa.z + slot0, z);
+            vload(pa.vx + slot0, vx); vload(pa.vy + slot0, vy); vload(pa.vz + slot0, vz);
+        } else {
 #pragma unroll
-        for (int p = 0; p < kPer; ++p) {
-            const int slot = chunk * kChunk + p * kBlock + (int)threadIdx.x;
-            x[p] = y[p] = z[p] = vx[p] = vy[p] = vz[p] = 0.f;
-            alive[p] = false;
-            if (slot < n) {
-                z[p] = pa.z[slot];
-                alive[p] = !is_dead(z[p]);
-                if (alive[p]) {
-                    x[p] = pa.x[slot]; y[p] = pa.y[slot];
+            for (int p = 0; p < kPer; ++p) {
+                const int slot = slot0 + p;
+                x[p] = y[p] = vx[p] = vy[p] = vz[p] = 0.f;
+                z[p] = __int_as_float(0x7fc00000);
+                if (slot < n) {
+                    x[p] = pa.x[slot]; y[p] = pa.y[slot]; z[p] = pa.z[slot];
                     vx[p] = pa.vx[slot]; vy[p] = pa.vy[slot]; vz[p] = pa.vz[slot];
                 }
             }
+        }
+#pragma unroll
+        for (int p = 0; p < kPer; ++p) {
+            const int slot = slot0 + p;
+            alive[p] = slot < n && !is_dead(z[p]);
             was_alive[p] = alive[p];
             absorbed[p] = false;
             ev[p] = 0;
@@ -620,168 +619,337 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                 cand[p] = u1[p] < ca.pmax;     // exact: p_T <= pmax for every particle
             }
         }
-        // ---- B: collisions of the candidates, densely, in rounds of kQCap -------
-        if (kCol) {
+        // ---- B: the wave's collision candidates, one per lane per round ----------
+        if constexpr (kCol) {
             int mine = 0;
 #pragma unroll
             for (int p = 0; p < kPer; ++p) mine += cand[p];
             int total;
-            const int off = block_excl_scan(mine, s_wave, total);
-            for (int base = 0; base < total; base += kQCap) {
+            const int off = wave_prefix_small<kPerBits>(mine, total);
+            float (*q_)[kQCapW] = s_q[wave];
+            for (int base = 0; base < total; base += kQCapW) {
                 int pos = off;
 #pragma unroll
                 for (int p = 0; p < kPer; ++p) {
                     if (cand[p]) {
                         const int q = pos - base;
-                        if (q >= 0 && q < kQCap) {
-                            s_q[0][q] = x[p]; s_q[1][q] = y[p]; s_q[2][q] = z[p];
-                            s_q[3][q] = vx[p]; s_q[4][q] = vy[p]; s_q[5][q] = vz[p];
-                            s_q[6][q] = u1[p]; s_q[7][q] = u2[p]; s_q[8][q] = u3[p]; s_q[9][q] = u4[p];
-                            s_q[10][q] = __int_as_float(chunk * kChunk + p * kBlock + (int)threadIdx.x);
+                        if (q >= 0 && q < kQCapW) {
+                            q_[0][q] = x[p]; q_[1][q] = y[p]; q_[2][q] = z[p];
+                            q_[3][q] = vx[p]; q_[4][q] = vy[p]; q_[5][q] = vz[p];
+                            q_[6][q] = u1[p]; q_[7][q] = u2[p]; q_[8][q] = u3[p]; q_[9][q] = u4[p];
+                            q_[10][q] = __int_as_float(slot0 + p);
                         }
                         ++pos;
                     }
                 }
-                __syncthreads();
-                const int nq = min(kQCap, total - base);
-                for (int q = threadIdx.x; q < nq; q += blockDim.x) {
-                    float qvx = s_q[3][q], qvy = s_q[4][q], qvz = s_q[5][q];
+                wave_sync();
+                if (lane < min(kQCapW, total - base)) {
+                    const int q = lane;
+                    float qvx = q_[3][q], qvy = q_[4][q], qvz = q_[5][q];
                     CollOut co;
-                    collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(s_q[10][q]), s_q[0][q],
-                                      s_q[1][q], s_q[2][q], qvx, qvy, qvz, s_q[6][q], s_q[7][q],
-                                      s_q[8][q], s_q[9][q], (F & kStageIon) != 0, co);
-                    s_q[3][q] = qvx; s_q[4][q] = qvy; s_q[5][q] = qvz;
-                    s_q[10][q] = __int_as_float(co.ev);
-                    if (co.ev & kEvSec) { s_q[6][q] = co.svx; s_q[7][q] = co.svy; s_q[8][q] = co.svz; }
-                    if (co.ev & kEvIon) { s_q[0][q] = co.ivx; s_q[1][q] = co.ivy; s_q[2][q] = co.ivz; }
+                    collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(q_[10][q]), q_[0][q], q_[1][q],
+                                      q_[2][q], qvx, qvy, qvz, q_[6][q], q_[7][q], q_[8][q], q_[9][q],
+                                      (F & kStageIon) != 0, co);
+                    q_[3][q] = qvx; q_[4][q] = qvy; q_[5][q] = qvz;
+                    q_[10][q] = __int_as_float(co.ev);
+                    if (co.ev & kEvSec) { q_[6][q] = co.svx; q_[7][q] = co.svy; q_[8][q] = co.svz; }
+                    if (co.ev & kEvIon) { q_[0][q] = co.ivx; q_[1][q] = co.ivy; q_[2][q] = co.ivz; }
                 }
-                __syncthreads();
+                wave_sync();
                 pos = off;
 #pragma unroll
                 for (int p = 0; p < kPer; ++p) {
                     if (cand[p]) {
                         const int q = pos - base;
-                        if (q >= 0 && q < kQCap) {
-                            vx[p] = s_q[3][q]; vy[p] = s_q[4][q]; vz[p] = s_q[5][q];
-                            ev[p] = __float_as_int(s_q[10][q]);
-                            if (ev[p] & kEvSec) { svx[p] = s_q[6][q]; svy[p] = s_q[7][q]; svz[p] = s_q[8][q]; }
-                            if (ev[p] & kEvIon) { ivx[p] = s_q[0][q]; ivy[p] = s_q[1][q]; ivz[p] = s_q[2][q]; }
+                        if (q >= 0 && q < kQCapW) {
+                            vx[p] = q_[3][q]; vy[p] = q_[4][q]; vz[p] = q_[5][q];
+                            ev[p] = __float_as_int(q_[10][q]);
+                            if (ev[p] & kEvSec) { svx[p] = q_[6][q]; svy[p] = q_[7][q]; svz[p] = q_[8][q]; }
+                            if (ev[p] & kEvIon) { ivx[p] = q_[0][q]; ivy[p] = q_[1][q]; ivz[p] = q_[2][q]; }
                         }
                         ++pos;
                     }
                 }
-                __syncthreads();                 // s_q is rewritten by the next round / chunk
+                wave_sync();                     // the queue is rewritten by the next round
             }
         }
-        // ---- C: next half drift, store ------------------------------------------
+        // ---- C: store; secondaries get their half drift (the primaries' is a
+        // later launch, k_drift_deposit) ------------------------------------------
         bool has_sec[kPer], has_ion[kPer];
         float sx[kPer], sy[kPer], sz[kPer];   // secondary position (drifted with it)
-        float cx[kPer], cy[kPer], cz[kPer];   // collision site (ions are placed there)
-        int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
 #pragma unroll
         for (int p = 0; p < kPer; ++p) {
-            const int slot = chunk * kChunk + p * kBlock + (int)threadIdx.x;
             has_sec[p] = (F & kStageSec) && (ev[p] & kEvSec);
             has_ion[p] = (F & kStageIon) && (ev[p] & kEvIon);
-            cx[p] = sx[p] = x[p]; cy[p] = sy[p] = y[p]; cz[p] = sz[p] = z[p];
-            if (F & kDrift) {
-                if (alive[p]) {
-                    half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
-                    if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) { alive[p] = false; absorbed[p] = true; }
-                }
-                if (has_sec[p]) {
-                    half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
-                    if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) { has_sec[p] = false; ++c_abs; }
-                }
+            sx[p] = x[p]; sy[p] = y[p]; sz[p] = z[p];
+            if ((F & kDriftSec) && has_sec[p]) {
+                half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
+                if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) { has_sec[p] = false; ++cnt[kCAbsorbed]; }
             }
-            if (slot < n && was_alive[p]) {
-                if (alive[p]) {
-                    pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
-                    pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
-                } else {
-                    pa.z[slot] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
-                }
-            }
-            if (kDep && alive[p]) {
-                int i0, i1, k0, k1;
-                deposit_box(g, x[p], y[p], z[p], i0, i1, k0, k1);
-                ilo = min(ilo, i0); ihi = max(ihi, i1); klo = min(klo, k0); khi = max(khi, k1);
-            }
+            if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
         }
-        // ---- deposit through the block-local tile ---------------------------------
-        // secondaries deposit in the append kernel, once their slot is known
-        // (a secondary dropped at capacity must not carry current)
-        if (kDep) {
-            const TileBox bx = tile_box(ilo, ihi, klo, khi, s_red);
-            const bool use_tile = bx.span > 0 && bx.span <= kTile;
-            if (use_tile) tile_zero(bx, s_tr, s_tz);
-            if (bx.span > 0) {
+        if (vec) {   // dead slots are written back unchanged
+            vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
+            vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
+        } else {
 #pragma unroll
-                for (int p = 0; p < kPer; ++p) {
+            for (int p = 0; p < kPer; ++p) {
+                const int slot = slot0 + p;
+                if (slot < n && was_alive[p]) {
                     if (alive[p]) {
-                        DepCells dc;
-                        deposit_cells(g, x[p], y[p], z[p], vx[p], vy[p], vz[p], pa.kq, dc);
-                        tile_add(bx, dc, g.nr, s_tr, s_tz, pa.qr, pa.qz);
+                        pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
+                        pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
+                    } else {
+                        pa.z[slot] = z[p];
                     }
                 }
             }
-            if (use_tile) tile_flush(bx, g.nr, s_tr, s_tz, pa.qr, pa.qz);
         }
-        // ---- staging in slot order -------------------------------------------------
+        // ---- staging in slot order (lane major, then p) ------------------------
         if (F & kStageSec) {
-            int rank[kPer];
-            const int tot = block_rank_slots(has_sec, rank, s_red);
+            int mine = 0;
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) mine += has_sec[p];
+            int tot;
+            int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
                 if (has_sec[p]) {
-                    const int dst = gchunk * kChunk + rank[p];
                     pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
                     pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
+                    ++dst;
                 }
             }
-            if (threadIdx.x == 0) pa.sec_cnt[gchunk] = tot;
+            if (lane == 0) pa.sec_cnt[gseg] = tot;
         }
         if (F & kStageIon) {
-            int rank[kPer];
-            const int tot = block_rank_slots(has_ion, rank, s_red);
+            int mine = 0;
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) mine += has_ion[p];
+            int tot;
+            int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
                 if (has_ion[p]) {
-                    const int dst = gchunk * kChunk + rank[p];
                     // ion at the ionization site (electron_collision.m:63)
-                    pa.ion[0][dst] = cx[p]; pa.ion[1][dst] = cy[p]; pa.ion[2][dst] = cz[p];
+                    pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
                     pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
+                    ++dst;
                 }
             }
-            if (threadIdx.x == 0) pa.ion_cnt[gchunk] = tot;
+            if (lane == 0) pa.ion_cnt[gseg] = tot;
         }
 #pragma unroll
         for (int p = 0; p < kPer; ++p) {
             if (kCol) {
                 const int typ = ev[p] & kEvMask;
-                c_el += typ == 1;
-                c_ion += typ == 2;
-                c_exc += typ == 3;
-                c_viol += (ev[p] & kEvViol) != 0;
-                c_sat += (ev[p] & kEvSat) != 0;
+                cnt[kCElastic] += typ == 1;
+                cnt[kCIonize] += typ == 2;
+                cnt[kCExcite] += typ == 3;
+                cnt[kCViol] += (ev[p] & kEvViol) != 0;
+                cnt[kCSat] += (ev[p] & kEvSat) != 0;
             }
-            c_abs += (unsigned)absorbed[p];
-            if (F & kPush) c_push += was_alive[p];
-        }
-        if constexpr (kBat) {   // per run: one atomic per counter and chunk
-            if (pa0.counters) {
-                const unsigned v[kNCounters] = {c_el, c_ion, c_exc, 0u, 0u, c_viol, c_abs, c_sat, c_push};
-                block_counts_add(v, pa0.counters + (size_t)blockIdx.x * kNCounters,
-                                 pa0.run_counters + (size_t)run * kNCounters);
-            }
+            cnt[kCAbsorbed] += (unsigned)absorbed[p];
+            if (F & kPush) cnt[kCPushed] += was_alive[p];
         }
     }
-    // one non-atomic update of this block's counter slot (launches on one
-    // stream never overlap, and a slot belongs to one block index)
-    if (!kBat && pa0.counters) {
-        const unsigned v[kNCounters] = {c_el, c_ion, c_exc, 0u, 0u, c_viol, c_abs, c_sat, c_push};
-        block_counts_add(v, pa0.counters + (size_t)blockIdx.x * kNCounters, nullptr);
+    if (pa0.counters && (!kBat || cnt_run >= 0))
+        wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
+                        kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
+}
+
+// ---- half drift + deposit (EM2.cu:23-57, EM7.cu:51-202) ----------------------
+// Workgroups take chunks of kChunk consecutive slots (thread t owns slots
+// chunk*kChunk + t*kPer .. + kPer-1, vector loads and stores).  Each thread
+// sums the contributions of its consecutive particles while they share a
+// stencil (cell-sorted slots mostly do), adds each run into the chunk's LDS
+// tile, and the tile is flushed into the workgroup's copy of the current.
+template <bool kDr, bool kDe, bool kBat>
+__global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
+{
+    constexpr int kWaves = kBlock / kWaveSize;
+    __shared__ int s_wave[kWaves];
+    __shared__ int s_red[4 * kWaves];
+    __shared__ unsigned long long s_tr[kDe ? kTile : 1], s_tz[kDe ? kTile : 1];
+    __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
+    int total_chunks;
+    if constexpr (kBat) {   // chunk prefix over the runs (run bases are multiples of kChunk)
+        constexpr int kRPer = kMaxRuns / kBlock;
+        int loc[kRPer], sum = 0;
+#pragma unroll
+        for (int j = 0; j < kRPer; ++j) {
+            const int r = (int)threadIdx.x * kRPer + j;
+            loc[j] = r < pa0.nruns ? (pa0.count[r] + kChunk - 1) / kChunk : 0;
+            sum += loc[j];
+        }
+        int tot;
+        int off = block_excl_scan(sum, s_wave, tot);
+#pragma unroll
+        for (int j = 0; j < kRPer; ++j) {
+            const int r = (int)threadIdx.x * kRPer + j;
+            if (r < pa0.nruns) s_cs[r] = off;
+            off += loc[j];
+        }
+        if (threadIdx.x == 0) s_cs[pa0.nruns] = tot;
+        __syncthreads();
+        total_chunks = tot;
+    } else {
+        total_chunks = (*pa0.count + kChunk - 1) / kChunk;
     }
+    const int nr_ = g.nr, nz_ = g.nz;
+    constexpr uintptr_t kAlign = sizeof(typename VecF<kPer>::T) - 1;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(pa0.x) | reinterpret_cast<uintptr_t>(pa0.y) |
+                           reinterpret_cast<uintptr_t>(pa0.z) | reinterpret_cast<uintptr_t>(pa0.vx) |
+                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz)) &
+                          kAlign) == 0;
+    unsigned cnt[kNCounters] = {};
+    int cnt_run = -1;
+    for (int chunk = blockIdx.x; chunk < total_chunks; chunk += gridDim.x) {   // workgroup-uniform
+        PartArgs pa = pa0;
+        int lchunk = chunk, n, run = 0;
+        if constexpr (kBat) {
+            int lo = 0, hi = pa0.nruns;            // s_cs[lo] <= chunk < s_cs[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_cs[mid] <= chunk) lo = mid; else hi = mid;
+            }
+            run = lo;
+            lchunk = chunk - s_cs[lo];
+            const int64_t base = pa0.runs[run].base;
+            pa.x += base; pa.y += base; pa.z += base;
+            pa.vx += base; pa.vy += base; pa.vz += base;
+            n = pa0.count[run];
+            if (pa.qr) {
+                const size_t nh = (size_t)nr_ * nz_;
+                pa.qr += (size_t)run * nh;
+                pa.qz += (size_t)run * nh;
+            }
+            if (run != cnt_run) {
+                if (cnt_run >= 0 && pa0.counters)
+                    wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
+                                    pa0.run_counters + (size_t)cnt_run * kNCounters);
+                cnt_run = run;
+            }
+        } else {
+            n = *pa0.count;
+        }
+        if (kDe && pa0.jstride) {                  // this workgroup's copy of the current
+            const size_t off = (size_t)(blockIdx.x % kJCopies) * pa0.jstride;
+            pa.qr += off;
+            pa.qz += off;
+        }
+        const int slot0 = lchunk * kChunk + (int)threadIdx.x * kPer;
+        const bool vec = aligned && slot0 + kPer <= n;
+        float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
+        bool alive[kPer], was_alive[kPer];
+        if (vec) {
+            vload(pa.x + slot0, x); vload(pa.y + slot0, y); vload(pa.z + slot0, z);
+            vload(pa.vx + slot0, vx); vload(pa.vy + slot0, vy); vload(pa.vz + slot0, vz);
+        } else {
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) {
+                const int slot = slot0 + p;
+                x[p] = y[p] = vx[p] = vy[p] = vz[p] = 0.f;
+                z[p] = __int_as_float(0x7fc00000);
+                if (slot < n) {
+                    x[p] = pa.x[slot]; y[p] = pa.y[slot]; z[p] = pa.z[slot];
+                    vx[p] = pa.vx[slot]; vy[p] = pa.vy[slot]; vz[p] = pa.vz[slot];
+                }
+            }
+        }
+        int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
+#pragma unroll
+        for (int p = 0; p < kPer; ++p) {
+            alive[p] = slot0 + p < n && !is_dead(z[p]);
+            was_alive[p] = alive[p];
+            if (kDr && alive[p]) {
+                half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+                    alive[p] = false;
+                    z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+                    ++cnt[kCAbsorbed];
+                }
+            }
+            if (kDe && alive[p]) {
+                int i0, i1, k0, k1;
+                deposit_box(g, x[p], y[p], z[p], i0, i1, k0, k1);
+                ilo = min(ilo, i0); ihi = max(ihi, i1); klo = min(klo, k0); khi = max(khi, k1);
+            }
+        }
+        if (kDr) {
+            if (vec) {   // dead slots are written back unchanged
+                vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
+                vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
+            } else {
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    const int slot = slot0 + p;
+                    if (slot < n && was_alive[p]) {
+                        if (alive[p]) {
+                            pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
+                            pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
+                        } else {
+                            pa.z[slot] = z[p];
+                        }
+                    }
+                }
+            }
+        }
+        if constexpr (kDe) {
+            const TileBox bx = tile_box(ilo, ihi, klo, khi, s_red);
+            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) { s_tr[t] = 0ull; s_tz[t] = 0ull; }
+            __syncthreads();
+            DepRun run_ = {-1, 0, 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
+#pragma unroll
+            for (int p = 0; p < kPer; ++p) {
+                if (alive[p]) {
+                    DepCells dc;
+                    deposit_cells(g, x[p], y[p], z[p], vx[p], vy[p], vz[p], pa.kq, dc);
+                    const int di = dc.i1 - dc.i0, dk = dc.k1 - dc.k0;
+                    if (dc.i0 != run_.i0 || dc.k0 != run_.k0 || di != run_.di || dk != run_.dk) {
+                        if (run_.i0 >= 0) run_add(bx, run_, nr_, s_tr, s_tz, pa.qr, pa.qz);
+                        run_.i0 = dc.i0; run_.k0 = dc.k0; run_.di = di; run_.dk = dk;
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) { run_.q[a] = dc.qr[a]; run_.q[4 + a] = dc.qz[a]; }
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 4; ++a) { run_.q[a] += dc.qr[a]; run_.q[4 + a] += dc.qz[a]; }
+                    }
+                }
+            }
+            if (run_.i0 >= 0) run_add(bx, run_, nr_, s_tr, s_tz, pa.qr, pa.qz);
+            __syncthreads();
+            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) {
+                const unsigned long long a = s_tr[t], c = s_tz[t];
+                const int cell = (bx.ilo + t % bx.wi) + (bx.klo + t / bx.wi) * nr_;
+                if (a) atomicAdd(&pa.qr[cell], a);
+                if (c) atomicAdd(&pa.qz[cell], c);
+            }
+            __syncthreads();                       // the tile is reused by the next chunk
+        }
+    }
+    if (pa0.counters && (!kBat || cnt_run >= 0))
+        wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
+                        kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
+}
+
+hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
+                                int blocks, hipStream_t s)
+{
+    blocks = blocks > 0 ? blocks : 1;
+    if (batch) {
+        if (!drift || !deposit) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+    } else if (drift && deposit) {
+        hipLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+    } else if (drift) {
+        hipLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+    } else if (deposit) {
+        hipLaunchKernelGGL((k_drift_deposit<false, true, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // ---- append: exclusive scan of per-chunk counts, then copy ------------------
@@ -797,7 +965,7 @@ __global__ __launch_bounds__(1024) void k_append_scan(const int32_t* src_count, 
     __shared__ int s_part[1024];
     const int ns = *src_count;
     const int n = *dst_count;
-    int nch = (ns + kChunk - 1) / kChunk;
+    int nch = (ns + kSeg - 1) / kSeg;
     if (nch > max_chunks) nch = max_chunks;
     const int per = (nch + blockDim.x - 1) / blockDim.x;
     const int beg = threadIdx.x * per;
@@ -837,15 +1005,19 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
                                                         const int32_t* stage_cnt,
                                                         const int32_t* offs, const int32_t* tmp2,
                                                         Geom g, float kq, unsigned long long* qr,
-                                                        unsigned long long* qz)
+                                                        unsigned long long* qz, size_t jstride)
 {
     const int base = tmp2[0];
     const int nch = tmp2[1];
+    if (qr) {                                      // this workgroup's copy of the current
+        qr += (size_t)(blockIdx.x % kJCopies) * jstride;
+        qz += (size_t)(blockIdx.x % kJCopies) * jstride;
+    }
     for (int c = blockIdx.x; c < nch; c += gridDim.x) {
         const int cnt = stage_cnt[c];
         for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
             const int dst = base + offs[c] + j;
-            const int src = c * kChunk + j;
+            const int src = c * kSeg + j;
             if (dst < cap) {
                 const float x = s0[src], y = s1[src], z = s2[src];
                 const float vx = s3[src], vy = s4[src], vz = s5[src];
@@ -865,7 +1037,7 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
                          int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, hipStream_t s)
+                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append_scan, dim3(1), dim3(1024), 0, s, src_count, dst_count, cap,
                        stage_cnt, scan_tmp, tmp2, max_chunks, dropped);
@@ -874,7 +1046,7 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
     const int blocks = max_chunks < 4096 ? (max_chunks > 0 ? max_chunks : 1) : 4096;
     hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, scan_tmp, tmp2, g, kq, qr, qz);
+                       stage[4], stage[5], stage_cnt, scan_tmp, tmp2, g, kq, qr, qz, jstride);
     return hipGetLastError();
 }
 
@@ -932,9 +1104,13 @@ __global__ __launch_bounds__(kBlock) void k_append_copy_batch(float* d0, float* 
                                                               const int32_t* chunk_start,
                                                               const int32_t* offs, const int32_t* tmp_run,
                                                               Geom g, float kq, unsigned long long* qr,
-                                                              unsigned long long* qz)
+                                                              unsigned long long* qz, size_t jstride)
 {
     __shared__ int s_cs[kMaxRuns + 1];
+    if (qr) {                                      // this workgroup's copy of the current
+        qr += (size_t)(blockIdx.x % kJCopies) * jstride;
+        qz += (size_t)(blockIdx.x % kJCopies) * jstride;
+    }
     for (int r = threadIdx.x; r <= nruns; r += blockDim.x) s_cs[r] = chunk_start[r];
     __syncthreads();
     const int total = s_cs[nruns];
@@ -953,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_append_copy_batch(float* d0, float* 
             const int local = first + j;
             if (local < br.cap) {
                 const int64_t dst = br.base + local;
-                const int src = c * kChunk + j;
+                const int src = c * kSeg + j;
                 const float x = s0[src], y = s1[src], z = s2[src];
                 const float vx = s3[src], vy = s4[src], vz = s5[src];
                 d0[dst] = x; d1[dst] = y; d2[dst] = z;
@@ -973,7 +1149,8 @@ hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const Batc
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
                                unsigned long long* dropped, const Geom& g, float kq,
-                               unsigned long long* qr, unsigned long long* qz, hipStream_t s)
+                               unsigned long long* qr, unsigned long long* qz, size_t jstride,
+                               hipStream_t s)
 {
     hipLaunchKernelGGL(k_append_scan_batch, dim3(nruns), dim3(1024), 0, s, counts, runs, stage_cnt,
                        chunk_start, offs, tmp_run, run_counters, dropped);
@@ -982,7 +1159,7 @@ hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const Batc
     const int blocks = total_chunks < 4096 ? (total_chunks > 0 ? total_chunks : 1) : 4096;
     hipLaunchKernelGGL(k_append_copy_batch, dim3(blocks), dim3(kBlock), 0, s, pool[0], pool[1], pool[2],
                        pool[3], pool[4], pool[5], runs, nruns, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, chunk_start, offs, tmp_run, g, kq, qr, qz);
+                       stage[4], stage[5], stage_cnt, chunk_start, offs, tmp_run, g, kq, qr, qz, jstride);
     return hipGetLastError();
 }
 
@@ -1037,32 +1214,22 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
                             int max_blocks, hipStream_t s)
 {
     const int blocks = max_blocks > 0 ? max_blocks : 1;
-    constexpr uint32_t P = kPush, Co = kCollide, D = kDrift, De = kDeposit, S = kStageSec, I = kStageIon,
-                       B = kBatch;
+    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec;
     switch (flags) {
-    case D: return launch_t<D>(g, pa, ca, blocks, s);
-    case De: return launch_t<De>(g, pa, ca, blocks, s);
-    case D | De: return launch_t<D | De>(g, pa, ca, blocks, s);
     case P: return launch_t<P>(g, pa, ca, blocks, s);
-    case P | D | De: return launch_t<P | D | De>(g, pa, ca, blocks, s);
     case Co | S: return launch_t<Co | S>(g, pa, ca, blocks, s);
     case Co | S | I: return launch_t<Co | S | I>(g, pa, ca, blocks, s);
     case P | Co: return launch_t<P | Co>(g, pa, ca, blocks, s);
     case P | Co | S: return launch_t<P | Co | S>(g, pa, ca, blocks, s);
     case P | Co | I: return launch_t<P | Co | I>(g, pa, ca, blocks, s);
     case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s);
-    case P | Co | D | De: return launch_t<P | Co | D | De>(g, pa, ca, blocks, s);
-    case P | Co | D | De | S: return launch_t<P | Co | D | De | S>(g, pa, ca, blocks, s);
-    case P | Co | D | De | I: return launch_t<P | Co | D | De | I>(g, pa, ca, blocks, s);
-    case P | Co | D | De | S | I: return launch_t<P | Co | D | De | S | I>(g, pa, ca, blocks, s);
+    case P | Co | S | DS: return launch_t<P | Co | S | DS>(g, pa, ca, blocks, s);
+    case P | Co | S | I | DS: return launch_t<P | Co | S | I | DS>(g, pa, ca, blocks, s);
     // batched ensemble
-    case B | D | De: return launch_t<B | D | De>(g, pa, ca, blocks, s);
     case B | P: return launch_t<B | P>(g, pa, ca, blocks, s);
-    case B | P | D | De: return launch_t<B | P | D | De>(g, pa, ca, blocks, s);
     case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s);
     case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s);
-    case B | P | Co | D | De: return launch_t<B | P | Co | D | De>(g, pa, ca, blocks, s);
-    case B | P | Co | D | De | S: return launch_t<B | P | Co | D | De | S>(g, pa, ca, blocks, s);
+    case B | P | Co | S | DS: return launch_t<B | P | Co | S | DS>(g, pa, ca, blocks, s);
     default: return hipErrorInvalidValue;
     }
 }

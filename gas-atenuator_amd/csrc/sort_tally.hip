@@ -7,8 +7,11 @@
 namespace gasx {
 
 // ---- cell sort ----------------------------------------------------------------
-// key = iz*nr + ir of the Hphi cell holding the particle; absorbed slots get
-// key nr*nz and sort last.  Stable radix sort keeps slot order within a cell.
+// key = k0*nr + i0, the first cell of the particle's EM7 deposit stencil
+// (cell-centred axes, clamped at the edges), so that neighbouring slots add
+// into the same four cells and the particle kernel can sum them before its
+// atomics; absorbed slots get key nr*nz and sort last.  Stable radix sort
+// keeps slot order within a key.
 __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
                             uint32_t* keys, uint32_t* vals, int32_t* n_live)
 {
@@ -20,8 +23,14 @@ __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float*
         if (is_dead(zz)) {
             key = (uint32_t)(g.nr * g.nz);
         } else {
-            float r = sqrtf(x[i] * x[i] + y[i] * y[i]);
-            int ir = (int)floorf(r * g.inv_dr), iz = (int)floorf(zz * g.inv_dz);
+            float r2 = x[i] * x[i];
+            r2 = r2 + y[i] * y[i];
+            const float r = sqrtf(r2);
+            float sr = r * g.inv_dr;
+            sr = sr - 0.5f;
+            float sz = zz * g.inv_dz;
+            sz = sz - 0.5f;
+            int ir = (int)floorf(sr), iz = (int)floorf(sz);
             ir = ir < 0 ? 0 : (ir >= g.nr ? g.nr - 1 : ir);
             iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
             key = (uint32_t)(iz * g.nr + ir);

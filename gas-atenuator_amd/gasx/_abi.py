@@ -88,6 +88,7 @@ class Timing(C.Structure):
         ("sort_ms", C.c_double), ("n_particle_launches", C.c_int64), ("n_fdtd_launches", C.c_int64),
         ("n_append_launches", C.c_int64), ("n_sorts", C.c_int64),
         ("graph_ms", C.c_double), ("n_graph_launches", C.c_int64),
+        ("drift_ms", C.c_double), ("n_drift_launches", C.c_int64),
     ]
 
 

@@ -114,3 +114,15 @@ def density_grid(w: Workload, gnr: int | None = None):
     rc = (np.arange(gnr) + 0.5) * (w.rp / gnr)
     d = w.density + (w.density_edge - w.density) * rc / w.rp
     return d.reshape(1, gnr).astype(np.float32), w.rp / gnr, w.lp
+
+
+def batched_runs(w: Workload, seed: int = 0):
+    """Run list of the batched ensemble: per-run density log-uniform in
+    [dens_lo, dens_hi] and electron count log-uniform in [n_lo, n_hi].
+    Returns (counts, densities) with densities as (1, 1) uniform grids."""
+    rng = np.random.default_rng(seed)
+    x = w.extra
+    runs = int(x["runs"])
+    dens = np.exp(rng.uniform(math.log(x["dens_lo"]), math.log(x["dens_hi"]), runs))
+    counts = np.exp(rng.uniform(math.log(x["n_lo"]), math.log(x["n_hi"]), runs)).astype(np.int64)
+    return counts, [np.full((1, 1), d, dtype=np.float32) for d in dens]

@@ -118,13 +118,15 @@ typedef struct gasx_counters {
 /* Kernel timing of the last gasx_run with time_kernels = 1 (HIP events on
  * the context's stream around every launch). */
 typedef struct gasx_timing {
-    double particle_ms;        /* sum over fused particle-kernel launches */
+    double particle_ms;        /* sum over particle launches (push + collide of a step, or fused passes) */
     double fdtd_ms;            /* sum over field-kernel launches */
     double append_ms;          /* sum over secondary/ion append launches */
     double sort_ms;            /* sum over cell sorts */
     int64_t n_particle_launches, n_fdtd_launches, n_append_launches, n_sorts;
     double graph_ms;           /* sum over graph launches (gasx_batch_run with use_graph) */
     int64_t n_graph_launches;
+    double drift_ms;           /* sum over drift + deposit launches of the collision loop */
+    int64_t n_drift_launches;
 } gasx_timing;
 
 /* Options of the fused production loop. */

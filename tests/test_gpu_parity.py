@@ -340,8 +340,12 @@ def test_sort_drops_absorbed_and_orders_cells(gpu):
     live = st[:, ~np.isnan(st[2])]
     assert g.shape[1] == live.shape[1]
     assert not np.isnan(g).any()
+    # key: first cell of the EM7 stencil (cell-centred axes), FP32 as the kernel
     r = np.sqrt(g[0] * g[0] + g[1] * g[1])
-    key = np.clip(np.floor(g[2] * np.float32(1 / p.dz)), 0, nz - 1) * nr + np.clip(np.floor(r * np.float32(1 / p.dr)), 0, nr - 1)
+    half = np.float32(0.5)
+    kr = np.clip(np.floor(r * np.float32(1 / p.dr) - half), 0, nr - 1)
+    kz = np.clip(np.floor(g[2] * np.float32(1 / p.dz) - half), 0, nz - 1)
+    key = kz * nr + kr
     assert np.all(np.diff(key) >= 0)
     a = np.lexsort(live)
     b = np.lexsort(g)
