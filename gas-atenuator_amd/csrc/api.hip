@@ -372,6 +372,8 @@ extern "C" gasx_status gasx_synchronize(gasx_ctx* ctx)
 static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
 {
     const int64_t slots = ((cap + kSeg - 1) / kSeg) * kSeg;
+    if (slots / kSeg > (int64_t)1024 * 1024)      // the append scan's group limit
+        return fail(ctx, GASX_E_ARG, "capacity above 2^28 slots is not supported by the append");
     if (slots > ctx->stage_slots || (ions && !ctx->istage[0])) {
         const int64_t ns = std::max(slots, ctx->stage_slots);
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
@@ -391,7 +393,8 @@ static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
         }
         HIPCHK(dalloc(&ctx->stage_cnt, ns / kSeg), "alloc stage cnt");
         HIPCHK(dalloc(&ctx->istage_cnt, ns / kSeg), "alloc ion stage cnt");
-        HIPCHK(dalloc(&ctx->scan_tmp, ns / kSeg), "alloc scan");
+        // segment offsets, then one total per group of 1024 segments
+        HIPCHK(dalloc(&ctx->scan_tmp, ns / kSeg + (ns / kSeg + 1023) / 1024 + 1), "alloc scan");
         ctx->stage_slots = ns;
     }
     return GASX_OK;

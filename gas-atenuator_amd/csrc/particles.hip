@@ -956,67 +956,145 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
 // src_count: slot count of the population whose kernel staged the entries
 // (it fixes the number of chunks); dst_count: the population appended to.
 // tmp2[0] <- old dst count, tmp2[1] <- number of staged chunks.
-__global__ __launch_bounds__(1024) void k_append_scan(const int32_t* src_count, int32_t* dst_count,
-                                                      int32_t cap, const int32_t* stage_cnt,
-                                                      int32_t* offs, int32_t* tmp2,
-                                                      int32_t max_chunks,
-                                                      unsigned long long* dropped)
+// Exclusive prefix of cnt[0..n) into offs (index order), returned total;
+// every thread of the 1024-thread workgroup must call it.  Tiles of 8192
+// counts: each thread scans 8 contiguous counts, a wave-shuffle scan and the
+// 16 wave totals combine the threads.
+__device__ int block_scan_counts(const int32_t* cnt, int32_t* offs, int n, int* s_wtot)
 {
-    __shared__ int s_part[1024];
-    const int ns = *src_count;
-    const int n = *dst_count;
-    int nch = (ns + kSeg - 1) / kSeg;
-    if (nch > max_chunks) nch = max_chunks;
-    const int per = (nch + blockDim.x - 1) / blockDim.x;
-    const int beg = threadIdx.x * per;
-    const int end = min(beg + per, nch);
-    int local = 0;
-    for (int i = beg; i < end; ++i) local += stage_cnt[i];
-    s_part[threadIdx.x] = local;
+    constexpr int kT = 8;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += (int)blockDim.x * kT) {
+        const int b0 = base + (int)threadIdx.x * kT;
+        int v[kT], sum = 0;
+#pragma unroll
+        for (int j = 0; j < kT; ++j) {
+            v[j] = b0 + j < n ? cnt[b0 + j] : 0;
+            sum += v[j];
+        }
+        int incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) s_wtot[wave] = incl;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int c = s_wtot[w];
+            if (w < wave) woff += c;
+            tot += c;
+        }
+        __syncthreads();                      // s_wtot is rewritten by the next tile
+        int acc = carry + woff + incl - sum;
+#pragma unroll
+        for (int j = 0; j < kT; ++j) {
+            if (b0 + j < n) offs[b0 + j] = acc;
+            acc += v[j];
+        }
+        carry += tot;
+    }
+    return carry;
+}
+
+// ---- append -------------------------------------------------------------------
+// Two launches, both parallel over the staging segments: k_append_scan gives
+// every segment its offset within a group of kScanGroup segments and every
+// group its total; k_append_copy prefixes the group totals in LDS, places
+// each staged entry at count + offset (parent slot order), deposits it, and
+// updates the count.  src_count fixes the number of staged segments.
+constexpr int kScanGroup = 1024;
+constexpr int kMaxScanGroups = 1024;     // staging of up to 2^20 segments
+
+__global__ __launch_bounds__(kScanGroup) void k_append_scan(const int32_t* src_count,
+                                                            const int32_t* dst_count,
+                                                            const int32_t* stage_cnt, int32_t* offs,
+                                                            int32_t* totals, int32_t* tmp2,
+                                                            int32_t max_segs)
+{
+    __shared__ int s_wtot[kScanGroup / 64];
+    int nseg = (*src_count + kSeg - 1) / kSeg;
+    if (nseg > max_segs) nseg = max_segs;
+    const int c = blockIdx.x * kScanGroup + (int)threadIdx.x;
+    const int v = c < nseg ? stage_cnt[c] : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wtot[wave] = incl;
     __syncthreads();
-    // Hillis-Steele inclusive scan over the 1024 partials
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        const int v = (int)threadIdx.x >= o ? s_part[threadIdx.x - o] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
+    int woff = 0, tot = 0;
+    for (int w = 0; w < kScanGroup / 64; ++w) {
+        const int t = s_wtot[w];
+        if (w < wave) woff += t;
+        tot += t;
     }
-    int run = s_part[threadIdx.x] - local;
-    for (int i = beg; i < end; ++i) {
-        offs[i] = run;
-        run += stage_cnt[i];
-    }
-    if (threadIdx.x == blockDim.x - 1) {
-        const int total = s_part[threadIdx.x];
-        const int room = cap - n;
-        const int app = total < room ? total : room;
-        tmp2[0] = n;
-        tmp2[1] = nch;
-        *dst_count = n + app;
-        if (total > app && dropped) atomicAdd(dropped, (unsigned long long)(total - app));
+    if (c < nseg) offs[c] = woff + incl - v;
+    if (threadIdx.x == 0) totals[blockIdx.x] = tot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tmp2[0] = *dst_count;
+        tmp2[1] = nseg;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, float* d2, float* d3,
                                                         float* d4, float* d5, int32_t cap,
+                                                        int32_t* dst_count,
                                                         const float* s0, const float* s1,
                                                         const float* s2, const float* s3,
                                                         const float* s4, const float* s5,
                                                         const int32_t* stage_cnt,
-                                                        const int32_t* offs, const int32_t* tmp2,
+                                                        const int32_t* offs, const int32_t* totals,
+                                                        int32_t ngroups, const int32_t* tmp2,
+                                                        unsigned long long* dropped,
                                                         Geom g, float kq, unsigned long long* qr,
                                                         unsigned long long* qz, size_t jstride)
 {
+    __shared__ int s_pre[kMaxScanGroups + 1];
+    __shared__ int s_wave[kBlock / 64];
     const int base = tmp2[0];
-    const int nch = tmp2[1];
+    const int nseg = tmp2[1];
+    // exclusive prefix of the group totals (four per thread)
+    {
+        constexpr int kG = kMaxScanGroups / kBlock;
+        int v[kG], sum = 0;
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            const int i = (int)threadIdx.x * kG + j;
+            v[j] = i < ngroups ? totals[i] : 0;
+            sum += v[j];
+        }
+        int tot;
+        int acc = block_excl_scan(sum, s_wave, tot);
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+            s_pre[threadIdx.x * kG + j] = acc;
+            acc += v[j];
+        }
+        if (threadIdx.x == 0) {
+            s_pre[kMaxScanGroups] = tot;
+            if (blockIdx.x == 0) {
+                const int room = cap - base;
+                const int app = tot < room ? tot : room;
+                *dst_count = base + app;
+                if (tot > app && dropped) atomicAdd(dropped, (unsigned long long)(tot - app));
+            }
+        }
+        __syncthreads();
+    }
     if (qr) {                                      // this workgroup's copy of the current
         qr += (size_t)(blockIdx.x % kJCopies) * jstride;
         qz += (size_t)(blockIdx.x % kJCopies) * jstride;
     }
-    for (int c = blockIdx.x; c < nch; c += gridDim.x) {
+    for (int c = blockIdx.x; c < nseg; c += gridDim.x) {
         const int cnt = stage_cnt[c];
+        if (cnt == 0) continue;
+        const int first = base + s_pre[c / kScanGroup] + offs[c];
         for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-            const int dst = base + offs[c] + j;
+            const int dst = first + j;
             const int src = c * kSeg + j;
             if (dst < cap) {
                 const float x = s0[src], y = s1[src], z = s2[src];
@@ -1033,20 +1111,25 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
     }
 }
 
+// scan_tmp: max_segs segment offsets followed by the group totals
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
-                         int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
+                         int32_t max_segs, unsigned long long* dropped, const Geom& g, float kq,
                          unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_append_scan, dim3(1), dim3(1024), 0, s, src_count, dst_count, cap,
-                       stage_cnt, scan_tmp, tmp2, max_chunks, dropped);
+    const int ngroups = (max_segs + kScanGroup - 1) / kScanGroup;
+    if (ngroups < 1 || ngroups > kMaxScanGroups) return hipErrorInvalidValue;
+    int32_t* totals = scan_tmp + max_segs;
+    hipLaunchKernelGGL(k_append_scan, dim3(ngroups), dim3(kScanGroup), 0, s, src_count, dst_count,
+                       stage_cnt, scan_tmp, totals, tmp2, max_segs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int blocks = max_chunks < 4096 ? (max_chunks > 0 ? max_chunks : 1) : 4096;
+    const int blocks = max_segs < 4096 ? (max_segs > 0 ? max_segs : 1) : 4096;
     hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
-                       dst[3], dst[4], dst[5], cap, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, scan_tmp, tmp2, g, kq, qr, qz, jstride);
+                       dst[3], dst[4], dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3],
+                       stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped, g, kq,
+                       qr, qz, jstride);
     return hipGetLastError();
 }
 
@@ -1060,30 +1143,12 @@ __global__ __launch_bounds__(1024) void k_append_scan_batch(int32_t* counts, con
                                                             unsigned long long* run_counters,
                                                             unsigned long long* dropped)
 {
-    __shared__ int s_part[1024];
+    __shared__ int s_wtot[16];
     const int r = blockIdx.x;
     const int c0 = chunk_start[r], nch = chunk_start[r + 1] - c0;
     const int n = counts[r];
-    const int per = (nch + blockDim.x - 1) / blockDim.x;
-    const int beg = threadIdx.x * per;
-    const int end = min(beg + per, nch);
-    int local = 0;
-    for (int i = beg; i < end; ++i) local += stage_cnt[c0 + i];
-    s_part[threadIdx.x] = local;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        const int v = (int)threadIdx.x >= o ? s_part[threadIdx.x - o] : 0;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int acc = s_part[threadIdx.x] - local;
-    for (int i = beg; i < end; ++i) {
-        offs[c0 + i] = acc;
-        acc += stage_cnt[c0 + i];
-    }
-    if (threadIdx.x == blockDim.x - 1) {
-        const int total = s_part[threadIdx.x];
+    const int total = block_scan_counts(stage_cnt + c0, offs + c0, nch, s_wtot);
+    if (threadIdx.x == 0) {
         const int room = runs[r].cap - n;
         const int app = total < room ? total : room;
         tmp_run[2 * r] = n;

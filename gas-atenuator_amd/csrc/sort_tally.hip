@@ -13,10 +13,9 @@ namespace gasx {
 // atomics; absorbed slots get key nr*nz and sort last.  Stable radix sort
 // keeps slot order within a key.
 __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
-                            uint32_t* keys, uint32_t* vals, int32_t* n_live)
+                            uint32_t* keys, uint32_t* vals)
 {
 #pragma clang fp contract(off)
-    int live = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float zz = z[i];
         uint32_t key;
@@ -34,48 +33,33 @@ __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float*
             ir = ir < 0 ? 0 : (ir >= g.nr ? g.nr - 1 : ir);
             iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
             key = (uint32_t)(iz * g.nr + ir);
-            ++live;
         }
         keys[i] = key;
         vals[i] = (uint32_t)i;
     }
-    // live count: wave sums, block sum, one atomic per block
-    for (int o = 32; o > 0; o >>= 1) live += __shfl_down(live, o, 64);
-    __shared__ int s_live[16];
-    if ((threadIdx.x & 63) == 0) s_live[threadIdx.x >> 6] = live;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int t = 0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += s_live[w];
-        if (t) atomicAdd(n_live, t);
-    }
 }
 
+// Gathers the live slots in key order; the slot at the boundary between live
+// and dead keys records the live count (no atomics).
 __global__ void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,
                           const float* s4, const float* s5, float* d0, float* d1, float* d2,
-                          float* d3, float* d4, float* d5, const uint32_t* perm, const int32_t* n_live)
+                          float* d3, float* d4, float* d5, const uint32_t* perm, const uint32_t* keys,
+                          uint32_t dead, int32_t n, int32_t* count)
 {
-    const int n = *n_live;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t s = perm[i];
-        d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
-        d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
+        const uint32_t k = keys[i];
+        if (k < dead) {
+            const uint32_t s = perm[i];
+            d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
+            d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
+            if (i + 1 == n || keys[i + 1] >= dead) *count = i + 1;
+        } else if (i == 0) {
+            *count = 0;
+        }
     }
 }
 
-__global__ void k_copy_back(float* d0, float* d1, float* d2, float* d3, float* d4, float* d5,
-                            const float* s0, const float* s1, const float* s2, const float* s3,
-                            const float* s4, const float* s5, const int32_t* n_live, int32_t* count)
-{
-    const int n = *n_live;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        d0[i] = s0[i]; d1[i] = s1[i]; d2[i] = s2[i];
-        d3[i] = s3[i]; d4[i] = s4[i]; d5[i] = s5[i];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) *count = n;
-}
-
-// scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][n_live][rocprim tmp]
+// scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][pad][rocprim tmp]
 hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
                        size_t* need_bytes, hipStream_t st)
 {
@@ -105,27 +89,27 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
     uint32_t* v_out = (uint32_t*)(base + 3 * a);
     float* tmp[6];
     for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * a);
-    int32_t* n_live = (int32_t*)(base + 10 * a);
     void* rtmp = (void*)(base + 10 * a + 256);
-    e = hipMemsetAsync(n_live, 0, sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
     int blocks = (n + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, n, k_in,
-                       v_in, n_live);
+                       v_in);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t tb = tmp_bytes;
     e = rocprim::radix_sort_pairs(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
     if (e != hipSuccess) return e;
+    const uint32_t dead = (uint32_t)(g.nr * g.nz);
     hipLaunchKernelGGL(k_permute, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
-                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], v_out, n_live);
+                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], v_out, k_out,
+                       dead, n, sp->count);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_copy_back, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
-                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], n_live,
-                       sp->count);
-    return hipGetLastError();
+    // copy back (slots past the live count hold stale values; the count excludes them)
+    float* dst[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
+    for (int c = 0; c < 6 && e == hipSuccess; ++c)
+        e = hipMemcpyAsync(dst[c], tmp[c], (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st);
+    return e;
 }
 
 // ---- FP64 tallies (electricmagneticfield5.m:167-177) ---------------------------
