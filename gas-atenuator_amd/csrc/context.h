@@ -29,6 +29,9 @@ struct BatchState {
     hipStream_t capture_stream = nullptr;       // private stream the sequence is captured on
 };
 
+// Recorded events, checkpoints and buffers of the differentiable run (adjoint.hip).
+struct DiffState;
+
 struct gasx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -66,6 +69,8 @@ struct gasx_ctx {
     // batched ensemble
     BatchState batch;
     gasx_batch_tuning tuning{16, 32, 16};
+    // differentiable run (at most one per context)
+    DiffState* diff = nullptr;
 };
 
 namespace gasx_host {
@@ -125,6 +130,8 @@ gasx_status make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, uint64_t
 int particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu);
 // Frees the batched-ensemble buffers and graph (batch.hip).
 void batch_release(gasx_ctx* ctx);
+// Frees the differentiable run (adjoint.hip).
+void diff_release(gasx_ctx* ctx);
 
 }  // namespace gasx_host
 

@@ -110,6 +110,14 @@ class BatchTuning(C.Structure):
     _fields_ = [("fdtd_ti", C.c_int32), ("fdtd_tk", C.c_int32), ("particle_blocks_per_cu", C.c_int32)]
 
 
+class DiffSpec(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int32), ("nodes0", C.POINTER(C.c_double)), ("n_steps", C.c_int32),
+        ("checkpoint_every", C.c_int32), ("seed", C.c_uint64), ("run_id", C.c_uint32),
+        ("r_frac", C.c_double),
+    ]
+
+
 class RunOpts(C.Structure):
     _fields_ = [
         ("n_steps", C.c_int32), ("collide", C.c_int32), ("secondaries", C.c_int32),
@@ -154,9 +162,14 @@ PROTOTYPES = {
     "gasx_batch_set_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "gasx_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "gasx_batch_counters": (C.c_int, [_P, C.c_int, C.POINTER(Counters), C.c_int]),
+    "gasx_diff_record": (C.c_int, [_P, C.POINTER(Species), C.POINTER(DiffSpec)]),
+    "gasx_diff_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]),
+    "gasx_diff_backward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "gasx_diff_events": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P]),
+    "gasx_diff_release": (C.c_int, [_P]),
 }
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib = None
 

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 2
+#define GASX_ABI_VERSION 3
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -258,6 +258,51 @@ gasx_status gasx_get_count(gasx_ctx* ctx, const gasx_species* s, int64_t* n);
 gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, double* pmax);
 /* Kernel times of the last timed gasx_run (synchronizes). */
 gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out);
+
+/* ---- Differentiable run and its adjoint ------------------------------------
+ * No counterpart in the reference, which has no gradients: these entry points
+ * serve callers that fit the gas profile or the source to an observed field
+ * (a MATLAB optimizer loop over the MEX shim, or torch.autograd).
+ *
+ * The model is the electron step of gasx_run (EM2.cu drift and wedge
+ * boundary map, EM7.cu deposit in FP64, EM3_1.cu / EM3.cu field cycle, EM4.cu
+ * gather and push) with the collisions of a recording pass frozen pathwise
+ * (common random numbers): each particle-step keeps its collision decision,
+ * velocity kick and the probability p0 = |v| ddt n0(r) sigma_T(E) at the
+ * reference profile n0; a replay at profile n applies the same kicks and
+ * multiplies the particle's weight by the likelihood ratio of the decision,
+ * n(r)/n0(r) after a collision and (1 - p0 n/n0)/(1 - p0) otherwise.  The
+ * density profile n(r) is linear between nodes at (j + 1/2) Rp / n_nodes and
+ * constant beyond the first and last.  Secondaries are not created.  The
+ * source amplitude A multiplies the deposit (FN -> A FN).  The objective is
+ * the electric energy of the outer shell at the last step,
+ * W = PHIp eps0 / 2 sum over Er cells with (i + 1/2) dr >= r_frac Rp of Er^2 r dr dz. */
+typedef struct gasx_diff_spec {
+    int32_t n_nodes;          /* profile nodes, 2..64 */
+    const double* nodes0;     /* host, n_nodes: reference profile n0 (m^-3), > 0 */
+    int32_t n_steps;          /* pushes */
+    int32_t checkpoint_every; /* steps between the adjoint's checkpoints */
+    uint64_t seed;            /* Philox key of the recording pass */
+    uint32_t run_id;          /* Philox counter word 2 */
+    double r_frac;            /* objective shell, in [0, 1) */
+} gasx_diff_spec;
+
+/* Records a run of the electrons in `e` (weight 1, fields zero at the start)
+ * at the reference profile; the species is only read.  Replaces any earlier
+ * recorded run of the context. */
+gasx_status gasx_diff_record(gasx_ctx* ctx, const gasx_species* e, const gasx_diff_spec* spec);
+/* Replays the recorded run at profile `nodes` (host, n_nodes) and amplitude;
+ * returns W (synchronizes). */
+gasx_status gasx_diff_forward(gasx_ctx* ctx, const double* nodes, double amplitude, double* objective);
+/* dW/dnodes (host, n_nodes) and dW/dA of the last gasx_diff_forward, by the
+ * hand-written adjoint with checkpoint recomputation (synchronizes). */
+gasx_status gasx_diff_backward(gasx_ctx* ctx, double* grad_nodes, double* grad_amplitude);
+/* Device copies of the recorded events of one step (caller's device arrays of
+ * the recorded particle count): kick, p0 and the collision flag. */
+gasx_status gasx_diff_events(gasx_ctx* ctx, int32_t step, float* dvx, float* dvy, float* dvz, float* p0,
+                             uint8_t* collided);
+/* Frees the recorded run. */
+gasx_status gasx_diff_release(gasx_ctx* ctx);
 
 #ifdef __cplusplus
 }
