@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -183,6 +184,65 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     return out
 
 
+def bench_adjoint(seed: int, steps: int | None = None):
+    """Differentiable run of the adjoint workload (128x256, 5e5 electrons,
+    200 pushes, 32-node radial profile falling linearly from 2.5e25 to
+    5e24 m^-3, checkpoints every 20 steps, shell r >= 0.25 Rp): the time of
+    one replayed forward and of one adjoint (backward) pass, against each
+    other and against the production step (gasx_run with collisions, same
+    electrons and steps)."""
+    import numpy as np
+    import torch
+
+    import gasx
+    from gasx.workloads import WORKLOADS, seed_electrons
+
+    w = WORKLOADS["adjoint_128x256"]
+    n_steps = steps or w.n_steps
+    nn, every = w.extra["nodes"], w.extra["checkpoint"]
+    p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
+    ctx = gasx.Context(p, device=0)
+    host = cell_sorted(seed_electrons(w.n_electrons, w.rp, w.lp, w.phip, w.sigma_r, seed=seed), w)
+    e = gasx.ParticleSet(w.n_electrons)
+    e.load(*host)
+    nodes0 = np.linspace(w.density, w.density / 5, nn)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = gasx.DiffRun(ctx, e, nodes0, n_steps=n_steps, checkpoint_every=every, seed=seed + 3, r_frac=0.25)
+    t_rec = time.perf_counter() - t0
+    run.forward(nodes0, 1.0)                      # warm
+    run.backward()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        wv = run.forward(nodes0, 1.0)
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g, ga = run.backward()
+    t_bwd = (time.perf_counter() - t0) / reps
+    run.release()
+    # production forward of the same electrons and steps (radial profile as a gas grid)
+    rc = (np.arange(w.nr) + 0.5) * w.dr
+    dens = np.interp(rc, (np.arange(nn) + 0.5) * w.rp / nn, nodes0).astype(np.float32).reshape(1, -1)
+    gas = gasx.GasGrid(dens, w.dr, w.lp, temperature=300.0)
+    f = gasx.FieldState(w.nr, w.nz)
+    ctx.run(e, f, gas, 5, collide=True, seed=seed + 3)
+    e.load(*host)
+    for t in (f.er, f.ez, f.hphi):
+        t.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run(e, f, gas, n_steps, collide=True, seed=seed + 3)
+    torch.cuda.synchronize()
+    t_prod = time.perf_counter() - t0
+    return {"workload": w.name, "grid": f"{w.nr}x{w.nz}", "electrons": w.n_electrons, "steps": n_steps,
+            "nodes": nn, "checkpoint_every": every, "record_s": t_rec, "forward_s": t_fwd,
+            "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
+            "production_forward_s": t_prod, "backward_over_production_forward": t_bwd / t_prod,
+            "objective_J": wv, "grad_amplitude": ga, "grad_nodes_norm": float(np.linalg.norm(g))}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -210,6 +270,15 @@ def main():
             "data": "synthetic", "config": {"workload": w.name, "runs": b["runs"], "grid": b["grid"],
                                             "electrons": b["electrons"]},
             "batched": b}))
+        return
+    if w.name == "adjoint_128x256":
+        a = bench_adjoint(args.seed, steps=args.steps if args.steps != 1000 else None)
+        print(json.dumps({
+            "metric": METRIC, "value": a["backward_over_forward"], "unit": "adjoint time / forward time",
+            "n_gpus": 1, "steps": a["steps"], "warmup": 1, "ms_per_step": a["backward_s"] * 1e3 / a["steps"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic", "config": {"workload": w.name, "grid": a["grid"], "electrons": a["electrons"]},
+            "adjoint": a}))
         return
     if w.n_electrons <= 0:
         raise SystemExit(f"workload {w.name} is not a single-run bench workload")
@@ -315,6 +384,9 @@ def main():
         del e, f, host_t
         torch.cuda.empty_cache()
         line["batched"] = bench_batched(args.seed)
+    if not args.no_adjoint:
+        torch.cuda.empty_cache()
+        line["adjoint"] = bench_adjoint(args.seed)
     print(json.dumps(line))
 
 

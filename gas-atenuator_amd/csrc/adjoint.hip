@@ -28,6 +28,7 @@
 // (scattering the field sensitivities), and the transposed field cycle
 // carries the field sensitivities one push back and yields the current's.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -187,11 +188,12 @@ __device__ __forceinline__ void push_fwd(const Geom& g, float qm, const Fld& f, 
 
 // Adjoint of push_fwd at input (x, y, z, v): lv holds the lambda of the pushed
 // velocity on entry and of the input velocity on return; the position lambdas
-// are added to lx, ly, lz; the field lambdas are scattered to lEr, lEz, lH.
+// are added to lx, ly, lz; the lambdas of the gathered Er, Ez, H (to be
+// spread over t.w's stencils) are returned in gEr, gEz, gH.
 __device__ __forceinline__ void push_adj(const Geom& g, float qm, const Fld& f, float x, float y, float z,
                                          float vx, float vy, float vz, const PushT& t, float& lvx,
                                          float& lvy, float& lvz, float& lx, float& ly, float& lz,
-                                         double* lEr, double* lEz, double* lH)
+                                         float& gEr, float& gEz, float& gH)
 {
     const float ic2 = g.inv_c2, dt = g.dt;
     // v' = igp u
@@ -234,19 +236,7 @@ __device__ __forceinline__ void push_adj(const Geom& g, float qm, const Fld& f, 
         lx += lr * x * ir + (lcph * y * y - lsph * x * y) * ir3;
         ly += lr * y * ir + (-lcph * x * y + lsph * x * x) * ir3;
     }
-    // field lambdas
-    atomicAdd(&lEr[w.i0 + w.k0 * nr], (double)(w.a0 * w.b0 * lEr_g));
-    atomicAdd(&lEr[w.i1 + w.k0 * nr], (double)(w.a1 * w.b0 * lEr_g));
-    atomicAdd(&lEr[w.i0 + w.k1 * nr], (double)(w.a0 * w.b1 * lEr_g));
-    atomicAdd(&lEr[w.i1 + w.k1 * nr], (double)(w.a1 * w.b1 * lEr_g));
-    atomicAdd(&lH[w.i0 + w.h0 * nr], (double)(w.a0 * w.c0 * lH_g));
-    atomicAdd(&lH[w.i1 + w.h0 * nr], (double)(w.a1 * w.c0 * lH_g));
-    atomicAdd(&lH[w.i0 + w.h1 * nr], (double)(w.a0 * w.c1 * lH_g));
-    atomicAdd(&lH[w.i1 + w.h1 * nr], (double)(w.a1 * w.c1 * lH_g));
-    atomicAdd(&lEz[w.j0 + w.h0 * ld], (double)(w.d0 * w.c0 * lEz_g));
-    atomicAdd(&lEz[w.j1 + w.h0 * ld], (double)(w.d1 * w.c0 * lEz_g));
-    atomicAdd(&lEz[w.j0 + w.h1 * ld], (double)(w.d0 * w.c1 * lEz_g));
-    atomicAdd(&lEz[w.j1 + w.h1 * ld], (double)(w.d1 * w.c1 * lEz_g));
+    gEr = lEr_g; gEz = lEz_g; gH = lH_g;
 }
 
 // EM7 deposit of weight w in FP64 A/m^2 (K = A FN e qnum), and its adjoint.
@@ -270,29 +260,11 @@ __device__ __forceinline__ void dep_geom(const Geom& g, float x, float y, float 
 
 __device__ __forceinline__ double cell_vol(double vcoef, int i) { return vcoef * (double)(2 * i + 1); }
 
-// EM7 deposit in int64 fixed point: each cell gets rint(W w v kq) with
-// kq = A qnum 2^32 / c, and the field cycle reads J = FN e c 2^-32 Q / V(i)
-// (= A FN e qnum w W v / V).  Integer sums make every replay bitwise
-// reproducible; the adjoint differentiates the unrounded map (the rounding
-// is below 2^-32 c of each contribution).
-__device__ __forceinline__ void dep_fwd(const Geom& g, double kq, float x, float y, float z, float vx,
-                                        float vy, float vz, float w, unsigned long long* qr,
-                                        unsigned long long* qz)
-{
-    DepT d;
-    dep_geom(g, x, y, z, vx, vy, d);
-    const int ci[4] = {d.i0, d.i1, d.i0, d.i1}, ck[4] = {d.k0, d.k0, d.k1, d.k1};
-    const float ww[4] = {d.wr0 * d.wz0, d.wr1 * d.wz0, d.wr0 * d.wz1, d.wr1 * d.wz1};
-    const double sr = kq * (double)w * (double)d.vr, sz = kq * (double)w * (double)vz;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        if (ww[a] == 0.0f) continue;
-        const long long qa = llrint((double)ww[a] * sr), qb = llrint((double)ww[a] * sz);
-        const int c = ci[a] + ck[a] * g.nr;
-        if (qa) atomicAdd(&qr[c], (unsigned long long)qa);
-        if (qb) atomicAdd(&qz[c], (unsigned long long)qb);
-    }
-}
+// EM7 deposit in int64 fixed point (dep_values / block_deposit below): each
+// cell gets rint(W w v kq) with kq = A qnum 2^32 / c, and the field cycle
+// reads J = FN e c 2^-32 Q / V(i) (= A FN e qnum w W v / V).  Integer sums
+// make every replay bitwise reproducible; the adjoint differentiates the
+// unrounded map (the rounding is below 2^-32 c of each contribution).
 
 // Adds the deposit's lambdas (of position, velocity, weight) given the current
 // lambdas ljr, ljz; returns dW/dA's share (<lambda_J, J> / A) of this particle.
@@ -374,21 +346,124 @@ struct DiffArgs {
     int32_t deposit;       // deposit J_{s+1} (not after the last step)
 };
 
+// ---- workgroup-tiled accumulation ---------------------------------------------
+// A workgroup's particles are mostly neighbours (the initial order is by
+// cell), so their stencils cover a small box of cells: sums go to an LDS tile
+// over that box first and then one global add per touched cell, instead of
+// one global atomic per particle and cell.  A box larger than the tile falls
+// back to global atomics.
+constexpr int kTileCells = 1024;
+
+struct Box {
+    int ilo, klo, wi, nk, ok;
+};
+
+// Bounding box [ilo, ihi] x [klo, khi] over the active lanes; every thread of
+// the workgroup calls it.
+__device__ Box block_box(bool act, int ilo, int ihi, int klo, int khi, int* s4)
+{
+    int a = act ? ilo : INT_MAX, b = act ? ihi : INT_MIN, c = act ? klo : INT_MAX, d = act ? khi : INT_MIN;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a = min(a, __shfl_xor(a, o));
+        b = max(b, __shfl_xor(b, o));
+        c = min(c, __shfl_xor(c, o));
+        d = max(d, __shfl_xor(d, o));
+    }
+    if (threadIdx.x == 0) { s4[0] = INT_MAX; s4[1] = INT_MIN; s4[2] = INT_MAX; s4[3] = INT_MIN; }
+    __syncthreads();
+    if ((threadIdx.x & (kWaveSize - 1)) == 0) {
+        atomicMin(&s4[0], a); atomicMax(&s4[1], b);
+        atomicMin(&s4[2], c); atomicMax(&s4[3], d);
+    }
+    __syncthreads();
+    Box bx;
+    const bool any = s4[1] >= s4[0];
+    bx.ilo = s4[0]; bx.klo = s4[2];
+    bx.wi = any ? s4[1] - s4[0] + 1 : 0;
+    bx.nk = any ? s4[3] - s4[2] + 1 : 0;
+    bx.ok = any && (long long)bx.wi * bx.nk <= kTileCells;
+    __syncthreads();   // s4 is reused by the next call
+    return bx;
+}
+
+// Fixed-point deposit contributions of one particle.
+struct DepQ {
+    int i0, i1, k0, k1;
+    long long qa[4], qb[4];   // cells (i0,k0) (i1,k0) (i0,k1) (i1,k1)
+};
+
+__device__ __forceinline__ void dep_values(const Geom& g, double kq, float x, float y, float z, float vx,
+                                           float vy, float vz, float w, DepQ& q)
+{
+    DepT d;
+    dep_geom(g, x, y, z, vx, vy, d);
+    q.i0 = d.i0; q.i1 = d.i1; q.k0 = d.k0; q.k1 = d.k1;
+    const float ww[4] = {d.wr0 * d.wz0, d.wr1 * d.wz0, d.wr0 * d.wz1, d.wr1 * d.wz1};
+    const double sr = kq * (double)w * (double)d.vr, sz = kq * (double)w * (double)vz;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        q.qa[a] = llrint((double)ww[a] * sr);
+        q.qb[a] = llrint((double)ww[a] * sz);
+    }
+}
+
+// Adds every active lane's contributions to qr, qz (one copy, nr x nz).
+__device__ void block_deposit(const Geom& g, bool act, const DepQ& q, unsigned long long* qr,
+                              unsigned long long* qz, unsigned long long* s_qr, unsigned long long* s_qz,
+                              int* s4)
+{
+    const Box bx = block_box(act, q.i0, q.i1, q.k0, q.k1, s4);
+    if (bx.ok) {
+        const int n = bx.wi * bx.nk;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) { s_qr[c] = 0ull; s_qz[c] = 0ull; }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int c = ((a & 1) ? q.i1 : q.i0) - bx.ilo + (((a & 2) ? q.k1 : q.k0) - bx.klo) * bx.wi;
+                if (q.qa[a]) atomicAdd(&s_qr[c], (unsigned long long)q.qa[a]);
+                if (q.qb[a]) atomicAdd(&s_qz[c], (unsigned long long)q.qb[a]);
+            }
+        }
+        __syncthreads();
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int gc = bx.ilo + c % bx.wi + (bx.klo + c / bx.wi) * g.nr;
+            if (s_qr[c]) atomicAdd(&qr[gc], s_qr[c]);
+            if (s_qz[c]) atomicAdd(&qz[gc], s_qz[c]);
+        }
+    } else if (act) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int c = ((a & 1) ? q.i1 : q.i0) + ((a & 2) ? q.k1 : q.k0) * g.nr;
+            if (q.qa[a]) atomicAdd(&qr[c], (unsigned long long)q.qa[a]);
+            if (q.qb[a]) atomicAdd(&qz[c], (unsigned long long)q.qb[a]);
+        }
+    }
+}
+
 // ---- prologue: half drift + deposit -----------------------------------------
 __global__ __launch_bounds__(kDiffBlock) void k_diff_prologue(DiffArgs a, DState in, DState out,
                                                               unsigned long long* qr, unsigned long long* qz)
 {
+    __shared__ unsigned long long s_qr[kTileCells], s_qz[kTileCells];
+    __shared__ int s4[4];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    float x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-    const float w = in.w[i];
-    if (!is_dead(z)) {
-        half_drift(a.g, x, y, z, vx, vy, vz);
-        const BcMap m = bc_apply(a.g, x, y, z, vx, vy, vz);
-        if (m.dead) z = __int_as_float(0x7fc00000);
-        else dep_fwd(a.g, a.kq, x, y, z, vx, vy, vz, w, qr, qz);
+    const size_t jo = (size_t)(blockIdx.x % kJCopies) * a.g.nr * a.g.nz;   // this workgroup's current copy
+    bool dep = false;
+    DepQ q{};
+    if (i < a.n) {
+        float x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+        const float w = in.w[i];
+        if (!is_dead(z)) {
+            half_drift(a.g, x, y, z, vx, vy, vz);
+            const BcMap m = bc_apply(a.g, x, y, z, vx, vy, vz);
+            if (m.dead) z = __int_as_float(0x7fc00000);
+            else { dep = true; dep_values(a.g, a.kq, x, y, z, vx, vy, vz, w, q); }
+        }
+        out.x[i] = x; out.y[i] = y; out.z[i] = z; out.vx[i] = vx; out.vy[i] = vy; out.vz[i] = vz; out.w[i] = w;
     }
-    out.x[i] = x; out.y[i] = y; out.z[i] = z; out.vx[i] = vx; out.vy[i] = vy; out.vz[i] = vz; out.w[i] = w;
+    block_deposit(a.g, dep, q, qr + jo, qz + jo, s_qr, s_qz, s4);
 }
 
 // ---- one step (record: live collisions at theta0; replay: frozen events) ----
@@ -398,6 +473,8 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_step(DiffArgs a, CollArgs c
                                                           unsigned long long* qz)
 {
     __shared__ uint32_t s_tab_raw[kRecord ? sizeof(SmallTabs) / 4 : 1];
+    __shared__ unsigned long long s_qr[kTileCells], s_qz[kTileCells];
+    __shared__ int s4[4];
     if (kRecord) {
         const uint32_t* src = reinterpret_cast<const uint32_t*>(&ca.tab->s);
         for (int i = threadIdx.x; i < (int)(sizeof(SmallTabs) / 4); i += blockDim.x) s_tab_raw[i] = src[i];
@@ -405,66 +482,74 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_step(DiffArgs a, CollArgs c
     }
     const SmallTabs* s_tab = reinterpret_cast<const SmallTabs*>(s_tab_raw);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    float x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-    float w = in.w[i];
-    if (!is_dead(z)) {
-        PushT t;
-        push_fwd(a.g, a.qm, f, x, y, z, vx, vy, vz, t);
-        half_drift(a.g, x, y, z, vx, vy, vz);
-        BcMap m = bc_apply(a.g, x, y, z, vx, vy, vz);
-        if (m.dead) {
-            z = __int_as_float(0x7fc00000);
-            if (kRecord) { ev.dvx[i] = ev.dvy[i] = ev.dvz[i] = 0.0f; ev.p0[i] = 0.0f; ev.col[i] = 0; }
-        } else {
-            float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f, p0 = 0.0f;
-            int col = 0;
-            int j0; float fr_, sl;
-            const float r = fsqrt(x * x + y * y);
-            const float n0r = prof(a.nodes0, a.nn, r, a.g.rp, j0, fr_, sl);
-            if (kRecord) {
-                // p_T of electron_transport.cu:74-77 with the profile density
-                const float v2 = vx * vx + vy * vy + vz * vz;
-                const float T = ev_from_speed2(v2, a.g.inv_c2, ca.ev_per_mc2_em);
-                if (T >= 15.6f && T <= 20000.0f) {
-                    float se, si, sx;
-                    cross_sections(s_tab, T, se, si, sx);
-                    p0 = fsqrt(v2) * ca.dt_coll * n0r * (se + si + sx);
-                }
-                const U4 R = philox4x32_10((uint32_t)i, ca.step, ca.run, 0u, ca.key0, ca.key1);
-                if (u01(R.a) < p0) {
-                    float cvx = vx, cvy = vy, cvz = vz;
-                    CollOut co;
-                    collide_candidate(a.g, ca, s_tab, (uint32_t)i, x, y, z, cvx, cvy, cvz, 0.0f, u01(R.b),
-                                      u01(R.c), u01(R.d), false, co);
-                    col = (co.ev & kEvMask) != 0;
-                    dvx = cvx - vx; dvy = cvy - vy; dvz = cvz - vz;
-                }
-                ev.dvx[i] = dvx; ev.dvy[i] = dvy; ev.dvz[i] = dvz; ev.p0[i] = p0; ev.col[i] = (uint8_t)col;
-            } else {
-                dvx = ev.dvx[i]; dvy = ev.dvy[i]; dvz = ev.dvz[i]; p0 = ev.p0[i]; col = ev.col[i];
-            }
-            vx += dvx; vy += dvy; vz += dvz;
-            // likelihood ratio of the frozen decision
-            if (p0 < 1.0f && n0r > 0.0f) {
-                const float nr_ = prof(a.nodes, a.nn, r, a.g.rp, j0, fr_, sl);
-                const float q = nr_ / n0r;
-                w *= col ? q : (1.0f - p0 * q) / (1.0f - p0);
-            }
+    const size_t jo = (size_t)(blockIdx.x % kJCopies) * a.g.nr * a.g.nz;   // this workgroup's current copy
+    bool dep = false;
+    DepQ q{};
+    if (i < a.n) {
+        float x = in.x[i], y = in.y[i], z = in.z[i], vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+        float w = in.w[i];
+        if (!is_dead(z)) {
+            PushT t;
+            push_fwd(a.g, a.qm, f, x, y, z, vx, vy, vz, t);
             half_drift(a.g, x, y, z, vx, vy, vz);
-            m = bc_apply(a.g, x, y, z, vx, vy, vz);
-            if (m.dead) z = __int_as_float(0x7fc00000);
-            else if (a.deposit) dep_fwd(a.g, a.kq, x, y, z, vx, vy, vz, w, qr, qz);
+            BcMap m = bc_apply(a.g, x, y, z, vx, vy, vz);
+            if (m.dead) {
+                z = __int_as_float(0x7fc00000);
+                if (kRecord) { ev.dvx[i] = ev.dvy[i] = ev.dvz[i] = 0.0f; ev.p0[i] = 0.0f; ev.col[i] = 0; }
+            } else {
+                float dvx = 0.0f, dvy = 0.0f, dvz = 0.0f, p0 = 0.0f;
+                int col = 0;
+                int j0;
+                float fr_, sl;
+                const float r = fsqrt(x * x + y * y);
+                const float n0r = prof(a.nodes0, a.nn, r, a.g.rp, j0, fr_, sl);
+                if (kRecord) {
+                    // p_T of electron_transport.cu:74-77 with the profile density
+                    const float v2 = vx * vx + vy * vy + vz * vz;
+                    const float T = ev_from_speed2(v2, a.g.inv_c2, ca.ev_per_mc2_em);
+                    if (T >= 15.6f && T <= 20000.0f) {
+                        float se, si, sx;
+                        cross_sections(s_tab, T, se, si, sx);
+                        p0 = fsqrt(v2) * ca.dt_coll * n0r * (se + si + sx);
+                    }
+                    const U4 R = philox4x32_10((uint32_t)i, ca.step, ca.run, 0u, ca.key0, ca.key1);
+                    if (u01(R.a) < p0) {
+                        float cvx = vx, cvy = vy, cvz = vz;
+                        CollOut co;
+                        collide_candidate(a.g, ca, s_tab, (uint32_t)i, x, y, z, cvx, cvy, cvz, 0.0f, u01(R.b),
+                                          u01(R.c), u01(R.d), false, co);
+                        col = (co.ev & kEvMask) != 0;
+                        dvx = cvx - vx; dvy = cvy - vy; dvz = cvz - vz;
+                    }
+                    ev.dvx[i] = dvx; ev.dvy[i] = dvy; ev.dvz[i] = dvz; ev.p0[i] = p0; ev.col[i] = (uint8_t)col;
+                } else {
+                    dvx = ev.dvx[i]; dvy = ev.dvy[i]; dvz = ev.dvz[i]; p0 = ev.p0[i]; col = ev.col[i];
+                }
+                vx += dvx; vy += dvy; vz += dvz;
+                // likelihood ratio of the frozen decision
+                if (p0 < 1.0f && n0r > 0.0f) {
+                    const float nr_ = prof(a.nodes, a.nn, r, a.g.rp, j0, fr_, sl);
+                    const float qq = nr_ / n0r;
+                    w *= col ? qq : (1.0f - p0 * qq) / (1.0f - p0);
+                }
+                half_drift(a.g, x, y, z, vx, vy, vz);
+                m = bc_apply(a.g, x, y, z, vx, vy, vz);
+                if (m.dead) z = __int_as_float(0x7fc00000);
+                else if (a.deposit) { dep = true; dep_values(a.g, a.kq, x, y, z, vx, vy, vz, w, q); }
+            }
         }
+        out.x[i] = x; out.y[i] = y; out.z[i] = z; out.vx[i] = vx; out.vy[i] = vy; out.vz[i] = vz; out.w[i] = w;
     }
-    out.x[i] = x; out.y[i] = y; out.z[i] = z; out.vx[i] = vx; out.vy[i] = vy; out.vz[i] = vz; out.w[i] = w;
+    if (a.deposit) block_deposit(a.g, dep, q, qr + jo, qz + jo, s_qr, s_qz, s4);
 }
 
 // ---- adjoint of one step -------------------------------------------------------
 // Recomputes step s from its input state and applies the transposed maps in
 // reverse.  lam_in: lambdas of the step's output state (x^{1/2}_{s+1}, v, w)
 // (without J_{s+1}'s share, added here from ljr, ljz); lam_out: lambdas of the
-// input state.  Field lambdas of F_{s+1} are scattered into lEr, lEz, lH.
+// input state.  Field lambdas of F_{s+1} are scattered into copy
+// blockIdx % kJCopies of lEr, lEz, lH (copies of the full arrays, back to
+// back), which spreads the atomics of the dense spot over as many addresses.
 struct Acc {
     double* gnodes;   // nn
     double* gA;       // 1
@@ -476,8 +561,21 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_adj_step(DiffArgs a, DState
                                                               double* lEr, double* lEz, double* lH, Acc acc)
 {
     __shared__ double s_g[kMaxNodes + 1];
+    __shared__ double s_er[kTileCells], s_ez[kTileCells], s_h[kTileCells];
+    __shared__ int s4[4];
     for (int j = threadIdx.x; j <= kMaxNodes; j += blockDim.x) s_g[j] = 0.0;
     __syncthreads();
+    {
+        const size_t cp = blockIdx.x % kJCopies, nr = a.g.nr, nz = a.g.nz;
+        lEr += cp * nr * (nz + 1);
+        lEz += cp * (nr + 1) * nz;
+        lH += cp * nr * nz;
+    }
+    // the gathered fields' lambdas of this lane and their stencils
+    bool scat = false;
+    Gath gw{};
+    float gEr = 0.0f, gEz = 0.0f, gH = 0.0f;
+    double gA = 0.0;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n) {
         float lx = 0.f, ly = 0.f, lz = 0.f, lvx = 0.f, lvy = 0.f, lvz = 0.f, lw = 0.f;
@@ -514,11 +612,9 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_adj_step(DiffArgs a, DState
                 if (!m2.dead) {
                     mx = lam_in.x[i]; my = lam_in.y[i]; mz = lam_in.z[i];
                     mvx = lam_in.vx[i]; mvy = lam_in.vy[i]; mvz = lam_in.vz[i]; mw = lam_in.w[i];
-                    if (a.deposit) {
-                        const double gA = dep_adj(a.g, a.K, a.A, a.vcoef, x, y, z, vx, vy, vz, w1, ljr, ljz, mx,
-                                                  my, mz, mvx, mvy, mvz, mw);
-                        atomicAdd(&s_g[kMaxNodes], gA);
-                    }
+                    if (a.deposit)
+                        gA = dep_adj(a.g, a.K, a.A, a.vcoef, x, y, z, vx, vy, vz, w1, ljr, ljz, mx, my, mz, mvx,
+                                     mvy, mvz, mw);
                     // second half drift + BC
                     bc_adjoint(m2, mx, my, mz, mvx, mvy, mvz);
                     const float h = 0.5f * a.g.dt;
@@ -545,11 +641,53 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_adj_step(DiffArgs a, DState
                 // push
                 lx = mx; ly = my; lz = mz;
                 lvx = mvx; lvy = mvy; lvz = mvz;
-                push_adj(a.g, a.qm, f, x0, y0, z0, vx0, vy0, vz0, t, lvx, lvy, lvz, lx, ly, lz, lEr, lEz, lH);
+                push_adj(a.g, a.qm, f, x0, y0, z0, vx0, vy0, vz0, t, lvx, lvy, lvz, lx, ly, lz, gEr, gEz, gH);
+                gw = t.w;
+                scat = true;
             }
         }
         lam_out.x[i] = lx; lam_out.y[i] = ly; lam_out.z[i] = lz;
         lam_out.vx[i] = lvx; lam_out.vy[i] = lvy; lam_out.vz[i] = lvz; lam_out.w[i] = lw;
+    }
+    // dW/dA: wave sums, then one LDS add per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) gA += __shfl_xor(gA, o);
+    if ((threadIdx.x & (kWaveSize - 1)) == 0 && gA != 0.0) atomicAdd(&s_g[kMaxNodes], gA);
+    // field lambdas: Er on (cells i, nodes k), H on (cells i, cells h), Ez on (nodes j, cells h)
+    const int nr = a.g.nr;
+    const Box bx = block_box(scat, min(gw.i0, gw.j0), max(gw.i1, gw.j1), min(gw.k0, gw.h0), max(gw.k1, gw.h1), s4);
+    const int eri[4] = {gw.i0, gw.i1, gw.i0, gw.i1}, erk[4] = {gw.k0, gw.k0, gw.k1, gw.k1};
+    const int hk[4] = {gw.h0, gw.h0, gw.h1, gw.h1}, ezj[4] = {gw.j0, gw.j1, gw.j0, gw.j1};
+    const float wer[4] = {gw.a0 * gw.b0, gw.a1 * gw.b0, gw.a0 * gw.b1, gw.a1 * gw.b1};
+    const float wh[4] = {gw.a0 * gw.c0, gw.a1 * gw.c0, gw.a0 * gw.c1, gw.a1 * gw.c1};
+    const float wez[4] = {gw.d0 * gw.c0, gw.d1 * gw.c0, gw.d0 * gw.c1, gw.d1 * gw.c1};
+    if (bx.ok) {
+        const int n = bx.wi * bx.nk;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) { s_er[c] = 0.0; s_ez[c] = 0.0; s_h[c] = 0.0; }
+        __syncthreads();
+        if (scat) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                atomicAdd(&s_er[eri[q] - bx.ilo + (erk[q] - bx.klo) * bx.wi], (double)(wer[q] * gEr));
+                atomicAdd(&s_h[eri[q] - bx.ilo + (hk[q] - bx.klo) * bx.wi], (double)(wh[q] * gH));
+                atomicAdd(&s_ez[ezj[q] - bx.ilo + (hk[q] - bx.klo) * bx.wi], (double)(wez[q] * gEz));
+            }
+        }
+        __syncthreads();
+        // a non-zero tile entry was added at an index that exists in that array
+        for (int c = threadIdx.x; c < n; c += blockDim.x) {
+            const int ii = bx.ilo + c % bx.wi, kk = bx.klo + c / bx.wi;
+            if (s_er[c] != 0.0) atomicAdd(&lEr[ii + kk * nr], s_er[c]);
+            if (s_h[c] != 0.0) atomicAdd(&lH[ii + kk * nr], s_h[c]);
+            if (s_ez[c] != 0.0) atomicAdd(&lEz[ii + kk * (nr + 1)], s_ez[c]);
+        }
+    } else if (scat) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            atomicAdd(&lEr[eri[q] + erk[q] * nr], (double)(wer[q] * gEr));
+            atomicAdd(&lH[eri[q] + hk[q] * nr], (double)(wh[q] * gH));
+            atomicAdd(&lEz[ezj[q] + hk[q] * (nr + 1)], (double)(wez[q] * gEz));
+        }
     }
     __syncthreads();
     for (int j = threadIdx.x; j < a.nn; j += blockDim.x)
@@ -685,6 +823,20 @@ __global__ void k_adj_objective(int nr, int nz, double coef, double dr, int i_mi
     if (threadIdx.x == 0) atomicAdd(w_out, s_w[0]);
 }
 
+// dst += sum of the kJCopies copies (stride n); the copies are zeroed.
+__global__ void k_fold_copies(double* dst, double* copies, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = dst[i];
+#pragma unroll
+        for (int k = 0; k < kJCopies; ++k) {
+            s += copies[k * n + i];
+            copies[k * n + i] = 0.0;
+        }
+        dst[i] = s;
+    }
+}
+
 __global__ void k_fill_f64(double* p, int64_t n)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -718,8 +870,9 @@ struct DiffState {
     std::vector<float*> seg_p;           // (every + 1) x 7: the states of one segment
     std::vector<float*> seg_f;           // every x 3: FP32 gather copies of F_{s+1}
     double *F[3] = {}, *Fb[3] = {};      // FP64 fields and their ping-pong partner
-    unsigned long long *qr = nullptr, *qz = nullptr;   // fixed-point current
+    unsigned long long *qr = nullptr, *qz = nullptr;   // fixed-point current, kJCopies copies of nr*nz
     double *lF[3] = {}, *ljr = nullptr, *ljz = nullptr;
+    double* lFc[3] = {};                 // kJCopies copies of each field lambda (scatter targets)
     float* lam[2][7] = {};               // particle lambdas (ping-pong)
     double* acc = nullptr;               // nn node gradients, dW/dA, W
     bool have_forward = false;
@@ -812,10 +965,19 @@ gasx_status copy_current(gasx_ctx* ctx, unsigned long long* dr_, unsigned long l
     return st == GASX_OK ? copy_d(ctx, dz_, sz, b) : st;
 }
 
-gasx_status zero_current(gasx_ctx* ctx, DiffState* d)
+// copies = kJCopies: every copy; 1: copy 0 only
+gasx_status zero_current(gasx_ctx* ctx, DiffState* d, int copies = kJCopies)
 {
-    HIPCHK(launch_fill_u64(d->qr, (int64_t)nh_of(ctx), ctx->stream), "fill");
-    HIPCHK(launch_fill_u64(d->qz, (int64_t)nh_of(ctx), ctx->stream), "fill");
+    const int64_t n = (int64_t)nh_of(ctx) * copies;
+    HIPCHK(launch_fill_u64(d->qr, n, ctx->stream), "fill");
+    HIPCHK(launch_fill_u64(d->qz, n, ctx->stream), "fill");
+    return GASX_OK;
+}
+
+// sums the current copies into copy 0 (and zeroes the others)
+gasx_status reduce_current(gasx_ctx* ctx, DiffState* d)
+{
+    HIPCHK(launch_j_reduce(d->qr, d->qz, (int64_t)nh_of(ctx), nh_of(ctx), ctx->stream), "current reduce");
     return GASX_OK;
 }
 
@@ -839,6 +1001,8 @@ gasx_status record_coll(gasx_ctx* ctx, const DiffState* d, CollArgs* ca)
 gasx_status diff_step(gasx_ctx* ctx, DiffState* d, int s, bool record, const CollArgs& ca0,
                       float* const* in, float* const* out, float* const* f32, int& parity)
 {
+    gasx_status st0 = reduce_current(ctx, d);
+    if (st0 != GASX_OK) return st0;
     FdtdRun r{};
     double* const* fin = parity ? d->Fb : d->F;
     double* const* fout = parity ? d->F : d->Fb;
@@ -850,7 +1014,7 @@ gasx_status diff_step(gasx_ctx* ctx, DiffState* d, int s, bool record, const Col
     parity ^= 1;
     const bool dep = s + 1 < d->steps;
     if (dep) {
-        gasx_status st = zero_current(ctx, d);
+        gasx_status st = zero_current(ctx, d, 1);   // the other copies were zeroed by the reduce
         if (st != GASX_OK) return st;
     }
     const DiffArgs a = diff_args(ctx, d, dep);
@@ -906,6 +1070,7 @@ gasx_status diff_forward_pass(gasx_ctx* ctx, DiffState* d, bool record, double* 
     for (int g = 0; g < d->nseg; ++g) {
         const int s0 = g * d->every, s1 = std::min(d->steps, s0 + d->every);
         st = copy_state(ctx, d, &d->ck_p[7 * g], work[cur]);
+        if (st == GASX_OK) st = reduce_current(ctx, d);
         if (st == GASX_OK) st = copy_fields(ctx, &d->ck_f[3 * g], parity ? d->Fb : d->F);
         if (st == GASX_OK) st = copy_current(ctx, d->ck_q[2 * g], d->ck_q[2 * g + 1], d->qr, d->qz);
         for (int s = s0; s < s1 && st == GASX_OK; ++s) {
@@ -992,8 +1157,9 @@ gasx_status diff_alloc(gasx_ctx* ctx, DiffState* d)
         HIPCHK(own(d, &d->Fb[c], fs[c]), "alloc fields");
         HIPCHK(own(d, &d->lF[c], fs[c]), "alloc field lambdas");
     }
-    HIPCHK(own(d, &d->qr, nh), "alloc current");
-    HIPCHK(own(d, &d->qz, nh), "alloc current");
+    HIPCHK(own(d, &d->qr, nh * kJCopies), "alloc current");
+    HIPCHK(own(d, &d->qz, nh * kJCopies), "alloc current");
+    for (int c = 0; c < 3; ++c) HIPCHK(own(d, &d->lFc[c], fs[c] * kJCopies), "alloc field lambda copies");
     HIPCHK(own(d, &d->ljr, nh), "alloc current lambdas");
     HIPCHK(own(d, &d->ljz, nh), "alloc current lambdas");
     for (auto& l : d->lam)
@@ -1108,6 +1274,8 @@ extern "C" gasx_status gasx_diff_backward(gasx_ctx* ctx, double* grad_nodes, dou
     if (st == GASX_OK) st = zero64(ctx, d->lF[2], nh);
     if (st == GASX_OK) st = zero64(ctx, d->ljr, nh);
     if (st == GASX_OK) st = zero64(ctx, d->ljz, nh);
+    const int64_t fsz[3] = {(int64_t)ner_of(ctx), (int64_t)nez_of(ctx), nh};
+    for (int c = 0; c < 3 && st == GASX_OK; ++c) st = zero64(ctx, d->lFc[c], fsz[c] * kJCopies);
     // dW/dF_S (Er only); W itself is recomputed into acc[nn + 1]
     if (st == GASX_OK) st = launch_objective(ctx, d, d->ck_f[3 * d->nseg], d->lF[0], d->acc + d->nn + 1);
     if (st != GASX_OK) return st;
@@ -1120,6 +1288,7 @@ extern "C" gasx_status gasx_diff_backward(gasx_ctx* ctx, double* grad_nodes, dou
         const int s0 = g * d->every, s1 = std::min(d->steps, s0 + d->every);
         // recompute the segment from its checkpoint, keeping every state and field copy
         st = copy_state(ctx, d, &d->seg_p[0], &d->ck_p[7 * g]);
+        if (st == GASX_OK) st = zero_current(ctx, d);
         if (st == GASX_OK) st = copy_fields(ctx, d->F, &d->ck_f[3 * g]);
         if (st == GASX_OK) st = copy_current(ctx, d->qr, d->qz, d->ck_q[2 * g], d->ck_q[2 * g + 1]);
         int parity = 0;
@@ -1137,9 +1306,14 @@ extern "C" gasx_status gasx_diff_backward(gasx_ctx* ctx, double* grad_nodes, dou
                 const Fld f{d->seg_f[3 * k], d->seg_f[3 * k + 1], d->seg_f[3 * k + 2]};
                 hipLaunchKernelGGL(k_diff_adj_step, dim3(blocks_for(d->n)), dim3(kDiffBlock), 0, ctx->stream, a,
                                    state_of(&d->seg_p[7 * k]), f, events_of(d, s), state_of(d->lam[cur]),
-                                   state_of(d->lam[cur ^ 1]), d->ljr, d->ljz, d->lF[0], d->lF[1], d->lF[2], acc);
+                                   state_of(d->lam[cur ^ 1]), d->ljr, d->ljz, d->lFc[0], d->lFc[1], d->lFc[2],
+                                   acc);
                 HIPCHK(hipGetLastError(), "adjoint step");
                 cur ^= 1;
+                for (int c = 0; c < 3; ++c)
+                    hipLaunchKernelGGL(k_fold_copies, dim3(blocks_for(fsz[c])), dim3(kDiffBlock), 0, ctx->stream,
+                                       d->lF[c], d->lFc[c], fsz[c]);
+                HIPCHK(hipGetLastError(), "fold field lambdas");
             }
             st = adj_cycle(ctx, d);
             if (st != GASX_OK) return st;
