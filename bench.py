@@ -186,11 +186,12 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
 
 def bench_adjoint(seed: int, steps: int | None = None):
     """Differentiable run of the adjoint workload (128x256, 5e5 electrons,
-    200 pushes, 32-node radial profile falling linearly from 2.5e25 to
-    5e24 m^-3, checkpoints every 20 steps, shell r >= 0.25 Rp): the time of
-    one replayed forward and of one adjoint (backward) pass, against each
-    other and against the production step (gasx_run with collisions, same
-    electrons and steps)."""
+    200 pushes, 32-node radial profile with nodes 2.5e25 (1 + 0.1 U(-1, 1)),
+    checkpoints every 20 steps, objective W_Er over r >= 0.9 Rp): the time of
+    one replayed forward and of one adjoint (backward, recomputation
+    included) pass, reported as (forward + adjoint) / forward, and against
+    the production step (gasx_run with collisions, same electrons, profile
+    and steps)."""
     import numpy as np
     import torch
 
@@ -205,10 +206,10 @@ def bench_adjoint(seed: int, steps: int | None = None):
     host = cell_sorted(seed_electrons(w.n_electrons, w.rp, w.lp, w.phip, w.sigma_r, seed=seed), w)
     e = gasx.ParticleSet(w.n_electrons)
     e.load(*host)
-    nodes0 = np.linspace(w.density, w.density / 5, nn)
+    nodes0 = w.density * (1 + 0.1 * np.random.default_rng(seed).uniform(-1.0, 1.0, nn))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run = gasx.DiffRun(ctx, e, nodes0, n_steps=n_steps, checkpoint_every=every, seed=seed + 3, r_frac=0.25)
+    run = gasx.DiffRun(ctx, e, nodes0, n_steps=n_steps, checkpoint_every=every, seed=seed + 3, r_frac=0.9)
     t_rec = time.perf_counter() - t0
     run.forward(nodes0, 1.0)                      # warm
     run.backward()
@@ -239,7 +240,9 @@ def bench_adjoint(seed: int, steps: int | None = None):
     return {"workload": w.name, "grid": f"{w.nr}x{w.nz}", "electrons": w.n_electrons, "steps": n_steps,
             "nodes": nn, "checkpoint_every": every, "record_s": t_rec, "forward_s": t_fwd,
             "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
+            "forward_plus_adjoint_over_forward": (t_fwd + t_bwd) / t_fwd,
             "production_forward_s": t_prod, "backward_over_production_forward": t_bwd / t_prod,
+            "forward_plus_adjoint_over_production_forward": (t_fwd + t_bwd) / t_prod,
             "objective_J": wv, "grad_amplitude": ga, "grad_nodes_norm": float(np.linalg.norm(g))}
 
 
@@ -274,7 +277,8 @@ def main():
     if w.name == "adjoint_128x256":
         a = bench_adjoint(args.seed, steps=args.steps if args.steps != 1000 else None)
         print(json.dumps({
-            "metric": METRIC, "value": a["backward_over_forward"], "unit": "adjoint time / forward time",
+            "metric": METRIC, "value": a["forward_plus_adjoint_over_forward"],
+            "unit": "(forward + adjoint) time / forward time",
             "n_gpus": 1, "steps": a["steps"], "warmup": 1, "ms_per_step": a["backward_s"] * 1e3 / a["steps"],
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic", "config": {"workload": w.name, "grid": a["grid"], "electrons": a["electrons"]},

@@ -713,92 +713,6 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_adj_prologue(DiffArgs a, DS
     if (threadIdx.x == 0 && s_a != 0.0) atomicAdd(acc.gA, s_a);
 }
 
-// ---- transposed field cycle ------------------------------------------------------
-struct AdjF {
-    int nr, nz;
-    double ch_r, ch_z, ce, cez, c4, dr;
-    int ez_metric, axis_sign;
-};
-
-// Transpose of H += s (ch_r dEz/dr - ch_z dEr/dz): H's lambda feeds Ez's and Er's.
-__global__ void k_adj_h(AdjF a, double s, const double* lH, double* lEr, double* lEz)
-{
-    const int nr = a.nr, nz = a.nz;
-    const int ner = nr * (nz + 1), nez = (nr + 1) * nz;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ner + nez; c += gridDim.x * blockDim.x) {
-        if (c < ner) {
-            const int i = c % nr, k = c / nr;
-            double v = 0.0;
-            if (k < nz) v += lH[i + k * nr];
-            if (k >= 1) v -= lH[i + (k - 1) * nr];
-            lEr[c] += s * a.ch_z * v;
-        } else {
-            const int e = c - ner, i = e % (nr + 1), k = e / (nr + 1);
-            double v = 0.0;
-            if (i >= 1) v += lH[(i - 1) + k * nr];
-            if (i < nr) v -= lH[i + k * nr];
-            lEz[e] += s * a.ch_r * v;
-        }
-    }
-}
-
-__device__ __forceinline__ double coef_a(const AdjF& a, int i)
-{
-    const double rs = (a.ez_metric ? (i + 1) : i) * a.dr;
-    return a.ce * (1 / (2 * rs) + 1 / a.dr);
-}
-
-__device__ __forceinline__ double coef_b(const AdjF& a, int i)
-{
-    const double rs = (a.ez_metric ? (i + 1) : i) * a.dr;
-    return a.ce * (1 / (2 * rs) - 1 / a.dr);
-}
-
-// Transpose of the E update (EM3.cu:14-42): H's lambda and the current's
-// lambda from E's.  E's own lambda passes through, except the Er boundary
-// rows, folded by k_adj_e_fold afterwards.
-__global__ void k_adj_e(AdjF a, double* lH, const double* lEr, const double* lEz, double* ljr, double* ljz)
-{
-    const int nr = a.nr, nz = a.nz;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nr * nz; c += gridDim.x * blockDim.x) {
-        const int i = c % nr, k = c / nr;
-        double h = 0.0;
-        // Er(i,k) and Er(i,k+1) use H(i,k) with -cez and +cez (interior rows)
-        const bool er_k = k >= 1 && k <= nz - 1, er_k1 = k + 1 >= 1 && k + 1 <= nz - 1;
-        if (er_k) h -= a.cez * lEr[i + k * nr];
-        if (er_k1) h += a.cez * lEr[i + (k + 1) * nr];
-        // Ez(i,k) uses H(i,k) (axis: c4; interior: cA), Ez(i+1,k) uses it with cB
-        const int ld = nr + 1;
-        if (i == 0) h += a.c4 * lEz[0 + k * ld];
-        else h += coef_a(a, i) * lEz[i + k * ld];
-        if (i + 1 <= nr - 1) h += coef_b(a, i + 1) * lEz[(i + 1) + k * ld];
-        lH[c] += h;
-        // currents: Er(i,k) and Er(i,k+1) take -ce/2 Jr(i,k); Ez(i,k), Ez(i+1,k) -ce/2 Jz(i,k); axis +sign ce Jz(0,k)
-        double jrv = 0.0, jzv = 0.0;
-        if (er_k) jrv -= 0.5 * a.ce * lEr[i + k * nr];
-        if (er_k1) jrv -= 0.5 * a.ce * lEr[i + (k + 1) * nr];
-        if (i == 0) jzv += (double)a.axis_sign * a.ce * lEz[0 + k * ld];
-        else jzv -= 0.5 * a.ce * lEz[i + k * ld];
-        if (i + 1 <= nr - 1) jzv -= 0.5 * a.ce * lEz[(i + 1) + k * ld];
-        ljr[c] += jrv;
-        ljz[c] += jzv;
-    }
-}
-
-// Er(i,0) <- Er0(i,1) and Er(i,nz) <- Er0(i,nz-1): the boundary rows' lambdas
-// move to rows 1 and nz-1.
-__global__ void k_adj_e_fold(AdjF a, double* lEr)
-{
-    const int nr = a.nr, nz = a.nz;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += gridDim.x * blockDim.x) {
-        const double top = lEr[i], bot = lEr[i + nz * nr];
-        lEr[i] = 0.0;
-        lEr[i + nz * nr] = 0.0;
-        lEr[i + 1 * nr] += top;
-        lEr[i + (nz - 1) * nr] += bot;
-    }
-}
-
 __global__ void k_adj_objective(int nr, int nz, double coef, double dr, int i_min, const double* er, double* lEr,
                                 double* w_out)
 {
@@ -871,7 +785,7 @@ struct DiffState {
     std::vector<float*> seg_f;           // every x 3: FP32 gather copies of F_{s+1}
     double *F[3] = {}, *Fb[3] = {};      // FP64 fields and their ping-pong partner
     unsigned long long *qr = nullptr, *qz = nullptr;   // fixed-point current, kJCopies copies of nr*nz
-    double *lF[3] = {}, *ljr = nullptr, *ljz = nullptr;
+    double *lF[3] = {}, *lFo[3] = {}, *ljr = nullptr, *ljz = nullptr;   // field lambdas (ping-pong)
     double* lFc[3] = {};                 // kJCopies copies of each field lambda (scatter targets)
     float* lam[2][7] = {};               // particle lambdas (ping-pong)
     double* acc = nullptr;               // nn node gradients, dW/dA, W
@@ -1091,31 +1005,13 @@ gasx_status diff_forward_pass(gasx_ctx* ctx, DiffState* d, bool record, double* 
     return GASX_OK;
 }
 
-// Transposed field cycle: lF <- M^T lF and (ljr, ljz) <- B^T lF.  The cycle
-// is H(-1/2), n_sub x {H(1), E}, H(+1/2); its transpose runs the transposed
-// updates in the reverse order.
+// Transposed field cycle: lF <- M^T lF (through lFo, then swapped) and
+// (ljr, ljz) <- B^T lF.
 gasx_status adj_cycle(gasx_ctx* ctx, DiffState* d)
 {
-    const FdtdArgs& fa = ctx->fa;
-    const AdjF a{ctx->p.nr, ctx->p.nz, fa.ch_r, fa.ch_z, fa.ce, fa.cez, fa.c4, fa.dr, fa.ez_metric, fa.axis_sign};
-    const int64_t nh = (int64_t)nh_of(ctx), nE = (int64_t)(ner_of(ctx) + nez_of(ctx));
-    gasx_status st = zero64(ctx, d->ljr, nh);
-    if (st == GASX_OK) st = zero64(ctx, d->ljz, nh);
-    if (st != GASX_OK) return st;
-    const int bh = blocks_for(nE), be = blocks_for(nh), bf = blocks_for(ctx->p.nr);
-    auto h = [&](double s) {
-        hipLaunchKernelGGL(k_adj_h, dim3(bh), dim3(kDiffBlock), 0, ctx->stream, a, s, d->lF[2], d->lF[0],
-                           d->lF[1]);
-    };
-    h(0.5);
-    for (int k = 0; k < ctx->p.n_sub; ++k) {
-        hipLaunchKernelGGL(k_adj_e, dim3(be), dim3(kDiffBlock), 0, ctx->stream, a, d->lF[2], d->lF[0], d->lF[1],
-                           d->ljr, d->ljz);
-        hipLaunchKernelGGL(k_adj_e_fold, dim3(bf), dim3(kDiffBlock), 0, ctx->stream, a, d->lF[0]);
-        h(1.0);
-    }
-    h(-0.5);
-    HIPCHK(hipGetLastError(), "transposed field cycle");
+    const AdjFdtdRun r{d->lF[0], d->lF[1], d->lF[2], d->lFo[0], d->lFo[1], d->lFo[2], d->ljr, d->ljz};
+    HIPCHK(launch_fdtd_adj(ctx->fa, r, ctx->stream), "transposed field cycle");
+    for (int c = 0; c < 3; ++c) std::swap(d->lF[c], d->lFo[c]);
     return GASX_OK;
 }
 
@@ -1156,6 +1052,7 @@ gasx_status diff_alloc(gasx_ctx* ctx, DiffState* d)
         HIPCHK(own(d, &d->F[c], fs[c]), "alloc fields");
         HIPCHK(own(d, &d->Fb[c], fs[c]), "alloc fields");
         HIPCHK(own(d, &d->lF[c], fs[c]), "alloc field lambdas");
+        HIPCHK(own(d, &d->lFo[c], fs[c]), "alloc field lambdas");
     }
     HIPCHK(own(d, &d->qr, nh * kJCopies), "alloc current");
     HIPCHK(own(d, &d->qz, nh * kJCopies), "alloc current");
@@ -1362,4 +1259,18 @@ extern "C" gasx_status gasx_diff_release(gasx_ctx* ctx)
     hipSetDevice(ctx->device);
     diff_release(ctx);
     return GASX_OK;
+}
+
+extern "C" gasx_status gasx_fdtd_transpose(gasx_ctx* ctx, gasx_fields* lam, double* ljr, double* ljz)
+{
+    if (!ctx) return GASX_E_ARG;
+    if (!lam || !lam->er || !lam->ez || !lam->hphi || !ljr || !ljz)
+        return fail(ctx, GASX_E_ARG, "gasx_fdtd_transpose: null array");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    const AdjFdtdRun r{lam->er, lam->ez, lam->hphi, ctx->er_b, ctx->ez_b, ctx->h_b, ljr, ljz};
+    HIPCHK(launch_fdtd_adj(ctx->fa, r, ctx->stream), "transposed field cycle");
+    gasx_status st = copy_d(ctx, lam->er, ctx->er_b, ner_of(ctx) * sizeof(double));
+    if (st == GASX_OK) st = copy_d(ctx, lam->ez, ctx->ez_b, nez_of(ctx) * sizeof(double));
+    if (st == GASX_OK) st = copy_d(ctx, lam->hphi, ctx->h_b, nh_of(ctx) * sizeof(double));
+    return st;
 }

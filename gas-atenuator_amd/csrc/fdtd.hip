@@ -195,13 +195,194 @@ static hipError_t launch_fdtd_t(const FdtdArgs& a, const FdtdRun& r0, const Fdtd
     return hipGetLastError();
 }
 
+// ---- transposed field cycle (adjoint of k_fdtd's cycle) -------------------------
+// The cycle is linear: F' = M F + B J.  This kernel applies M^T to the field
+// sensitivities (lEr, lEz, lH) and returns B^T of them as the current's
+// sensitivities (ljr, ljz, overwritten), with the same temporal blocking as
+// k_fdtd: the transposed updates run in reverse order -- H(+1/2)^T,
+// n_sub x {E^T, H(1)^T}, H(-1/2)^T -- each one reading the neighbours on the
+// opposite side to its forward update, so the same halo keeps the interior
+// exact.
+//   H(s)^T:  lEz(i,k) += s ch_r (lH(i-1,k) - lH(i,k)), lEr(i,k) += s ch_z (lH(i,k) - lH(i,k-1))
+//            (terms outside the H grid are absent)
+//   E^T:     lH(i,k) += -cez lEr(i,k) + cez lEr(i,k+1)   (Er rows 1..nz-1 only)
+//                      + (i == 0 ? c4 : cA(i)) lEz(i,k) + cB(i+1) lEz(i+1,k)   (i+1 <= nr-1)
+//            ljr(i,k) -= ce/2 (lEr(i,k) + lEr(i,k+1)) over the same rows,
+//            ljz(i,k) += (i == 0 ? axis ce lEz(0,k) : -ce/2 lEz(i,k)) - ce/2 lEz(i+1,k) (i+1 <= nr-1),
+//            then the Jacobi boundary copies Er(i,0) <- Er(i,1), Er(i,nz) <- Er(i,nz-1)
+//            move the boundary rows' sensitivities onto rows 1 and nz-1.
+template <int BS, int MAXP>
+__global__ __launch_bounds__(BS) void k_fdtd_adj(FdtdArgs a, AdjFdtdRun R)
+{
+    extern __shared__ double lds[];
+    const int nr = a.nr, nz = a.nz;
+    const int t = blockIdx.x;
+    const int ti_idx = t % a.ntr, tk_idx = t / a.ntr;
+    const int i0 = ti_idx * a.ti, i1 = min(nr, i0 + a.ti);
+    const int k0 = tk_idx * a.tk, k1 = min(nz, k0 + a.tk);
+    const int ri0 = max(0, i0 - a.halo), ri1 = min(nr, i1 + a.halo);
+    const int rk0 = max(0, k0 - a.halo), rk1 = min(nz, k1 + a.halo);
+    const int W = ri1 - ri0, Hh = rk1 - rk0;
+    double* sH = lds;
+    double* sEr = sH + W * Hh;
+    double* sEz = sEr + W * (Hh + 1);
+    const int ldz = W + 1;
+    const int npos = (W + 1) * (Hh + 1);
+
+    for (int p = threadIdx.x; p < npos; p += BS) {
+        const int li = p % (W + 1), lk = p / (W + 1);
+        const int gi = ri0 + li, gk = rk0 + lk;
+        if (li < W && lk < Hh) sH[li + lk * W] = R.lh_in[gi + gk * nr];
+        if (li < W) sEr[li + lk * W] = R.ler_in[gi + gk * nr];
+        if (lk < Hh) sEz[li + lk * ldz] = R.lez_in[gi + gk * (nr + 1)];
+    }
+    // per H position: the E^T coefficients and the current accumulators
+    double cAe[MAXP], cBe[MAXP], accR[MAXP], accZ[MAXP];
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = threadIdx.x + q * BS;
+        cAe[q] = 0.0; cBe[q] = 0.0; accR[q] = 0.0; accZ[q] = 0.0;
+        if (p < npos) {
+            const int gi = ri0 + p % (W + 1);
+            auto rs = [&](int i) { return (a.ez_metric ? (i + 1) : i) * a.dr; };
+            if (gi == 0) cAe[q] = a.c4;
+            else if (gi <= nr - 1) cAe[q] = a.ce * (1 / (2 * rs(gi)) + 1 / a.dr);
+            if (gi + 1 <= nr - 1) cBe[q] = a.ce * (1 / (2 * rs(gi + 1)) - 1 / a.dr);
+        }
+    }
+    __syncthreads();
+
+    auto h_adj = [&](double s) {
+        const double cr = s * a.ch_r, cz = s * a.ch_z;
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                const int gi = ri0 + li, gk = rk0 + lk;
+                if (li < W) {
+                    double v = 0.0;
+                    if (gk < nz && lk < Hh) v += sH[li + lk * W];
+                    if (gk >= 1 && lk >= 1) v -= sH[li + (lk - 1) * W];
+                    sEr[li + lk * W] += cz * v;
+                }
+                if (lk < Hh) {
+                    double v = 0.0;
+                    if (gi >= 1 && li >= 1) v += sH[(li - 1) + lk * W];
+                    if (gi < nr && li < W) v -= sH[li + lk * W];
+                    sEz[li + lk * ldz] += cr * v;
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    auto e_adj = [&]() {
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                const int gi = ri0 + li, gk = rk0 + lk;
+                if (li < W && lk < Hh) {
+                    double h = sH[li + lk * W];
+                    if (gk >= 1) {                        // Er row gk is interior (gk <= nz-1 always)
+                        const double e = sEr[li + lk * W];
+                        h -= a.cez * e;
+                        accR[q] -= 0.5 * a.ce * e;
+                    }
+                    if (gk + 1 <= nz - 1) {
+                        const double e = sEr[li + (lk + 1) * W];
+                        h += a.cez * e;
+                        accR[q] -= 0.5 * a.ce * e;
+                    }
+                    const double ez0 = sEz[li + lk * ldz];
+                    h += cAe[q] * ez0;
+                    accZ[q] += (gi == 0 ? (double)a.axis_sign * a.ce : -0.5 * a.ce) * ez0;
+                    if (gi + 1 <= nr - 1) {
+                        const double ez1 = sEz[(li + 1) + lk * ldz];
+                        h += cBe[q] * ez1;
+                        accZ[q] -= 0.5 * a.ce * ez1;
+                    }
+                    sH[li + lk * W] = h;
+                }
+            }
+        }
+        __syncthreads();
+        // boundary rows of Er onto rows 1 and nz-1 (atomic: with nz = 2 both land on row 1)
+#pragma unroll
+        for (int q = 0; q < MAXP; ++q) {
+            const int p = threadIdx.x + q * BS;
+            if (p < npos) {
+                const int li = p % (W + 1), lk = p / (W + 1);
+                const int gk = rk0 + lk;
+                if (li < W && gk == 0 && lk + 1 <= Hh) {
+                    const double v = sEr[li + lk * W];
+                    sEr[li + lk * W] = 0.0;
+                    atomicAdd(&sEr[li + (lk + 1) * W], v);
+                }
+                if (li < W && gk == nz && lk >= 1) {
+                    const double v = sEr[li + lk * W];
+                    sEr[li + lk * W] = 0.0;
+                    atomicAdd(&sEr[li + (lk - 1) * W], v);
+                }
+            }
+        }
+        __syncthreads();
+    };
+
+    h_adj(0.5);
+    for (int s = 0; s < a.n_sub; ++s) {
+        e_adj();
+        h_adj(1.0);
+    }
+    h_adj(-0.5);
+
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = threadIdx.x + q * BS;
+        if (p >= npos) continue;
+        const int li = p % (W + 1), lk = p / (W + 1);
+        const int gi = ri0 + li, gk = rk0 + lk;
+        const bool in_i = gi >= i0 && gi < i1;
+        const bool in_k = gk >= k0 && gk < k1;
+        if (li < W && lk < Hh && in_i && in_k) {
+            R.lh_out[gi + gk * nr] = sH[li + lk * W];
+            R.ljr[gi + gk * nr] = accR[q];
+            R.ljz[gi + gk * nr] = accZ[q];
+        }
+        if (li < W && in_i && (in_k || (gk == nz && k1 == nz))) R.ler_out[gi + gk * nr] = sEr[li + lk * W];
+        if (lk < Hh && in_k && (in_i || (gi == nr && i1 == nr))) R.lez_out[gi + gk * (nr + 1)] = sEz[li + lk * ldz];
+    }
+}
+
+template <int BS, int MAXP>
+static hipError_t launch_fdtd_adj_t(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_fdtd_adj<BS, MAXP>), dim3(a.ntr * a.ntk), dim3(BS), fdtd_lds_bytes(a), s, a, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s)
+{
+    const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
+    const int npos = (W + 1) * (Hh + 1);
+    if (fdtd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
+    if (npos <= 256 * 4) return launch_fdtd_adj_t<256, 4>(a, r, s);
+    if (npos <= 512 * 4) return launch_fdtd_adj_t<512, 4>(a, r, s);
+    if (npos <= 512 * 8) return launch_fdtd_adj_t<512, 8>(a, r, s);
+    if (npos <= 1024 * 8) return launch_fdtd_adj_t<1024, 8>(a, r, s);
+    return hipErrorInvalidValue;
+}
+
 // Raises the dynamic LDS limit of every field-kernel variant to the CU's
 // 160 KB (once per device, at context creation: not a stream operation, so it
 // must not wait for the first launch, which may be under graph capture).
 hipError_t fdtd_prepare()
 {
-    const void* fns[] = {(const void*)k_fdtd<256, 4>, (const void*)k_fdtd<512, 4>,
-                         (const void*)k_fdtd<512, 8>, (const void*)k_fdtd<1024, 8>};
+    const void* fns[] = {(const void*)k_fdtd<256, 4>,     (const void*)k_fdtd<512, 4>,
+                         (const void*)k_fdtd<512, 8>,     (const void*)k_fdtd<1024, 8>,
+                         (const void*)k_fdtd_adj<256, 4>, (const void*)k_fdtd_adj<512, 4>,
+                         (const void*)k_fdtd_adj<512, 8>, (const void*)k_fdtd_adj<1024, 8>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;

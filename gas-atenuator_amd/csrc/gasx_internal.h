@@ -170,6 +170,13 @@ struct FdtdRun {
     unsigned long long *qr_clear, *qz_clear;  // accumulators to zero for the next deposit (or null)
 };
 
+// Transposed field cycle: field sensitivities in -> out, current sensitivities out.
+struct AdjFdtdRun {
+    const double *ler_in, *lez_in, *lh_in;
+    double *ler_out, *lez_out, *lh_out;
+    double *ljr, *ljz;
+};
+
 // ---------------------------------------------------------------------------
 // Launchers (defined in the .hip files)
 // ---------------------------------------------------------------------------
@@ -177,6 +184,8 @@ struct FdtdRun {
 hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
                        int fixed_j, hipStream_t s);
 size_t fdtd_lds_bytes(const FdtdArgs& a);
+// transposed cycle (adjoint), single run, tiles of a
+hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s);
 // push / collide / stage (kPush, kCollide, kStageSec, kStageIon, kDriftSec, kBatch)
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s);

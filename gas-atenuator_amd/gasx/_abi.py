@@ -167,6 +167,7 @@ PROTOTYPES = {
     "gasx_diff_backward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "gasx_diff_events": (C.c_int, [_P, C.c_int32, _P, _P, _P, _P, _P]),
     "gasx_diff_release": (C.c_int, [_P]),
+    "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
 ABI_VERSION = 3

@@ -192,6 +192,12 @@ class Context:
         fs = f.struct()
         self._check(self.lib.gasx_fdtd(self.ctx, C.byref(fs), _ptr(jr), _ptr(jz), n_push))
 
+    def fdtd_transpose(self, lam: FieldState, ljr: torch.Tensor, ljz: torch.Tensor):
+        """Transposed field cycle of one push (the adjoint's): lam <- M^T lam,
+        (ljr, ljz) <- B^T lam."""
+        fs = lam.struct()
+        self._check(self.lib.gasx_fdtd_transpose(self.ctx, C.byref(fs), _ptr(ljr), _ptr(ljz)))
+
     def fields_sync(self, f: FieldState):
         fs = f.struct()
         self._check(self.lib.gasx_fields_sync(self.ctx, C.byref(fs)))

@@ -303,6 +303,10 @@ gasx_status gasx_diff_events(gasx_ctx* ctx, int32_t step, float* dvx, float* dvy
                              uint8_t* collided);
 /* Frees the recorded run. */
 gasx_status gasx_diff_release(gasx_ctx* ctx);
+/* The adjoint's transposed field cycle on its own (for tests): with the
+ * cycle F' = M F + B J of gasx_fdtd (one push), replaces lam by M^T lam and
+ * writes B^T lam to ljr, ljz (device, nr*nz). */
+gasx_status gasx_fdtd_transpose(gasx_ctx* ctx, gasx_fields* lam, double* ljr, double* ljz);
 
 #ifdef __cplusplus
 }

@@ -158,3 +158,42 @@ def test_adjoint_errors(case):
     with pytest.raises(GasxError):
         case["run"].backward()
     case["run"].forward(case["nodes0"], 1.0)
+
+
+@pytest.mark.parametrize("nr,nz,kw", [
+    (40, 70, {}),
+    (37, 101, {"ez_metric": 1, "axis_j_sign": -1}),
+    (16, 32, {"n_sub": 4}),
+    (90, 33, {"n_sub": 1}),
+    (24, 2, {"n_sub": 3}),
+])
+def test_transposed_field_cycle_dot_product(gpu, nr, nz, kw):
+    """<lam, M F + B J> = <M^T lam, F> + <B^T lam, J> for random F, J, lam,
+    with the forward cycle of gasx_fdtd (pinned to the oracle in
+    test_gpu_parity)."""
+    import gasx
+    p = gasx.make_params(nr, nz, **kw)
+    ctx = gasx.Context(p)
+    rng = np.random.default_rng(nr * 1000 + nz)
+    F = [rng.standard_normal((nz + 1, nr)), rng.standard_normal((nz, nr + 1)), rng.standard_normal((nz, nr))]
+    L = [rng.standard_normal((nz + 1, nr)), rng.standard_normal((nz, nr + 1)), rng.standard_normal((nz, nr))]
+    jr, jz = rng.standard_normal((nz, nr)) * 1e3, rng.standard_normal((nz, nr)) * 1e3
+    fs = gasx.FieldState(nr, nz)
+    for t, a in zip((fs.er, fs.ez, fs.hphi), F):
+        t.copy_(torch.from_numpy(a))
+    ctx.fdtd(fs, torch.from_numpy(jr).cuda(), torch.from_numpy(jz).cuda(), 1)
+    MF = fs.numpy()
+    lam = gasx.FieldState(nr, nz)
+    for t, a in zip((lam.er, lam.ez, lam.hphi), L):
+        t.copy_(torch.from_numpy(a))
+    ljr = torch.zeros((nz, nr), dtype=torch.float64, device="cuda")
+    ljz = torch.zeros_like(ljr)
+    ctx.fdtd_transpose(lam, ljr, ljz)
+    MtL = lam.numpy()
+    lhs = sum(float((l * m).sum()) for l, m in zip(L, MF))
+    rhs = sum(float((a * b).sum()) for a, b in zip(MtL, F)) + float((ljr.cpu().numpy() * jr).sum()) + \
+        float((ljz.cpu().numpy() * jz).sum())
+    scale = sum(float(np.abs(l * m).sum()) for l, m in zip(L, MF))
+    assert abs(lhs - rhs) <= 1e-11 * scale
+    # and the current part alone (F = 0): the transposed map's B^T is not zero
+    assert np.abs(ljr.cpu().numpy()).max() > 0 and np.abs(ljz.cpu().numpy()).max() > 0
