@@ -187,7 +187,7 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
 def bench_adjoint(seed: int, steps: int | None = None):
     """Differentiable run of the adjoint workload (128x256, 5e5 electrons,
     200 pushes, 32-node radial profile with nodes 2.5e25 (1 + 0.1 U(-1, 1)),
-    checkpoints every 20 steps, objective W_Er over r >= 0.9 Rp): the time of
+    checkpoints every 20 steps, objective W_Er over r >= 0.25 Rp): the time of
     one replayed forward and of one adjoint (backward, recomputation
     included) pass, reported as (forward + adjoint) / forward, and against
     the production step (gasx_run with collisions, same electrons, profile
@@ -209,7 +209,8 @@ def bench_adjoint(seed: int, steps: int | None = None):
     nodes0 = w.density * (1 + 0.1 * np.random.default_rng(seed).uniform(-1.0, 1.0, nn))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run = gasx.DiffRun(ctx, e, nodes0, n_steps=n_steps, checkpoint_every=every, seed=seed + 3, r_frac=0.9)
+    run = gasx.DiffRun(ctx, e, nodes0, n_steps=n_steps, checkpoint_every=every, seed=seed + 3,
+                       r_frac=w.extra["shell"])
     t_rec = time.perf_counter() - t0
     run.forward(nodes0, 1.0)                      # warm
     run.backward()
@@ -238,7 +239,7 @@ def bench_adjoint(seed: int, steps: int | None = None):
     torch.cuda.synchronize()
     t_prod = time.perf_counter() - t0
     return {"workload": w.name, "grid": f"{w.nr}x{w.nz}", "electrons": w.n_electrons, "steps": n_steps,
-            "nodes": nn, "checkpoint_every": every, "record_s": t_rec, "forward_s": t_fwd,
+            "nodes": nn, "checkpoint_every": every, "shell_r_over_rp": w.extra["shell"], "record_s": t_rec, "forward_s": t_fwd,
             "backward_s": t_bwd, "backward_over_forward": t_bwd / t_fwd,
             "forward_plus_adjoint_over_forward": (t_fwd + t_bwd) / t_fwd,
             "production_forward_s": t_prod, "backward_over_production_forward": t_bwd / t_prod,

@@ -88,8 +88,8 @@ def test_raw_tables_shapes():
     assert t["t_sdel"].shape == (182, 8)
     assert t["t_ion"].shape == (38, 2)         # 15.6 eV .. 20 keV
     assert t["t_exc"].shape == (7, 5)          # CR line endings handled
-    assert t["t_ion"][0, 0] == pytest.approx(15.6)
-    assert t["t_ion"][-1, 0] == pytest.approx(20000.0)
+    assert t["t_ion"][0, 0] == pytest.approx(15.6, rel=1e-6, abs=0)
+    assert t["t_ion"][-1, 0] == pytest.approx(20000.0, rel=1e-6, abs=0)
 
 
 def test_read_csv_handles_cr_line_endings(tmp_path):

@@ -75,19 +75,19 @@ def test_collision_free_run_matches_oracle_em_loop():
     er, ez, h, _ = oracle.em_step(op, ost, z((g.nz + 1) * g.nr), z(g.nz * (g.nr + 1)), z(g.nz * g.nr), steps)
     w_or = dm.objective(g, torch.as_tensor(er).view(g.nz + 1, g.nr), 0.0).item()
     assert w_or > 0
-    assert w.item() == pytest.approx(w_or, rel=1e-9)
+    assert w.item() == pytest.approx(w_or, rel=1e-9, abs=0)
 
 
 def test_amplitude_scales_linear_regime():
-    """Weak source (no self-field feedback on the push at this FN): Er is linear
-    in A, so W(A) = A^2 W(1)."""
+    """Weak source: at FN = 1 the self-field moves the particles by ~1e-8 of
+    their paths, so Er is linear in A to that level and W(A) = A^2 W(1)."""
     g, _ = grid_and_params(16, 24, fn=1.0)
     st = electrons(200, g, seed=6)
     nodes0 = np.full(8, 1e25)
     ev = no_events(st.shape[1], 4)
     w1 = dm.run(g, st, ev, torch.as_tensor(nodes0), nodes0, torch.tensor(1.0, dtype=torch.float64), 4, 0.0)
     w3 = dm.run(g, st, ev, torch.as_tensor(nodes0), nodes0, torch.tensor(3.0, dtype=torch.float64), 4, 0.0)
-    assert w3.item() == pytest.approx(9 * w1.item(), rel=1e-9)
+    assert w3.item() == pytest.approx(9 * w1.item(), rel=1e-6, abs=0)
 
 
 def synthetic_events(n, steps, seed):
@@ -118,7 +118,7 @@ def test_uniform_scaling_derivative_matches_richardson_differences():
     lams = np.array([0.98, 0.99, 1.01, 1.02])
     ws = [dm.run(g, st, ev, float(l) * torch.as_tensor(nodes0), nodes0, amp, steps, 0.0).item() for l in lams]
     fd = (8 * (ws[2] - ws[1]) - (ws[3] - ws[0])) / (12 * 0.01)
-    assert dw.item() == pytest.approx(fd, rel=1e-6)
+    assert dw.item() == pytest.approx(fd, rel=1e-6, abs=0)
 
 
 def test_autograd_matches_central_differences():
@@ -145,4 +145,4 @@ def test_autograd_matches_central_differences():
         fd = (W(nodes0 + e, 1.0) - W(nodes0 - e, 1.0)) / (2 * e[j])
         assert gn[j].item() == pytest.approx(fd, rel=1e-5, abs=1e-6 * np.abs(gn.numpy()).max())
     fda = (W(nodes0, 1 + h) - W(nodes0, 1 - h)) / (2 * h)
-    assert ga.item() == pytest.approx(fda, rel=1e-6)
+    assert ga.item() == pytest.approx(fda, rel=1e-6, abs=0)

@@ -66,7 +66,7 @@ def test_recorded_events_are_collisions(case):
 def test_objective_matches_fp64_model(case):
     w_ref, _, _ = reference(case)
     assert case["w"] > 0
-    assert case["w"] == pytest.approx(w_ref, rel=1e-5)
+    assert case["w"] == pytest.approx(w_ref, rel=1e-5, abs=0)
 
 
 def test_gradient_matches_fp64_autograd(case):
@@ -75,7 +75,7 @@ def test_gradient_matches_fp64_autograd(case):
     assert np.linalg.norm(g_ref) > 0
     err = np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref)
     assert err < 2e-3, (g, g_ref)
-    assert case["ga"] == pytest.approx(ga_ref, rel=2e-3)
+    assert case["ga"] == pytest.approx(ga_ref, rel=2e-3, abs=0)
 
 
 def test_gradient_matches_central_differences(case):
@@ -85,14 +85,14 @@ def test_gradient_matches_central_differences(case):
     h = 1e-3
     d = g * nodes0 / np.linalg.norm(g * nodes0) * nodes0        # relative direction, nodes-weighted
     fd = (run.forward(nodes0 + h * d, 1.0) - run.forward(nodes0 - h * d, 1.0)) / (2 * h)
-    assert fd == pytest.approx(float(g @ d), rel=1e-2)
+    assert fd == pytest.approx(float(g @ d), rel=1e-2, abs=0)
     j = int(np.argmax(np.abs(g * nodes0)))
     e = np.zeros_like(nodes0)
     e[j] = h * nodes0[j]
     fdj = (run.forward(nodes0 + e, 1.0) - run.forward(nodes0 - e, 1.0)) / (2 * e[j])
-    assert fdj == pytest.approx(g[j], rel=2e-2)
+    assert fdj == pytest.approx(g[j], rel=2e-2, abs=0)
     fda = (run.forward(nodes0, 1 + h) - run.forward(nodes0, 1 - h)) / (2 * h)
-    assert fda == pytest.approx(case["ga"], rel=1e-3)
+    assert fda == pytest.approx(case["ga"], rel=1e-3, abs=0)
     run.forward(nodes0, 1.0)
 
 
@@ -105,9 +105,9 @@ def test_checkpoint_interval_does_not_change_gradient(case):
     base = case["run"] = record(gasx, ctx, e, nodes0)   # restore the module's recorded run
     base.forward(nodes0, 1.0)
     for w, g, ga in outs:
-        assert w == pytest.approx(case["w"], rel=1e-6)
+        assert w == pytest.approx(case["w"], rel=1e-6, abs=0)
         assert np.linalg.norm(g - case["g"]) <= 1e-4 * np.linalg.norm(case["g"])
-        assert ga == pytest.approx(case["ga"], rel=1e-5)
+        assert ga == pytest.approx(case["ga"], rel=1e-5, abs=0)
 
 
 def test_gradient_off_reference_profile(case):
@@ -118,10 +118,28 @@ def test_gradient_off_reference_profile(case):
     w = run.forward(nodes, 1.7)
     g, ga = run.backward()
     w_ref, g_ref, ga_ref = reference(case, nodes=nodes, amp=1.7)
-    assert w == pytest.approx(w_ref, rel=1e-5)
+    assert w == pytest.approx(w_ref, rel=1e-5, abs=0)
     assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 2e-3
-    assert ga == pytest.approx(ga_ref, rel=2e-3)
+    assert ga == pytest.approx(ga_ref, rel=2e-3, abs=0)
     run.forward(nodes0, 1.0)
+
+
+def test_tiny_objective_gradient(gpu):
+    """FN = 1e-16 scales the current (and every field) by ~3e-21 against the
+    other cases, so W ~ 1e-41 of theirs: the adjoint's power-of-two seed
+    scaling keeps the FP32 sensitivities representable, and objective and
+    gradient still match the FP64 model."""
+    gasx, p, ctx, st, e, nodes0 = setup(fn=1e-16)
+    run = record(gasx, ctx, e, nodes0)
+    c = {"p": p, "st": st, "nodes0": nodes0, "events": [run.events(s) for s in range(STEPS)]}
+    w = run.forward(nodes0, 1.0)
+    g, ga = run.backward()
+    w_ref, g_ref, ga_ref = reference(c)
+    assert 0 < w_ref < 1e-50
+    assert w == pytest.approx(w_ref, rel=1e-5, abs=0)
+    assert np.linalg.norm(g_ref) > 0
+    assert np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref) < 2e-3
+    assert ga == pytest.approx(ga_ref, rel=2e-3, abs=0)
 
 
 def test_autograd_function(case):
@@ -132,10 +150,10 @@ def test_autograd_function(case):
     # a second forward in between: backward restores the first one's point
     run.forward(nodes0 * 1.1, 1.0)
     (2.0 * w).backward()
-    assert w.item() == pytest.approx(case["w"], rel=1e-6)
+    assert w.item() == pytest.approx(case["w"], rel=1e-6, abs=0)
     np.testing.assert_allclose(nodes.grad.numpy(), 2.0 * case["g"], rtol=1e-6,
                                atol=1e-6 * np.abs(case["g"]).max())
-    assert amp.grad.item() == pytest.approx(2.0 * case["ga"], rel=1e-6)
+    assert amp.grad.item() == pytest.approx(2.0 * case["ga"], rel=1e-6, abs=0)
 
 
 def test_adjoint_errors(case):
