@@ -1,12 +1,13 @@
-// Fused particle kernel: one pass over the FP32 SoA per step.
+// Particle pass of a step: two kernels over the FP32 SoA.
 //
+// k_particles<F> (the flags select the phases):
 //   kPush     EM4.cu:36-177   gather E,H at x^{n+1/2}, relativistic push, half drift, BC
 //   kCollide  electron_transport.cu:64-227 (+ find_CS0.m cross sections)
-//   kDrift    EM2.cu:23-57    next half drift + BC  (x^{n+3/2})
-//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+3/2} with v^{n+1}
 //   kStageSec / kStageIon     electron_collision.m:63-75 append, staged per wave
 //                             segment in slot order so the final order is deterministic
-//
+//   kDriftSec                 half drift + BC of the staged secondaries (the
+//                             primaries get theirs in k_drift_deposit)
+//   kBatch                    segments belong to the runs of a batched ensemble
 // Each wave owns segments of kSeg consecutive slots, kPer consecutive slots
 // per lane (vector loads and stores), and grid-strides over segments with no
 // workgroup barrier, so waves hide each other's latency.  Per segment:
@@ -15,12 +16,16 @@
 //   B  the wave's candidates are compacted into its LDS queue and processed
 //      one per lane (cross sections, decision, elastic / ionization /
 //      excitation), so the collision work is dense instead of divergent
-//   C  every lane: read back, next half drift, store; the fixed-point current
-//      is summed over runs of equal deposit stencils (per lane, then across
-//      the wave) and added with one global atomic per run, cell and component;
-//      secondaries and ions are staged
+//   C  every lane: read back, store; secondaries and ions are staged
+//
+// k_drift_deposit<kDr, kDe, kBat>:
+//   kDrift    EM2.cu:23-57    next half drift + BC  (x^{n+3/2})
+//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+3/2} with v^{n+1},
+//                             summed in a workgroup LDS tile and flushed into
+//                             the workgroup's copy of the current
+//
 // The slot count is read once at kernel start; appends land in a later
-// kernel (launch_append), so the count is stable inside this kernel.
+// kernel (launch_append), so the count is stable inside these kernels.
 #include "particle_physics.h"
 
 namespace gasx {
