@@ -1,6 +1,11 @@
 // Device helpers shared by the particle kernels: Philox4x32-10, wedge
 // boundary conditions, clamped area weights, table lookups and the collision
 // physics of electron_transport.cu / find_CS0.m.
+//
+// Every function of the particle step turns FP contraction off: the single-run
+// and the batched-ensemble kernels inline them into differently shaped code,
+// and only uncontracted arithmetic is guaranteed to round the same in both
+// (a batched run is bit-identical to the same run alone).
 #pragma once
 
 #include "gasx_internal.h"
@@ -51,6 +56,7 @@ __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x)
 __device__ __forceinline__ void wedge_wrap_axis(const Geom& g, float r2, float& x, float& y,
                                              float& vx, float& vy)
 {
+#pragma clang fp contract(off)
     const float r = fsqrt(r2);
     float phi = atanf(y / x);
     float rot;
@@ -86,6 +92,7 @@ __device__ __forceinline__ void wedge_wrap_axis(const Geom& g, float r2, float& 
 __device__ __forceinline__ bool wedge_bc(const Geom& g, float& x, float& y, float& z,
                                          float& vx, float& vy, float& vz)
 {
+#pragma clang fp contract(off)
     const float r2 = x * x + y * y;
     if (r2 > g.rp * g.rp) return false;
     if (x > 0.0f) {
@@ -115,6 +122,7 @@ __device__ __forceinline__ bool wedge_bc(const Geom& g, float& x, float& y, floa
 // collapse onto the edge cell with weight 1.
 __device__ __forceinline__ void half_axis(float s, int n, int& i0, int& i1, float& w0, float& w1)
 {
+#pragma clang fp contract(off)
     // s = x/d - 0.5
     const float fl = floorf(s);
     const int a = (int)fl;
@@ -129,6 +137,7 @@ __device__ __forceinline__ void half_axis(float s, int n, int& i0, int& i1, floa
 // Linear weights on a node axis (nodes at i*d, i in [0,n]): EM4.cu:41-80.
 __device__ __forceinline__ void node_axis(float s, int n, int& i0, int& i1, float& w0, float& w1)
 {
+#pragma clang fp contract(off)
     const float fl = floorf(s);
     int a = (int)fl;
     float f = s - fl;
@@ -146,6 +155,7 @@ __device__ __forceinline__ void node_axis(float s, int n, int& i0, int& i1, floa
 // interp1 'linear' on a sorted table; caller guarantees x[0] <= xq <= x[n-1].
 __device__ __forceinline__ float interp_in(const float* x, const float* y, int n, float xq)
 {
+#pragma clang fp contract(off)
     int lo = 0, hi = n - 1;
     while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
@@ -158,6 +168,7 @@ __device__ __forceinline__ float interp_in(const float* x, const float* y, int n
 // find_CS0.m:14-56 for an energy T (eV) inside [15.6, 20000].
 __device__ __forceinline__ void cross_sections(const SmallTabs* t, float T, float& se, float& si, float& sx)
 {
+#pragma clang fp contract(off)
     if (T >= 50.0f) se = interp_in(t->el_e, t->el_s, t->n_el, T);
     else se = interp_in(t->sel_e, t->sel_s, t->n_sel, T);
     si = interp_in(t->ion_e, t->ion_s, t->n_ion, T);
@@ -176,6 +187,7 @@ __device__ __forceinline__ void cross_sections(const SmallTabs* t, float T, floa
 // Speed from kinetic energy E (eV), findV.m:5-8 in cancellation-free form.
 __device__ __forceinline__ float speed_from_ev(float E, float mc2_ev, float c)
 {
+#pragma clang fp contract(off)
     const float g1 = E * frcp(mc2_ev);
     return c * fsqrt(g1 * (g1 + 2.0f)) * frcp(1.0f + g1);
 }
@@ -184,6 +196,7 @@ __device__ __forceinline__ float speed_from_ev(float E, float mc2_ev, float c)
 // electricmagneticfield6.m:172, as m c^2 beta^2 gamma^2 / (1+gamma).
 __device__ __forceinline__ float ev_from_speed2(float v2, float inv_c2, float mc2_ev)
 {
+#pragma clang fp contract(off)
     const float b2 = v2 * inv_c2;
     const float g2 = frcp(1.0f - b2);
     const float gm = fsqrt(g2);
@@ -193,6 +206,7 @@ __device__ __forceinline__ float ev_from_speed2(float v2, float inv_c2, float mc
 // rotate.m:4-13 with v^2 - vz^2 evaluated as vx^2 + vy^2.
 __device__ __forceinline__ void rotate_v(float& vx, float& vy, float& vz, float th, float ph, float eps)
 {
+#pragma clang fp contract(off)
     const float x = vx, y = vy, zz = vz;
     const float vp2 = x * x + y * y;
     const float v = fsqrt(vp2 + zz * zz);
@@ -217,6 +231,7 @@ __device__ __forceinline__ void rotate_v(float& vx, float& vy, float& vz, float 
 // electron_transport.cu:212-219 (form 1).
 __device__ __forceinline__ float exc_edep(const SmallTabs* t, float T, int form)
 {
+#pragma clang fp contract(off)
     float k1 = 0.0f, k2 = 0.0f;
     const float l2T = __log2f(T);
     for (int i = 0; i < t->n_exc; ++i) {

@@ -72,6 +72,7 @@ __device__ __forceinline__ void deposit_global(const DepCells& d, int nr, unsign
 __device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, float x, float y,
                                          float z, float& vx, float& vy, float& vz)
 {
+#pragma clang fp contract(off)
     const float r2 = x * x + y * y;
     float cph = 1.0f, sph = 0.0f, r = 0.0f;
     if (r2 > 0.0f) {
@@ -116,6 +117,7 @@ __device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, floa
 __device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, float& z, float vx,
                                            float vy, float vz)
 {
+#pragma clang fp contract(off)
     const float h = 0.5f * g.dt;
     x = x + h * vx;
     y = y + h * vy;
@@ -134,6 +136,7 @@ __device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, fl
 __device__ inline float elastic_theta_exact(const DevTables* t, const SmallTabs* st, int i,
                                                  float w, float u)
 {
+#pragma clang fp contract(off)
     const float* v0 = &t->dcs_logv[i * kNAng];
     const float* v1 = &t->dcs_logv[(i + 1) * kNAng];
     float tot = 0.0f, prev = 0.0f;
@@ -164,6 +167,7 @@ __device__ inline float elastic_theta_exact(const DevTables* t, const SmallTabs*
 __device__ inline float elastic_theta(const DevTables* t, const SmallTabs* st, const CollArgs& ca,
                                uint32_t slot, float e_kev)
 {
+#pragma clang fp contract(off)
     const float lx = logf(e_kev);
     int i = 0, hi = st->n_dcs - 1;                 // bin: dcs_loge[i] <= lx (clamped to n-2)
     while (hi - i > 1) {
@@ -212,12 +216,16 @@ struct CollOut {
 // find_CS0.m cross sections, interp1 NaN gating (collide only for
 // 15.6 eV <= T <= 20 keV), linear p_T, branch choice by u2, and the elastic /
 // ionization / excitation kinematics of electron_transport.cu with the
-// MATLAB choices.
+// MATLAB choices.  Contraction is off in the collision math (here and in
+// its callees): the kernels that inline it (single run, batched ensemble)
+// then cannot fuse differently, and a batched run stays bit-identical to the
+// same run alone.
 __device__ inline void collide_candidate(const Geom& g, const CollArgs& ca, const SmallTabs* stab,
                                   uint32_t slot, float x, float y, float z, float& vx, float& vy,
                                   float& vz, float u1, float u2, float u3, float u4, bool want_ion,
                                   CollOut& o)
 {
+#pragma clang fp contract(off)
     o.ev = 0;
     o.pt = 0.0f;
     const float v2 = vx * vx + vy * vy + vz * vz;

@@ -165,7 +165,7 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
 }
 
 constexpr int kQW = 11;     // LDS queue words per candidate
-constexpr int kQCapW = 64;  // candidates per wave round (one per lane)
+constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes 64)
 
 // Event counts of a wave added to the block's slot and, for an ensemble, to
 // the run's slot (lane 0 adds; several waves share the block slot).
@@ -208,15 +208,27 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[kPer])
 // Push and collision pass.  Waves work independently: wave segments of kSeg
 // consecutive slots (lane l owns slots seg*kSeg + l*kPer .. + kPer-1),
 // grid-stride over segments; no workgroup barrier inside the loop.
+//
+// Collision candidates (u1 < pmax, energy inside the tables' window) are
+// queued in the wave's LDS ring across segments and processed 64 at a time,
+// one per lane, so a round is full however few candidates a segment has.
+// A segment's slots are stored when it is pushed; a round then rewrites the
+// velocities of the particles that collided (after a fence on the wave's
+// earlier stores) and stages their secondaries and ions.
+// The queue is in (segment, slot) order, so each staged entry's rank inside
+// its segment -- the append order -- is the parent slot order.  Batched
+// ensembles give each wave a contiguous range of segments, so its ring rarely
+// has to drain at a run boundary.
 template <uint32_t F>
 __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, CollArgs ca0)
 {
     static_assert((F & (kDrift | kDeposit)) == 0, "drift and deposit run in k_drift_deposit");
+    static_assert((F & (kStageSec | kStageIon)) == 0 || (F & kCollide), "staging needs collisions");
     constexpr bool kCol = (F & kCollide) != 0;
     constexpr bool kBat = (F & kBatch) != 0;
     constexpr int kWaves = kBlock / kWaveSize;
     __shared__ int s_wave[kWaves];
-    __shared__ float s_q[kCol ? kWaves : 1][kCol ? kQW : 1][kCol ? kQCapW : 1];
+    __shared__ float s_q[kCol ? kWaves * kQW * kQCap : 1];   // per wave: kQW rows of kQCap
     __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
     static_assert(sizeof(SmallTabs) % 4 == 0, "SmallTabs is copied in 32-bit words");
     __shared__ uint32_t s_tab_raw[kCol ? sizeof(SmallTabs) / 4 : 1];
@@ -264,6 +276,86 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                           kAlign) == 0;
     unsigned cnt[kNCounters] = {};
     int cnt_run = -1;
+
+    // ---- the wave's candidate ring (wave-uniform state) ---------------------
+    float (*Q)[kQCap] = reinterpret_cast<float (*)[kQCap]>(s_q + (kCol ? wave * kQW * kQCap : 0));
+    int q_head = 0, q_n = 0;
+    int c_seg = -1, c_sec = 0, c_ion = 0;   // staging counts carried for the last round's last segment
+    PartArgs qpa = pa0;                       // the queued entries' run (one run at a time)
+    CollArgs qca = ca0;
+    int q_cs = 0;                             // first staging segment of that run
+    // Processes the m (1..64) oldest queued candidates, one per lane.
+    auto round = [&](int m) {
+        // the segments' stores (by other lanes) complete before this round rewrites velocities
+        __threadfence_block();
+        int qi = q_head + lane;
+        if (qi >= kQCap) qi -= kQCap;
+        const bool act = lane < m;
+        int ev = 0, slot = 0, gs = 0;
+        bool seg_last = false;   // the last candidate of its segment: records the segment's counts
+        float px = 0.f, py = 0.f, pz = 0.f;
+        CollOut co{};
+        if (act) {
+            px = Q[0][qi]; py = Q[1][qi]; pz = Q[2][qi];
+            float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
+            const int raw = __float_as_int(Q[10][qi]);
+            slot = raw & 0x7fffffff;
+            seg_last = raw < 0;
+            collide_candidate(g, qca, s_tab, (uint32_t)slot, px, py, pz, vx, vy, vz, Q[6][qi], Q[7][qi],
+                              Q[8][qi], Q[9][qi], (F & kStageIon) != 0, co);
+            ev = co.ev;
+            if (ev & kEvMask) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
+            gs = q_cs + slot / kSeg;
+        }
+        wave_sync();   // the ring entries are free once read
+        bool has_sec = (F & kStageSec) && act && (ev & kEvSec);
+        const bool has_ion = (F & kStageIon) && act && (ev & kEvIon);
+        float sx = px, sy = py, sz = pz, svx = co.svx, svy = co.svy, svz = co.svz;
+        if ((F & kDriftSec) && has_sec) {
+            half_drift(g, sx, sy, sz, svx, svy, svz);
+            if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; ++cnt[kCAbsorbed]; }
+        }
+        // rank of each staged entry inside its segment: entries of earlier rounds
+        // (carried) plus the earlier lanes of this round in the same segment
+        const int gprev = __shfl_up(gs, 1, 64);
+        const unsigned long long heads = __ballot(act && (lane == 0 || gs != gprev));
+        const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+        const int head = 63 - __clzll(heads & (below | (1ull << lane)));
+        const unsigned long long span = below & ~((head == 0) ? 0ull : ((~0ull) >> (64 - head)));
+        int rs = __popcll(__ballot(has_sec) & span), ri = __popcll(__ballot(has_ion) & span);
+        if (gs == c_seg) { rs += c_sec; ri += c_ion; }
+        if (has_sec) {
+            const int d = gs * kSeg + rs;
+            qpa.sec[0][d] = sx; qpa.sec[1][d] = sy; qpa.sec[2][d] = sz;
+            qpa.sec[3][d] = svx; qpa.sec[4][d] = svy; qpa.sec[5][d] = svz;
+        }
+        if (has_ion) {   // ion at the ionization site (electron_collision.m:63)
+            const int d = gs * kSeg + ri;
+            qpa.ion[0][d] = px; qpa.ion[1][d] = py; qpa.ion[2][d] = pz;
+            qpa.ion[3][d] = co.ivx; qpa.ion[4][d] = co.ivy; qpa.ion[5][d] = co.ivz;
+        }
+        const int rs1 = rs + has_sec, ri1 = ri + has_ion;
+        if (seg_last) {
+            if (F & kStageSec) qpa.sec_cnt[gs] = rs1;
+            if (F & kStageIon) qpa.ion_cnt[gs] = ri1;
+        }
+        c_seg = __shfl(gs, m - 1, 64);
+        c_sec = __shfl(rs1, m - 1, 64);
+        c_ion = __shfl(ri1, m - 1, 64);
+        const int typ = ev & kEvMask;
+        cnt[kCElastic] += typ == 1;
+        cnt[kCIonize] += typ == 2;
+        cnt[kCExcite] += typ == 3;
+        cnt[kCViol] += (ev & kEvViol) != 0;
+        cnt[kCSat] += (ev & kEvSat) != 0;
+        q_head += m;
+        if (q_head >= kQCap) q_head -= kQCap;
+        q_n -= m;
+    };
+    auto drain = [&]() {
+        while (q_n > 0) round(q_n < kWaveSize ? q_n : kWaveSize);
+    };
+
     const int n_waves = gridDim.x * kWaves;
     for (int gseg = blockIdx.x * kWaves + wave; gseg < total_segs; gseg += n_waves) {
         // seg: index within the population (slots, Philox counters);
@@ -304,10 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
         const int slot0 = seg * kSeg + lane * kPer;
         const bool vec = aligned && slot0 + kPer <= n;
         float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
-        bool alive[kPer], was_alive[kPer], absorbed[kPer], cand[kPer];
-        int ev[kPer];
-        float svx[kPer], svy[kPer], svz[kPer];
-        float ivx[kPer], ivy[kPer], ivz[kPer];
+        bool alive[kPer], was_alive[kPer], cand[kPer];
         float u1[kPer], u2[kPer], u3[kPer], u4[kPer];
         if (vec) {
             vload(pa.x + slot0, x); vload(pa.y + slot0, y); vload(pa.z + slot0, z);
@@ -324,151 +413,110 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                 }
             }
         }
+        // ---- A: push + first Philox block; candidates ---------------------------
 #pragma unroll
         for (int p = 0; p < kPer; ++p) {
             const int slot = slot0 + p;
             alive[p] = slot < n && !is_dead(z[p]);
             was_alive[p] = alive[p];
-            absorbed[p] = false;
-            ev[p] = 0;
             cand[p] = false;
-            svx[p] = svy[p] = svz[p] = ivx[p] = ivy[p] = ivz[p] = 0.f;
-            // ---- A: push + first Philox block ------------------------------
             if ((F & kPush) && alive[p]) {
                 push_one(g, pa, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
-                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) { alive[p] = false; absorbed[p] = true; }
+                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+                    alive[p] = false;
+                    ++cnt[kCAbsorbed];
+                }
             }
             u1[p] = 1.f; u2[p] = u3[p] = u4[p] = 0.f;
             if (kCol && alive[p]) {
                 const U4 R = philox4x32_10((uint32_t)slot, ca.step, ca.run, 0u, ca.key0, ca.key1);
                 u1[p] = u01(R.a); u2[p] = u01(R.b); u3[p] = u01(R.c); u4[p] = u01(R.d);
-                cand[p] = u1[p] < ca.pmax;     // exact: p_T <= pmax for every particle
+                // exact: p_T <= pmax for every particle, and collide_candidate does
+                // nothing outside the tables' energy window (the window is widened
+                // by 1e-6 here, so rounding of this copy of T never drops a particle
+                // that collide_candidate would test)
+                const float T = ev_from_speed2(vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p], g.inv_c2,
+                                               ca.ev_per_mc2_em);
+                cand[p] = u1[p] < ca.pmax && T >= 15.6f * (1.0f - 1e-6f) && T <= 20000.0f * (1.0f + 1e-6f);
             }
-        }
-        // ---- B: the wave's collision candidates, one per lane per round ----------
-        if constexpr (kCol) {
-            int mine = 0;
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) mine += cand[p];
-            int total;
-            const int off = wave_prefix_small<kPerBits>(mine, total);
-            float (*q_)[kQCapW] = s_q[wave];
-            for (int base = 0; base < total; base += kQCapW) {
-                int pos = off;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (cand[p]) {
-                        const int q = pos - base;
-                        if (q >= 0 && q < kQCapW) {
-                            q_[0][q] = x[p]; q_[1][q] = y[p]; q_[2][q] = z[p];
-                            q_[3][q] = vx[p]; q_[4][q] = vy[p]; q_[5][q] = vz[p];
-                            q_[6][q] = u1[p]; q_[7][q] = u2[p]; q_[8][q] = u3[p]; q_[9][q] = u4[p];
-                            q_[10][q] = __int_as_float(slot0 + p);
-                        }
-                        ++pos;
-                    }
-                }
-                wave_sync();
-                if (lane < min(kQCapW, total - base)) {
-                    const int q = lane;
-                    float qvx = q_[3][q], qvy = q_[4][q], qvz = q_[5][q];
-                    CollOut co;
-                    collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(q_[10][q]), q_[0][q], q_[1][q],
-                                      q_[2][q], qvx, qvy, qvz, q_[6][q], q_[7][q], q_[8][q], q_[9][q],
-                                      (F & kStageIon) != 0, co);
-                    q_[3][q] = qvx; q_[4][q] = qvy; q_[5][q] = qvz;
-                    q_[10][q] = __int_as_float(co.ev);
-                    if (co.ev & kEvSec) { q_[6][q] = co.svx; q_[7][q] = co.svy; q_[8][q] = co.svz; }
-                    if (co.ev & kEvIon) { q_[0][q] = co.ivx; q_[1][q] = co.ivy; q_[2][q] = co.ivz; }
-                }
-                wave_sync();
-                pos = off;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (cand[p]) {
-                        const int q = pos - base;
-                        if (q >= 0 && q < kQCapW) {
-                            vx[p] = q_[3][q]; vy[p] = q_[4][q]; vz[p] = q_[5][q];
-                            ev[p] = __float_as_int(q_[10][q]);
-                            if (ev[p] & kEvSec) { svx[p] = q_[6][q]; svy[p] = q_[7][q]; svz[p] = q_[8][q]; }
-                            if (ev[p] & kEvIon) { ivx[p] = q_[0][q]; ivy[p] = q_[1][q]; ivz[p] = q_[2][q]; }
-                        }
-                        ++pos;
-                    }
-                }
-                wave_sync();                     // the queue is rewritten by the next round
-            }
-        }
-        // ---- C: store; secondaries get their half drift (the primaries' is a
-        // later launch, k_drift_deposit) ------------------------------------------
-        bool has_sec[kPer], has_ion[kPer];
-        float sx[kPer], sy[kPer], sz[kPer];   // secondary position (drifted with it)
-#pragma unroll
-        for (int p = 0; p < kPer; ++p) {
-            has_sec[p] = (F & kStageSec) && (ev[p] & kEvSec);
-            has_ion[p] = (F & kStageIon) && (ev[p] & kEvIon);
-            sx[p] = x[p]; sy[p] = y[p]; sz[p] = z[p];
-            if ((F & kDriftSec) && has_sec[p]) {
-                half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
-                if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) { has_sec[p] = false; ++cnt[kCAbsorbed]; }
-            }
+            if (F & kPush) cnt[kCPushed] += was_alive[p];
             if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
         }
-        if (vec) {   // dead slots are written back unchanged
-            vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
-            vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
-        } else {
+        if constexpr (kBat) {
+            // ---- B (ensemble): the segment's candidates in rounds of 64 before its
+            // store, results handed back to their owners through the LDS queue --
+            // a wave of an ensemble meets a new run at most segments, where a ring
+            // would have to drain anyway
+            int ev[kPer] = {};
+            float svx[kPer] = {}, svy[kPer] = {}, svz[kPer] = {}, ivx[kPer] = {}, ivy[kPer] = {}, ivz[kPer] = {};
+            if constexpr (kCol) {
+                int mine = 0;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) mine += cand[p];
+                int total;
+                const int off = wave_prefix_small<kPerBits>(mine, total);
+                for (int base = 0; base < total; base += kWaveSize) {
+                    int pos = off;
+#pragma unroll
+                    for (int p = 0; p < kPer; ++p) {
+                        if (cand[p]) {
+                            const int q = pos - base;
+                            if (q >= 0 && q < kWaveSize) {
+                                Q[0][q] = x[p]; Q[1][q] = y[p]; Q[2][q] = z[p];
+                                Q[3][q] = vx[p]; Q[4][q] = vy[p]; Q[5][q] = vz[p];
+                                Q[6][q] = u1[p]; Q[7][q] = u2[p]; Q[8][q] = u3[p]; Q[9][q] = u4[p];
+                                Q[10][q] = __int_as_float(slot0 + p);
+                            }
+                            ++pos;
+                        }
+                    }
+                    wave_sync();
+                    if (lane < min(kWaveSize, total - base)) {
+                        const int q = lane;
+                        float qvx = Q[3][q], qvy = Q[4][q], qvz = Q[5][q];
+                        CollOut co;
+                        collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(Q[10][q]), Q[0][q], Q[1][q],
+                                          Q[2][q], qvx, qvy, qvz, Q[6][q], Q[7][q], Q[8][q], Q[9][q],
+                                          (F & kStageIon) != 0, co);
+                        Q[3][q] = qvx; Q[4][q] = qvy; Q[5][q] = qvz;
+                        Q[10][q] = __int_as_float(co.ev);
+                        if (co.ev & kEvSec) { Q[6][q] = co.svx; Q[7][q] = co.svy; Q[8][q] = co.svz; }
+                        if (co.ev & kEvIon) { Q[0][q] = co.ivx; Q[1][q] = co.ivy; Q[2][q] = co.ivz; }
+                    }
+                    wave_sync();
+                    pos = off;
+#pragma unroll
+                    for (int p = 0; p < kPer; ++p) {
+                        if (cand[p]) {
+                            const int q = pos - base;
+                            if (q >= 0 && q < kWaveSize) {
+                                vx[p] = Q[3][q]; vy[p] = Q[4][q]; vz[p] = Q[5][q];
+                                ev[p] = __float_as_int(Q[10][q]);
+                                if (ev[p] & kEvSec) { svx[p] = Q[6][q]; svy[p] = Q[7][q]; svz[p] = Q[8][q]; }
+                                if (ev[p] & kEvIon) { ivx[p] = Q[0][q]; ivy[p] = Q[1][q]; ivz[p] = Q[2][q]; }
+                            }
+                            ++pos;
+                        }
+                    }
+                    wave_sync();   // the queue is rewritten by the next round
+                }
+            }
+            // ---- C (ensemble): store; stage in slot order -------------------------
+            bool has_sec[kPer], has_ion[kPer];
+            float sx[kPer], sy[kPer], sz[kPer];
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
-                const int slot = slot0 + p;
-                if (slot < n && was_alive[p]) {
-                    if (alive[p]) {
-                        pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
-                        pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
-                    } else {
-                        pa.z[slot] = z[p];
+                has_sec[p] = (F & kStageSec) && (ev[p] & kEvSec);
+                has_ion[p] = (F & kStageIon) && (ev[p] & kEvIon);
+                sx[p] = x[p]; sy[p] = y[p]; sz[p] = z[p];
+                if ((F & kDriftSec) && has_sec[p]) {
+                    half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
+                    if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) {
+                        has_sec[p] = false;
+                        ++cnt[kCAbsorbed];
                     }
                 }
-            }
-        }
-        // ---- staging in slot order (lane major, then p) ------------------------
-        if (F & kStageSec) {
-            int mine = 0;
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) mine += has_sec[p];
-            int tot;
-            int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) {
-                if (has_sec[p]) {
-                    pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
-                    pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
-                    ++dst;
-                }
-            }
-            if (lane == 0) pa.sec_cnt[gseg] = tot;
-        }
-        if (F & kStageIon) {
-            int mine = 0;
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) mine += has_ion[p];
-            int tot;
-            int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) {
-                if (has_ion[p]) {
-                    // ion at the ionization site (electron_collision.m:63)
-                    pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
-                    pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
-                    ++dst;
-                }
-            }
-            if (lane == 0) pa.ion_cnt[gseg] = tot;
-        }
-#pragma unroll
-        for (int p = 0; p < kPer; ++p) {
-            if (kCol) {
                 const int typ = ev[p] & kEvMask;
                 cnt[kCElastic] += typ == 1;
                 cnt[kCIonize] += typ == 2;
@@ -476,10 +524,114 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                 cnt[kCViol] += (ev[p] & kEvViol) != 0;
                 cnt[kCSat] += (ev[p] & kEvSat) != 0;
             }
-            cnt[kCAbsorbed] += (unsigned)absorbed[p];
-            if (F & kPush) cnt[kCPushed] += was_alive[p];
+            if (vec) {
+                vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
+                vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
+            } else {
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    const int slot = slot0 + p;
+                    if (slot < n && was_alive[p]) {
+                        if (alive[p]) {
+                            pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
+                            pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
+                        } else {
+                            pa.z[slot] = z[p];
+                        }
+                    }
+                }
+            }
+            if (F & kStageSec) {
+                int mine = 0;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) mine += has_sec[p];
+                int tot;
+                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (has_sec[p]) {
+                        pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
+                        pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
+                        ++dst;
+                    }
+                }
+                if (lane == 0) pa.sec_cnt[gseg] = tot;
+            }
+            if (F & kStageIon) {
+                int mine = 0;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) mine += has_ion[p];
+                int tot;
+                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (has_ion[p]) {   // ion at the ionization site (electron_collision.m:63)
+                        pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
+                        pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
+                        ++dst;
+                    }
+                }
+                if (lane == 0) pa.ion_cnt[gseg] = tot;
+            }
+        } else {
+            // ---- store (a later round rewrites the velocities of those that collide) --
+            if (vec) {   // dead slots are written back unchanged
+                vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
+                vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
+            } else {
+    #pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    const int slot = slot0 + p;
+                    if (slot < n && was_alive[p]) {
+                        if (alive[p]) {
+                            pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
+                            pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
+                        } else {
+                            pa.z[slot] = z[p];
+                        }
+                    }
+                }
+            }
+            // ---- B: queue the candidates; full rounds as the ring fills -------------
+            if constexpr (kCol) {
+                int mine = 0;
+    #pragma unroll
+                for (int p = 0; p < kPer; ++p) mine += cand[p];
+                int total;
+                const int off = wave_prefix_small<kPerBits>(mine, total);
+                if (lane == 0 && total == 0) {   // otherwise the round of its last candidate writes them
+                    if (F & kStageSec) pa.sec_cnt[gseg] = 0;
+                    if (F & kStageIon) pa.ion_cnt[gseg] = 0;
+                }
+                int done = 0;
+                while (done < total) {
+                    const int take = min(total - done, kQCap - q_n);
+                    int pos = off;
+    #pragma unroll
+                    for (int p = 0; p < kPer; ++p) {
+                        if (cand[p]) {
+                            const int r = pos - done;
+                            if (r >= 0 && r < take) {
+                                int qi = q_head + q_n + r;
+                                if (qi >= kQCap) qi -= kQCap;
+                                Q[0][qi] = x[p]; Q[1][qi] = y[p]; Q[2][qi] = z[p];
+                                Q[3][qi] = vx[p]; Q[4][qi] = vy[p]; Q[5][qi] = vz[p];
+                                Q[6][qi] = u1[p]; Q[7][qi] = u2[p]; Q[8][qi] = u3[p]; Q[9][qi] = u4[p];
+                                // slot, and in the sign bit: the segment's last candidate
+                                Q[10][qi] = __int_as_float((slot0 + p) | (pos == total - 1 ? (int)0x80000000 : 0));
+                            }
+                            ++pos;
+                        }
+                    }
+                    q_n += take;
+                    done += take;
+                    wave_sync();
+                    while (q_n >= kWaveSize) round(kWaveSize);
+                }
+            }
         }
     }
+    if constexpr (kCol) drain();
     if (pa0.counters && (!kBat || cnt_run >= 0))
         wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
                         kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
