@@ -939,6 +939,7 @@ gasx_status diff_step(gasx_ctx* ctx, DiffState* d, int s, bool record, const Col
     r.er_in = fin[0]; r.ez_in = fin[1]; r.h_in = fin[2];
     r.er_out = fout[0]; r.ez_out = fout[1]; r.h_out = fout[2];
     r.er32 = f32[0]; r.ez32 = f32[1]; r.h32 = f32[2];
+    r.er_mid = ctx->er_m; r.ez_mid = ctx->ez_m; r.h_mid = ctx->h_m;
     r.qr = d->qr; r.qz = d->qz;
     HIPCHK(launch_fdtd(ctx->fa, r, nullptr, 1, 1, ctx->stream), "field cycle");
     parity ^= 1;

@@ -30,6 +30,7 @@ void dfree(T*& p)
 void free_run_buffers(BatchState& b)
 {
     dfree(b.er_b); dfree(b.ez_b); dfree(b.h_b);
+    dfree(b.er_m); dfree(b.ez_m); dfree(b.h_m);
     dfree(b.er32); dfree(b.ez32); dfree(b.h32);
     for (auto& qq : b.q) { dfree(qq[0]); dfree(qq[1]); }
     dfree(b.runs); dfree(b.fdtd_runs); dfree(b.chunk_start); dfree(b.tmp_run); dfree(b.run_counters);
@@ -72,6 +73,9 @@ gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs)
         HIPCHK(dalloc(&b.er_b, n * z.ner), "alloc batch er");
         HIPCHK(dalloc(&b.ez_b, n * z.nez), "alloc batch ez");
         HIPCHK(dalloc(&b.h_b, n * z.nh), "alloc batch h");
+        HIPCHK(dalloc(&b.er_m, n * z.ner), "alloc batch er mid");
+        HIPCHK(dalloc(&b.ez_m, n * z.nez), "alloc batch ez mid");
+        HIPCHK(dalloc(&b.h_m, n * z.nh), "alloc batch h mid");
         HIPCHK(dalloc(&b.er32, n * z.ner), "alloc batch er32");
         HIPCHK(dalloc(&b.ez32, n * z.nez), "alloc batch ez32");
         HIPCHK(dalloc(&b.h32, n * z.nh), "alloc batch h32");
@@ -186,6 +190,9 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
                 R.er32 = st.er32 + (size_t)r * z.ner;
                 R.ez32 = st.ez32 + (size_t)r * z.nez;
                 R.h32 = st.h32 + (size_t)r * z.nh;
+                R.er_mid = st.er_m + (size_t)r * z.ner;
+                R.ez_mid = st.ez_m + (size_t)r * z.nez;
+                R.h_mid = st.h_m + (size_t)r * z.nh;
                 const int cur = parity, nxt = parity ^ 1;   // step s uses q[s & 1]
                 R.qr = st.q[cur][0] + (size_t)r * z.nh;
                 R.qz = st.q[cur][1] + (size_t)r * z.nh;

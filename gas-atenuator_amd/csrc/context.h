@@ -13,6 +13,7 @@ struct BatchState {
     int nruns_cap = 0;            // runs the buffers below hold
     int64_t stage_segs = 0;       // staging segments allocated
     double *er_b = nullptr, *ez_b = nullptr, *h_b = nullptr;      // FP64 ping-pong partners, run-major
+    double *er_m = nullptr, *ez_m = nullptr, *h_m = nullptr;      // mid buffers of a split field cycle
     float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies, run-major
     unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // int64 current, run-major
     gasx::BatchRun* runs = nullptr;             // device descriptors
@@ -43,6 +44,7 @@ struct gasx_ctx {
     gasx::DevTables* d_tab = nullptr;
     float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies
     double *er_b = nullptr, *ez_b = nullptr, *h_b = nullptr;      // FP64 ping-pong partner
+    double *er_m = nullptr, *ez_m = nullptr, *h_m = nullptr;      // mid buffers of a split field cycle
     unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buf][r/z]
     double *jr_tmp = nullptr, *jz_tmp = nullptr;
     unsigned long long* d_counters = nullptr;

@@ -22,7 +22,12 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
     const int tiles_per_run = a.ntr * a.ntk;
     const int run = blockIdx.x / tiles_per_run;
     const int t = blockIdx.x - run * tiles_per_run;
-    const FdtdRun R = runs ? runs[run] : r0;
+    FdtdRun R = runs ? runs[run] : r0;
+    // a cycle split over two launches: the first writes the mid buffers, the
+    // second reads them (a.part: 0 whole cycle, 1 first, 2 second)
+    if (a.part == 1) { R.er_out = R.er_mid; R.ez_out = R.ez_mid; R.h_out = R.h_mid; }
+    if (a.part == 2) { R.er_in = R.er_mid; R.ez_in = R.ez_mid; R.h_in = R.h_mid; }
+    const bool final_part = a.part != 1;
     const int nr = a.nr, nz = a.nz;
     const int ti_idx = t % a.ntr, tk_idx = t / a.ntr;
     const int i0 = ti_idx * a.ti, i1 = min(nr, i0 + a.ti);
@@ -144,12 +149,13 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
         __syncthreads();
     };
 
-    h_update(-0.5);
-    for (int s = 0; s < a.n_sub; ++s) {
+    const int s_lo = a.part == 2 ? a.split : 0, s_hi = a.part == 1 ? a.split : a.n_sub;
+    if (a.part != 2) h_update(-0.5);
+    for (int s = s_lo; s < s_hi; ++s) {
         h_update(1.0);
         e_update();
     }
-    h_update(0.5);
+    if (final_part) h_update(0.5);
 
     // ---- write the tile interior ------------------------------------------
     for (int p = threadIdx.x; p < npos; p += BS) {
@@ -160,8 +166,8 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
         if (li < W && lk < Hh && in_i && in_k) {
             const double v = sH[li + lk * W];
             R.h_out[gi + gk * nr] = v;
-            R.h32[gi + gk * nr] = (float)v;
-            if (R.qr_clear) {
+            if (final_part) R.h32[gi + gk * nr] = (float)v;
+            if (final_part && R.qr_clear) {
                 R.qr_clear[gi + gk * nr] = 0ull;
                 R.qz_clear[gi + gk * nr] = 0ull;
             }
@@ -169,12 +175,12 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
         if (li < W && in_i && (in_k || (gk == nz && k1 == nz))) {
             const double v = sEr[li + lk * W];
             R.er_out[gi + gk * nr] = v;
-            R.er32[gi + gk * nr] = (float)v;
+            if (final_part) R.er32[gi + gk * nr] = (float)v;
         }
         if (lk < Hh && in_k && (in_i || (gi == nr && i1 == nr))) {
             const double v = sEz[li + lk * ldz];
             R.ez_out[gi + gk * (nr + 1)] = v;
-            R.ez32[gi + gk * (nr + 1)] = (float)v;
+            if (final_part) R.ez32[gi + gk * (nr + 1)] = (float)v;
         }
     }
 }
@@ -390,8 +396,8 @@ hipError_t fdtd_prepare()
     return hipSuccess;
 }
 
-hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
-                       int fixed_j, hipStream_t s)
+static hipError_t launch_fdtd_part(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
+                                   int fixed_j, hipStream_t s)
 {
     const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
     const int npos = (W + 1) * (Hh + 1);
@@ -401,6 +407,25 @@ hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs
     if (npos <= 512 * 8) return launch_fdtd_t<512, 8>(a, r0, runs_dev, nruns, fixed_j, s);
     if (npos <= 1024 * 8) return launch_fdtd_t<1024, 8>(a, r0, runs_dev, nruns, fixed_j, s);
     return hipErrorInvalidValue;
+}
+
+// One field cycle; with 0 < a.split < n_sub, as two launches of split and
+// n_sub - split substeps through the mid buffers, each with the halo its own
+// substeps need (less redundant halo work for one more pass over the fields).
+hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
+                       int fixed_j, hipStream_t s)
+{
+    if (a.split <= 0 || a.split >= a.n_sub) {
+        FdtdArgs w = a;
+        w.part = 0;
+        w.halo = a.n_sub + 3;
+        return launch_fdtd_part(w, r0, runs_dev, nruns, fixed_j, s);
+    }
+    FdtdArgs p1 = a, p2 = a;
+    p1.part = 1; p1.halo = a.split + 2;
+    p2.part = 2; p2.halo = a.n_sub - a.split + 2;
+    hipError_t e = launch_fdtd_part(p1, r0, runs_dev, nruns, fixed_j, s);
+    return e == hipSuccess ? launch_fdtd_part(p2, r0, runs_dev, nruns, fixed_j, s) : e;
 }
 
 // ---- fixed point -> A/m^2 ------------------------------------------------------

@@ -152,6 +152,8 @@ struct FdtdArgs {
     int32_t ti, tk, halo;        // tile interior size and halo
     int32_t ntr, ntk;            // tiles per run in r and z
     int32_t n_sub;
+    int32_t split;               // 0: one launch per cycle; else substeps of the first of two
+    int32_t part;                // set per launch: 0 whole cycle, 1 first part, 2 second part
     double dtf;                  // field substep
     double ch_r, ch_z;           // dtf/(mu dr), dtf/(mu dz)
     double ce, cez, c4;          // dtf/eps, dtf/(eps dz), 4 dtf/(eps dr)
@@ -164,6 +166,7 @@ struct FdtdArgs {
 struct FdtdRun {
     const double *er_in, *ez_in, *h_in;
     double *er_out, *ez_out, *h_out;
+    double *er_mid, *ez_mid, *h_mid;       // between the two launches of a split cycle
     float *er32, *ez32, *h32;
     const unsigned long long *qr, *qz;     // fixed-point current (or null)
     const double *jr, *jz;                 // FP64 current (or null)
