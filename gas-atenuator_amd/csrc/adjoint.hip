@@ -802,6 +802,7 @@ struct DiffState {
     double *F[3] = {}, *Fb[3] = {};      // FP64 fields and their ping-pong partner
     unsigned long long *qr = nullptr, *qz = nullptr;   // fixed-point current, kJCopies copies of nr*nz
     double *lF[3] = {}, *lFo[3] = {}, *ljr = nullptr, *ljz = nullptr;   // field lambdas (ping-pong)
+    double* lFm[3] = {};                 // mid buffers of a split transposed cycle
     double* lFc[3] = {};                 // kJCopies copies of each field lambda (scatter targets)
     float* lam[2][7] = {};               // particle lambdas (ping-pong)
     double* acc = nullptr;               // nn node gradients, dW/dA, W
@@ -1027,7 +1028,8 @@ gasx_status diff_forward_pass(gasx_ctx* ctx, DiffState* d, bool record, double* 
 // (ljr, ljz) <- B^T lF.
 gasx_status adj_cycle(gasx_ctx* ctx, DiffState* d)
 {
-    const AdjFdtdRun r{d->lF[0], d->lF[1], d->lF[2], d->lFo[0], d->lFo[1], d->lFo[2], d->ljr, d->ljz};
+    const AdjFdtdRun r{d->lF[0], d->lF[1], d->lF[2], d->lFo[0], d->lFo[1], d->lFo[2], d->ljr, d->ljz,
+                       d->lFm[0], d->lFm[1], d->lFm[2]};
     HIPCHK(launch_fdtd_adj(ctx->fa, r, ctx->stream), "transposed field cycle");
     for (int c = 0; c < 3; ++c) std::swap(d->lF[c], d->lFo[c]);
     return GASX_OK;
@@ -1071,6 +1073,7 @@ gasx_status diff_alloc(gasx_ctx* ctx, DiffState* d)
         HIPCHK(own(d, &d->Fb[c], fs[c]), "alloc fields");
         HIPCHK(own(d, &d->lF[c], fs[c]), "alloc field lambdas");
         HIPCHK(own(d, &d->lFo[c], fs[c]), "alloc field lambdas");
+        HIPCHK(own(d, &d->lFm[c], fs[c]), "alloc field lambdas");
     }
     HIPCHK(own(d, &d->qr, nh * kJCopies), "alloc current");
     HIPCHK(own(d, &d->qz, nh * kJCopies), "alloc current");
@@ -1299,7 +1302,8 @@ extern "C" gasx_status gasx_fdtd_transpose(gasx_ctx* ctx, gasx_fields* lam, doub
     if (!lam || !lam->er || !lam->ez || !lam->hphi || !ljr || !ljz)
         return fail(ctx, GASX_E_ARG, "gasx_fdtd_transpose: null array");
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    const AdjFdtdRun r{lam->er, lam->ez, lam->hphi, ctx->er_b, ctx->ez_b, ctx->h_b, ljr, ljz};
+    const AdjFdtdRun r{lam->er, lam->ez, lam->hphi, ctx->er_b, ctx->ez_b, ctx->h_b, ljr, ljz,
+                       ctx->er_m, ctx->ez_m, ctx->h_m};
     HIPCHK(launch_fdtd_adj(ctx->fa, r, ctx->stream), "transposed field cycle");
     gasx_status st = copy_d(ctx, lam->er, ctx->er_b, ner_of(ctx) * sizeof(double));
     if (st == GASX_OK) st = copy_d(ctx, lam->ez, ctx->ez_b, nez_of(ctx) * sizeof(double));

@@ -221,6 +221,11 @@ template <int BS, int MAXP>
 __global__ __launch_bounds__(BS) void k_fdtd_adj(FdtdArgs a, AdjFdtdRun R)
 {
     extern __shared__ double lds[];
+    // split as the forward cycle (a.part 1: H(+1/2)^T and n_sub - split
+    // substeps into the mid buffers; 2: the other split substeps and
+    // H(-1/2)^T, adding to the current sensitivities of part 1)
+    if (a.part == 1) { R.ler_out = R.ler_mid; R.lez_out = R.lez_mid; R.lh_out = R.lh_mid; }
+    if (a.part == 2) { R.ler_in = R.ler_mid; R.lez_in = R.lez_mid; R.lh_in = R.lh_mid; }
     const int nr = a.nr, nz = a.nz;
     const int t = blockIdx.x;
     const int ti_idx = t % a.ntr, tk_idx = t / a.ntr;
@@ -337,12 +342,13 @@ __global__ __launch_bounds__(BS) void k_fdtd_adj(FdtdArgs a, AdjFdtdRun R)
         __syncthreads();
     };
 
-    h_adj(0.5);
-    for (int s = 0; s < a.n_sub; ++s) {
+    const int n_here = a.part == 0 ? a.n_sub : (a.part == 1 ? a.n_sub - a.split : a.split);
+    if (a.part != 2) h_adj(0.5);
+    for (int s = 0; s < n_here; ++s) {
         e_adj();
         h_adj(1.0);
     }
-    h_adj(-0.5);
+    if (a.part != 1) h_adj(-0.5);
 
     for (int q = 0; q < MAXP; ++q) {
         const int p = threadIdx.x + q * BS;
@@ -353,8 +359,10 @@ __global__ __launch_bounds__(BS) void k_fdtd_adj(FdtdArgs a, AdjFdtdRun R)
         const bool in_k = gk >= k0 && gk < k1;
         if (li < W && lk < Hh && in_i && in_k) {
             R.lh_out[gi + gk * nr] = sH[li + lk * W];
-            R.ljr[gi + gk * nr] = accR[q];
-            R.ljz[gi + gk * nr] = accZ[q];
+            const double pr = a.part == 2 ? R.ljr[gi + gk * nr] : 0.0;   // part 1's share
+            const double pz = a.part == 2 ? R.ljz[gi + gk * nr] : 0.0;
+            R.ljr[gi + gk * nr] = pr + accR[q];
+            R.ljz[gi + gk * nr] = pz + accZ[q];
         }
         if (li < W && in_i && (in_k || (gk == nz && k1 == nz))) R.ler_out[gi + gk * nr] = sEr[li + lk * W];
         if (lk < Hh && in_k && (in_i || (gi == nr && i1 == nr))) R.lez_out[gi + gk * (nr + 1)] = sEz[li + lk * ldz];
@@ -368,7 +376,7 @@ static hipError_t launch_fdtd_adj_t(const FdtdArgs& a, const AdjFdtdRun& r, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s)
+static hipError_t launch_fdtd_adj_part(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s)
 {
     const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
     const int npos = (W + 1) * (Hh + 1);
@@ -378,6 +386,21 @@ hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s
     if (npos <= 512 * 8) return launch_fdtd_adj_t<512, 8>(a, r, s);
     if (npos <= 1024 * 8) return launch_fdtd_adj_t<1024, 8>(a, r, s);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s)
+{
+    if (a.split <= 0 || a.split >= a.n_sub) {
+        FdtdArgs w = a;
+        w.part = 0;
+        w.halo = a.n_sub + 3;
+        return launch_fdtd_adj_part(w, r, s);
+    }
+    FdtdArgs p1 = a, p2 = a;
+    p1.part = 1; p1.halo = a.n_sub - a.split + 2;
+    p2.part = 2; p2.halo = a.split + 2;
+    hipError_t e = launch_fdtd_adj_part(p1, r, s);
+    return e == hipSuccess ? launch_fdtd_adj_part(p2, r, s) : e;
 }
 
 // Raises the dynamic LDS limit of every field-kernel variant to the CU's

@@ -178,6 +178,7 @@ struct AdjFdtdRun {
     const double *ler_in, *lez_in, *lh_in;
     double *ler_out, *lez_out, *lh_out;
     double *ljr, *ljz;
+    double *ler_mid, *lez_mid, *lh_mid;    // between the two launches of a split cycle
 };
 
 // ---------------------------------------------------------------------------
