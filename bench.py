@@ -305,19 +305,26 @@ def main():
 
     run_kw = dict(collide=True, secondaries=w.secondaries, sort_every=w.sort_every, seed=args.seed + 1)
     reset()
-    ctx.run(e, f, gas, args.warmup, time_kernels=True, **run_kw)
+    ctx.run(e, f, gas, args.warmup, time_kernels=2, **run_kw)
     torch.cuda.synchronize()
     reset()
     ctx.counters(reset=True)
     torch.cuda.synchronize()
+    # timed region: HIP events around the particle-pass launches only (the
+    # roofline kernels); events around every launch would cost ~6% of the step
     t0 = time.perf_counter()
-    ctx.run(e, f, gas, args.steps, time_kernels=True, **run_kw)
+    ctx.run(e, f, gas, args.steps, time_kernels=2, **run_kw)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     c = ctx.counters()
     tm = ctx.timing()
     psteps = c.n_pushed
     value = psteps / wall
+    # per-kernel breakdown: the next steps of the same run with every launch timed
+    n_bd = min(args.steps, 200)
+    ctx.run(e, f, gas, n_bd, time_kernels=1, step0=args.steps, **run_kw)
+    torch.cuda.synchronize()
+    tb = ctx.timing()
 
     # The particle pass of a step is two kernels: k_particles (push + collide,
     # the dominant one, reported as the roofline kernel) and k_drift_deposit
@@ -376,12 +383,14 @@ def main():
             "ms_per_step": pass_ms,
             "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps / args.steps / (pass_ms * 1e-3) / 1e9,
         },
-        "breakdown_ms": {k: tm[k] for k in ("particle_ms", "drift_ms", "fdtd_ms", "append_ms", "sort_ms")},
+        "breakdown_us_per_step": {k[:-3]: tb[k] * 1e3 / n_bd
+                                  for k in ("particle_ms", "drift_ms", "fdtd_ms", "append_ms", "sort_ms")},
+        "breakdown_note": f"every launch timed, {n_bd} further steps of the same run",
         "events": {"elastic": c.n_elastic, "ionize": c.n_ionize, "excite": c.n_excite,
                    "absorbed": c.n_absorbed, "dropped_secondaries": c.n_dropped_sec,
                    "bound_violations": c.n_bound_violations, "final_slots": e.n},
-        "fdtd_cell_updates_per_s": w.nr * w.nz * w.n_sub * args.steps / (tm["fdtd_ms"] * 1e-3)
-        if tm["fdtd_ms"] > 0 else None,
+        "fdtd_cell_updates_per_s": w.nr * w.nz * w.n_sub * n_bd / (tb["fdtd_ms"] * 1e-3)
+        if tb["fdtd_ms"] > 0 else None,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, host, args.cpu_particles, args.cpu_steps, args.seed + 1)

@@ -18,7 +18,9 @@ size_t gasx_host::next_event(gasx_ctx* c)
 {
     if (c->ev_used == c->events.size()) {
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return (size_t)-1;
+        // timestamps only (read after a stream synchronize): no system-scope fence
+        // on record, which would otherwise add ~3 us per recorded launch
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return (size_t)-1;
         c->events.push_back(e);
     }
     return c->ev_used++;
@@ -414,6 +416,7 @@ static PartArgs make_part(gasx_ctx* ctx, const gasx_species* s)
 {
     PartArgs pa{};
     pa.x = s->x; pa.y = s->y; pa.z = s->z; pa.vx = s->vx; pa.vy = s->vy; pa.vz = s->vz;
+    pa.xi = s->x; pa.yi = s->y; pa.zi = s->z; pa.vxi = s->vx; pa.vyi = s->vy; pa.vzi = s->vz;
     pa.count = s->count;
     pa.cap = (int32_t)s->capacity;
     pa.q_over_m = (float)(s->qnum * ctx->p.e_em / s->mass);
@@ -528,9 +531,10 @@ extern "C" gasx_status gasx_fields_sync(gasx_ctx* ctx, const gasx_fields* f)
 // jstride > 0: qr, qz are the context's kJCopies-fold accumulators
 static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_species* s,
                                  unsigned long long* qr, unsigned long long* qz, const CollArgs& ca,
-                                 size_t jstride = 0, int timed_kind = kTParticle)
+                                 size_t jstride = 0, int timed_kind = kTParticle, float* const* in = nullptr)
 {
     PartArgs pa = make_part(ctx, s);
+    if (in) { pa.xi = in[0]; pa.yi = in[1]; pa.zi = in[2]; pa.vxi = in[3]; pa.vyi = in[4]; pa.vzi = in[5]; }
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
     TimedScope ts(ctx, timed_kind);
@@ -655,6 +659,10 @@ extern "C" gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, do
 // Fused loop shared by gasx_em_step (several pushed species, no collisions)
 // and gasx_run (electrons with collisions, optional appended ions).
 // ---------------------------------------------------------------------------
+// Cell sort of one species; sorted_out non-null leaves the permuted particles
+// in the sort scratch (their pointers returned) instead of copying them back.
+static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_out);
+
 struct LoopOpts {
     int n_steps = 0;
     bool collide = false, secondaries = false, ions = false;
@@ -695,6 +703,11 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         gasx_status st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[0][0], ctx->q[0][1], nh, kTDrift);
         if (st != GASX_OK) return st;
     }
+    // after an in-loop sort the last species sorted is read from the sort's
+    // permuted copies by its next particle launch, which writes it back in
+    // place of the sort's own copy back
+    float* sorted[6] = {};
+    int sorted_k = -1;
     for (int s = 0; s < o.n_steps; ++s) {
         const int cur = s & 1, nxt = cur ^ 1;
         const bool last = s == o.n_steps - 1;
@@ -724,8 +737,10 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                 if (o.secondaries) flags |= kStageSec | (last ? 0u : kDriftSec);
                 if (o.ions) flags |= kStageIon;
             }
-            st = run_particles(ctx, flags, &sp[k], nullptr, nullptr, ca);
+            st = run_particles(ctx, flags, &sp[k], nullptr, nullptr, ca, 0, kTParticle,
+                               k == sorted_k ? sorted : nullptr);
             if (st != GASX_OK) return st;
+            if (k == sorted_k) sorted_k = -1;
             if (!last) {
                 st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
                 if (st != GASX_OK) return st;
@@ -744,9 +759,10 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         }
         if (o.sort_every > 0 && !last && ((s + 1) % o.sort_every) == 0) {
             for (int k = 0; k < nsp; ++k) {
-                st = gasx_sort(ctx, &sp[k]);
+                st = sort_species(ctx, &sp[k], k == nsp - 1 ? sorted : nullptr);
                 if (st != GASX_OK) return st;
             }
+            sorted_k = nsp - 1;
         }
     }
     if (o.n_steps % 2 == 1) {
@@ -788,6 +804,8 @@ extern "C" gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* io
     o.step0 = ro->step0;
     if (ro->time_kernels) {
         ctx->timing = true;
+        // 2: the particle-pass launches only (the roofline kernels); else every launch
+        ctx->timing_kinds = ro->time_kernels == 2 ? (1u << kTParticle) | (1u << kTDrift) : ~0u;
         ctx->ev_used = 0;
         for (auto& v : ctx->tpairs) v.clear();
     }
@@ -800,6 +818,13 @@ extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
 {
     if (!ctx || !valid_species(s)) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    return sort_species(ctx, s, nullptr);
+}
+
+// sorted_out non-null: the permuted particles stay in the sort scratch (for
+// the next particle launch to read) instead of being copied back
+static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_out)
+{
     size_t need = 0;
     hipError_t e = launch_sort(ctx->geom, s, nullptr, 0, &need, ctx->stream);
     if (e != hipSuccess && e != hipErrorInvalidValue) return hip_fail(ctx, e, "sort size");
@@ -811,7 +836,7 @@ extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
         ctx->sort_bytes = need;
     }
     TimedScope ts(ctx, kTSort);
-    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream), "sort");
+    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream, sorted_out), "sort");
     return GASX_OK;
 }
 

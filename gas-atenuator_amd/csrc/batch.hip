@@ -231,6 +231,7 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
     }
     PartArgs pa{};
     pa.x = b->x; pa.y = b->y; pa.z = b->z; pa.vx = b->vx; pa.vy = b->vy; pa.vz = b->vz;
+    pa.xi = b->x; pa.yi = b->y; pa.zi = b->z; pa.vxi = b->vx; pa.vyi = b->vy; pa.vzi = b->vz;
     pa.count = b->counts;
     pa.q_over_m = (float)(-1.0 * ctx->p.e_em / ctx->p.me_em);
     pa.kq = -(float)(4294967296.0 / ctx->p.c);
@@ -401,6 +402,8 @@ extern "C" gasx_status gasx_batch_run(gasx_ctx* ctx, gasx_batch* b, const gasx_r
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
     if (o->time_kernels) {
         ctx->timing = true;
+        // 2: the particle-pass launches only (the roofline kernels); else every launch
+        ctx->timing_kinds = o->time_kernels == 2 ? (1u << kTParticle) | (1u << kTDrift) : ~0u;
         ctx->ev_used = 0;
         for (auto& v : ctx->tpairs) v.clear();
     }

@@ -65,6 +65,7 @@ struct gasx_ctx {
     std::string err;
     // kernel timing (gasx_run_opts.time_kernels): event pairs per launch kind
     bool timing = false;
+    unsigned timing_kinds = ~0u;   // bit per TimedKind: the kinds that get event pairs
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
     std::vector<std::pair<size_t, size_t>> tpairs[6];   // particle, fdtd, append, sort, graph, drift
@@ -88,7 +89,7 @@ struct TimedScope {
     size_t a = (size_t)-1;
     TimedScope(gasx_ctx* c_, int k) : c(c_), kind(k)
     {
-        if (c->timing) {
+        if (c->timing && (c->timing_kinds >> k & 1u)) {
             a = next_event(c);
             if (a != (size_t)-1) (void)hipEventRecord(c->events[a], c->stream);
         }

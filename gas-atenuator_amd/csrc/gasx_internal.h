@@ -121,6 +121,9 @@ struct CollArgs {
 
 struct PartArgs {
     float *x, *y, *z, *vx, *vy, *vz;
+    // where k_particles reads the particles (normally x..vz; after a cell sort
+    // inside the loop, the sort's permuted copies, saving its copy back)
+    const float *xi, *yi, *zi, *vxi, *vyi, *vzi;
     int32_t* count;
     int32_t cap;
     float q_over_m;        // qnum*e_em/mass
@@ -223,7 +226,9 @@ hipError_t launch_j_reduce(unsigned long long* qr, unsigned long long* qz, int64
 hipError_t launch_tallies(const gasx_params& p, const double* er, const double* ez, const double* h,
                           const gasx_species* sp, int nsp, double* partial, double* out,
                           hipStream_t s);
+// sorted_out non-null: the permuted arrays are left in the scratch (their six
+// pointers returned) instead of being copied back into s
 hipError_t launch_sort(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st);
+                       size_t* need_bytes, hipStream_t st, float** sorted_out = nullptr);
 
 }  // namespace gasx

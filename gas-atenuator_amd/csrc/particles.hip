@@ -272,8 +272,14 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
     constexpr uintptr_t kAlign = sizeof(typename VecF<kPer>::T) - 1;
     const bool aligned = ((reinterpret_cast<uintptr_t>(pa0.x) | reinterpret_cast<uintptr_t>(pa0.y) |
                            reinterpret_cast<uintptr_t>(pa0.z) | reinterpret_cast<uintptr_t>(pa0.vx) |
-                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz)) &
+                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz) |
+                           reinterpret_cast<uintptr_t>(pa0.xi) | reinterpret_cast<uintptr_t>(pa0.yi) |
+                           reinterpret_cast<uintptr_t>(pa0.zi) | reinterpret_cast<uintptr_t>(pa0.vxi) |
+                           reinterpret_cast<uintptr_t>(pa0.vyi) | reinterpret_cast<uintptr_t>(pa0.vzi)) &
                           kAlign) == 0;
+    // reading from other arrays than those written (the permuted copies a cell
+    // sort left): every slot below the count is written, dead ones included
+    const bool moved = pa0.xi != pa0.x;
     unsigned cnt[kNCounters] = {};
     int cnt_run = -1;
 
@@ -374,6 +380,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
             const BatchRun br = pa0.runs[run];
             pa.x += br.base; pa.y += br.base; pa.z += br.base;
             pa.vx += br.base; pa.vy += br.base; pa.vz += br.base;
+            pa.xi += br.base; pa.yi += br.base; pa.zi += br.base;
+            pa.vxi += br.base; pa.vyi += br.base; pa.vzi += br.base;
             n = pa0.count[run];
             const size_t nh = (size_t)nr_ * nz_;
             pa.er += (size_t)run * (nh + nr_);
@@ -399,8 +407,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
         bool alive[kPer], was_alive[kPer], cand[kPer];
         float u1[kPer], u2[kPer], u3[kPer], u4[kPer];
         if (vec) {
-            vload(pa.x + slot0, x); vload(pa.y + slot0, y); vload(pa.z + slot0, z);
-            vload(pa.vx + slot0, vx); vload(pa.vy + slot0, vy); vload(pa.vz + slot0, vz);
+            vload(pa.xi + slot0, x); vload(pa.yi + slot0, y); vload(pa.zi + slot0, z);
+            vload(pa.vxi + slot0, vx); vload(pa.vyi + slot0, vy); vload(pa.vzi + slot0, vz);
         } else {
 #pragma unroll
             for (int p = 0; p < kPer; ++p) {
@@ -408,8 +416,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                 x[p] = y[p] = vx[p] = vy[p] = vz[p] = 0.f;
                 z[p] = __int_as_float(0x7fc00000);
                 if (slot < n) {
-                    x[p] = pa.x[slot]; y[p] = pa.y[slot]; z[p] = pa.z[slot];
-                    vx[p] = pa.vx[slot]; vy[p] = pa.vy[slot]; vz[p] = pa.vz[slot];
+                    x[p] = pa.xi[slot]; y[p] = pa.yi[slot]; z[p] = pa.zi[slot];
+                    vx[p] = pa.vxi[slot]; vy[p] = pa.vyi[slot]; vz[p] = pa.vzi[slot];
                 }
             }
         }
@@ -531,8 +539,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
 #pragma unroll
                 for (int p = 0; p < kPer; ++p) {
                     const int slot = slot0 + p;
-                    if (slot < n && was_alive[p]) {
-                        if (alive[p]) {
+                    if (slot < n && (was_alive[p] || moved)) {
+                        if (alive[p] || moved) {
                             pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
                             pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
                         } else {
@@ -582,8 +590,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
     #pragma unroll
                 for (int p = 0; p < kPer; ++p) {
                     const int slot = slot0 + p;
-                    if (slot < n && was_alive[p]) {
-                        if (alive[p]) {
+                    if (slot < n && (was_alive[p] || moved)) {
+                        if (alive[p] || moved) {
                             pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
                             pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
                         } else {

@@ -61,7 +61,7 @@ __global__ void k_permute(const float* s0, const float* s1, const float* s2, con
 
 // scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][pad][rocprim tmp]
 hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st)
+                       size_t* need_bytes, hipStream_t st, float** sorted_out)
 {
     int32_t n = 0;
     hipError_t e = hipMemcpyAsync(&n, sp->count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
@@ -81,7 +81,13 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
     const size_t need = 4 * a + 6 * a + 256 + ((tmp_bytes + 255) & ~(size_t)255);
     if (need_bytes) *need_bytes = need;
     if (!scratch || scratch_bytes < need) return hipErrorInvalidValue;
-    if (n <= 0) return hipSuccess;
+    if (n <= 0) {
+        if (sorted_out) {   // nothing to permute: the particles stay where they are
+            float* own[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
+            for (int c = 0; c < 6; ++c) sorted_out[c] = own[c];
+        }
+        return hipSuccess;
+    }
     char* base = (char*)scratch;
     uint32_t* k_in = (uint32_t*)(base);
     uint32_t* k_out = (uint32_t*)(base + a);
@@ -105,6 +111,10 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
                        dead, n, sp->count);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (sorted_out) {
+        for (int c = 0; c < 6; ++c) sorted_out[c] = tmp[c];
+        return e;
+    }
     // copy back (slots past the live count hold stale values; the count excludes them)
     float* dst[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
     for (int c = 0; c < 6 && e == hipSuccess; ++c)

@@ -258,7 +258,7 @@ class Context:
     def run(self, e: ParticleSet, f: FieldState, gas: GasGrid | None, n_steps: int,
             collide: bool = True, secondaries: bool = False, sort_every: int = 0, seed: int = 0,
             run_id: int = 0, step0: int = 0, ions: ParticleSet | None = None, use_graph: bool = False,
-            time_kernels: bool = False):
+            time_kernels: bool | int = False):
         s = e.struct()
         fs = f.struct()
         g = gas.struct() if gas is not None else None
