@@ -464,9 +464,12 @@ int gasx_host::particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu)
     return (int)std::max<int64_t>(1, std::min(need, persistent));
 }
 
-static int particle_blocks(gasx_ctx* ctx, int64_t cap)
+// One generation of resident workgroups: each wave then walks the most
+// segments, so its candidate ring fills the most collision rounds (a round
+// costs the same however few lanes it uses).
+static int particle_blocks(gasx_ctx* ctx, int64_t cap, uint32_t flags)
 {
-    return particle_grid(ctx, (cap + kSeg - 1) / kSeg, 16);
+    return particle_grid(ctx, (cap + kSeg - 1) / kSeg, particle_occupancy(flags));
 }
 
 // one field cycle: reads (in) writes (out) and refreshes the FP32 copies
@@ -538,7 +541,7 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
     TimedScope ts(ctx, timed_kind);
-    HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity), ctx->stream),
+    HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity, flags), ctx->stream),
            "particles");
     return GASX_OK;
 }
@@ -552,7 +555,7 @@ static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool 
     PartArgs pa = make_part(ctx, s);
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
-    const int64_t chunks = (s->capacity + kChunk - 1) / kChunk;
+    const int64_t chunks = (s->capacity + kChunkD - 1) / kChunkD;
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, std::min<int64_t>(
                                                                              (int64_t)ctx->cu_count * 16,
                                                                              kCounterSlots)));

@@ -503,10 +503,15 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
         s = trial(c, &ms);
         if (s == GASX_OK && ms < best_ms) { best = c; best_ms = ms; }
     }
-    const int per_cu[] = {4, 8, 16, 32};
-    for (int pc : per_cu) {
+    // one or two generations of resident workgroups, or a fixed oversubscription
+    const int occ = particle_occupancy(kBatch | kPush | kCollide | kStageSec | kDriftSec);
+    const int per_cu[] = {occ, 2 * occ, 4, 8, 16, 32};
+    for (int k = 0; k < (int)(sizeof(per_cu) / sizeof(per_cu[0])); ++k) {
+        const int pc = per_cu[k];
         if (s != GASX_OK) break;
-        if (pc == best.particle_blocks_per_cu) continue;
+        bool seen = pc == best.particle_blocks_per_cu || pc > 64;
+        for (int j = 0; j < k; ++j) seen = seen || per_cu[j] == pc;
+        if (seen) continue;
         gasx_batch_tuning c = best;
         c.particle_blocks_per_cu = pc;
         float ms = 0.0f;

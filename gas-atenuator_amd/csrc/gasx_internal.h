@@ -20,6 +20,14 @@ constexpr int kWaveSize = 64;
 constexpr int kPer = GASX_KPER;  // consecutive slots per lane
 constexpr int kSeg = kWaveSize * kPer;  // slots per wave segment: the unit of work and of staging
 constexpr int kChunk = kBlock * kPer;   // alignment unit of pooled batch runs (a multiple of kSeg)
+#ifndef GASX_KPER_DRIFT
+#define GASX_KPER_DRIFT 8
+#endif
+// consecutive slots per lane of the half drift + deposit pass (a multiple of 4:
+// float4 loads); more slots per lane merge more same-cell contributions in
+// registers before they reach the shared current tile
+constexpr int kPerD = GASX_KPER_DRIFT;
+constexpr int kChunkD = kBlock * kPerD;  // slots per drift/deposit workgroup chunk
 constexpr int kCounterSlots = 8192;  // per-block event counter slots (>= particle kernel grid)
 // Copies of the fixed-point current accumulators: workgroup b adds into copy
 // b % kJCopies, which spreads the atomics on the hot cells of a dense spot
@@ -194,6 +202,8 @@ size_t fdtd_lds_bytes(const FdtdArgs& a);
 // transposed cycle (adjoint), single run, tiles of a
 hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s);
 // push / collide / stage (kPush, kCollide, kStageSec, kStageIon, kDriftSec, kBatch)
+// Workgroups of the particle kernel variant `flags` resident per CU at once.
+int particle_occupancy(uint32_t flags);
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s);
 // EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk chunks

@@ -89,7 +89,10 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 // cells and flushed with one global atomic per non-zero cell and component;
 // a stencil cell outside the box (or a box too large) goes to global memory
 // directly.  Integer sums: the result does not depend on the grouping.
-constexpr int kTile = 512;    // cells (2 x 4 KB of LDS)
+#ifndef GASX_DEP_TILE
+#define GASX_DEP_TILE 512
+#endif
+constexpr int kTile = GASX_DEP_TILE;   // cells (16 bytes of LDS each)
 
 struct TileBox {
     int ilo, klo, wi, nk;     // nk == 0: no tile (nothing to deposit, or the box exceeds kTile)
@@ -203,6 +206,27 @@ __device__ __forceinline__ void vstore(float* p, const float (&v)[kPer])
     T t;
     __builtin_memcpy(&t, v, sizeof(T));
     *reinterpret_cast<T*>(p) = t;
+}
+
+// N consecutive floats (N a multiple of 4) as float4 loads / stores.
+template <int N>
+__device__ __forceinline__ void vload4n(const float* p, float (&v)[N])
+{
+    static_assert(N % 4 == 0, "float4 pieces");
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+        const float4 t = reinterpret_cast<const float4*>(p)[q];
+        v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void vstore4n(float* p, const float (&v)[N])
+{
+    static_assert(N % 4 == 0, "float4 pieces");
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q)
+        reinterpret_cast<float4*>(p)[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
 // Push and collision pass.  Waves work independently: wave segments of kSeg
@@ -646,8 +670,8 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
 }
 
 // ---- half drift + deposit (EM2.cu:23-57, EM7.cu:51-202) ----------------------
-// Workgroups take chunks of kChunk consecutive slots (thread t owns slots
-// chunk*kChunk + t*kPer .. + kPer-1, vector loads and stores).  Each thread
+// Workgroups take chunks of kChunkD consecutive slots (thread t owns slots
+// chunk*kChunkD + t*kPerD .. + kPerD-1, float4 loads and stores).  Each thread
 // sums the contributions of its consecutive particles while they share a
 // stencil (cell-sorted slots mostly do), adds each run into the chunk's LDS
 // tile, and the tile is flushed into the workgroup's copy of the current.
@@ -666,7 +690,7 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
 #pragma unroll
         for (int j = 0; j < kRPer; ++j) {
             const int r = (int)threadIdx.x * kRPer + j;
-            loc[j] = r < pa0.nruns ? (pa0.count[r] + kChunk - 1) / kChunk : 0;
+            loc[j] = r < pa0.nruns ? (pa0.count[r] + kChunkD - 1) / kChunkD : 0;
             sum += loc[j];
         }
         int tot;
@@ -681,10 +705,10 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
         __syncthreads();
         total_chunks = tot;
     } else {
-        total_chunks = (*pa0.count + kChunk - 1) / kChunk;
+        total_chunks = (*pa0.count + kChunkD - 1) / kChunkD;
     }
     const int nr_ = g.nr, nz_ = g.nz;
-    constexpr uintptr_t kAlign = sizeof(typename VecF<kPer>::T) - 1;
+    constexpr uintptr_t kAlign = sizeof(float4) - 1;
     const bool aligned = ((reinterpret_cast<uintptr_t>(pa0.x) | reinterpret_cast<uintptr_t>(pa0.y) |
                            reinterpret_cast<uintptr_t>(pa0.z) | reinterpret_cast<uintptr_t>(pa0.vx) |
                            reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz)) &
@@ -725,16 +749,16 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
             pa.qr += off;
             pa.qz += off;
         }
-        const int slot0 = lchunk * kChunk + (int)threadIdx.x * kPer;
-        const bool vec = aligned && slot0 + kPer <= n;
-        float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
-        bool alive[kPer], was_alive[kPer];
+        const int slot0 = lchunk * kChunkD + (int)threadIdx.x * kPerD;
+        const bool vec = aligned && slot0 + kPerD <= n;
+        float x[kPerD], y[kPerD], z[kPerD], vx[kPerD], vy[kPerD], vz[kPerD];
+        bool alive[kPerD], was_alive[kPerD];
         if (vec) {
-            vload(pa.x + slot0, x); vload(pa.y + slot0, y); vload(pa.z + slot0, z);
-            vload(pa.vx + slot0, vx); vload(pa.vy + slot0, vy); vload(pa.vz + slot0, vz);
+            vload4n(pa.x + slot0, x); vload4n(pa.y + slot0, y); vload4n(pa.z + slot0, z);
+            vload4n(pa.vx + slot0, vx); vload4n(pa.vy + slot0, vy); vload4n(pa.vz + slot0, vz);
         } else {
 #pragma unroll
-            for (int p = 0; p < kPer; ++p) {
+            for (int p = 0; p < kPerD; ++p) {
                 const int slot = slot0 + p;
                 x[p] = y[p] = vx[p] = vy[p] = vz[p] = 0.f;
                 z[p] = __int_as_float(0x7fc00000);
@@ -746,7 +770,7 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
         }
         int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
 #pragma unroll
-        for (int p = 0; p < kPer; ++p) {
+        for (int p = 0; p < kPerD; ++p) {
             alive[p] = slot0 + p < n && !is_dead(z[p]);
             was_alive[p] = alive[p];
             if (kDr && alive[p]) {
@@ -765,11 +789,11 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
         }
         if (kDr) {
             if (vec) {   // dead slots are written back unchanged
-                vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
-                vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
+                vstore4n(pa.x + slot0, x); vstore4n(pa.y + slot0, y); vstore4n(pa.z + slot0, z);
+                vstore4n(pa.vx + slot0, vx); vstore4n(pa.vy + slot0, vy); vstore4n(pa.vz + slot0, vz);
             } else {
 #pragma unroll
-                for (int p = 0; p < kPer; ++p) {
+                for (int p = 0; p < kPerD; ++p) {
                     const int slot = slot0 + p;
                     if (slot < n && was_alive[p]) {
                         if (alive[p]) {
@@ -788,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
             __syncthreads();
             DepRun run_ = {-1, 0, 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
 #pragma unroll
-            for (int p = 0; p < kPer; ++p) {
+            for (int p = 0; p < kPerD; ++p) {
                 if (alive[p]) {
                     DepCells dc;
                     deposit_cells(g, x[p], y[p], z[p], vx[p], vy[p], vz[p], pa.kq, dc);
@@ -1160,6 +1184,40 @@ static hipError_t launch_t(const Geom& g, const PartArgs& pa, const CollArgs& ca
 {
     hipLaunchKernelGGL(k_particles<F>, dim3(blocks), dim3(kBlock), 0, s, g, pa, ca);
     return hipGetLastError();
+}
+
+template <uint32_t F>
+static int occupancy_t()
+{
+    static int cached = 0;          // per instantiation; the device set is homogeneous
+    if (cached == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_particles<F>, kBlock, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        cached = nb;
+    }
+    return cached;
+}
+
+int particle_occupancy(uint32_t flags)
+{
+    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec;
+    switch (flags) {
+    case P: return occupancy_t<P>();
+    case Co | S: return occupancy_t<Co | S>();
+    case Co | S | I: return occupancy_t<Co | S | I>();
+    case P | Co: return occupancy_t<P | Co>();
+    case P | Co | S: return occupancy_t<P | Co | S>();
+    case P | Co | I: return occupancy_t<P | Co | I>();
+    case P | Co | S | I: return occupancy_t<P | Co | S | I>();
+    case P | Co | S | DS: return occupancy_t<P | Co | S | DS>();
+    case P | Co | S | I | DS: return occupancy_t<P | Co | S | I | DS>();
+    case B | P: return occupancy_t<B | P>();
+    case B | P | Co: return occupancy_t<B | P | Co>();
+    case B | P | Co | S: return occupancy_t<B | P | Co | S>();
+    case B | P | Co | S | DS: return occupancy_t<B | P | Co | S | DS>();
+    default: return 1;
+    }
 }
 
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
