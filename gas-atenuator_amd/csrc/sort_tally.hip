@@ -12,52 +12,96 @@ namespace gasx {
 // into the same four cells and the particle kernel can sum them before its
 // atomics; absorbed slots get key nr*nz and sort last.  Stable radix sort
 // keeps slot order within a key.
+__device__ __forceinline__ uint32_t cell_key(const Geom& g, float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    if (is_dead(z)) return (uint32_t)(g.nr * g.nz);
+    float r2 = x * x;
+    r2 = r2 + y * y;
+    const float r = sqrtf(r2);
+    float sr = r * g.inv_dr;
+    sr = sr - 0.5f;
+    float sz = z * g.inv_dz;
+    sz = sz - 0.5f;
+    int ir = (int)floorf(sr), iz = (int)floorf(sz);
+    ir = ir < 0 ? 0 : (ir >= g.nr ? g.nr - 1 : ir);
+    iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
+    return (uint32_t)(iz * g.nr + ir);
+}
+
+// Four consecutive slots per thread: float4 loads where the arrays allow,
+// uint4 stores into the (256-byte aligned) sort scratch.
 __global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
                             uint32_t* keys, uint32_t* vals)
 {
-#pragma clang fp contract(off)
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float zz = z[i];
-        uint32_t key;
-        if (is_dead(zz)) {
-            key = (uint32_t)(g.nr * g.nz);
+    const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
+                           reinterpret_cast<uintptr_t>(z)) & 15) == 0;
+    const int ng = (n + 3) >> 2;
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x) {
+        const int i0 = gi * 4;
+        if (aligned && i0 + 4 <= n) {
+            const float4 a = *reinterpret_cast<const float4*>(x + i0);
+            const float4 b = *reinterpret_cast<const float4*>(y + i0);
+            const float4 c = *reinterpret_cast<const float4*>(z + i0);
+            *reinterpret_cast<uint4*>(keys + i0) = make_uint4(cell_key(g, a.x, b.x, c.x), cell_key(g, a.y, b.y, c.y),
+                                                             cell_key(g, a.z, b.z, c.z), cell_key(g, a.w, b.w, c.w));
+            *reinterpret_cast<uint4*>(vals + i0) = make_uint4(i0, i0 + 1, i0 + 2, i0 + 3);
         } else {
-            float r2 = x[i] * x[i];
-            r2 = r2 + y[i] * y[i];
-            const float r = sqrtf(r2);
-            float sr = r * g.inv_dr;
-            sr = sr - 0.5f;
-            float sz = zz * g.inv_dz;
-            sz = sz - 0.5f;
-            int ir = (int)floorf(sr), iz = (int)floorf(sz);
-            ir = ir < 0 ? 0 : (ir >= g.nr ? g.nr - 1 : ir);
-            iz = iz < 0 ? 0 : (iz >= g.nz ? g.nz - 1 : iz);
-            key = (uint32_t)(iz * g.nr + ir);
+            for (int i = i0; i < i0 + 4 && i < n; ++i) {
+                keys[i] = cell_key(g, x[i], y[i], z[i]);
+                vals[i] = (uint32_t)i;
+            }
         }
-        keys[i] = key;
-        vals[i] = (uint32_t)i;
     }
 }
 
-// Gathers the live slots in key order; the slot at the boundary between live
-// and dead keys records the live count (no atomics).
+// Gathers the sorted order into the scratch copies (four slots per thread,
+// uint4 / float4 accesses of the scratch) and sets the live count: the keys
+// are ascending with the dead slots (key == dead) last.
 __global__ void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,
                           const float* s4, const float* s5, float* d0, float* d1, float* d2,
                           float* d3, float* d4, float* d5, const uint32_t* perm, const uint32_t* keys,
                           uint32_t dead, int32_t n, int32_t* count)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t k = keys[i];
-        if (k < dead) {
-            const uint32_t s = perm[i];
-            d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
-            d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
-            if (i + 1 == n || keys[i + 1] >= dead) *count = i + 1;
-        } else if (i == 0) {
-            *count = 0;
+    const int ng = (n + 3) >> 2;
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x) {
+        const int i0 = gi * 4;
+        if (i0 + 4 <= n) {
+            const uint4 k4 = *reinterpret_cast<const uint4*>(keys + i0);
+            const uint4 p4 = *reinterpret_cast<const uint4*>(perm + i0);
+            const uint32_t k[4] = {k4.x, k4.y, k4.z, k4.w}, q[4] = {p4.x, p4.y, p4.z, p4.w};
+            // a permutation of [0, n): every source index is in range
+            *reinterpret_cast<float4*>(d0 + i0) = make_float4(s0[q[0]], s0[q[1]], s0[q[2]], s0[q[3]]);
+            *reinterpret_cast<float4*>(d1 + i0) = make_float4(s1[q[0]], s1[q[1]], s1[q[2]], s1[q[3]]);
+            *reinterpret_cast<float4*>(d2 + i0) = make_float4(s2[q[0]], s2[q[1]], s2[q[2]], s2[q[3]]);
+            *reinterpret_cast<float4*>(d3 + i0) = make_float4(s3[q[0]], s3[q[1]], s3[q[2]], s3[q[3]]);
+            *reinterpret_cast<float4*>(d4 + i0) = make_float4(s4[q[0]], s4[q[1]], s4[q[2]], s4[q[3]]);
+            *reinterpret_cast<float4*>(d5 + i0) = make_float4(s5[q[0]], s5[q[1]], s5[q[2]], s5[q[3]]);
+            const uint32_t after = i0 + 4 < n ? keys[i0 + 4] : dead;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t nx = j < 3 ? k[j + 1] : after;
+                if (k[j] < dead && nx >= dead) *count = i0 + j + 1;
+            }
+            if (i0 == 0 && k[0] >= dead) *count = 0;
+        } else {
+            for (int i = i0; i < n; ++i) {
+                const uint32_t kk = keys[i], s = perm[i];
+                d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
+                d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
+                if (kk < dead && (i + 1 == n || keys[i + 1] >= dead)) *count = i + 1;
+                if (i == 0 && kk >= dead) *count = 0;
+            }
         }
     }
 }
+
+// Onesweep with 9-bit digits: the cell keys of grids up to 2^18 cells (dead
+// slots take key nr*nz) sort in two passes instead of three of 8 bits.
+using SortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 9,
+                                        rocprim::block_radix_rank_algorithm::match>>;
 
 // scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][pad][rocprim tmp]
 hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
@@ -74,7 +118,7 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
     int bits = 1;
     while ((1u << bits) <= nkeys && bits < 32) ++bits;
     size_t tmp_bytes = 0;
-    e = rocprim::radix_sort_pairs((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+    e = rocprim::radix_sort_pairs<SortConfig>((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                   (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)(cap > 0 ? cap : 1),
                                   0, bits, st);
     if (e != hipSuccess) return e;
@@ -96,14 +140,14 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
     float* tmp[6];
     for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * a);
     void* rtmp = (void*)(base + 10 * a + 256);
-    int blocks = (n + 255) / 256;
+    int blocks = ((n + 3) / 4 + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, n, k_in,
                        v_in);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     size_t tb = tmp_bytes;
-    e = rocprim::radix_sort_pairs(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
+    e = rocprim::radix_sort_pairs<SortConfig>(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
     if (e != hipSuccess) return e;
     const uint32_t dead = (uint32_t)(g.nr * g.nz);
     hipLaunchKernelGGL(k_permute, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
