@@ -540,8 +540,9 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
     if (in) { pa.xi = in[0]; pa.yi = in[1]; pa.zi = in[2]; pa.vxi = in[3]; pa.vyi = in[4]; pa.vzi = in[5]; }
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
-    TimedScope ts(ctx, timed_kind);
-    HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity, flags), ctx->stream),
+    const KernelEvents ke = kernel_events(ctx, timed_kind);
+    HIPCHK(launch_particles(flags, ctx->geom, pa, ca, particle_blocks(ctx, s->capacity, flags), ctx->stream,
+                            ke.start, ke.stop),
            "particles");
     return GASX_OK;
 }
@@ -559,8 +560,9 @@ static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool 
     const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, std::min<int64_t>(
                                                                              (int64_t)ctx->cu_count * 16,
                                                                              kCounterSlots)));
-    TimedScope ts(ctx, timed_kind);
-    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream), "drift/deposit");
+    const KernelEvents ke = kernel_events(ctx, timed_kind);
+    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream, ke.start, ke.stop),
+           "drift/deposit");
     return GASX_OK;
 }
 

@@ -106,6 +106,27 @@ struct TimedScope {
     }
 };
 
+// An event pair that a single launch fills with its own start and end times
+// (hipExtLaunchKernel), registered under `kind`; both null when that kind is
+// not timed.
+struct KernelEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+inline KernelEvents kernel_events(gasx_ctx* c, int kind)
+{
+    KernelEvents k;
+    if (!c->timing || !(c->timing_kinds >> kind & 1u)) return k;
+    const size_t a = next_event(c);
+    if (a == (size_t)-1) return k;
+    const size_t b = next_event(c);
+    if (b == (size_t)-1) return k;
+    k.start = c->events[a];
+    k.stop = c->events[b];
+    c->tpairs[kind].push_back({a, b});
+    return k;
+}
+
 inline gasx_status fail(gasx_ctx* c, gasx_status s, const std::string& msg)
 {
     if (c) c->err = msg;

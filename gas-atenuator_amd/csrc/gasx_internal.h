@@ -204,11 +204,12 @@ hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s
 // push / collide / stage (kPush, kCollide, kStageSec, kStageIon, kDriftSec, kBatch)
 // Workgroups of the particle kernel variant `flags` resident per CU at once.
 int particle_occupancy(uint32_t flags);
+// ev0 / ev1 (optional) receive the kernel's own start and end times.
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
-                            int max_blocks, hipStream_t s);
+                            int max_blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 // EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk chunks
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s);
+                                int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,

@@ -26,6 +26,8 @@
 //
 // The slot count is read once at kernel start; appends land in a later
 // kernel (launch_append), so the count is stable inside these kernels.
+#include <hip/hip_ext.h>
+
 #include "particle_physics.h"
 
 namespace gasx {
@@ -845,18 +847,20 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
 }
 
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s)
+                                int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     blocks = blocks > 0 ? blocks : 1;
+    // the events (if any) take the kernel's own start and end times: no marker
+    // packets between the launches of the step
     if (batch) {
         if (!drift || !deposit) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+        hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift && deposit) {
-        hipLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+        hipExtLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift) {
-        hipLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+        hipExtLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (deposit) {
-        hipLaunchKernelGGL((k_drift_deposit<false, true, false>), dim3(blocks), dim3(kBlock), 0, s, g, pa);
+        hipExtLaunchKernelGGL((k_drift_deposit<false, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1180,9 +1184,9 @@ hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t
 // ---- dispatch -----------------------------------------------------------------
 template <uint32_t F>
 static hipError_t launch_t(const Geom& g, const PartArgs& pa, const CollArgs& ca, int blocks,
-                           hipStream_t s)
+                           hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
-    hipLaunchKernelGGL(k_particles<F>, dim3(blocks), dim3(kBlock), 0, s, g, pa, ca);
+    hipExtLaunchKernelGGL(k_particles<F>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa, ca);
     return hipGetLastError();
 }
 
@@ -1221,25 +1225,25 @@ int particle_occupancy(uint32_t flags)
 }
 
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
-                            int max_blocks, hipStream_t s)
+                            int max_blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int blocks = max_blocks > 0 ? max_blocks : 1;
     constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec;
     switch (flags) {
-    case P: return launch_t<P>(g, pa, ca, blocks, s);
-    case Co | S: return launch_t<Co | S>(g, pa, ca, blocks, s);
-    case Co | S | I: return launch_t<Co | S | I>(g, pa, ca, blocks, s);
-    case P | Co: return launch_t<P | Co>(g, pa, ca, blocks, s);
-    case P | Co | S: return launch_t<P | Co | S>(g, pa, ca, blocks, s);
-    case P | Co | I: return launch_t<P | Co | I>(g, pa, ca, blocks, s);
-    case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s);
-    case P | Co | S | DS: return launch_t<P | Co | S | DS>(g, pa, ca, blocks, s);
-    case P | Co | S | I | DS: return launch_t<P | Co | S | I | DS>(g, pa, ca, blocks, s);
+    case P: return launch_t<P>(g, pa, ca, blocks, s, ev0, ev1);
+    case Co | S: return launch_t<Co | S>(g, pa, ca, blocks, s, ev0, ev1);
+    case Co | S | I: return launch_t<Co | S | I>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co: return launch_t<P | Co>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S: return launch_t<P | Co | S>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | I: return launch_t<P | Co | I>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S | DS: return launch_t<P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S | I | DS: return launch_t<P | Co | S | I | DS>(g, pa, ca, blocks, s, ev0, ev1);
     // batched ensemble
-    case B | P: return launch_t<B | P>(g, pa, ca, blocks, s);
-    case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s);
-    case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s);
-    case B | P | Co | S | DS: return launch_t<B | P | Co | S | DS>(g, pa, ca, blocks, s);
+    case B | P: return launch_t<B | P>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | S | DS: return launch_t<B | P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
     default: return hipErrorInvalidValue;
     }
 }
