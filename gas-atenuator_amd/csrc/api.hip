@@ -320,6 +320,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
         chk(dalloc(&ctx->base_tmp, 4), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
+        chk(hipHostMalloc((void**)&ctx->h_count, sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
         if (st == GASX_OK) {
             chk(hipMemcpy(ctx->d_tab, &dt, sizeof(dt), hipMemcpyHostToDevice), "upload tables");
             chk(hipMemset(ctx->d_counters, 0, (size_t)kNCounters * kCounterSlots * sizeof(unsigned long long)),
@@ -354,6 +355,7 @@ extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
         if (ctx->stage[c]) hipFree(ctx->stage[c]);
         if (ctx->istage[c]) hipFree(ctx->istage[c]);
     }
+    if (ctx->h_count) hipHostFree(ctx->h_count);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     batch_release(ctx);
     diff_release(ctx);
@@ -841,7 +843,9 @@ static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_o
         ctx->sort_bytes = need;
     }
     TimedScope ts(ctx, kTSort);
-    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream, sorted_out), "sort");
+    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream, sorted_out,
+                       ctx->h_count),
+           "sort");
     return GASX_OK;
 }
 

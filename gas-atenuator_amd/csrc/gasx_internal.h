@@ -239,7 +239,9 @@ hipError_t launch_tallies(const gasx_params& p, const double* er, const double* 
                           hipStream_t s);
 // sorted_out non-null: the permuted arrays are left in the scratch (their six
 // pointers returned) instead of being copied back into s
+// pinned_count (optional): a pinned host word the slot count is read through
 hipError_t launch_sort(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st, float** sorted_out = nullptr);
+                       size_t* need_bytes, hipStream_t st, float** sorted_out = nullptr,
+                       int32_t* pinned_count = nullptr);
 
 }  // namespace gasx

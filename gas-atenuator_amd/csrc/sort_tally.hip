@@ -105,26 +105,30 @@ using SortConfig = rocprim::radix_sort_config<
 
 // scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][pad][rocprim tmp]
 hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st, float** sorted_out)
+                       size_t* need_bytes, hipStream_t st, float** sorted_out, int32_t* pinned_count)
 {
-    int32_t n = 0;
-    hipError_t e = hipMemcpyAsync(&n, sp->count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
     const size_t cap = (size_t)sp->capacity;
     const size_t a = (cap * 4 + 255) & ~(size_t)255;
     uint32_t nkeys = (uint32_t)(g.nr * g.nz);
     int bits = 1;
     while ((1u << bits) <= nkeys && bits < 32) ++bits;
     size_t tmp_bytes = 0;
-    e = rocprim::radix_sort_pairs<SortConfig>((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+    hipError_t e = rocprim::radix_sort_pairs<SortConfig>((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                   (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)(cap > 0 ? cap : 1),
                                   0, bits, st);
     if (e != hipSuccess) return e;
     const size_t need = 4 * a + 6 * a + 256 + ((tmp_bytes + 255) & ~(size_t)255);
     if (need_bytes) *need_bytes = need;
     if (!scratch || scratch_bytes < need) return hipErrorInvalidValue;
+    // the slot count (the one host synchronisation of the sort: rocPRIM takes
+    // the size on the host)
+    int32_t n_local = 0;
+    int32_t* hn = pinned_count ? pinned_count : &n_local;
+    e = hipMemcpyAsync(hn, sp->count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const int32_t n = *hn;
     if (n <= 0) {
         if (sorted_out) {   // nothing to permute: the particles stay where they are
             float* own[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};

@@ -114,7 +114,8 @@ def test_autotune_restores_state_and_picks_a_valid_shape(gpu):
     ctx, ens = make_ensemble(params, dens, parts, caps)
     before = [ens.run_state(r) for r in range(len(sizes))]
     t = ens.autotune(trial_steps=2, collide=True, secondaries=True, seed=1)
-    assert t["fdtd_ti"] >= 1 and t["fdtd_tk"] >= 1 and t["particle_blocks_per_cu"] in (4, 8, 16, 32)
+    # particle grid: one or two generations of resident workgroups, or 4..32 per CU
+    assert t["fdtd_ti"] >= 1 and t["fdtd_tk"] >= 1 and 1 <= t["particle_blocks_per_cu"] <= 64
     assert ens.tuning() == t
     for r in range(len(sizes)):
         assert np.array_equal(ens.run_state(r), before[r], equal_nan=True)
