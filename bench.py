@@ -310,8 +310,8 @@ def main():
     reset()
     ctx.counters(reset=True)
     torch.cuda.synchronize()
-    # timed region: HIP events around the particle-pass launches only (the
-    # roofline kernels); events around every launch would cost ~6% of the step
+    # timed region: each k_particles launch (the roofline kernel) timed by its
+    # own dispatch; timing every launch would add ~5% to the step
     t0 = time.perf_counter()
     ctx.run(e, f, gas, args.steps, time_kernels=2, **run_kw)
     torch.cuda.synchronize()
@@ -325,6 +325,7 @@ def main():
     ctx.run(e, f, gas, n_bd, time_kernels=1, step0=args.steps, **run_kw)
     torch.cuda.synchronize()
     tb = ctx.timing()
+    psteps_bd = ctx.counters().n_pushed - psteps
 
     # The particle pass of a step is two kernels: k_particles (push + collide,
     # the dominant one, reported as the roofline kernel) and k_drift_deposit
@@ -333,9 +334,10 @@ def main():
     avg_ms = tm["particle_ms"] / n_launch
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * psteps / n_launch
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    n_dd = max(1, tm["n_drift_launches"])
-    dd_ms = tm["drift_ms"] / n_dd
-    pass_ms = (tm["particle_ms"] + tm["drift_ms"]) / args.steps
+    # the drift/deposit kernel and the whole particle pass from the breakdown steps
+    n_dd = max(1, tb["n_drift_launches"])
+    dd_ms = tb["drift_ms"] / n_dd
+    pass_ms = (tb["particle_ms"] + tb["drift_ms"]) / n_bd
     # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE passes
     # (calibrated against a plain copy), stored per particle-step and scaled
     # to this run's particle-steps per launch.
@@ -375,13 +377,13 @@ def main():
             "avg_launch_ms": avg_ms,
             "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
         },
-        "drift_deposit_kernel": {
+        "drift_deposit_kernel": {   # breakdown steps
             "avg_launch_ms": dd_ms,
-            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps / n_dd / (dd_ms * 1e-3) / 1e9,
+            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps_bd / n_dd / (dd_ms * 1e-3) / 1e9,
         },
         "particle_pass": {   # both kernels of a step against the step's 48 algorithmic bytes per particle
             "ms_per_step": pass_ms,
-            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps / args.steps / (pass_ms * 1e-3) / 1e9,
+            "achieved_GBs": BYTES_PER_PARTICLE_STEP * psteps_bd / n_bd / (pass_ms * 1e-3) / 1e9,
         },
         "breakdown_us_per_step": {k[:-3]: tb[k] * 1e3 / n_bd
                                   for k in ("particle_ms", "drift_ms", "fdtd_ms", "append_ms", "sort_ms")},

@@ -812,7 +812,7 @@ extern "C" gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* io
     if (ro->time_kernels) {
         ctx->timing = true;
         // 2: the particle-pass launches only (the roofline kernels); else every launch
-        ctx->timing_kinds = ro->time_kernels == 2 ? (1u << kTParticle) | (1u << kTDrift) : ~0u;
+        ctx->timing_kinds = ro->time_kernels == 2 ? (1u << kTParticle) : ~0u;
         ctx->ev_used = 0;
         for (auto& v : ctx->tpairs) v.clear();
     }

@@ -403,7 +403,7 @@ extern "C" gasx_status gasx_batch_run(gasx_ctx* ctx, gasx_batch* b, const gasx_r
     if (o->time_kernels) {
         ctx->timing = true;
         // 2: the particle-pass launches only (the roofline kernels); else every launch
-        ctx->timing_kinds = o->time_kernels == 2 ? (1u << kTParticle) | (1u << kTDrift) : ~0u;
+        ctx->timing_kinds = o->time_kernels == 2 ? (1u << kTParticle) : ~0u;
         ctx->ev_used = 0;
         for (auto& v : ctx->tpairs) v.clear();
     }

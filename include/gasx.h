@@ -140,7 +140,8 @@ typedef struct gasx_run_opts {
     uint32_t step0;             /* Philox counter word 1 of the first step */
     int32_t use_graph;          /* 1: capture the step sequence in a hipGraph and replay */
     int32_t time_kernels;       /* 1: record HIP events around every launch (gasx_get_timing);
-                                   2: around the particle-pass launches only (less overhead) */
+                                   2: the push + collision launches only (the least
+                                   overhead between launches) */
 } gasx_run_opts;
 
 /* ---- context ------------------------------------------------------------ */
