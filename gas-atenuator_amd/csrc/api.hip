@@ -466,12 +466,13 @@ int gasx_host::particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu)
     return (int)std::max<int64_t>(1, std::min(need, persistent));
 }
 
-// One generation of resident workgroups: each wave then walks the most
-// segments, so its candidate ring fills the most collision rounds (a round
-// costs the same however few lanes it uses).
+// With collisions, one generation of resident workgroups: each wave then
+// walks the most segments, so its candidate ring fills the most collision
+// rounds (a round costs the same however few lanes it uses).  A push alone
+// streams better from an oversubscribed grid.
 static int particle_blocks(gasx_ctx* ctx, int64_t cap, uint32_t flags)
 {
-    return particle_grid(ctx, (cap + kSeg - 1) / kSeg, particle_occupancy(flags));
+    return particle_grid(ctx, (cap + kSeg - 1) / kSeg, (flags & kCollide) ? particle_occupancy(flags) : 16);
 }
 
 // one field cycle: reads (in) writes (out) and refreshes the FP32 copies

@@ -95,6 +95,15 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 #define GASX_DEP_TILE 512
 #endif
 constexpr int kTile = GASX_DEP_TILE;   // cells (16 bytes of LDS each)
+#ifndef GASX_DEP_COPIES
+#define GASX_DEP_COPIES 4
+#endif
+// Copies of the tile, lane l adding into copy l % kTCopies: neighbouring lanes
+// of a cell-sorted chunk mostly add into the same cells, and the LDS
+// serialises atomics on one address; the copies (offset by two elements so
+// that they sit in different banks) are summed when the tile is flushed.
+constexpr int kTCopies = GASX_DEP_COPIES;
+constexpr int kTStride = kTile + (kTCopies > 1 ? 2 : 0);
 
 struct TileBox {
     int ilo, klo, wi, nk;     // nk == 0: no tile (nothing to deposit, or the box exceeds kTile)
@@ -683,7 +692,7 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
     constexpr int kWaves = kBlock / kWaveSize;
     __shared__ int s_wave[kWaves];
     __shared__ int s_red[4 * kWaves];
-    __shared__ unsigned long long s_tr[kDe ? kTile : 1], s_tz[kDe ? kTile : 1];
+    __shared__ unsigned long long s_tr[kDe ? kTCopies * kTStride : 1], s_tz[kDe ? kTCopies * kTStride : 1];
     __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
     int total_chunks;
     if constexpr (kBat) {   // chunk prefix over the runs (run bases are multiples of kChunk)
@@ -810,7 +819,12 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
         }
         if constexpr (kDe) {
             const TileBox bx = tile_box(ilo, ihi, klo, khi, s_red);
-            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) { s_tr[t] = 0ull; s_tz[t] = 0ull; }
+            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) {
+#pragma unroll
+                for (int c = 0; c < kTCopies; ++c) { s_tr[c * kTStride + t] = 0ull; s_tz[c * kTStride + t] = 0ull; }
+            }
+            unsigned long long* const my_tr = s_tr + (threadIdx.x % kTCopies) * kTStride;
+            unsigned long long* const my_tz = s_tz + (threadIdx.x % kTCopies) * kTStride;
             __syncthreads();
             DepRun run_ = {-1, 0, 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
 #pragma unroll
@@ -820,7 +834,7 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
                     deposit_cells(g, x[p], y[p], z[p], vx[p], vy[p], vz[p], pa.kq, dc);
                     const int di = dc.i1 - dc.i0, dk = dc.k1 - dc.k0;
                     if (dc.i0 != run_.i0 || dc.k0 != run_.k0 || di != run_.di || dk != run_.dk) {
-                        if (run_.i0 >= 0) run_add(bx, run_, nr_, s_tr, s_tz, pa.qr, pa.qz);
+                        if (run_.i0 >= 0) run_add(bx, run_, nr_, my_tr, my_tz, pa.qr, pa.qz);
                         run_.i0 = dc.i0; run_.k0 = dc.k0; run_.di = di; run_.dk = dk;
 #pragma unroll
                         for (int a = 0; a < 4; ++a) { run_.q[a] = dc.qr[a]; run_.q[4 + a] = dc.qz[a]; }
@@ -830,10 +844,12 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
                     }
                 }
             }
-            if (run_.i0 >= 0) run_add(bx, run_, nr_, s_tr, s_tz, pa.qr, pa.qz);
+            if (run_.i0 >= 0) run_add(bx, run_, nr_, my_tr, my_tz, pa.qr, pa.qz);
             __syncthreads();
             for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) {
-                const unsigned long long a = s_tr[t], c = s_tz[t];
+                unsigned long long a = s_tr[t], c = s_tz[t];
+#pragma unroll
+                for (int k = 1; k < kTCopies; ++k) { a += s_tr[k * kTStride + t]; c += s_tz[k * kTStride + t]; }
                 const int cell = (bx.ilo + t % bx.wi) + (bx.klo + t / bx.wi) * nr_;
                 if (a) atomicAdd(&pa.qr[cell], a);
                 if (c) atomicAdd(&pa.qz[cell], c);
