@@ -1016,15 +1016,20 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
         }
         __syncthreads();
     }
+    if (base >= cap) return;                       // full: everything staged was dropped
     if (qr) {                                      // this workgroup's copy of the current
         qr += (size_t)(blockIdx.x % kJCopies) * jstride;
         qz += (size_t)(blockIdx.x % kJCopies) * jstride;
     }
-    for (int c = blockIdx.x; c < nseg; c += gridDim.x) {
+    // a wave per staged segment (a segment holds a few entries on average)
+    const int lane = (int)threadIdx.x & 63;
+    const int nwaves = (int)gridDim.x * (kBlock / 64);
+    for (int c = (int)blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6); c < nseg; c += nwaves) {
         const int cnt = stage_cnt[c];
         if (cnt == 0) continue;
         const int first = base + s_pre[c / kScanGroup] + offs[c];
-        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
+        if (first >= cap) continue;
+        for (int j = lane; j < cnt; j += 64) {
             const int dst = first + j;
             const int src = c * kSeg + j;
             if (dst < cap) {
@@ -1056,7 +1061,8 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                        stage_cnt, scan_tmp, totals, tmp2, max_segs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int blocks = max_segs < 4096 ? (max_segs > 0 ? max_segs : 1) : 4096;
+    const int wseg = (max_segs + kBlock / 64 - 1) / (kBlock / 64);   // a wave per segment
+    const int blocks = wseg < 4096 ? (wseg > 0 ? wseg : 1) : 4096;
     hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3],
                        stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped, g, kq,
