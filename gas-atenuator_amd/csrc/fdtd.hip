@@ -510,7 +510,7 @@ __global__ void k_j_reduce(unsigned long long* qr, unsigned long long* qz, int64
 hipError_t launch_j_reduce(unsigned long long* qr, unsigned long long* qz, int64_t ncells, size_t stride,
                            hipStream_t s)
 {
-    if (ncells <= 0) return hipSuccess;
+    if (ncells <= 0 || kJCopies == 1) return hipSuccess;   // one copy: nothing to fold
     int64_t blocks = (ncells + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_j_reduce, dim3((unsigned)blocks), dim3(256), 0, s, qr, qz, ncells, stride);

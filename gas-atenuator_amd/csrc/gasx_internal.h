@@ -30,10 +30,12 @@ constexpr int kPerD = GASX_KPER_DRIFT;
 constexpr int kChunkD = kBlock * kPerD;  // slots per drift/deposit workgroup chunk
 constexpr int kCounterSlots = 8192;  // per-block event counter slots (>= particle kernel grid)
 // Copies of the fixed-point current accumulators: workgroup b adds into copy
-// b % kJCopies, which spreads the atomics on the hot cells of a dense spot
-// over as many addresses; the copies are summed (exact integer adds) before
-// the field cycle reads the current.
-constexpr int kJCopies = 8;
+// b % kJCopies, which would spread the atomics on the hot cells of a dense
+// spot over as many addresses; the copies are summed (exact integer adds)
+// before the field cycle reads the current.  The deposit's LDS tiles leave
+// one global atomic per touched cell and chunk, so one copy (no fold pass
+// at all) is faster than 2, 4 or 8 on the 256x512 production grid.
+constexpr int kJCopies = 1;
 
 constexpr int kGuide = 32;       // guide entries per energy node of the elastic-angle CDF
 
