@@ -56,9 +56,9 @@ class Workload:
 WORKLOADS = {
     "plain_cpu_64x128": Workload("plain_cpu_64x128", 64, 128, 100_000, 200, 2.5e25),
     # cell sort every 40 steps: the median electron moves ~1e-3 cell per step,
-    # and on this workload 5 / 10 / 20 / 40 / 80 / never measured 0.350 / 0.318 /
-    # 0.303 / 0.295 / 0.284 / 0.293 ms per step (the sort's cost against the
-    # locality it restores)
+    # and on this workload 5 / 10 / 20 / 40 measured 0.350 / 0.318 / 0.303 /
+    # 0.295 ms per step (eight current copies), 40 / 80 / never 0.287 / 0.284 /
+    # 0.293 (two copies): the sort's cost against the locality it restores
     "plain_gpu_256x512": Workload("plain_gpu_256x512", 256, 512, 4_000_000, 1000, 2.5e25,
                                   capacity_factor=2.0, secondaries=True, sort_every=40),
     "batched_128x256": Workload("batched_128x256", 128, 256, 0, 500, 2.5e25,
