@@ -42,29 +42,38 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
     const int ldz = W + 1;
     const int npos = (W + 1) * (Hh + 1);   // positions (i,k), i in [ri0,ri1], k in [rk0,rk1]
 
-    // ---- load --------------------------------------------------------------
-    for (int p = threadIdx.x; p < npos; p += BS) {
+    // ---- load ------------------------------------------------------------------
+    // Every global read of the thread is issued before any is used (indices
+    // clamped into the arrays, so the reads need no guard; a value read for a
+    // position that does not use it is dropped below).
+    double vH[MAXP], vEr[MAXP], vEz[MAXP], vr0[MAXP], vr1[MAXP], vz0[MAXP], vz1[MAXP];
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = min((int)threadIdx.x + q * BS, npos - 1);
         const int li = p % (W + 1), lk = p / (W + 1);
-        const int gi = ri0 + li, gk = rk0 + lk;
-        if (li < W && lk < Hh) sH[li + lk * W] = R.h_in[gi + gk * nr];
-        if (li < W) sEr[li + lk * W] = R.er_in[gi + gk * nr];
-        if (lk < Hh) sEz[li + lk * ldz] = R.ez_in[gi + gk * (nr + 1)];
+        const int gi = ri0 + li, gk = rk0 + lk;              // gi <= nr, gk <= nz
+        const int ci = min(gi, nr - 1), ck = min(gk, nz - 1);
+        vH[q] = R.h_in[ci + ck * nr];
+        vEr[q] = R.er_in[ci + gk * nr];
+        vEz[q] = R.ez_in[gi + ck * (nr + 1)];
+        const int km = max(gk - 1, 0), im = max(gi - 1, 0);
+        if (fixed_j) {
+            vr0[q] = (double)(long long)R.qr[ci + km * nr];
+            vr1[q] = (double)(long long)R.qr[ci + ck * nr];
+            vz0[q] = (double)(long long)R.qz[min(im, nr - 1) + ck * nr];
+            vz1[q] = (double)(long long)R.qz[ci + ck * nr];
+        } else {
+            vr0[q] = R.jr[ci + km * nr];
+            vr1[q] = R.jr[ci + ck * nr];
+            vz0[q] = R.jz[min(im, nr - 1) + ck * nr];
+            vz1[q] = R.jz[ci + ck * nr];
+        }
     }
 
-    // ---- per-position sources and coefficients (registers) -----------------
-    auto jr_at = [&](int i, int k) -> double {
-        if (fixed_j) {
-            const double V = a.vcoef * (double)((i + 1) * (i + 1) - i * i);
-            return a.junit * (double)(long long)R.qr[i + k * nr] / V;
-        }
-        return R.jr[i + k * nr];
-    };
-    auto jz_at = [&](int i, int k) -> double {
-        if (fixed_j) {
-            const double V = a.vcoef * (double)((i + 1) * (i + 1) - i * i);
-            return a.junit * (double)(long long)R.qz[i + k * nr] / V;
-        }
-        return R.jz[i + k * nr];
+    // ---- LDS tile, per-position sources and coefficients (registers) ---------
+    // J = junit * Q / V(i) for the fixed-point current, as read for FP64 J
+    auto jscale = [&](int i) -> double {
+        return fixed_j ? a.junit / (a.vcoef * (double)((i + 1) * (i + 1) - i * i)) : 1.0;
     };
     double srcR[MAXP], srcZ[MAXP], cA[MAXP], cB[MAXP];
 #pragma unroll
@@ -74,13 +83,18 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
         if (p < npos) {
             const int li = p % (W + 1), lk = p / (W + 1);
             const int gi = ri0 + li, gk = rk0 + lk;
-            if (li < W && gk >= 1 && gk <= nz - 1)   // EM3.cu:29 (jr at k-1 and k)
-                srcR[q] = a.ce * (jr_at(gi, gk - 1) + jr_at(gi, gk)) / 2;
+            if (li < W && lk < Hh) sH[li + lk * W] = vH[q];
+            if (li < W) sEr[li + lk * W] = vEr[q];
+            if (lk < Hh) sEz[li + lk * ldz] = vEz[q];
+            if (li < W && gk >= 1 && gk <= nz - 1) {   // EM3.cu:29 (jr at k-1 and k)
+                const double js = jscale(gi);
+                srcR[q] = a.ce * (js * vr0[q] + js * vr1[q]) / 2;
+            }
             if (lk < Hh) {
                 if (gi == 0)                           // EM3.cu:35
-                    srcZ[q] = ((double)a.axis_sign * a.ce) * jz_at(0, gk);
+                    srcZ[q] = ((double)a.axis_sign * a.ce) * (jscale(0) * vz1[q]);
                 else if (gi <= nr - 1) {               // EM3.cu:39
-                    srcZ[q] = a.ce * (jz_at(gi - 1, gk) + jz_at(gi, gk)) / 2;
+                    srcZ[q] = a.ce * (jscale(gi - 1) * vz0[q] + jscale(gi) * vz1[q]) / 2;
                     const double rs = (a.ez_metric ? (gi + 1) : gi) * a.dr;
                     cA[q] = a.ce * (1 / (2 * rs) + 1 / a.dr);
                     cB[q] = a.ce * (1 / (2 * rs) - 1 / a.dr);
