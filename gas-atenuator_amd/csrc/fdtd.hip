@@ -254,12 +254,27 @@ __global__ __launch_bounds__(BS) void k_fdtd_adj(FdtdArgs a, AdjFdtdRun R)
     const int ldz = W + 1;
     const int npos = (W + 1) * (Hh + 1);
 
-    for (int p = threadIdx.x; p < npos; p += BS) {
+    // every global read issued before use (clamped indices, as in k_fdtd)
+    double vH[MAXP], vEr[MAXP], vEz[MAXP];
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = min((int)threadIdx.x + q * BS, npos - 1);
         const int li = p % (W + 1), lk = p / (W + 1);
-        const int gi = ri0 + li, gk = rk0 + lk;
-        if (li < W && lk < Hh) sH[li + lk * W] = R.lh_in[gi + gk * nr];
-        if (li < W) sEr[li + lk * W] = R.ler_in[gi + gk * nr];
-        if (lk < Hh) sEz[li + lk * ldz] = R.lez_in[gi + gk * (nr + 1)];
+        const int gi = ri0 + li, gk = rk0 + lk;              // gi <= nr, gk <= nz
+        const int ci = min(gi, nr - 1), ck = min(gk, nz - 1);
+        vH[q] = R.lh_in[ci + ck * nr];
+        vEr[q] = R.ler_in[ci + gk * nr];
+        vEz[q] = R.lez_in[gi + ck * (nr + 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < MAXP; ++q) {
+        const int p = threadIdx.x + q * BS;
+        if (p < npos) {
+            const int li = p % (W + 1), lk = p / (W + 1);
+            if (li < W && lk < Hh) sH[li + lk * W] = vH[q];
+            if (li < W) sEr[li + lk * W] = vEr[q];
+            if (lk < Hh) sEz[li + lk * ldz] = vEz[q];
+        }
     }
     // per H position: the E^T coefficients and the current accumulators
     double cAe[MAXP], cBe[MAXP], accR[MAXP], accZ[MAXP];
