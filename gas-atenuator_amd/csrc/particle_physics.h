@@ -69,8 +69,16 @@ __device__ __forceinline__ void deposit_global(const DepCells& d, int nr, unsign
 }
 
 // ---- gather + push (EM4.cu:36-148) ------------------------------------------
-__device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, float x, float y,
-                                         float z, float& vx, float& vy, float& vz)
+// In two parts so that a lane can have several particles' field reads in
+// flight together: gather_issue computes the stencils and reads the 12 field
+// values, push_apply interpolates and pushes.  push_one is the two in a row.
+struct FieldGather {
+    float er[4], h[4], ez[4];
+    float a0, a1, b0, b1, c0, c1, d0, d1, cph, sph;
+};
+
+__device__ __forceinline__ void gather_issue(const Geom& g, const PartArgs& pa, float x, float y, float z,
+                                             FieldGather& G)
 {
 #pragma clang fp contract(off)
     const float r2 = x * x + y * y;
@@ -80,28 +88,35 @@ __device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, floa
         r = r2 * ir;
         cph = x * ir; sph = y * ir;
     }
-    const float vr = vx * cph + vy * sph;
+    G.cph = cph; G.sph = sph;
     const float srn = r * g.inv_dr, szn = z * g.inv_dz;
     const float srh = srn - 0.5f, szh = szn - 0.5f;
-    int i0, i1, k0, k1;
-    float a0, a1, b0, b1;
-    const int nr = g.nr;
+    int i0, i1, k0, k1, kh0, kh1, j0, j1;
+    const int nr = g.nr, ld = nr + 1;
     // Er: cell-centred in r, nodes in z (EM4.cu:41-60)
-    half_axis(srh, g.nr, i0, i1, a0, a1);
-    node_axis(szn, g.nz, k0, k1, b0, b1);
-    const float Er = b0 * (a0 * pa.er[i0 + k0 * nr] + a1 * pa.er[i1 + k0 * nr]) +
-                     b1 * (a0 * pa.er[i0 + k1 * nr] + a1 * pa.er[i1 + k1 * nr]);
+    half_axis(srh, g.nr, i0, i1, G.a0, G.a1);
+    node_axis(szn, g.nz, k0, k1, G.b0, G.b1);
     // Hphi: cell-centred in both (EM4.cu:82-122); shares the r weights of Er
-    int kh0, kh1;
-    float c0, c1;
-    half_axis(szh, g.nz, kh0, kh1, c0, c1);
-    const float H = c0 * (a0 * pa.h[i0 + kh0 * nr] + a1 * pa.h[i1 + kh0 * nr]) +
-                    c1 * (a0 * pa.h[i0 + kh1 * nr] + a1 * pa.h[i1 + kh1 * nr]);
+    half_axis(szh, g.nz, kh0, kh1, G.c0, G.c1);
     // Ez: nodes in r, cell-centred in z (EM4.cu:61-80)
-    node_axis(srn, g.nr, i0, i1, a0, a1);
-    const int ld = nr + 1;
-    const float Ez = c0 * (a0 * pa.ez[i0 + kh0 * ld] + a1 * pa.ez[i1 + kh0 * ld]) +
-                     c1 * (a0 * pa.ez[i0 + kh1 * ld] + a1 * pa.ez[i1 + kh1 * ld]);
+    node_axis(srn, g.nr, j0, j1, G.d0, G.d1);
+    G.er[0] = pa.er[i0 + k0 * nr]; G.er[1] = pa.er[i1 + k0 * nr];
+    G.er[2] = pa.er[i0 + k1 * nr]; G.er[3] = pa.er[i1 + k1 * nr];
+    G.h[0] = pa.h[i0 + kh0 * nr]; G.h[1] = pa.h[i1 + kh0 * nr];
+    G.h[2] = pa.h[i0 + kh1 * nr]; G.h[3] = pa.h[i1 + kh1 * nr];
+    G.ez[0] = pa.ez[j0 + kh0 * ld]; G.ez[1] = pa.ez[j1 + kh0 * ld];
+    G.ez[2] = pa.ez[j0 + kh1 * ld]; G.ez[3] = pa.ez[j1 + kh1 * ld];
+}
+
+__device__ __forceinline__ void push_apply(const Geom& g, const PartArgs& pa, const FieldGather& G, float& vx,
+                                           float& vy, float& vz)
+{
+#pragma clang fp contract(off)
+    const float cph = G.cph, sph = G.sph;
+    const float vr = vx * cph + vy * sph;
+    const float Er = G.b0 * (G.a0 * G.er[0] + G.a1 * G.er[1]) + G.b1 * (G.a0 * G.er[2] + G.a1 * G.er[3]);
+    const float H = G.c0 * (G.a0 * G.h[0] + G.a1 * G.h[1]) + G.c1 * (G.a0 * G.h[2] + G.a1 * G.h[3]);
+    const float Ez = G.c0 * (G.d0 * G.ez[0] + G.d1 * G.ez[1]) + G.c1 * (G.d0 * G.ez[2] + G.d1 * G.ez[3]);
     // EM4.cu:126-144: magnetic term uses v^n; explicit (not Boris) update
     const float fr = pa.q_over_m * (Er - vz * g.mu0 * H);
     const float fz = pa.q_over_m * (Ez + vr * g.mu0 * H);
@@ -112,6 +127,14 @@ __device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, floa
     const float uz = gm * vz + fz * g.dt;
     const float igp = rsqrtf(1.0f + (ux * ux + uy * uy + uz * uz) * g.inv_c2);
     vx = ux * igp; vy = uy * igp; vz = uz * igp;
+}
+
+__device__ __forceinline__ void push_one(const Geom& g, const PartArgs& pa, float x, float y,
+                                         float z, float& vx, float& vy, float& vz)
+{
+    FieldGather G;
+    gather_issue(g, pa, x, y, z, G);
+    push_apply(g, pa, G, vx, vy, vz);
 }
 
 __device__ __forceinline__ void half_drift(const Geom& g, float& x, float& y, float& z, float vx,

@@ -178,6 +178,13 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
     }
 }
 
+#ifndef GASX_GATHER_BATCH
+#define GASX_GATHER_BATCH 1
+#endif
+// particles of a lane whose field reads are in flight together (divides kPer)
+constexpr int kGatherBatch = GASX_GATHER_BATCH;
+static_assert(kPer % kGatherBatch == 0, "gather batch divides the slots per lane");
+
 constexpr int kQW = 11;     // LDS queue words per candidate
 constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes 64)
 
@@ -457,14 +464,24 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
             }
         }
         // ---- A: push + first Philox block; candidates ---------------------------
+        // the field reads of kGatherBatch particles are issued together (a
+        // slot that is not pushed reads at the origin and drops the values)
+        FieldGather G[kGatherBatch];
 #pragma unroll
         for (int p = 0; p < kPer; ++p) {
             const int slot = slot0 + p;
             alive[p] = slot < n && !is_dead(z[p]);
             was_alive[p] = alive[p];
             cand[p] = false;
+            if ((F & kPush) && p % kGatherBatch == 0) {
+#pragma unroll
+                for (int q = p; q < p + kGatherBatch; ++q) {
+                    const bool on = slot0 + q < n && !is_dead(z[q]);
+                    gather_issue(g, pa, on ? x[q] : 0.f, on ? y[q] : 0.f, on ? z[q] : 0.f, G[q - p]);
+                }
+            }
             if ((F & kPush) && alive[p]) {
-                push_one(g, pa, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                push_apply(g, pa, G[p % kGatherBatch], vx[p], vy[p], vz[p]);
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
                 if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
                     alive[p] = false;
