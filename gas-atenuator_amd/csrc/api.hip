@@ -452,6 +452,15 @@ gasx_status gasx_host::make_coll(gasx_ctx* ctx, const gasx_gas* g, double dt_col
     ca->key0 = (uint32_t)seed; ca->key1 = (uint32_t)(seed >> 32);
     ca->step = step; ca->run = run;
     ca->ev_per_mc2_em = (float)(p.me_em * p.c * p.c / p.e_em);
+    {
+        const double mc2_ev = p.me_em * p.c * p.c / p.e_em;
+        auto v2_of = [&](double T) {   // push-side electron: v^2 = c^2 (1 - 1/gamma^2)
+            const double gm = 1.0 + T / mc2_ev;
+            return p.c * p.c * (1.0 - 1.0 / (gm * gm));
+        };
+        ca->v2_lo = (float)v2_of(15.6 * (1.0 - 1e-4));
+        ca->v2_hi = (float)v2_of(20000.0 * (1.0 + 1e-4));
+    }
     ca->mc2_mc_ev = (float)(p.me_mc * p.c * p.c / p.e_mc);
     ca->ion_vth_k = (float)std::sqrt(p.k_b / p.m_ion);
     ca->exc_form = p.exc_form;

@@ -127,6 +127,10 @@ struct CollArgs {
     float ion_vth_k;       // sqrt(k_b / m_ion)
     int32_t exc_form;
     float rot_eps;
+    // squared speeds bracketing the tables' energy window [15.6 eV, 20 keV],
+    // widened by 1e-4 in energy: a candidate prefilter that is a superset of
+    // collide_candidate's own window test on its FP32 energy
+    float v2_lo, v2_hi;
 };
 
 struct PartArgs {

@@ -493,12 +493,12 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                 const U4 R = philox4x32_10((uint32_t)slot, ca.step, ca.run, 0u, ca.key0, ca.key1);
                 u1[p] = u01(R.a); u2[p] = u01(R.b); u3[p] = u01(R.c); u4[p] = u01(R.d);
                 // exact: p_T <= pmax for every particle, and collide_candidate does
-                // nothing outside the tables' energy window (the window is widened
-                // by 1e-6 here, so rounding of this copy of T never drops a particle
-                // that collide_candidate would test)
-                const float T = ev_from_speed2(vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p], g.inv_c2,
-                                               ca.ev_per_mc2_em);
-                cand[p] = u1[p] < ca.pmax && T >= 15.6f * (1.0f - 1e-6f) && T <= 20000.0f * (1.0f + 1e-6f);
+                // nothing outside the tables' energy window; the window is tested on
+                // v^2 here, widened by 1e-4 in energy (far above the few-ulp error
+                // of collide_candidate's FP32 energy), so no particle that
+                // collide_candidate would test is dropped
+                const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
+                cand[p] = u1[p] < ca.pmax && v2 >= ca.v2_lo && v2 <= ca.v2_hi;
             }
             if (F & kPush) cnt[kCPushed] += was_alive[p];
             if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
