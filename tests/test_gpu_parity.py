@@ -394,3 +394,37 @@ def test_in_loop_sort_keeps_trajectories(gpu, n):
     assert np.array_equal(a[:, np.lexsort(a)], b[:, np.lexsort(b)])
     for x, y in zip(fa, fb):
         assert np.array_equal(x, y)
+
+
+def test_collision_window_edges_match_oracle(gpu, otab):
+    """Energies within 1e-3 of the tables' window ends (15.6 eV, 20 keV): the
+    kernel's candidate prefilter tests the window on v^2 with host-computed,
+    widened bounds, and must never drop a particle that the exact FP32 test in
+    the collision itself would keep. Per particle, 'collided' (velocity
+    changed) must agree with the oracle on the same Philox streams, except for
+    energies within 1e-5 of an edge (FP32 vs FP64 energy)."""
+    import gasx
+    from gasx.workloads import speed_from_ev
+    log("collision window edges")
+    n, dens = 20000, 2.5e26
+    p, ctx, op = mk(40, 70)
+    rng = np.random.default_rng(21)
+    st = wedge_particles(n, 40, 70, p.dr, p.dz, p.phip, 21, edge_frac=0.0)
+    edge = np.where(rng.random(n) < 0.5, 15.6, 20000.0)
+    ev = edge * (1.0 + rng.uniform(-1e-3, 1e-3, n))
+    d = st[3:6] / np.linalg.norm(st[3:6], axis=0)
+    st[3:6] = (d * speed_from_ev(ev)).astype(np.float32)
+    e = gasx.ParticleSet(2 * n).load(*st)
+    gas = gasx.GasGrid.uniform(dens, 40 * p.dr, 70 * p.dz)
+    dt = 2e-14
+    ctx.collide(e, gas, dt, seed=99, step=3, run=0)
+    g = e.state()[:, :n]
+    o = oracle.State(st)
+    og = oracle.make_gas(np.full((1, 1), dens), 40 * p.dr, 70 * p.dz)
+    oev, _, _ = oracle.collide(op, otab, og, o, dt, 99, 3, 0)
+    moved = np.any(g[3:6] != st[3:6], axis=0)
+    collided = oev[:n] != 0
+    near = np.abs(ev / edge - 1.0) < 1e-5
+    assert collided.sum() > 0.05 * n and (~collided).sum() > 0.05 * n
+    assert np.array_equal(moved[~near], collided[~near])
+    assert ctx.counters().n_bound_violations == 0
