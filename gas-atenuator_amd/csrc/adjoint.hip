@@ -512,8 +512,12 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff_step(DiffArgs a, CollArgs c
                         cross_sections(s_tab, T, se, si, sx);
                         p0 = fsqrt(v2) * ca.dt_coll * n0r * (se + si + sx);
                     }
-                    const U4 R = philox4x32_10((uint32_t)i, ca.step, ca.run, 0u, ca.key0, ca.key1);
-                    if (u01(R.a) < p0) {
+                    // the production step's counters: u1 from the decision block of
+                    // slot i's group of four, u2..u4 from block 0 of slot i
+                    const U4 D = philox4x32_10((uint32_t)i >> 2, ca.step, ca.run, kDecisionBlock, ca.key0, ca.key1);
+                    const uint32_t dw[4] = {D.a, D.b, D.c, D.d};
+                    if (u01(dw[i & 3]) < p0) {
+                        const U4 R = philox4x32_10((uint32_t)i, ca.step, ca.run, 0u, ca.key0, ca.key1);
                         float cvx = vx, cvy = vy, cvz = vz;
                         CollOut co;
                         collide_candidate(a.g, ca, s_tab, (uint32_t)i, x, y, z, cvx, cvy, cvz, 0.0f, u01(R.b),

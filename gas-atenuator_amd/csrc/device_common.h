@@ -35,6 +35,13 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
     return U4{c0, c1, c2, c3};
 }
 
+// Counter layout (key = seed): (slot, step, run, block).  The collision
+// decision uniform u1 of slot s is word s % 4 of the block (s / 4, step, run,
+// kDecisionBlock), one draw for four consecutive slots; a candidate's u2..u4
+// are words 1..3 of block 0 of its own slot; block 1 is the ion Maxwellian,
+// blocks 2..66 the elastic-angle sampler.
+constexpr uint32_t kDecisionBlock = 0xFFFFFFFFu;
+
 // (2k+1)*2^-24 with k the top 23 bits: strictly inside (0,1), exact in FP32.
 __device__ __forceinline__ float u01(uint32_t x)
 {

@@ -185,6 +185,7 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
 constexpr int kGatherBatch = GASX_GATHER_BATCH;
 static_assert(kPer % kGatherBatch == 0, "gather batch divides the slots per lane");
 
+static_assert(kPer == 4, "one decision draw covers a lane's four slots");
 constexpr int kQW = 11;     // LDS queue words per candidate
 constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes 64)
 
@@ -349,8 +350,9 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
             const int raw = __float_as_int(Q[10][qi]);
             slot = raw & 0x7fffffff;
             seg_last = raw < 0;
-            collide_candidate(g, qca, s_tab, (uint32_t)slot, px, py, pz, vx, vy, vz, Q[6][qi], Q[7][qi],
-                              Q[8][qi], Q[9][qi], (F & kStageIon) != 0, co);
+            const U4 R = philox4x32_10((uint32_t)slot, qca.step, qca.run, 0u, qca.key0, qca.key1);
+            collide_candidate(g, qca, s_tab, (uint32_t)slot, px, py, pz, vx, vy, vz, Q[6][qi], u01(R.b),
+                              u01(R.c), u01(R.d), (F & kStageIon) != 0, co);
             ev = co.ev;
             if (ev & kEvMask) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
             gs = q_cs + slot / kSeg;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
         const bool vec = aligned && slot0 + kPer <= n;
         float x[kPer], y[kPer], z[kPer], vx[kPer], vy[kPer], vz[kPer];
         bool alive[kPer], was_alive[kPer], cand[kPer];
-        float u1[kPer], u2[kPer], u3[kPer], u4[kPer];
+        float u1[kPer];
         if (vec) {
             vload(pa.xi + slot0, x); vload(pa.yi + slot0, y); vload(pa.zi + slot0, z);
             vload(pa.vxi + slot0, vx); vload(pa.vyi + slot0, vy); vload(pa.vzi + slot0, vz);
@@ -488,10 +490,11 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                     ++cnt[kCAbsorbed];
                 }
             }
-            u1[p] = 1.f; u2[p] = u3[p] = u4[p] = 0.f;
+            if (kCol && p == 0) {   // the decision uniforms of the lane's four slots: one draw
+                const U4 D = philox4x32_10((uint32_t)slot0 >> 2, ca.step, ca.run, kDecisionBlock, ca.key0, ca.key1);
+                u1[0] = u01(D.a); u1[1] = u01(D.b); u1[2] = u01(D.c); u1[3] = u01(D.d);
+            }
             if (kCol && alive[p]) {
-                const U4 R = philox4x32_10((uint32_t)slot, ca.step, ca.run, 0u, ca.key0, ca.key1);
-                u1[p] = u01(R.a); u2[p] = u01(R.b); u3[p] = u01(R.c); u4[p] = u01(R.d);
                 // exact: p_T <= pmax for every particle, and collide_candidate does
                 // nothing outside the tables' energy window; the window is tested on
                 // v^2 here, widened by 1e-4 in energy (far above the few-ulp error
@@ -525,7 +528,7 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                             if (q >= 0 && q < kWaveSize) {
                                 Q[0][q] = x[p]; Q[1][q] = y[p]; Q[2][q] = z[p];
                                 Q[3][q] = vx[p]; Q[4][q] = vy[p]; Q[5][q] = vz[p];
-                                Q[6][q] = u1[p]; Q[7][q] = u2[p]; Q[8][q] = u3[p]; Q[9][q] = u4[p];
+                                Q[6][q] = u1[p];
                                 Q[10][q] = __int_as_float(slot0 + p);
                             }
                             ++pos;
@@ -536,9 +539,10 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                         const int q = lane;
                         float qvx = Q[3][q], qvy = Q[4][q], qvz = Q[5][q];
                         CollOut co;
-                        collide_candidate(g, ca, s_tab, (uint32_t)__float_as_int(Q[10][q]), Q[0][q], Q[1][q],
-                                          Q[2][q], qvx, qvy, qvz, Q[6][q], Q[7][q], Q[8][q], Q[9][q],
-                                          (F & kStageIon) != 0, co);
+                        const uint32_t qslot = (uint32_t)__float_as_int(Q[10][q]);
+                        const U4 R = philox4x32_10(qslot, ca.step, ca.run, 0u, ca.key0, ca.key1);
+                        collide_candidate(g, ca, s_tab, qslot, Q[0][q], Q[1][q], Q[2][q], qvx, qvy, qvz,
+                                          Q[6][q], u01(R.b), u01(R.c), u01(R.d), (F & kStageIon) != 0, co);
                         Q[3][q] = qvx; Q[4][q] = qvy; Q[5][q] = qvz;
                         Q[10][q] = __int_as_float(co.ev);
                         if (co.ev & kEvSec) { Q[6][q] = co.svx; Q[7][q] = co.svy; Q[8][q] = co.svz; }
@@ -676,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_particles(Geom g, PartArgs pa0, Coll
                                 if (qi >= kQCap) qi -= kQCap;
                                 Q[0][qi] = x[p]; Q[1][qi] = y[p]; Q[2][qi] = z[p];
                                 Q[3][qi] = vx[p]; Q[4][qi] = vy[p]; Q[5][qi] = vz[p];
-                                Q[6][qi] = u1[p]; Q[7][qi] = u2[p]; Q[8][qi] = u3[p]; Q[9][qi] = u4[p];
+                                Q[6][qi] = u1[p];
                                 // slot, and in the sign bit: the segment's last candidate
                                 Q[10][qi] = __int_as_float((slot0 + p) | (pos == total - 1 ? (int)0x80000000 : 0));
                             }

@@ -630,7 +630,8 @@ static void gas_lookup(const orc_gas* g, double r, double z, double* n, double* 
 
 /* electron_transport.cu:64-227 with the MATLAB choices (find_CS0.m cross
  * sections, NaN gating, get_ElasticSc_sub1.m angles, get_ImpExc2.m loss).
- * Random numbers: Philox4x32-10, key = seed, counter = (slot, step, run, block). */
+ * Random numbers: Philox4x32-10, key = seed, counter = (slot, step, run, block);
+ * the decision uniform from block 0xFFFFFFFF of the slot's group of four. */
 void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
                  double* x, double* y, double* z, double* vx, double* vy, double* vz,
                  const int32_t* del, int64_t n, double dt_coll,
@@ -647,9 +648,13 @@ void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
         double v2 = vx[m] * vx[m] + vy[m] * vy[m] + vz[m] * vz[m];
         double ve = sqrt(v2);
         double T = ev_from_speed(v2, p->e_em, p->me_em, c);     /* eV */
+        /* u1: word m % 4 of the decision block of slot group m / 4; u2..u4:
+         * words 1..3 of block 0 of slot m */
+        uint32_t dctr[4] = { (uint32_t)m >> 2, step, run, 0xFFFFFFFFu }, d[4];
+        orc_philox4x32_10(dctr, key, d);
         uint32_t ctr[4] = { (uint32_t)m, step, run, 0 }, o[4];
         orc_philox4x32_10(ctr, key, o);
-        double u1 = orc_u01(o[0]), u2 = orc_u01(o[1]), u3 = orc_u01(o[2]), u4 = orc_u01(o[3]);
+        double u1 = orc_u01(d[m & 3]), u2 = orc_u01(o[1]), u3 = orc_u01(o[2]), u4 = orc_u01(o[3]);
         if (!(T >= 15.6 && T <= 20000.0)) continue;    /* interp1 NaN gating */
         double se, si, sx;
         cs_one(t, T / 1000.0, &se, &si, &sx);
