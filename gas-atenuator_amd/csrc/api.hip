@@ -748,7 +748,10 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
             // drifted), then drift + deposit of the existing slots; the
             // collision phase's registers would otherwise limit the occupancy
             // of the streaming half drift and deposit
-            uint32_t flags = kPush;
+            // the particle launch also makes the step's second half drift
+            // (except after the last push), so the deposit that follows only
+            // reads the particles
+            uint32_t flags = kPush | (last ? 0u : kDrift2);
             if (k == 0 && o.collide) {
                 flags |= kCollide;
                 if (o.secondaries) flags |= kStageSec | (last ? 0u : kDriftSec);
@@ -759,7 +762,7 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
             if (st != GASX_OK) return st;
             if (k == sorted_k) sorted_k = -1;
             if (!last) {
-                st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
+                st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
                 if (st != GASX_OK) return st;
             }
             if (k == 0 && o.collide) {

@@ -270,9 +270,10 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
                                ctx->stream),
                    "fdtd");
         }
-        // push (+ collide, secondaries staged already drifted), then drift +
-        // deposit of the existing slots, as the single-run loop does
-        uint32_t flags = kBatch | kPush;
+        // push + half drift (+ collide, secondaries staged already drifted) +
+        // the second half drift, then the deposit of the existing slots, as
+        // the single-run loop does
+        uint32_t flags = kBatch | kPush | (last ? 0u : kDrift2);
         if (o->collide) flags |= kCollide | (sec ? (kStageSec | (last ? 0u : kDriftSec)) : 0u);
         PartArgs ps = pa;
         ps.qr = last ? nullptr : st.q[nxt][0];
@@ -287,7 +288,7 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
         }
         if (!last) {
             TimedScope ts(ctx, kTDrift);
-            HIPCHK(launch_drift_deposit(true, true, true, ctx->geom, ps, grid, ctx->stream), "drift + deposit");
+            HIPCHK(launch_drift_deposit(false, true, true, ctx->geom, ps, grid, ctx->stream), "deposit");
         }
         if (sec) {
             TimedScope ts(ctx, kTAppend);

@@ -267,6 +267,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
 {
     static_assert((F & (kDrift | kDeposit)) == 0, "drift and deposit run in k_drift_deposit");
     static_assert((F & (kStageSec | kStageIon)) == 0 || (F & kCollide), "staging needs collisions");
+    static_assert((F & kDrift2) == 0 || (F & kPush), "the second half drift follows a push");
     constexpr bool kCol = (F & kCollide) != 0;
     constexpr bool kBat = (F & kBatch) != 0;
     constexpr int kWaves = kBlock / kWaveSize;
@@ -354,7 +355,20 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
             collide_candidate(g, qca, s_tab, (uint32_t)slot, px, py, pz, vx, vy, vz, Q[6][qi], u01(R.b),
                               u01(R.c), u01(R.d), (F & kStageIon) != 0, co);
             ev = co.ev;
-            if (ev & kEvMask) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
+            if constexpr ((F & kDrift2) != 0) {
+                // the candidate's second half drift from its mid-step position,
+                // with its velocity after the collision (null or not)
+                float x2 = px, y2 = py, z2 = pz;
+                half_drift(g, x2, y2, z2, vx, vy, vz);
+                if (!wedge_bc(g, x2, y2, z2, vx, vy, vz)) {
+                    z2 = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+                    ++cnt[kCAbsorbed];
+                }
+                qpa.x[slot] = x2; qpa.y[slot] = y2; qpa.z[slot] = z2;
+                qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz;
+            } else {
+                if (ev & kEvMask) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
+            }
             gs = q_cs + slot / kSeg;
         }
         wave_sync();   // the ring entries are free once read
@@ -503,6 +517,13 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
                 cand[p] = u1[p] < ca.pmax && v2 >= ca.v2_lo && v2 <= ca.v2_hi;
             }
+            if ((F & kDrift2) && alive[p] && !cand[p]) {   // a candidate's is done by its round
+                half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+                    alive[p] = false;
+                    ++cnt[kCAbsorbed];
+                }
+            }
             if (F & kPush) cnt[kCPushed] += was_alive[p];
             if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
         }
@@ -588,6 +609,35 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 cnt[kCViol] += (ev[p] & kEvViol) != 0;
                 cnt[kCSat] += (ev[p] & kEvSat) != 0;
             }
+            if (F & kStageIon) {
+                int mine = 0;
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) mine += has_ion[p];
+                int tot;
+                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (has_ion[p]) {   // ion at the ionization site (electron_collision.m:63)
+                        pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
+                        pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
+                        ++dst;
+                    }
+                }
+                if (lane == 0) pa.ion_cnt[gseg] = tot;
+            }
+            if constexpr ((F & kDrift2) != 0) {   // the candidates' second half drift, after their collision
+#pragma unroll
+                for (int p = 0; p < kPer; ++p) {
+                    if (cand[p] && alive[p]) {
+                        half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
+                        if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+                            alive[p] = false;
+                            z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+                            ++cnt[kCAbsorbed];
+                        }
+                    }
+                }
+            }
             if (vec) {
                 vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
                 vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
@@ -620,22 +670,6 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                     }
                 }
                 if (lane == 0) pa.sec_cnt[gseg] = tot;
-            }
-            if (F & kStageIon) {
-                int mine = 0;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) mine += has_ion[p];
-                int tot;
-                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (has_ion[p]) {   // ion at the ionization site (electron_collision.m:63)
-                        pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
-                        pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
-                        ++dst;
-                    }
-                }
-                if (lane == 0) pa.ion_cnt[gseg] = tot;
             }
         } else {
             // ---- store (a later round rewrites the velocities of those that collide) --
@@ -907,8 +941,11 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
     // the events (if any) take the kernel's own start and end times: no marker
     // packets between the launches of the step
     if (batch) {
-        if (!drift || !deposit) return hipErrorInvalidValue;
-        hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+        if (!deposit) return hipErrorInvalidValue;
+        if (drift)
+            hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+        else
+            hipExtLaunchKernelGGL((k_drift_deposit<false, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift && deposit) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift) {
@@ -1272,7 +1309,8 @@ static int occupancy_t()
 
 int particle_occupancy(uint32_t flags)
 {
-    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec;
+    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec,
+                       D2 = kDrift2;
     switch (flags) {
     case P: return occupancy_t<P>();
     case Co | S: return occupancy_t<Co | S>();
@@ -1283,10 +1321,18 @@ int particle_occupancy(uint32_t flags)
     case P | Co | S | I: return occupancy_t<P | Co | S | I>();
     case P | Co | S | DS: return occupancy_t<P | Co | S | DS>();
     case P | Co | S | I | DS: return occupancy_t<P | Co | S | I | DS>();
+    case P | D2: return occupancy_t<P | D2>();
+    case P | Co | D2: return occupancy_t<P | Co | D2>();
+    case P | Co | I | D2: return occupancy_t<P | Co | I | D2>();
+    case P | Co | S | DS | D2: return occupancy_t<P | Co | S | DS | D2>();
+    case P | Co | S | I | DS | D2: return occupancy_t<P | Co | S | I | DS | D2>();
     case B | P: return occupancy_t<B | P>();
     case B | P | Co: return occupancy_t<B | P | Co>();
     case B | P | Co | S: return occupancy_t<B | P | Co | S>();
     case B | P | Co | S | DS: return occupancy_t<B | P | Co | S | DS>();
+    case B | P | D2: return occupancy_t<B | P | D2>();
+    case B | P | Co | D2: return occupancy_t<B | P | Co | D2>();
+    case B | P | Co | S | DS | D2: return occupancy_t<B | P | Co | S | DS | D2>();
     default: return 1;
     }
 }
@@ -1295,7 +1341,8 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
                             int max_blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     const int blocks = max_blocks > 0 ? max_blocks : 1;
-    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec;
+    constexpr uint32_t P = kPush, Co = kCollide, S = kStageSec, I = kStageIon, B = kBatch, DS = kDriftSec,
+                       D2 = kDrift2;
     switch (flags) {
     case P: return launch_t<P>(g, pa, ca, blocks, s, ev0, ev1);
     case Co | S: return launch_t<Co | S>(g, pa, ca, blocks, s, ev0, ev1);
@@ -1306,11 +1353,20 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
     case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s, ev0, ev1);
     case P | Co | S | DS: return launch_t<P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
     case P | Co | S | I | DS: return launch_t<P | Co | S | I | DS>(g, pa, ca, blocks, s, ev0, ev1);
+    // with the step's second half drift
+    case P | D2: return launch_t<P | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | D2: return launch_t<P | Co | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | I | D2: return launch_t<P | Co | I | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S | DS | D2: return launch_t<P | Co | S | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case P | Co | S | I | DS | D2: return launch_t<P | Co | S | I | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
     // batched ensemble
     case B | P: return launch_t<B | P>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | S | DS: return launch_t<B | P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | D2: return launch_t<B | P | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | D2: return launch_t<B | P | Co | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | S | DS | D2: return launch_t<B | P | Co | S | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
     default: return hipErrorInvalidValue;
     }
 }
