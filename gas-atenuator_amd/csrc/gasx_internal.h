@@ -21,11 +21,12 @@ constexpr int kPer = GASX_KPER;  // consecutive slots per lane
 constexpr int kSeg = kWaveSize * kPer;  // slots per wave segment: the unit of work and of staging
 constexpr int kChunk = kBlock * kPer;   // alignment unit of pooled batch runs (a multiple of kSeg)
 #ifndef GASX_KPER_DRIFT
-#define GASX_KPER_DRIFT 8
+#define GASX_KPER_DRIFT 4
 #endif
 // consecutive slots per lane of the half drift + deposit pass (a multiple of 4:
 // float4 loads); more slots per lane merge more same-cell contributions in
-// registers before they reach the shared current tile
+// registers before they reach the shared current tile, fewer leave room for
+// more resident waves (4 measured best for the deposit alone)
 constexpr int kPerD = GASX_KPER_DRIFT;
 constexpr int kChunkD = kBlock * kPerD;  // slots per drift/deposit workgroup chunk
 constexpr int kCounterSlots = 8192;  // per-block event counter slots (>= particle kernel grid)

@@ -92,7 +92,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 // a stencil cell outside the box (or a box too large) goes to global memory
 // directly.  Integer sums: the result does not depend on the grouping.
 #ifndef GASX_DEP_TILE
-#define GASX_DEP_TILE 512
+#define GASX_DEP_TILE 384
 #endif
 constexpr int kTile = GASX_DEP_TILE;   // cells (16 bytes of LDS each)
 #ifndef GASX_DEP_COPIES
@@ -256,12 +256,14 @@ __device__ __forceinline__ void vstore4n(float* p, const float (&v)[N])
 // queued in the wave's LDS ring across segments and processed 64 at a time,
 // one per lane, so a round is full however few candidates a segment has.
 // A segment's slots are stored when it is pushed; a round then rewrites the
-// velocities of the particles that collided (after a fence on the wave's
-// earlier stores) and stages their secondaries and ions.
-// The queue is in (segment, slot) order, so each staged entry's rank inside
-// its segment -- the append order -- is the parent slot order.  Batched
-// ensembles give each wave a contiguous range of segments, so its ring rarely
-// has to drain at a run boundary.
+// particles that were candidates (after a fence on the wave's earlier stores:
+// their velocity after the collision and, with kDrift2, their second half
+// drift) and stages their secondaries and ions.  The queue is in (segment,
+// slot) order, so each staged entry's rank inside its segment -- the append
+// order -- is the parent slot order.  Batched ensembles, whose waves meet a
+// new run at most segments (where a ring would have to drain), process each
+// segment's candidates in rounds before its store and hand the results back
+// to their owner lanes.
 template <uint32_t F>
 __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa0, const CollArgs& ca0)
 {
@@ -759,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_particles_batch(Geom g, PartArgs pa0
 // stencil (cell-sorted slots mostly do), adds each run into the chunk's LDS
 // tile, and the tile is flushed into the workgroup's copy of the current.
 template <bool kDr, bool kDe, bool kBat>
-__global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
+__device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs& pa0)
 {
     constexpr int kWaves = kBlock / kWaveSize;
     __shared__ int s_wave[kWaves];
@@ -934,6 +936,21 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
                         kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
 }
 
+template <bool kDr, bool kDe, bool kBat>
+__global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
+{
+    drift_deposit_pass<kDr, kDe, kBat>(g, pa0);
+}
+
+// The deposit alone (every push of the production loop): six waves per SIMD
+// (80 registers, a 384-cell tile in four copies) hide more of its LDS and
+// memory latency than four.
+template <bool kBat>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deposit(Geom g, PartArgs pa0)
+{
+    drift_deposit_pass<false, true, kBat>(g, pa0);
+}
+
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
                                 int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
@@ -945,13 +962,13 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
         if (drift)
             hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
         else
-            hipExtLaunchKernelGGL((k_drift_deposit<false, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+            hipExtLaunchKernelGGL(k_deposit<true>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift && deposit) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (deposit) {
-        hipExtLaunchKernelGGL((k_drift_deposit<false, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+        hipExtLaunchKernelGGL(k_deposit<false>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else {
         return hipErrorInvalidValue;
     }
