@@ -6,7 +6,7 @@ Workload (default): the plain GPU step at scale -- axisymmetric r-z grid
 spot (sigma_r = 50 um) with the 70/30 400 +- 20 eV / 30 +- 10 eV energy mix,
 uniform N2 at 2.5e25 m^-3, one collision check per push (dt = 1e-14 s,
 10 field substeps), ionization secondaries on (capacity 2x), cell sort every
-10 steps, FP32 particles, FP64 field solver.
+40 steps, FP32 particles, FP64 field solver.
 
 A step = one push of every electron + its collision check + the field cycle.
 value = particle-steps/s over the K timed steps (inputs resident in HBM).

@@ -186,6 +186,7 @@ constexpr int kGatherBatch = GASX_GATHER_BATCH;
 static_assert(kPer % kGatherBatch == 0, "gather batch divides the slots per lane");
 
 static_assert(kPer == 4, "one decision draw covers a lane's four slots");
+
 constexpr int kQW = 11;     // LDS queue words per candidate
 constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes 64)
 
@@ -260,10 +261,9 @@ __device__ __forceinline__ void vstore4n(float* p, const float (&v)[N])
 // their velocity after the collision and, with kDrift2, their second half
 // drift) and stages their secondaries and ions.  The queue is in (segment,
 // slot) order, so each staged entry's rank inside its segment -- the append
-// order -- is the parent slot order.  Batched ensembles, whose waves meet a
-// new run at most segments (where a ring would have to drain), process each
-// segment's candidates in rounds before its store and hand the results back
-// to their owner lanes.
+// order -- is the parent slot order.  In a batched ensemble each wave takes a
+// contiguous range of segments, so consecutive segments mostly share a run and
+// the ring is drained only where the run changes.
 template <uint32_t F>
 __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa0, const CollArgs& ca0)
 {
@@ -423,7 +423,17 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
     };
 
     const int n_waves = gridDim.x * kWaves;
-    for (int gseg = blockIdx.x * kWaves + wave; gseg < total_segs; gseg += n_waves) {
+    // ensembles on the ring: each wave takes a contiguous range of segments, so
+    // consecutive segments mostly share a run and the ring drains only where
+    // the run changes; otherwise waves grid-stride over the segments
+    int g_begin = blockIdx.x * kWaves + wave, g_end = total_segs, g_step = n_waves;
+    if constexpr (kBat) {
+        const int per = (total_segs + n_waves - 1) / n_waves;
+        g_begin = (blockIdx.x * kWaves + wave) * per;
+        g_end = min(total_segs, g_begin + per);
+        g_step = 1;
+    }
+    for (int gseg = g_begin; gseg < g_end; gseg += g_step) {
         // seg: index within the population (slots, Philox counters);
         // gseg: index of the staging area
         PartArgs pa = pa0;
@@ -453,6 +463,14 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
             ca.pmax = br.pmax;
             ca.run = br.run;
             if (run != cnt_run) {
+                if constexpr (kCol) {
+                    // the queued candidates belong to the previous run: process
+                    // them (counted for it), then queue this run's
+                    drain();
+                    qpa = pa;
+                    qca = ca;
+                    q_cs = s_cs[run];
+                }
                 if (cnt_run >= 0 && pa0.counters)
                     wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
                                     pa0.run_counters + (size_t)cnt_run * kNCounters);
@@ -529,152 +547,8 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
             if (F & kPush) cnt[kCPushed] += was_alive[p];
             if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
         }
-        if constexpr (kBat) {
-            // ---- B (ensemble): the segment's candidates in rounds of 64 before its
-            // store, results handed back to their owners through the LDS queue --
-            // a wave of an ensemble meets a new run at most segments, where a ring
-            // would have to drain anyway
-            int ev[kPer] = {};
-            float svx[kPer] = {}, svy[kPer] = {}, svz[kPer] = {}, ivx[kPer] = {}, ivy[kPer] = {}, ivz[kPer] = {};
-            if constexpr (kCol) {
-                int mine = 0;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) mine += cand[p];
-                int total;
-                const int off = wave_prefix_small<kPerBits>(mine, total);
-                for (int base = 0; base < total; base += kWaveSize) {
-                    int pos = off;
-#pragma unroll
-                    for (int p = 0; p < kPer; ++p) {
-                        if (cand[p]) {
-                            const int q = pos - base;
-                            if (q >= 0 && q < kWaveSize) {
-                                Q[0][q] = x[p]; Q[1][q] = y[p]; Q[2][q] = z[p];
-                                Q[3][q] = vx[p]; Q[4][q] = vy[p]; Q[5][q] = vz[p];
-                                Q[6][q] = u1[p];
-                                Q[10][q] = __int_as_float(slot0 + p);
-                            }
-                            ++pos;
-                        }
-                    }
-                    wave_sync();
-                    if (lane < min(kWaveSize, total - base)) {
-                        const int q = lane;
-                        float qvx = Q[3][q], qvy = Q[4][q], qvz = Q[5][q];
-                        CollOut co;
-                        const uint32_t qslot = (uint32_t)__float_as_int(Q[10][q]);
-                        const U4 R = philox4x32_10(qslot, ca.step, ca.run, 0u, ca.key0, ca.key1);
-                        collide_candidate(g, ca, s_tab, qslot, Q[0][q], Q[1][q], Q[2][q], qvx, qvy, qvz,
-                                          Q[6][q], u01(R.b), u01(R.c), u01(R.d), (F & kStageIon) != 0, co);
-                        Q[3][q] = qvx; Q[4][q] = qvy; Q[5][q] = qvz;
-                        Q[10][q] = __int_as_float(co.ev);
-                        if (co.ev & kEvSec) { Q[6][q] = co.svx; Q[7][q] = co.svy; Q[8][q] = co.svz; }
-                        if (co.ev & kEvIon) { Q[0][q] = co.ivx; Q[1][q] = co.ivy; Q[2][q] = co.ivz; }
-                    }
-                    wave_sync();
-                    pos = off;
-#pragma unroll
-                    for (int p = 0; p < kPer; ++p) {
-                        if (cand[p]) {
-                            const int q = pos - base;
-                            if (q >= 0 && q < kWaveSize) {
-                                vx[p] = Q[3][q]; vy[p] = Q[4][q]; vz[p] = Q[5][q];
-                                ev[p] = __float_as_int(Q[10][q]);
-                                if (ev[p] & kEvSec) { svx[p] = Q[6][q]; svy[p] = Q[7][q]; svz[p] = Q[8][q]; }
-                                if (ev[p] & kEvIon) { ivx[p] = Q[0][q]; ivy[p] = Q[1][q]; ivz[p] = Q[2][q]; }
-                            }
-                            ++pos;
-                        }
-                    }
-                    wave_sync();   // the queue is rewritten by the next round
-                }
-            }
-            // ---- C (ensemble): store; stage in slot order -------------------------
-            bool has_sec[kPer], has_ion[kPer];
-            float sx[kPer], sy[kPer], sz[kPer];
-#pragma unroll
-            for (int p = 0; p < kPer; ++p) {
-                has_sec[p] = (F & kStageSec) && (ev[p] & kEvSec);
-                has_ion[p] = (F & kStageIon) && (ev[p] & kEvIon);
-                sx[p] = x[p]; sy[p] = y[p]; sz[p] = z[p];
-                if ((F & kDriftSec) && has_sec[p]) {
-                    half_drift(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p]);
-                    if (!wedge_bc(g, sx[p], sy[p], sz[p], svx[p], svy[p], svz[p])) {
-                        has_sec[p] = false;
-                        ++cnt[kCAbsorbed];
-                    }
-                }
-                const int typ = ev[p] & kEvMask;
-                cnt[kCElastic] += typ == 1;
-                cnt[kCIonize] += typ == 2;
-                cnt[kCExcite] += typ == 3;
-                cnt[kCViol] += (ev[p] & kEvViol) != 0;
-                cnt[kCSat] += (ev[p] & kEvSat) != 0;
-            }
-            if (F & kStageIon) {
-                int mine = 0;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) mine += has_ion[p];
-                int tot;
-                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (has_ion[p]) {   // ion at the ionization site (electron_collision.m:63)
-                        pa.ion[0][dst] = x[p]; pa.ion[1][dst] = y[p]; pa.ion[2][dst] = z[p];
-                        pa.ion[3][dst] = ivx[p]; pa.ion[4][dst] = ivy[p]; pa.ion[5][dst] = ivz[p];
-                        ++dst;
-                    }
-                }
-                if (lane == 0) pa.ion_cnt[gseg] = tot;
-            }
-            if constexpr ((F & kDrift2) != 0) {   // the candidates' second half drift, after their collision
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (cand[p] && alive[p]) {
-                        half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
-                        if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
-                            alive[p] = false;
-                            z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
-                            ++cnt[kCAbsorbed];
-                        }
-                    }
-                }
-            }
-            if (vec) {
-                vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
-                vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
-            } else {
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    const int slot = slot0 + p;
-                    if (slot < n && (was_alive[p] || moved)) {
-                        if (alive[p] || moved) {
-                            pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
-                            pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
-                        } else {
-                            pa.z[slot] = z[p];
-                        }
-                    }
-                }
-            }
-            if (F & kStageSec) {
-                int mine = 0;
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) mine += has_sec[p];
-                int tot;
-                int dst = gseg * kSeg + wave_prefix_small<kPerBits>(mine, tot);
-#pragma unroll
-                for (int p = 0; p < kPer; ++p) {
-                    if (has_sec[p]) {
-                        pa.sec[0][dst] = sx[p]; pa.sec[1][dst] = sy[p]; pa.sec[2][dst] = sz[p];
-                        pa.sec[3][dst] = svx[p]; pa.sec[4][dst] = svy[p]; pa.sec[5][dst] = svz[p];
-                        ++dst;
-                    }
-                }
-                if (lane == 0) pa.sec_cnt[gseg] = tot;
-            }
-        } else {
-            // ---- store (a later round rewrites the velocities of those that collide) --
+        {
+            // ---- store (a later round rewrites the candidates) ----------------------
             if (vec) {   // dead slots are written back unchanged
                 vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
                 vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
