@@ -954,6 +954,9 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
     __shared__ int s_wave[kBlock / 64];
     const int base = tmp2[0];
     const int nseg = tmp2[1];
+    // full: everything staged is dropped; only workgroup 0 (the count and the
+    // dropped tally) has work
+    if (base >= cap && blockIdx.x != 0) return;
     // exclusive prefix of the group totals (four per thread)
     {
         constexpr int kG = kMaxScanGroups / kBlock;
