@@ -43,12 +43,21 @@ __device__ __forceinline__ void deposit_cells(const Geom& g, float x, float y, f
     const float w[4] = {wr0 * wz0, wr1 * wz0, wr0 * wz1, wr1 * wz1};
     // |w q v / c| 2^32 < 2^31 whenever |q v| < c/2 (always for the electrons
     // here): one float->int32 conversion instead of the longer int64 one
+    // (a branch, not a select: the int64 conversions run only in the rare
+    // wave that holds such a particle)
     const bool small = fabsf(qrv) < 2147483648.0f && fabsf(qzv) < 2147483648.0f;
+    if (small) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const float tr = rintf(w[a] * qrv), tz = rintf(w[a] * qzv);
-        d.qr[a] = small ? (long long)(int)tr : (long long)tr;
-        d.qz[a] = small ? (long long)(int)tz : (long long)tz;
+        for (int a = 0; a < 4; ++a) {
+            d.qr[a] = (long long)(int)rintf(w[a] * qrv);
+            d.qz[a] = (long long)(int)rintf(w[a] * qzv);
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            d.qr[a] = (long long)rintf(w[a] * qrv);
+            d.qz[a] = (long long)rintf(w[a] * qzv);
+        }
     }
 }
 
