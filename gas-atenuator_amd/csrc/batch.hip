@@ -505,7 +505,9 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
         if (s == GASX_OK && ms < best_ms) { best = c; best_ms = ms; }
     }
     // one or two generations of resident workgroups, or a fixed oversubscription
-    const int occ = particle_occupancy(kBatch | kPush | kCollide | kStageSec | kDriftSec);
+    const uint32_t step_flags = kBatch | kPush | kDrift2 |
+                                (o->collide ? kCollide | (o->secondaries ? kStageSec | kDriftSec : 0u) : 0u);
+    const int occ = particle_occupancy(step_flags);
     const int per_cu[] = {occ, 2 * occ, 4, 8, 16, 32};
     for (int k = 0; k < (int)(sizeof(per_cu) / sizeof(per_cu[0])); ++k) {
         const int pc = per_cu[k];

@@ -1213,8 +1213,6 @@ int particle_occupancy(uint32_t flags)
     case P | Co | S: return occupancy_t<P | Co | S>();
     case P | Co | I: return occupancy_t<P | Co | I>();
     case P | Co | S | I: return occupancy_t<P | Co | S | I>();
-    case P | Co | S | DS: return occupancy_t<P | Co | S | DS>();
-    case P | Co | S | I | DS: return occupancy_t<P | Co | S | I | DS>();
     case P | D2: return occupancy_t<P | D2>();
     case P | Co | D2: return occupancy_t<P | Co | D2>();
     case P | Co | I | D2: return occupancy_t<P | Co | I | D2>();
@@ -1223,7 +1221,6 @@ int particle_occupancy(uint32_t flags)
     case B | P: return occupancy_t<B | P>();
     case B | P | Co: return occupancy_t<B | P | Co>();
     case B | P | Co | S: return occupancy_t<B | P | Co | S>();
-    case B | P | Co | S | DS: return occupancy_t<B | P | Co | S | DS>();
     case B | P | D2: return occupancy_t<B | P | D2>();
     case B | P | Co | D2: return occupancy_t<B | P | Co | D2>();
     case B | P | Co | S | DS | D2: return occupancy_t<B | P | Co | S | DS | D2>();
@@ -1245,8 +1242,6 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
     case P | Co | S: return launch_t<P | Co | S>(g, pa, ca, blocks, s, ev0, ev1);
     case P | Co | I: return launch_t<P | Co | I>(g, pa, ca, blocks, s, ev0, ev1);
     case P | Co | S | I: return launch_t<P | Co | S | I>(g, pa, ca, blocks, s, ev0, ev1);
-    case P | Co | S | DS: return launch_t<P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
-    case P | Co | S | I | DS: return launch_t<P | Co | S | I | DS>(g, pa, ca, blocks, s, ev0, ev1);
     // with the step's second half drift
     case P | D2: return launch_t<P | D2>(g, pa, ca, blocks, s, ev0, ev1);
     case P | Co | D2: return launch_t<P | Co | D2>(g, pa, ca, blocks, s, ev0, ev1);
@@ -1257,7 +1252,6 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
     case B | P: return launch_t<B | P>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co: return launch_t<B | P | Co>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | S: return launch_t<B | P | Co | S>(g, pa, ca, blocks, s, ev0, ev1);
-    case B | P | Co | S | DS: return launch_t<B | P | Co | S | DS>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | D2: return launch_t<B | P | D2>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | D2: return launch_t<B | P | Co | D2>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | S | DS | D2: return launch_t<B | P | Co | S | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
