@@ -410,9 +410,9 @@ static hipError_t launch_fdtd_adj_part(const FdtdArgs& a, const AdjFdtdRun& r, h
     const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
     const int npos = (W + 1) * (Hh + 1);
     if (fdtd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
-    if (npos <= 256 * 4) return launch_fdtd_adj_t<256, 4>(a, r, s);
-    if (npos <= 512 * 4) return launch_fdtd_adj_t<512, 4>(a, r, s);
-    if (npos <= 512 * 8) return launch_fdtd_adj_t<512, 8>(a, r, s);
+    if (npos <= 512 * 2) return launch_fdtd_adj_t<512, 2>(a, r, s);
+    if (npos <= 1024 * 2) return launch_fdtd_adj_t<1024, 2>(a, r, s);
+    if (npos <= 1024 * 4) return launch_fdtd_adj_t<1024, 4>(a, r, s);
     if (npos <= 1024 * 8) return launch_fdtd_adj_t<1024, 8>(a, r, s);
     return hipErrorInvalidValue;
 }
@@ -437,10 +437,10 @@ hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s
 // must not wait for the first launch, which may be under graph capture).
 hipError_t fdtd_prepare()
 {
-    const void* fns[] = {(const void*)k_fdtd<256, 4>,     (const void*)k_fdtd<512, 4>,
-                         (const void*)k_fdtd<512, 8>,     (const void*)k_fdtd<1024, 8>,
-                         (const void*)k_fdtd_adj<256, 4>, (const void*)k_fdtd_adj<512, 4>,
-                         (const void*)k_fdtd_adj<512, 8>, (const void*)k_fdtd_adj<1024, 8>};
+    const void* fns[] = {(const void*)k_fdtd<512, 2>,     (const void*)k_fdtd<1024, 2>,
+                         (const void*)k_fdtd<1024, 4>,    (const void*)k_fdtd<1024, 8>,
+                         (const void*)k_fdtd_adj<512, 2>, (const void*)k_fdtd_adj<1024, 2>,
+                         (const void*)k_fdtd_adj<1024, 4>, (const void*)k_fdtd_adj<1024, 8>};
     for (const void* f : fns) {
         hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
@@ -454,9 +454,11 @@ static hipError_t launch_fdtd_part(const FdtdArgs& a, const FdtdRun& r0, const F
     const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
     const int npos = (W + 1) * (Hh + 1);
     if (fdtd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
-    if (npos <= 256 * 4) return launch_fdtd_t<256, 4>(a, r0, runs_dev, nruns, fixed_j, s);
-    if (npos <= 512 * 4) return launch_fdtd_t<512, 4>(a, r0, runs_dev, nruns, fixed_j, s);
-    if (npos <= 512 * 8) return launch_fdtd_t<512, 8>(a, r0, runs_dev, nruns, fixed_j, s);
+    // the most threads a tile can use, a few positions each: one workgroup per
+    // CU on small grids, so resident waves are what hides the latency
+    if (npos <= 512 * 2) return launch_fdtd_t<512, 2>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (npos <= 1024 * 2) return launch_fdtd_t<1024, 2>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (npos <= 1024 * 4) return launch_fdtd_t<1024, 4>(a, r0, runs_dev, nruns, fixed_j, s);
     if (npos <= 1024 * 8) return launch_fdtd_t<1024, 8>(a, r0, runs_dev, nruns, fixed_j, s);
     return hipErrorInvalidValue;
 }
