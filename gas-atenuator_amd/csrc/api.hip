@@ -729,7 +729,7 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         const int cur = s & 1, nxt = cur ^ 1;
         const bool last = s == o.n_steps - 1;
         gasx_status st;
-        {
+        if (kJCopies > 1) {   // fold the current copies into copy 0
             TimedScope ts(ctx, kTFdtd);
             HIPCHK(launch_j_reduce(ctx->q[cur][0], ctx->q[cur][1], (int64_t)nh, nh, ctx->stream), "j reduce");
         }
