@@ -109,13 +109,12 @@ struct TileBox {
     int ilo, klo, wi, nk;     // nk == 0: no tile (nothing to deposit, or the box exceeds kTile)
 };
 
-// A box that contains the particle's deposit stencil: the stencil cells
-// computed with fast arithmetic, widened by one cell (the exact arithmetic of
-// deposit_cells can land one cell over at a boundary).
-__device__ __forceinline__ void deposit_box(const Geom& g, float x, float y, float z, int& i0, int& i1,
-                                            int& k0, int& k1)
+// A box that contains the deposit stencil of a particle at radius r, height z:
+// the stencil cells computed with fast arithmetic, widened by one cell (the
+// exact arithmetic of deposit_cells can land one cell over at a boundary).
+__device__ __forceinline__ void deposit_box(const Geom& g, float r, float z, int& i0, int& i1, int& k0,
+                                            int& k1)
 {
-    const float r = fsqrt(x * x + y * y);
     const int a = (int)floorf(r * g.inv_dr - 0.5f), b = (int)floorf(z * g.inv_dz - 0.5f);
     i0 = max(min(a - 1, g.nr - 1), 0); i1 = max(min(a + 2, g.nr - 1), 0);
     k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
@@ -167,9 +166,9 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
         const unsigned long long vr = (unsigned long long)d.q[a], vz = (unsigned long long)d.q[4 + a];
         const int ti = i - b.ilo, tk = k - b.klo;
         if ((unsigned)ti < (unsigned)b.wi && (unsigned)tk < (unsigned)b.nk) {
-            const int t = ti + tk * b.wi;
-            if (vr) atomicAdd(&s_tr[t], vr);
-            if (vz) atomicAdd(&s_tz[t], vz);
+            const int t = ti + tk * b.wi;     // (adding a zero costs less than testing for it)
+            atomicAdd(&s_tr[t], vr);
+            atomicAdd(&s_tz[t], vz);
         } else {                                 // outside the box: straight to global
             const int c = i + k * nr;
             if (vr) atomicAdd(&qr[c], vr);
@@ -727,7 +726,9 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
                 }
             }
         }
-        int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
+        // the thread's extent in r^2 and z; its stencil box once, from the extremes
+        // (the box is monotone in both)
+        float r2lo = INFINITY, r2hi = -INFINITY, zlo = INFINITY, zhi = -INFINITY;
 #pragma unroll
         for (int p = 0; p < kPerD; ++p) {
             alive[p] = slot0 + p < n && !is_dead(z[p]);
@@ -741,10 +742,18 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
                 }
             }
             if (kDe && alive[p]) {
-                int i0, i1, k0, k1;
-                deposit_box(g, x[p], y[p], z[p], i0, i1, k0, k1);
-                ilo = min(ilo, i0); ihi = max(ihi, i1); klo = min(klo, k0); khi = max(khi, k1);
+                const float r2 = x[p] * x[p] + y[p] * y[p];
+                r2lo = fminf(r2lo, r2); r2hi = fmaxf(r2hi, r2);
+                zlo = fminf(zlo, z[p]); zhi = fmaxf(zhi, z[p]);
             }
+        }
+        int ilo = 0x7fffffff, ihi = -1, klo = 0x7fffffff, khi = -1;
+        if (kDe && r2lo <= r2hi) {
+            int i0, i1, k0, k1;
+            deposit_box(g, fsqrt(r2lo), zlo, i0, i1, k0, k1);
+            ilo = i0; klo = k0;
+            deposit_box(g, fsqrt(r2hi), zhi, i0, i1, k0, k1);
+            ihi = i1; khi = k1;
         }
         if (kDr) {
             if (vec) {   // dead slots are written back unchanged
