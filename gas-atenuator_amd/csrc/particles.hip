@@ -1039,8 +1039,10 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                        stage_cnt, scan_tmp, totals, tmp2, max_segs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int wseg = (max_segs + kBlock / 64 - 1) / (kBlock / 64);   // a wave per segment
-    const int blocks = wseg < 4096 ? (wseg > 0 ? wseg : 1) : 4096;
+    // a wave per segment, at most 1024 workgroups (with a full population all
+    // but workgroup 0 return at once, and launching them is most of the cost)
+    const int wseg = (max_segs + kBlock / 64 - 1) / (kBlock / 64);
+    const int blocks = wseg < 1024 ? (wseg > 0 ? wseg : 1) : 1024;
     hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3],
                        stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped, g, kq,

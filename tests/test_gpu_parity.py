@@ -428,3 +428,22 @@ def test_collision_window_edges_match_oracle(gpu, otab):
     assert collided.sum() > 0.05 * n and (~collided).sum() > 0.05 * n
     assert np.array_equal(moved[~near], collided[~near])
     assert ctx.counters().n_bound_violations == 0
+
+
+def test_deposit_bit_exact_fast_particles(gpu):
+    """Speeds up to 0.97 c, mixed with slow ones: contributions with |q v| >= c/2
+    exceed int32 before the weights and take the int64 conversion, which the
+    kernel runs as a branch (a wave of slow particles skips it). Bit for bit
+    against the oracle's FP32 emulation."""
+    import gasx
+    log("deposit fast particles")
+    nr, nz = 40, 70
+    p, ctx, op = mk(nr, nz, fn=3.0)
+    st = wedge_particles(40000, nr, nz, p.dr, p.dz, p.phip, 6, vmax=2.9e8)
+    st[3:6, ::3] *= np.float32(0.01)            # every third slow
+    ps = gasx.ParticleSet(st.shape[1]).load(*st)
+    speed = np.sqrt(np.sum(st[3:6].astype(np.float64) ** 2, axis=0))
+    assert (speed > 0.5 * 299792458.0).mean() > 0.2
+    qr, qz = ctx.deposit(ps)
+    oqr, oqz = oracle.deposit_fx32(op, st, -1.0)
+    assert np.array_equal(qr.cpu().numpy(), oqr) and np.array_equal(qz.cpu().numpy(), oqz)
