@@ -100,7 +100,7 @@ enum : uint32_t {
     kStageSec = 16u,   // stage ionization secondaries for append
     kStageIon = 32u,   // stage N2+ ions for append
     kBatch = 64u,      // chunks belong to the runs of a batched ensemble
-    kDriftSec = 128u,  // half drift + BC of staged secondaries (when kDrift runs in a later launch)
+    kDriftSec = 128u,  // half drift + BC of staged secondaries (appended after the deposit has read the slots)
     kDrift2 = 256u,    // the step's second half drift + BC of every electron, after its collision
                        // (the launch that follows then deposits without writing the particles)
 };

@@ -5,22 +5,25 @@
 //   kCollide  electron_transport.cu:64-227 (+ find_CS0.m cross sections)
 //   kStageSec / kStageIon     electron_collision.m:63-75 append, staged per wave
 //                             segment in slot order so the final order is deterministic
-//   kDriftSec                 half drift + BC of the staged secondaries (the
-//                             primaries get theirs in k_drift_deposit)
+//   kDriftSec                 half drift + BC of the staged secondaries
+//   kDrift2   EM2.cu:23-57    the step's second half drift + BC of every electron,
+//                             after its collision (the next launch only deposits)
 //   kBatch                    segments belong to the runs of a batched ensemble
 // Each wave owns segments of kSeg consecutive slots, kPer consecutive slots
-// per lane (vector loads and stores), and grid-strides over segments with no
-// workgroup barrier, so waves hide each other's latency.  Per segment:
-//   A  every lane: load, push, boundary conditions, first Philox block;
-//      slots with u1 < pmax (the exact rejection bound on p_T) are candidates
-//   B  the wave's candidates are compacted into its LDS queue and processed
-//      one per lane (cross sections, decision, elastic / ionization /
-//      excitation), so the collision work is dense instead of divergent
-//   C  every lane: read back, store; secondaries and ions are staged
+// per lane (vector loads and stores), with no workgroup barrier, so waves hide
+// each other's latency.  Per segment:
+//   A  every lane: load, push, boundary conditions, the decision draw; slots
+//      with u1 < pmax (the exact rejection bound on p_T) inside the tables'
+//      energy window are candidates; the others make their second half drift
+//   B  the segment is stored; its candidates go into the wave's LDS ring and
+//      are processed 64 at a time, one per lane (cross sections, decision,
+//      elastic / ionization / excitation, second half drift), so the
+//      collision work is dense instead of divergent; secondaries and ions are
+//      staged
 //
-// k_drift_deposit<kDr, kDe, kBat>:
-//   kDrift    EM2.cu:23-57    next half drift + BC  (x^{n+3/2})
-//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+3/2} with v^{n+1},
+// k_drift_deposit<kDr, kDe, kBat> (k_deposit<kBat>: the deposit alone):
+//   kDrift    EM2.cu:23-57    half drift + BC (the prologue of a call)
+//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+1} with v^{n+1/2},
 //                             summed in a workgroup LDS tile and flushed into
 //                             the workgroup's copy of the current
 //
