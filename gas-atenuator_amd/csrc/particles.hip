@@ -23,7 +23,7 @@
 //
 // k_drift_deposit<kDr, kDe, kBat> (k_deposit<kBat>: the deposit alone):
 //   kDrift    EM2.cu:23-57    half drift + BC (the prologue of a call)
-//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+1} with v^{n+1/2},
+//   kDeposit  EM7.cu:51-202   fixed-point current at x^{n+3/2} with v^{n+1},
 //                             summed in a workgroup LDS tile and flushed into
 //                             the workgroup's copy of the current
 //
