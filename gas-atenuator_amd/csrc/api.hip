@@ -822,13 +822,7 @@ extern "C" gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* io
     o.seed = ro->seed;
     o.run_id = ro->run_id;
     o.step0 = ro->step0;
-    if (ro->time_kernels) {
-        ctx->timing = true;
-        // 2: the particle-pass launches only (the roofline kernels); else every launch
-        ctx->timing_kinds = ro->time_kernels == 2 ? (1u << kTParticle) : ~0u;
-        ctx->ev_used = 0;
-        for (auto& v : ctx->tpairs) v.clear();
-    }
+    if (ro->time_kernels) start_timing(ctx, ro->time_kernels, ro->time_every);
     gasx_status st = fused_loop(ctx, e, 1, ions, f, g, o);
     ctx->timing = false;
     return st;

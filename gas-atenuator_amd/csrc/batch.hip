@@ -323,6 +323,7 @@ std::vector<unsigned char> graph_key(const gasx_batch* b, const gasx_run_opts* o
     if (o->collide) put(b->n_max, b->nruns * sizeof(double));
     gasx_run_opts oo = *o;
     oo.time_kernels = 0;
+    oo.time_every = 0;
     put(&oo, sizeof(oo));
     put(&t, sizeof(t));
     return k;
@@ -401,13 +402,7 @@ extern "C" gasx_status gasx_batch_run(gasx_ctx* ctx, gasx_batch* b, const gasx_r
 {
     if (!ctx || !b || !o) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    if (o->time_kernels) {
-        ctx->timing = true;
-        // 2: the particle-pass launches only (the roofline kernels); else every launch
-        ctx->timing_kinds = o->time_kernels == 2 ? (1u << kTParticle) : ~0u;
-        ctx->ev_used = 0;
-        for (auto& v : ctx->tpairs) v.clear();
-    }
+    if (o->time_kernels) start_timing(ctx, o->time_kernels, o->time_every);
     gasx_status s = run_batch(ctx, b, o, ctx->tuning);
     ctx->timing = false;
     return s;

@@ -67,6 +67,8 @@ struct gasx_ctx {
     // kernel timing (gasx_run_opts.time_kernels): event pairs per launch kind
     bool timing = false;
     unsigned timing_kinds = ~0u;   // bit per TimedKind: the kinds that get event pairs
+    int timing_every = 1;          // time launches 0, k, 2k, ... of each kind
+    int64_t timing_calls[6] = {};  // launches of each kind since timing was switched on
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
     std::vector<std::pair<size_t, size_t>> tpairs[6];   // particle, fdtd, append, sort, graph, drift
@@ -90,7 +92,7 @@ struct TimedScope {
     size_t a = (size_t)-1;
     TimedScope(gasx_ctx* c_, int k) : c(c_), kind(k)
     {
-        if (c->timing && (c->timing_kinds >> k & 1u)) {
+        if (c->timing && (c->timing_kinds >> k & 1u) && c->timing_calls[k]++ % c->timing_every == 0) {
             a = next_event(c);
             if (a != (size_t)-1) (void)hipEventRecord(c->events[a], c->stream);
         }
@@ -118,6 +120,7 @@ inline KernelEvents kernel_events(gasx_ctx* c, int kind)
 {
     KernelEvents k;
     if (!c->timing || !(c->timing_kinds >> kind & 1u)) return k;
+    if (c->timing_calls[kind]++ % c->timing_every != 0) return k;
     const size_t a = next_event(c);
     if (a == (size_t)-1) return k;
     const size_t b = next_event(c);
@@ -126,6 +129,19 @@ inline KernelEvents kernel_events(gasx_ctx* c, int kind)
     k.stop = c->events[b];
     c->tpairs[kind].push_back({a, b});
     return k;
+}
+
+// Switches timing on for one run (gasx_run_opts.time_kernels / time_every):
+// 2 times the particle-pass launches only (the roofline kernels), any other
+// non-zero value every launch; every `every`-th launch of a kind is timed.
+inline void start_timing(gasx_ctx* c, int32_t kinds, int32_t every)
+{
+    c->timing = true;
+    c->timing_kinds = kinds == 2 ? (1u << kTParticle) : ~0u;
+    c->timing_every = every > 1 ? every : 1;
+    for (auto& n : c->timing_calls) n = 0;
+    c->ev_used = 0;
+    for (auto& v : c->tpairs) v.clear();
 }
 
 inline gasx_status fail(gasx_ctx* c, gasx_status s, const std::string& msg)

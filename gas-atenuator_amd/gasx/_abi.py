@@ -123,6 +123,7 @@ class RunOpts(C.Structure):
         ("n_steps", C.c_int32), ("collide", C.c_int32), ("secondaries", C.c_int32),
         ("sort_every", C.c_int32), ("seed", C.c_uint64), ("run_id", C.c_uint32),
         ("step0", C.c_uint32), ("use_graph", C.c_int32), ("time_kernels", C.c_int32),
+        ("time_every", C.c_int32),
     ]
 
 
@@ -170,7 +171,7 @@ PROTOTYPES = {
     "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _lib = None
 

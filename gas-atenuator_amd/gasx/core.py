@@ -258,13 +258,13 @@ class Context:
     def run(self, e: ParticleSet, f: FieldState, gas: GasGrid | None, n_steps: int,
             collide: bool = True, secondaries: bool = False, sort_every: int = 0, seed: int = 0,
             run_id: int = 0, step0: int = 0, ions: ParticleSet | None = None, use_graph: bool = False,
-            time_kernels: bool | int = False):
+            time_kernels: bool | int = False, time_every: int = 1):
         s = e.struct()
         fs = f.struct()
         g = gas.struct() if gas is not None else None
         si = ions.struct() if ions is not None else None
         o = RunOpts(n_steps, int(collide), int(secondaries), sort_every, seed, run_id, step0,
-                    int(use_graph), int(time_kernels))
+                    int(use_graph), int(time_kernels), time_every)
         self._check(self.lib.gasx_run(self.ctx, C.byref(s), C.byref(si) if si is not None else None,
                                       C.byref(fs), C.byref(g) if g is not None else None, C.byref(o)))
 
@@ -362,7 +362,7 @@ class Ensemble:
     @staticmethod
     def _opts(n_steps, collide, secondaries, seed, run_id, step0, use_graph, time_kernels):
         return RunOpts(n_steps, int(collide), int(secondaries), 0, seed, run_id, step0,
-                       int(use_graph), int(time_kernels))
+                       int(use_graph), int(time_kernels), 1)
 
     def run(self, n_steps: int, collide: bool = True, secondaries: bool = False, seed: int = 0,
             run_id: int = 0, step0: int = 0, use_graph: bool = False, time_kernels: bool = False):

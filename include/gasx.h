@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 3
+#define GASX_ABI_VERSION 4
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -116,7 +116,8 @@ typedef struct gasx_counters {
 } gasx_counters;
 
 /* Kernel timing of the last gasx_run with time_kernels = 1 (HIP events on
- * the context's stream around every launch). */
+ * the context's stream around every launch).  The counts are those of the
+ * timed launches (every time_every-th launch of a kind). */
 typedef struct gasx_timing {
     double particle_ms;        /* sum over particle launches (push + collide of a step, or fused passes) */
     double fdtd_ms;            /* sum over field-kernel launches */
@@ -142,6 +143,8 @@ typedef struct gasx_run_opts {
     int32_t time_kernels;       /* 1: record HIP events around every launch (gasx_get_timing);
                                    2: the push + collision launches only (the least
                                    overhead between launches) */
+    int32_t time_every;         /* with time_kernels: time launches 0, k, 2k, ... of each kind
+                                   (k = time_every; 0 or 1: every launch) */
 } gasx_run_opts;
 
 /* ---- context ------------------------------------------------------------ */

@@ -396,6 +396,32 @@ def test_in_loop_sort_keeps_trajectories(gpu, n):
         assert np.array_equal(x, y)
 
 
+def test_sampled_kernel_timing(gpu):
+    """time_kernels = 2 with time_every = k times launches 0, k, 2k, ... of
+    the particle pass and nothing else; time_kernels = 1 times every launch;
+    neither changes the results of the run."""
+    import gasx
+    log("sampled kernel timing")
+    nr, nz, n = 24, 40, 4000
+    p, _, _ = mk(nr, nz, fn=5e4)
+    st = wedge_particles(n, nr, nz, p.dr, p.dz, p.phip, 23, vmax=3e7, edge_frac=0.1)
+    out = []
+    for tk, every in ((0, 1), (2, 4), (1, 1)):
+        ctx = gasx.Context(p)
+        ps = gasx.ParticleSet(n).load(*st)
+        fs = gasx.FieldState(nr, nz)
+        ctx.run(ps, fs, None, 10, collide=False, time_kernels=tk, time_every=every)
+        out.append((ps.state(), fs.numpy(), ctx.timing()))
+    (a, fa, t0), (b, fb, t2), (c, fc, t1) = out
+    assert np.array_equal(a, b, equal_nan=True) and np.array_equal(a, c, equal_nan=True)
+    for x, y, z in zip(fa, fb, fc):
+        assert np.array_equal(x, y) and np.array_equal(x, z)
+    assert t0["n_particle_launches"] == 0
+    assert t2["n_particle_launches"] == 3 and t2["particle_ms"] > 0   # launches 0, 4, 8
+    assert t2["n_fdtd_launches"] == 0 and t2["n_append_launches"] == 0
+    assert t1["n_particle_launches"] == 10 and t1["n_fdtd_launches"] >= 10
+
+
 def test_collision_window_edges_match_oracle(gpu, otab):
     """Energies within 1e-3 of the tables' window ends (15.6 eV, 20 keV): the
     kernel's candidate prefilter tests the window on v^2 with host-computed,
