@@ -563,7 +563,7 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
 // are the context's kJCopies-fold accumulators
 static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool drift, bool deposit,
                                      unsigned long long* qr, unsigned long long* qz, size_t jstride = 0,
-                                     int timed_kind = kTParticle)
+                                     int timed_kind = kTParticle, const AppendScan* scan = nullptr)
 {
     PartArgs pa = make_part(ctx, s);
     pa.qr = qr; pa.qz = qz;
@@ -573,7 +573,8 @@ static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool 
                                                                              (int64_t)ctx->cu_count * 16,
                                                                              kCounterSlots)));
     const KernelEvents ke = kernel_events(ctx, timed_kind);
-    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream, ke.start, ke.stop),
+    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream, ke.start, ke.stop,
+                                scan),
            "drift/deposit");
     return GASX_OK;
 }
@@ -624,9 +625,24 @@ extern "C" gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, in
 // Appends what the last particle kernel over `src` staged.  Ions must be
 // appended before the electrons' own secondaries (the electron count fixes
 // the number of staged chunks).
+// The arguments of the electron secondaries' append scan (src: the species
+// whose particle launch staged them, appended to itself).
+static AppendScan secondary_scan(gasx_ctx* ctx, const gasx_species* src)
+{
+    AppendScan a;
+    a.src_count = src->count;
+    a.dst_count = src->count;
+    a.stage_cnt = ctx->stage_cnt;
+    a.scan_tmp = ctx->scan_tmp;
+    a.tmp2 = ctx->base_tmp;
+    a.max_segs = (int32_t)(ctx->stage_slots / kSeg);
+    return a;
+}
+
+// scanned: the step's deposit launch made the scan (secondary_scan, electrons)
 static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion,
                                      unsigned long long* qr = nullptr, unsigned long long* qz = nullptr,
-                                     size_t jstride = 0)
+                                     size_t jstride = 0, bool scanned = false)
 {
     float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
     const int32_t max_chunks = (int32_t)(ctx->stage_slots / kSeg);
@@ -636,7 +652,7 @@ static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gas
                          ion ? ctx->istage_cnt : ctx->stage_cnt, src->count, ctx->scan_tmp,
                          ctx->base_tmp + (ion ? 2 : 0), max_chunks,
                          &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz, jstride,
-                         ctx->stream),
+                         ctx->stream, scanned),
            "append");
     return GASX_OK;
 }
@@ -761,8 +777,13 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                                k == sorted_k ? sorted : nullptr);
             if (st != GASX_OK) return st;
             if (k == sorted_k) sorted_k = -1;
+            // without ions (whose append shares the scan buffers) the deposit
+            // launch also makes the secondaries' append scan
+            const bool fuse_scan = !last && k == 0 && o.collide && o.secondaries && !o.ions;
             if (!last) {
-                st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
+                const AppendScan sc = secondary_scan(ctx, &sp[k]);
+                st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift,
+                                       fuse_scan ? &sc : nullptr);
                 if (st != GASX_OK) return st;
             }
             if (k == 0 && o.collide) {
@@ -772,7 +793,7 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                 }
                 if (o.secondaries) {
                     st = append_from_stage(ctx, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
-                                           last ? nullptr : ctx->q[nxt][1], nh);
+                                           last ? nullptr : ctx->q[nxt][1], nh, fuse_scan);
                     if (st != GASX_OK) return st;
                 }
             }

@@ -216,14 +216,29 @@ int particle_occupancy(uint32_t flags);
 // ev0 / ev1 (optional) receive the kernel's own start and end times.
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-// EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk chunks
+// The first half of an append (the exclusive scan of the staging segments'
+// counts in groups, launch_append's first launch), made by extra workgroups
+// of a deposit launch instead: same inputs and outputs as that launch.
+struct AppendScan {
+    const int32_t* src_count;     // fixes the number of staged segments
+    const int32_t* dst_count;     // tmp2[0] <- *dst_count
+    const int32_t* stage_cnt;
+    int32_t* scan_tmp;            // max_segs segment offsets, then the group totals
+    int32_t* tmp2;                // [old dst count, staged segments]
+    int32_t max_segs;
+};
+// EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk
+// chunks.  scan (deposit alone, not batched): the launch also makes that scan.
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+                                int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
+                                const AppendScan* scan = nullptr);
+// scanned: the scan was made by the step's deposit launch (same arguments)
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
                          int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s);
+                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s,
+                         bool scanned = false);
 // batched append: per-run scan over the chunks recorded in chunk_start, then copy
 hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,
                                float* const stage[6], const int32_t* stage_cnt,

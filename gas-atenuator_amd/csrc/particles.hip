@@ -209,6 +209,30 @@ __device__ __forceinline__ void wave_counts_add(unsigned (&v)[kNCounters], unsig
     }
 }
 
+// A lane's event counts in one particle launch, two 16-bit counts per
+// register (seven counts live across the whole kernel otherwise take seven
+// registers).  A lane counts at most a few events per slot of a segment it
+// handles (pushes, absorptions, collision types), and the counts are flushed
+// at least every kFlushSegs segments a wave handles, so none reaches 2^16.
+constexpr int kFlushSegs = 1024;
+struct LaneCounts {
+    unsigned ei = 0;       // elastic | ionization << 16
+    unsigned xv = 0;       // excitation | bound violation << 16
+    unsigned sa = 0;       // saturated | absorbed << 16
+    unsigned pushed = 0;
+    // every lane of the wave calls it (wave_counts_add)
+    __device__ __forceinline__ void flush(unsigned long long* dst_block, unsigned long long* dst_run)
+    {
+        unsigned v[kNCounters] = {};
+        v[kCElastic] = ei & 0xffffu; v[kCIonize] = ei >> 16;
+        v[kCExcite] = xv & 0xffffu; v[kCViol] = xv >> 16;
+        v[kCSat] = sa & 0xffffu; v[kCAbsorbed] = sa >> 16;
+        v[kCPushed] = pushed;
+        wave_counts_add(v, dst_block, dst_run);
+        ei = xv = sa = pushed = 0;
+    }
+};
+
 template <int N> struct VecF;
 template <> struct VecF<1> { using T = float; };
 template <> struct VecF<2> { using T = float2; };
@@ -328,8 +352,9 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
     // reading from other arrays than those written (the permuted copies a cell
     // sort left): every slot below the count is written, dead ones included
     const bool moved = pa0.xi != pa0.x;
-    unsigned cnt[kNCounters] = {};
+    LaneCounts lc;
     int cnt_run = -1;
+    int segs_done = 0;                        // segments since the last flush of lc (wave-uniform)
 
     // ---- the wave's candidate ring (wave-uniform state) ---------------------
     float (*Q)[kQCap] = reinterpret_cast<float (*)[kQCap]>(s_q + (kCol ? wave * kQW * kQCap : 0));
@@ -366,7 +391,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 half_drift(g, x2, y2, z2, vx, vy, vz);
                 if (!wedge_bc(g, x2, y2, z2, vx, vy, vz)) {
                     z2 = __int_as_float(0x7fc00000);   // absorbed: z = NaN
-                    ++cnt[kCAbsorbed];
+                    lc.sa += 1u << 16;
                 }
                 qpa.x[slot] = x2; qpa.y[slot] = y2; qpa.z[slot] = z2;
                 qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz;
@@ -381,7 +406,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
         float sx = px, sy = py, sz = pz, svx = co.svx, svy = co.svy, svz = co.svz;
         if ((F & kDriftSec) && has_sec) {
             half_drift(g, sx, sy, sz, svx, svy, svz);
-            if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; ++cnt[kCAbsorbed]; }
+            if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; lc.sa += 1u << 16; }
         }
         // rank of each staged entry inside its segment: entries of earlier rounds
         // (carried) plus the earlier lanes of this round in the same segment
@@ -411,11 +436,9 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
         c_sec = __shfl(rs1, m - 1, 64);
         c_ion = __shfl(ri1, m - 1, 64);
         const int typ = ev & kEvMask;
-        cnt[kCElastic] += typ == 1;
-        cnt[kCIonize] += typ == 2;
-        cnt[kCExcite] += typ == 3;
-        cnt[kCViol] += (ev & kEvViol) != 0;
-        cnt[kCSat] += (ev & kEvSat) != 0;
+        lc.ei += (unsigned)(typ == 1) | (unsigned)(typ == 2) << 16;
+        lc.xv += (unsigned)(typ == 3) | (unsigned)((ev & kEvViol) != 0) << 16;
+        lc.sa += (unsigned)((ev & kEvSat) != 0);
         q_head += m;
         if (q_head >= kQCap) q_head -= kQCap;
         q_n -= m;
@@ -436,6 +459,12 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
         g_step = 1;
     }
     for (int gseg = g_begin; gseg < g_end; gseg += g_step) {
+        if (++segs_done > kFlushSegs) {           // the 16-bit counts stay below 2^16
+            if (pa0.counters && (!kBat || cnt_run >= 0))
+                lc.flush(pa0.counters + (size_t)blockIdx.x * kNCounters,
+                         kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
+            segs_done = 1;
+        }
         // seg: index within the population (slots, Philox counters);
         // gseg: index of the staging area
         PartArgs pa = pa0;
@@ -474,9 +503,10 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                     q_cs = s_cs[run];
                 }
                 if (cnt_run >= 0 && pa0.counters)
-                    wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
-                                    pa0.run_counters + (size_t)cnt_run * kNCounters);
+                    lc.flush(pa0.counters + (size_t)blockIdx.x * kNCounters,
+                             pa0.run_counters + (size_t)cnt_run * kNCounters);
                 cnt_run = run;
+                segs_done = 0;
             }
         } else {
             n = *pa0.count;
@@ -523,7 +553,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
                 if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
                     alive[p] = false;
-                    ++cnt[kCAbsorbed];
+                    lc.sa += 1u << 16;
                 }
             }
             if (kCol && p == 0) {   // the decision uniforms of the lane's four slots: one draw
@@ -543,10 +573,10 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
                 if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
                     alive[p] = false;
-                    ++cnt[kCAbsorbed];
+                    lc.sa += 1u << 16;
                 }
             }
-            if (F & kPush) cnt[kCPushed] += was_alive[p];
+            if (F & kPush) lc.pushed += was_alive[p];
             if (was_alive[p] && !alive[p]) z[p] = __int_as_float(0x7fc00000);   // absorbed: z = NaN
         }
         {
@@ -609,8 +639,8 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
     }
     if constexpr (kCol) drain();
     if (pa0.counters && (!kBat || cnt_run >= 0))
-        wave_counts_add(cnt, pa0.counters + (size_t)blockIdx.x * kNCounters,
-                        kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
+        lc.flush(pa0.counters + (size_t)blockIdx.x * kNCounters,
+                 kBat ? pa0.run_counters + (size_t)cnt_run * kNCounters : nullptr);
 }
 
 // Single run: at least four waves per SIMD.  The 128-register budget that
@@ -637,7 +667,7 @@ __global__ __launch_bounds__(kBlock) void k_particles_batch(Geom g, PartArgs pa0
 // stencil (cell-sorted slots mostly do), adds each run into the chunk's LDS
 // tile, and the tile is flushed into the workgroup's copy of the current.
 template <bool kDr, bool kDe, bool kBat>
-__device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs& pa0)
+__device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs& pa0, int grid)
 {
     constexpr int kWaves = kBlock / kWaveSize;
     __shared__ int s_wave[kWaves];
@@ -676,7 +706,7 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
                           kAlign) == 0;
     unsigned cnt[kNCounters] = {};
     int cnt_run = -1;
-    for (int chunk = blockIdx.x; chunk < total_chunks; chunk += gridDim.x) {   // workgroup-uniform
+    for (int chunk = blockIdx.x; chunk < total_chunks; chunk += grid) {   // workgroup-uniform
         PartArgs pa = pa0;
         int lchunk = chunk, n, run = 0;
         if constexpr (kBat) {
@@ -825,22 +855,72 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
 template <bool kDr, bool kDe, bool kBat>
 __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
 {
-    drift_deposit_pass<kDr, kDe, kBat>(g, pa0);
+    drift_deposit_pass<kDr, kDe, kBat>(g, pa0, (int)gridDim.x);
+}
+
+constexpr int kScanGroup = 1024;         // staging segments per group of the append's scan
+constexpr int kMaxScanGroups = 1024;     // staging of up to 2^20 segments
+
+// One group of the append's scan by a kBlock workgroup (kScanGroup / kBlock
+// segments per thread): every segment's exclusive offset within its group,
+// the group's total, and (group 0) the destination's old count and the
+// number of staged segments.
+__device__ void append_scan_group(const AppendScan& sc, int grp)
+{
+    __shared__ int s_w[kBlock / kWaveSize];
+    constexpr int kT = kScanGroup / kBlock;
+    int nseg = (*sc.src_count + kSeg - 1) / kSeg;
+    if (nseg > sc.max_segs) nseg = sc.max_segs;
+    const int c0 = grp * kScanGroup + (int)threadIdx.x * kT;
+    int v[kT], sum = 0;
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        v[j] = c0 + j < nseg ? sc.stage_cnt[c0 + j] : 0;
+        sum += v[j];
+    }
+    int tot;
+    int off = block_excl_scan(sum, s_w, tot);
+#pragma unroll
+    for (int j = 0; j < kT; ++j) {
+        if (c0 + j < nseg) sc.scan_tmp[c0 + j] = off;
+        off += v[j];
+    }
+    if (threadIdx.x == 0) {
+        sc.scan_tmp[sc.max_segs + grp] = tot;
+        if (grp == 0) {
+            sc.tmp2[0] = *sc.dst_count;
+            sc.tmp2[1] = nseg;
+        }
+    }
 }
 
 // The deposit alone (every push of the production loop): six waves per SIMD
 // (80 registers, a 384-cell tile in four copies) hide more of its LDS and
-// memory latency than four.
+// memory latency than four.  Not batched, workgroups ndep.. (if any) make
+// the append's scan (one group each) instead.
 template <bool kBat>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deposit(Geom g, PartArgs pa0)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deposit(Geom g, PartArgs pa0,
+                                                                                           AppendScan sc, int ndep)
 {
-    drift_deposit_pass<false, true, kBat>(g, pa0);
+    if (!kBat && (int)blockIdx.x >= ndep) {
+        append_scan_group(sc, (int)blockIdx.x - ndep);
+        return;
+    }
+    drift_deposit_pass<false, true, kBat>(g, pa0, ndep);
 }
 
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
+                                int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const AppendScan* scan)
 {
     blocks = blocks > 0 ? blocks : 1;
+    AppendScan sc{};
+    int nscan = 0;                                 // extra workgroups: one per scan group
+    if (scan) {
+        if (batch || drift || !deposit) return hipErrorInvalidValue;
+        nscan = (scan->max_segs + kScanGroup - 1) / kScanGroup;
+        if (nscan < 1 || nscan > kMaxScanGroups) return hipErrorInvalidValue;
+        sc = *scan;
+    }
     // the events (if any) take the kernel's own start and end times: no marker
     // packets between the launches of the step
     if (batch) {
@@ -848,13 +928,15 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
         if (drift)
             hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
         else
-            hipExtLaunchKernelGGL(k_deposit<true>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+            hipExtLaunchKernelGGL(k_deposit<true>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa, sc,
+                                  blocks);
     } else if (drift && deposit) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (deposit) {
-        hipExtLaunchKernelGGL(k_deposit<false>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
+        hipExtLaunchKernelGGL(k_deposit<false>, dim3(blocks + nscan), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa,
+                              sc, blocks);
     } else {
         return hipErrorInvalidValue;
     }
@@ -912,9 +994,8 @@ __device__ int block_scan_counts(const int32_t* cnt, int32_t* offs, int n, int* 
 // every segment its offset within a group of kScanGroup segments and every
 // group its total; k_append_copy prefixes the group totals in LDS, places
 // each staged entry at count + offset (parent slot order), deposits it, and
-// updates the count.  src_count fixes the number of staged segments.
-constexpr int kScanGroup = 1024;
-constexpr int kMaxScanGroups = 1024;     // staging of up to 2^20 segments
+// updates the count.  src_count fixes the number of staged segments.  The
+// scan can instead be made by a deposit launch (k_deposit, AppendScan).
 
 __global__ __launch_bounds__(kScanGroup) void k_append_scan(const int32_t* src_count,
                                                             const int32_t* dst_count,
@@ -1033,15 +1114,18 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
                          int32_t max_segs, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s)
+                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s,
+                         bool scanned)
 {
     const int ngroups = (max_segs + kScanGroup - 1) / kScanGroup;
     if (ngroups < 1 || ngroups > kMaxScanGroups) return hipErrorInvalidValue;
     int32_t* totals = scan_tmp + max_segs;
-    hipLaunchKernelGGL(k_append_scan, dim3(ngroups), dim3(kScanGroup), 0, s, src_count, dst_count,
-                       stage_cnt, scan_tmp, totals, tmp2, max_segs);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (!scanned) {
+        hipLaunchKernelGGL(k_append_scan, dim3(ngroups), dim3(kScanGroup), 0, s, src_count, dst_count,
+                           stage_cnt, scan_tmp, totals, tmp2, max_segs);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     // a wave per segment, at most 1024 workgroups (with a full population all
     // but workgroup 0 return at once, and launching them is most of the cost)
     const int wseg = (max_segs + kBlock / 64 - 1) / (kBlock / 64);
