@@ -548,10 +548,11 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                     gather_issue(g, pa, on ? x[q] : 0.f, on ? y[q] : 0.f, on ? z[q] : 0.f, G[q - p]);
                 }
             }
-            if ((F & kPush) && alive[p]) {
+            if (F & kPush) {
+                // without a branch: a dead slot's pushed values are never read
                 push_apply(g, pa, G[p % kGatherBatch], vx[p], vy[p], vz[p]);
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
-                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+                if (alive[p] && !wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
                     alive[p] = false;
                     lc.sa += 1u << 16;
                 }
