@@ -570,9 +570,12 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 const float v2 = vx[p] * vx[p] + vy[p] * vy[p] + vz[p] * vz[p];
                 cand[p] = u1[p] < ca.pmax && v2 >= ca.v2_lo && v2 <= ca.v2_hi;
             }
-            if ((F & kDrift2) && alive[p] && !cand[p]) {   // a candidate's is done by its round
-                half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
-                if (!wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
+            if (F & kDrift2) {   // a candidate's is done by its round: selected, not branched
+                const bool d2 = alive[p] && !cand[p];
+                float x2 = x[p], y2 = y[p], z2 = z[p];
+                half_drift(g, x2, y2, z2, vx[p], vy[p], vz[p]);
+                x[p] = d2 ? x2 : x[p]; y[p] = d2 ? y2 : y[p]; z[p] = d2 ? z2 : z[p];
+                if (d2 && !wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
                     alive[p] = false;
                     lc.sa += 1u << 16;
                 }
