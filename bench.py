@@ -314,8 +314,8 @@ def main():
     ctx.counters(reset=True)
     torch.cuda.synchronize()
     # timed region: every TIME_EVERY-th k_particles launch (the roofline
-    # kernel) timed by its own dispatch; a timed launch adds ~1.4 us of idle
-    # time around it (0.7% of a step), and the average over every launch reads
+    # kernel) timed by its own dispatch; a timed launch adds 0.3-1.4 us of idle
+    # time around it (under 0.7% of a step), and the average over every launch reads
     # closer to a kernel trace than a 1-in-8 sample did
     t0 = time.perf_counter()
     ctx.run(e, f, gas, args.steps, time_kernels=2, time_every=TIME_EVERY, **run_kw)
