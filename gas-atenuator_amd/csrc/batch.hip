@@ -35,6 +35,7 @@ void free_run_buffers(BatchState& b)
     for (auto& qq : b.q) { dfree(qq[0]); dfree(qq[1]); }
     dfree(b.runs); dfree(b.fdtd_runs); dfree(b.chunk_start); dfree(b.tmp_run); dfree(b.run_counters);
     b.nruns_cap = 0;
+    b.uploaded_key.clear();
 }
 
 void free_stage_buffers(BatchState& b)
@@ -148,6 +149,26 @@ bool tile_fits(const gasx_ctx* ctx, int ti, int tk)
     return fdtd_lds_bytes(a) <= 160 * 1024 && (W + 1) * (Hh + 1) <= 1024 * 8;
 }
 
+// Everything upload() writes into the device descriptors depends on.
+std::vector<unsigned char> upload_key(const gasx_batch* b, const gasx_run_opts* o)
+{
+    std::vector<unsigned char> k;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        k.insert(k.end(), c, c + n);
+    };
+    put(&b->nruns, sizeof(b->nruns));
+    put(&b->er, sizeof(b->er));
+    put(&b->ez, sizeof(b->ez));
+    put(&b->hphi, sizeof(b->hphi));
+    put(b->base, b->nruns * sizeof(int64_t));
+    put(b->capacity, b->nruns * sizeof(int64_t));
+    put(&o->collide, sizeof(o->collide));
+    if (o->collide) put(b->n_max, b->nruns * sizeof(double));
+    put(&o->run_id, sizeof(o->run_id));
+    return k;
+}
+
 // Uploads the run descriptors and the field-kernel run tables (outside any
 // capture: they come from pageable host memory).
 gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
@@ -207,6 +228,7 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
                           ctx->stream),
            "upload field runs");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    st.uploaded_key = upload_key(b, o);
     return GASX_OK;
 }
 
@@ -380,6 +402,11 @@ gasx_status run_batch(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o, cons
             return hip_fail(ctx, e, "graph instantiate");
         }
         st.graph_key = std::move(key);
+    } else if (st.uploaded_key != upload_key(b, o)) {
+        // an eager run or an autotune trial since the capture left other
+        // descriptors in the buffers the graph reads
+        s = upload(ctx, b, o);
+        if (s != GASX_OK) return s;
     }
     TimedScope ts(ctx, kTGraph);
     HIPCHK(hipGraphLaunch(st.exec, ctx->stream), "graph launch");

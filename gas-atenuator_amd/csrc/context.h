@@ -27,6 +27,10 @@ struct BatchState {
     // captured step sequence (use_graph), replayed while the key matches
     hipGraphExec_t exec = nullptr;
     std::vector<unsigned char> graph_key;
+    // what the device descriptors (runs, fdtd_runs) were last uploaded for:
+    // an eager run or an autotune trial in between re-uploads other contents,
+    // and a replay then uploads its own again first
+    std::vector<unsigned char> uploaded_key;
     hipStream_t capture_stream = nullptr;       // private stream the sequence is captured on
 };
 

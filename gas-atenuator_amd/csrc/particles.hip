@@ -549,7 +549,8 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 }
             }
             if (F & kPush) {
-                // without a branch: a dead slot's pushed values are never read
+                // without a branch: a dead slot is pushed too (its z stays NaN, and
+                // nothing reads its other values)
                 push_apply(g, pa, G[p % kGatherBatch], vx[p], vy[p], vz[p]);
                 half_drift(g, x[p], y[p], z[p], vx[p], vy[p], vz[p]);
                 if (alive[p] && !wedge_bc(g, x[p], y[p], z[p], vx[p], vy[p], vz[p])) {
@@ -585,7 +586,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
         }
         {
             // ---- store (a later round rewrites the candidates) ----------------------
-            if (vec) {   // dead slots are written back unchanged
+            if (vec) {   // a dead slot keeps z = NaN; its other values are the unbranched push's
                 vstore(pa.x + slot0, x); vstore(pa.y + slot0, y); vstore(pa.z + slot0, z);
                 vstore(pa.vx + slot0, vx); vstore(pa.vy + slot0, vy); vstore(pa.vz + slot0, vz);
             } else {

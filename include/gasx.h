@@ -19,7 +19,9 @@
  *
  * Particles are FP32 structure-of-arrays (x, y, z, vx, vy, vz) in the wedge's
  * Cartesian frame.  A slot whose z is NaN holds an absorbed particle (the
- * reference's parDelete = 1, EM2.cu:31-33); gasx_sort removes them.
+ * reference's parDelete = 1, EM2.cu:31-33); gasx_sort removes them.  The
+ * other five values of an absorbed slot are unspecified: the push kernels
+ * run on every slot below the count without a branch and may overwrite them.
  * Field state is FP64 (the reference's double arrays); the particle push
  * gathers from FP32 copies the library refreshes after every field cycle.
  */

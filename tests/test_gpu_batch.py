@@ -106,6 +106,44 @@ def test_batch_graph_replays_match_eager(gpu):
     assert ctx2.timing()["n_graph_launches"] == 0        # timing was not requested
 
 
+def test_graph_replay_after_eager_run_uses_its_own_descriptors(gpu):
+    """A captured graph, then an eager run of another description (other run
+    ids and density bounds) on the same context, then a replay of the first:
+    the replay must compute what an all-eager sequence computes, bit for bit
+    (the eager run overwrites the device descriptors the graph reads)."""
+    import torch
+    import gasx
+    log("batch graph -> eager (other run ids, n_max) -> graph")
+    params = gasx.make_params(NR, NZ, fn=2e3)
+    sizes, dens, parts, caps, _ = ensemble_case(3)
+    dens_b = [d * 0.5 for d in dens]                      # other n_max per run
+    kw_a = dict(collide=True, secondaries=True, seed=5, run_id=10)
+    kw_b = dict(collide=True, secondaries=True, seed=6, run_id=500)
+
+    def sequence(use_graph):
+        ctx = gasx.Context(params)
+        a = gasx.Ensemble(ctx, caps, dens, 4e-5, 4e-5)
+        b = gasx.Ensemble(ctx, caps, dens_b, 4e-5, 4e-5)
+        for r, p in enumerate(parts):
+            if p.shape[1]:
+                a.load_run(r, *p)
+                b.load_run(r, *p)
+        a.run(3, use_graph=use_graph, **kw_a)             # capture (or eager)
+        b.run(3, use_graph=False, **kw_b)                 # eager: uploads b's descriptors
+        a.run(3, use_graph=use_graph, **kw_a)             # replay of a's graph
+        torch.cuda.synchronize()
+        return ([a.run_state(r) for r in range(len(sizes))],
+                [a.er[r].cpu().numpy() for r in range(len(sizes))],
+                [b.run_state(r) for r in range(len(sizes))])
+
+    ref_a, ref_er, ref_b = sequence(False)
+    got_a, got_er, got_b = sequence(True)
+    for r in range(len(sizes)):
+        assert np.array_equal(got_a[r], ref_a[r], equal_nan=True), r
+        assert np.array_equal(got_er[r], ref_er[r]), r
+        assert np.array_equal(got_b[r], ref_b[r], equal_nan=True), r
+
+
 def test_autotune_restores_state_and_picks_a_valid_shape(gpu):
     import gasx
     log("batch autotune")
