@@ -1,0 +1,79 @@
+"""Secondary append at production-like capacities (GPU).
+
+The append scan works in groups of 1024 staging segments (2^18 slots); the
+tests here use populations above 3 * 2^18 slots, so its multi-group path
+(group totals prefixed across groups) runs:
+
+* gasx_collide: every secondary is placed at its parent's position with the
+  velocity the collision gave it, and the appended block must list the
+  parents in increasing slot order (electron_collision.m:68-75 appends the
+  secondaries of the logical mask in index order);
+* gasx_run: the scan fused into the deposit launch (no ions) and the
+  two-launch scan (with ions, whose append shares the scan buffers) must
+  give the same electrons and fields bit for bit.
+"""
+import numpy as np
+import pytest
+
+from conftest import log
+from test_gpu_parity import wedge_particles
+
+pytestmark = pytest.mark.gpu
+
+NR, NZ = 32, 64
+
+
+def test_collide_appends_in_parent_slot_order_across_scan_groups(gpu):
+    import gasx
+    n, cap = 900_000, 1_200_000                            # 3516 staging segments: 4 groups
+    log(f"append order n={n} cap={cap}")
+    params = gasx.make_params(NR, NZ, fn=1e3)
+    ctx = gasx.Context(params)
+    part = wedge_particles(n, NR, NZ, 1e-5, 1e-5, 0.1, 77, vmax=2.5e7, edge_frac=0.0)
+    e = gasx.ParticleSet(cap).load(*part)
+    gas = gasx.GasGrid.uniform(1e26, NR * 1e-5, NZ * 1e-5)
+    ctx.counters(reset=True)
+    ctx.collide(e, gas, 1e-14, seed=5, step=2, run=0)
+    c = ctx.counters()
+    st = e.state()
+    n_sec = st.shape[1] - n
+    assert n_sec == c.n_ionize - c.n_dropped_sec and n_sec > 20_000
+    # collisions change velocities only: a parent keeps its position
+    assert np.array_equal(st[:3, :n], part[:3])
+    key = {tuple(v): i for i, v in enumerate(part[:3].T.view(np.uint32).reshape(n, 3).tolist())}
+    sec_pos = st[:3, n:].T.copy().view(np.uint32).reshape(n_sec, 3)
+    parents = np.array([key[tuple(v)] for v in sec_pos.tolist()])
+    assert np.all(np.diff(parents) > 0)                   # strictly increasing parent slots
+    # every parent's speed dropped by at least the ionization energy (15.6 eV)
+    v0 = np.linalg.norm(part[3:, parents].astype(np.float64), axis=0)
+    v1 = np.linalg.norm(st[3:, parents].astype(np.float64), axis=0)
+    assert np.all(v1 < v0)
+    # and staged segments beyond the first scan group hold parents
+    assert parents.max() > 3 * 1024 * 256
+
+
+def test_fused_and_two_launch_append_scans_agree(gpu):
+    import gasx
+    from gasx.core import M_ION
+    n, cap = 800_000, 1_000_000                           # 3907 segments: 4 scan groups
+    log(f"fused vs two-launch append scan n={n} cap={cap}")
+    params = gasx.make_params(NR, NZ, fn=1e2)
+    part = wedge_particles(n, NR, NZ, 1e-5, 1e-5, 0.1, 78, vmax=2.5e7, edge_frac=0.05)
+    gas_d = np.full((1, 1), 1e26, np.float32)
+    out = []
+    for with_ions in (False, True):
+        ctx = gasx.Context(params)
+        e = gasx.ParticleSet(cap).load(*part)
+        f = gasx.FieldState(NR, NZ)
+        gas = gasx.GasGrid(gas_d, NR * 1e-5, NZ * 1e-5)
+        ions = gasx.ParticleSet(cap, qnum=1.0, mass=M_ION) if with_ions else None
+        ctx.run(e, f, gas, 4, collide=True, secondaries=True, sort_every=0, seed=8, ions=ions)
+        c = ctx.counters()
+        out.append((e.state(), f.numpy(), c, ions.n if ions is not None else None))
+    (s0, f0, c0, _), (s1, f1, c1, n_ions) = out
+    assert c0.n_ionize == c1.n_ionize > 20_000
+    assert n_ions == c1.n_ionize - c1.n_dropped_ion
+    assert s0.shape == s1.shape and s0.shape[1] > n
+    assert np.array_equal(s0, s1, equal_nan=True)
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
