@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -13,6 +14,53 @@
 
 using namespace gasx;
 using namespace gasx_host;
+
+// ---------------------------------------------------------------------------
+// Fixed staging for gasx_collide_noalloc: one array per GPU in the code
+// object, held until the process ends.  Layout in 32-bit words: six FP32
+// secondary staging arrays and six ion staging arrays of kNaSlots each, the
+// per-segment secondary and ion counts, the append scan (segment offsets,
+// group totals) and the four append bases.  Every word a call reads is
+// written by that call first (each segment below the slot count gets its
+// count, the scan writes its offsets, totals and bases), so nothing needs
+// clearing between calls.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNaSlots = GASX_NOALLOC_MAX_SLOTS;
+constexpr int64_t kNaSegs = kNaSlots / kSeg;
+static_assert(kNaSlots % kSeg == 0, "whole staging segments");
+constexpr int64_t kNaScanWords = ((kNaSegs + (kNaSegs + 1023) / 1024 + 1 + 63) / 64) * 64;
+constexpr int64_t kNaStageCnt = 12 * kNaSlots;            // word offsets
+constexpr int64_t kNaIonCnt = kNaStageCnt + kNaSegs;
+constexpr int64_t kNaScan = kNaIonCnt + kNaSegs;
+constexpr int64_t kNaBase = kNaScan + kNaScanWords;
+constexpr int64_t kNaWords = kNaBase + 64;
+constexpr int kNaMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint32_t scratch[kNaWords];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kNaMaxDevices] = {};   // this device's address of `scratch`
+
+// With the context's device current: look up (once per device) where this
+// device holds `scratch`.  Host code uses only the returned address.
+gasx_status bind_scratch(gasx_ctx* ctx)
+{
+    if (ctx->device >= kNaMaxDevices) return GASX_OK;      // gasx_collide_noalloc then refuses
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(ctx, GASX_E_HIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    ctx->scratch = g_scratch_ptr[ctx->device];
+    return GASX_OK;
+}
+}  // namespace
 
 size_t gasx_host::next_event(gasx_ctx* c)
 {
@@ -294,6 +342,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         if (e != hipSuccess && st == GASX_OK) st = hip_fail(ctx, e, w);
     };
     chk(hipSetDevice(device), "hipSetDevice");
+    if (st == GASX_OK) st = bind_scratch(ctx);
     if (st == GASX_OK) chk(fdtd_prepare(), "field kernel LDS limit");
     if (st == GASX_OK) {
         hipDeviceProp_t prop;
@@ -412,6 +461,30 @@ static gasx_status ensure_staging(gasx_ctx* ctx, int64_t cap, bool ions)
         ctx->stage_slots = ns;
     }
     return GASX_OK;
+}
+
+// Where a collision launch stages its secondaries and ions and where their
+// append keeps its scan: the context's buffers, or the fixed array.
+struct Staging {
+    float* sec[6];
+    int32_t* sec_cnt;
+    float* ion[6];
+    int32_t* ion_cnt;
+    int32_t* scan_tmp;    // max_segs segment offsets, then the group totals
+    int32_t* base_tmp;    // [0..1] secondary append, [2..3] ion append
+    int32_t max_segs;
+};
+
+static Staging ctx_staging(const gasx_ctx* ctx)
+{
+    Staging t{};
+    for (int c = 0; c < 6; ++c) { t.sec[c] = ctx->stage[c]; t.ion[c] = ctx->istage[c]; }
+    t.sec_cnt = ctx->stage_cnt;
+    t.ion_cnt = ctx->istage_cnt;
+    t.scan_tmp = ctx->scan_tmp;
+    t.base_tmp = ctx->base_tmp;
+    t.max_segs = (int32_t)(ctx->stage_slots / kSeg);
+    return t;
 }
 
 static PartArgs make_part(gasx_ctx* ctx, const gasx_species* s)
@@ -546,9 +619,15 @@ extern "C" gasx_status gasx_fields_sync(gasx_ctx* ctx, const gasx_fields* f)
 // jstride > 0: qr, qz are the context's kJCopies-fold accumulators
 static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_species* s,
                                  unsigned long long* qr, unsigned long long* qz, const CollArgs& ca,
-                                 size_t jstride = 0, int timed_kind = kTParticle, float* const* in = nullptr)
+                                 size_t jstride = 0, int timed_kind = kTParticle, float* const* in = nullptr,
+                                 const Staging* stg = nullptr)
 {
     PartArgs pa = make_part(ctx, s);
+    if (stg) {
+        for (int c = 0; c < 6; ++c) { pa.sec[c] = stg->sec[c]; pa.ion[c] = stg->ion[c]; }
+        pa.sec_cnt = stg->sec_cnt;
+        pa.ion_cnt = stg->ion_cnt;
+    }
     if (in) { pa.xi = in[0]; pa.yi = in[1]; pa.zi = in[2]; pa.vxi = in[3]; pa.vyi = in[4]; pa.vzi = in[5]; }
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
@@ -640,21 +719,36 @@ static AppendScan secondary_scan(gasx_ctx* ctx, const gasx_species* src)
 }
 
 // scanned: the step's deposit launch made the scan (secondary_scan, electrons)
-static gasx_status append_from_stage(gasx_ctx* ctx, const gasx_species* src, gasx_species* dst, bool ion,
-                                     unsigned long long* qr = nullptr, unsigned long long* qz = nullptr,
-                                     size_t jstride = 0, bool scanned = false)
+static gasx_status append_from_stage(gasx_ctx* ctx, const Staging& stg, const gasx_species* src,
+                                     gasx_species* dst, bool ion, unsigned long long* qr = nullptr,
+                                     unsigned long long* qz = nullptr, size_t jstride = 0,
+                                     bool scanned = false)
 {
     float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
-    const int32_t max_chunks = (int32_t)(ctx->stage_slots / kSeg);
     const float kq = (float)(4294967296.0 / ctx->p.c) * dst->qnum;
     TimedScope ts(ctx, kTAppend);
-    HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? ctx->istage : ctx->stage,
-                         ion ? ctx->istage_cnt : ctx->stage_cnt, src->count, ctx->scan_tmp,
-                         ctx->base_tmp + (ion ? 2 : 0), max_chunks,
+    HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? stg.ion : stg.sec,
+                         ion ? stg.ion_cnt : stg.sec_cnt, src->count, stg.scan_tmp,
+                         stg.base_tmp + (ion ? 2 : 0), stg.max_segs,
                          &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz, jstride,
                          ctx->stream, scanned),
            "append");
     return GASX_OK;
+}
+
+// One collision check of every electron of e, staging into stg, then the
+// appends (ions first: they read the same per-segment ranks as the electrons).
+static gasx_status collide_staged(gasx_ctx* ctx, const Staging& stg, gasx_species* e, gasx_species* ions,
+                                  const CollArgs& ca)
+{
+    const uint32_t flags = kCollide | kStageSec | (ions ? kStageIon : 0u);
+    gasx_status st = run_particles(ctx, flags, e, nullptr, nullptr, ca, 0, kTParticle, nullptr, &stg);
+    if (st != GASX_OK) return st;
+    if (ions) {
+        st = append_from_stage(ctx, stg, e, ions, true);
+        if (st != GASX_OK) return st;
+    }
+    return append_from_stage(ctx, stg, e, e, false);
 }
 
 extern "C" gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species* ions,
@@ -668,14 +762,35 @@ extern "C" gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species
     CollArgs ca;
     st = make_coll(ctx, g, dt_coll, seed, step, run, &ca);
     if (st != GASX_OK) return st;
-    uint32_t flags = kCollide | kStageSec | (ions ? kStageIon : 0u);
-    st = run_particles(ctx, flags, e, nullptr, nullptr, ca);
+    const Staging stg = ctx_staging(ctx);
+    return collide_staged(ctx, stg, e, ions, ca);
+}
+
+extern "C" gasx_status gasx_collide_noalloc(gasx_ctx* ctx, gasx_species* e, gasx_species* ions,
+                                            const gasx_gas* g, double dt_coll, uint64_t seed,
+                                            uint32_t step, uint32_t run)
+{
+    if (!ctx || !valid_species(e) || !g || (ions && !valid_species(ions))) return GASX_E_ARG;
+    if (e->capacity > kNaSlots)
+        return fail(ctx, GASX_E_ARG, "gasx_collide_noalloc: electron capacity " + std::to_string(e->capacity) +
+                                         " exceeds GASX_NOALLOC_MAX_SLOTS (" + std::to_string(kNaSlots) + ")");
+    if (!ctx->scratch) return fail(ctx, GASX_E_STATE, "gasx_collide_noalloc: no fixed staging on this device");
+    HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
+    CollArgs ca;
+    gasx_status st = make_coll(ctx, g, dt_coll, seed, step, run, &ca);
     if (st != GASX_OK) return st;
-    if (ions) {   // ions first: they read the same per-chunk ranks as the electrons
-        st = append_from_stage(ctx, e, ions, true);
-        if (st != GASX_OK) return st;
+    uint32_t* w = static_cast<uint32_t*>(ctx->scratch);   // this device's address of the array
+    Staging stg{};
+    for (int c = 0; c < 6; ++c) {
+        stg.sec[c] = reinterpret_cast<float*>(w + (size_t)c * kNaSlots);
+        stg.ion[c] = reinterpret_cast<float*>(w + (size_t)(6 + c) * kNaSlots);
     }
-    return append_from_stage(ctx, e, e, false);
+    stg.sec_cnt = reinterpret_cast<int32_t*>(w + kNaStageCnt);
+    stg.ion_cnt = reinterpret_cast<int32_t*>(w + kNaIonCnt);
+    stg.scan_tmp = reinterpret_cast<int32_t*>(w + kNaScan);
+    stg.base_tmp = reinterpret_cast<int32_t*>(w + kNaBase);
+    stg.max_segs = (int32_t)((e->capacity + kSeg - 1) / kSeg);   // <= kNaSegs
+    return collide_staged(ctx, stg, e, ions, ca);
 }
 
 extern "C" gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_coll, double* pmax)
@@ -787,12 +902,13 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                 if (st != GASX_OK) return st;
             }
             if (k == 0 && o.collide) {
+                const Staging stg = ctx_staging(ctx);
                 if (o.ions) {
-                    st = append_from_stage(ctx, &sp[0], ions, true);
+                    st = append_from_stage(ctx, stg, &sp[0], ions, true);
                     if (st != GASX_OK) return st;
                 }
                 if (o.secondaries) {
-                    st = append_from_stage(ctx, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
+                    st = append_from_stage(ctx, stg, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
                                            last ? nullptr : ctx->q[nxt][1], nh, fuse_scan);
                     if (st != GASX_OK) return st;
                 }

@@ -60,6 +60,9 @@ struct gasx_ctx {
     int32_t* istage_cnt = nullptr;
     int32_t* scan_tmp = nullptr;
     int32_t* base_tmp = nullptr;
+    // this device's copy of the library's fixed staging array (gasx_collide_noalloc),
+    // from hipGetSymbolAddress at context creation; null if unavailable
+    void* scratch = nullptr;
     // sort scratch
     void* sort_tmp = nullptr;
     size_t sort_bytes = 0;

@@ -147,6 +147,8 @@ PROTOTYPES = {
     "gasx_cross_sections": (C.c_int, [_P, _P, C.c_int64, _P, _P, _P]),
     "gasx_collide": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Gas),
                                C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "gasx_collide_noalloc": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Gas),
+                                       C.c_double, C.c_uint64, C.c_uint32, C.c_uint32]),
     "gasx_run": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Fields),
                            C.POINTER(Gas), C.POINTER(RunOpts)]),
     "gasx_sort": (C.c_int, [_P, C.POINTER(Species)]),
@@ -171,7 +173,7 @@ PROTOTYPES = {
     "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 _lib = None
 

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 4
+#define GASX_ABI_VERSION 5
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -197,6 +197,19 @@ gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, int64_t n,
  * come from Philox4x32-10 with key = seed and counter = (slot, step, run, block). */
 gasx_status gasx_collide(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, const gasx_gas* g,
                          double dt_coll, uint64_t seed, uint32_t step, uint32_t run);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gasx_collide with the same arguments and results, whose
+ * staging of secondaries and ions and whose append scan live in a fixed
+ * device array of the library (one per GPU, 48 MiB, held until the process
+ * ends) instead of buffers the context allocates on first use; the call
+ * allocates nothing.  e->capacity must not exceed GASX_NOALLOC_MAX_SLOTS
+ * (GASX_E_ARG otherwise).  All calls on one GPU share that array: calls
+ * that overlap on one GPU (two streams, two contexts) race. */
+#define GASX_NOALLOC_MAX_SLOTS (1 << 20)
+gasx_status gasx_collide_noalloc(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, const gasx_gas* g,
+                                 double dt_coll, uint64_t seed, uint32_t step, uint32_t run);
 
 /* ---- production loop --------------------------------------------------------
  * electron_transport_field3.m:61-116 with one push per collision step, fused:
