@@ -421,3 +421,295 @@ def test_collision_branch_frequencies(otab):
     g1 = (400.0 - edep) * 1.6022e-19 / (9.109e-31 * 9e16)
     vexp = 3e8 * math.sqrt(g1 * (g1 + 2)) / (1 + g1)
     assert np.allclose(np.linalg.norm(st.v[3:6, ex], axis=0), vexp, rtol=1e-12)
+
+
+# ---- EM4 gathers: fields linear on each staggered grid ---------------------------
+# Bilinear interpolation reproduces a field that is linear in r and z exactly at
+# interior points; the EM4 edge rules (EM4.cu:48-53, 68-73, 89-116) hold the
+# value of the nearest node row / column outside the node range, i.e. the
+# linear field evaluated at the position clamped into the node range.  A
+# half-cell offset error misses by (slope * cell / 2); a transposed index or a
+# wrong edge rule misses by more.
+Q_E, M_E = -1.6e-19, 9.1096e-31
+
+
+def _gather_case(nr=16, nz=20, n=3000, seed=4):
+    p = params(nr, nz)
+    rng = np.random.default_rng(seed)
+    r0 = rng.random(n) * nr * p.dr * 0.999999
+    z0 = rng.random(n) * nz * p.dz
+    # force a share of the points into the edge bands of every grid
+    k = n // 6
+    r0[:k] = rng.random(k) * 0.5 * p.dr                         # r < dr/2
+    r0[k:2 * k] = nr * p.dr - rng.random(k) * 0.5 * p.dr * 0.999  # r > Rp - dr/2
+    z0[2 * k:3 * k] = rng.random(k) * 0.5 * p.dz                # z < dz/2
+    z0[3 * k:4 * k] = nz * p.dz - rng.random(k) * 0.5 * p.dz     # z > Lp - dz/2
+    phi = p.phip * (0.25 + 0.5 * rng.random(n))
+    return p, r0, z0, phi
+
+
+def _kick(p, r0, z0, phi, v, er, ez, h, vr):
+    """One EM4 push of particles at (r0, phi, z0) with velocity v (3 x n);
+    returns the momentum change per unit mass u' - u (3 x n)."""
+    n = r0.size
+    st = oracle.State(np.vstack([r0 * np.cos(phi), r0 * np.sin(phi), z0, v]))
+    c = 3e8
+    u0 = v / np.sqrt(1 - np.sum(v ** 2, axis=0) / c ** 2)
+    oracle.em4(p, st, r0.copy(), phi.copy(), vr.copy(), er, ez, h)
+    v1 = st.v[3:6, :n]
+    u1 = v1 / np.sqrt(1 - np.sum(v1 ** 2, axis=0) / c ** 2)
+    return u1 - u0
+
+
+@pytest.mark.parametrize("which", ["er", "ez"])
+def test_em4_gather_reproduces_linear_electric_fields(which):
+    p, r0, z0, phi = _gather_case()
+    nr, nz, dr, dz = p.nr, p.nz, p.dr, p.dz
+    a0, gr, gz = 700.0, 3.1e8, -1.7e8                            # V/m, V/m^2
+    f = lambda r, z: a0 + gr * r + gz * z
+    er = np.zeros((nz + 1, nr))
+    ez = np.zeros((nz, nr + 1))
+    if which == "er":        # Er nodes at r = (i + 1/2) dr, z = k dz (EM3.cu:18)
+        ri, zk = (np.arange(nr) + 0.5) * dr, np.arange(nz + 1) * dz
+        er[:] = f(ri[None, :], zk[:, None])
+        expect = f(np.clip(r0, 0.5 * dr, (nr - 0.5) * dr), z0)
+    else:                    # Ez nodes at r = i dr, z = (k + 1/2) dz (EM3.cu:19)
+        ri, zk = np.arange(nr + 1) * dr, (np.arange(nz) + 0.5) * dz
+        ez[:] = f(ri[None, :], zk[:, None])
+        expect = f(r0, np.clip(z0, 0.5 * dz, (nz - 0.5) * dz))
+    h = np.zeros((nz, nr))
+    du = _kick(p, r0, z0, phi, np.zeros((3, r0.size)), er.ravel(), ez.ravel(), h.ravel(), np.zeros(r0.size))
+    k = Q_E * p.ddt / M_E
+    got_r = (du[0] * np.cos(phi) + du[1] * np.sin(phi)) / k
+    got_z = du[2] / k
+    got, other = (got_r, got_z) if which == "er" else (got_z, got_r)
+    assert np.allclose(got, expect, rtol=1e-9, atol=1e-9 * a0)
+    assert np.allclose(other, 0.0, atol=1e-9 * a0)
+    # a half-cell offset would be visible: slope * cell / 2 >> tolerance
+    assert min(abs(gr) * dr, abs(gz) * dz) / 2 > 1e3 * 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("vel", ["axial", "radial"])
+def test_em4_gather_and_magnetic_force_of_linear_hphi(vel):
+    """Hphi linear on its grid (r = (i+1/2) dr, z = (k+1/2) dz, EM3_1.cu:12):
+    with E = 0 the kick is (q dt / m) mu0 H (vr z^ - vz r^) (EM4.cu:126-129),
+    H at the clamped position."""
+    p, r0, z0, phi = _gather_case(seed=5)
+    nr, nz, dr, dz = p.nr, p.nz, p.dr, p.dz
+    z0 = np.clip(z0, 1e-7, nz * dz - 1e-7)                        # no z reflection in the drift
+    h0, hr, hz = 2.0, 4e4, 6e4                                     # A/m, A/m^2
+    f = lambda r, z: h0 + hr * r + hz * z
+    ri, zk = (np.arange(nr) + 0.5) * dr, (np.arange(nz) + 0.5) * dz
+    h = f(ri[None, :], zk[:, None])
+    expect = f(np.clip(r0, 0.5 * dr, (nr - 0.5) * dr), np.clip(z0, 0.5 * dz, (nz - 0.5) * dz))
+    n, v0, mu0 = r0.size, 1e6, 4e-7 * math.pi
+    if vel == "axial":        # vr = 0, vz = v0: kick along -r^
+        v = np.vstack([np.zeros(n), np.zeros(n), np.full(n, v0)])
+        vr = np.zeros(n)
+    else:                     # vr = v0, vz = 0: kick along +z^
+        v = np.vstack([v0 * np.cos(phi), v0 * np.sin(phi), np.zeros(n)])
+        vr = np.full(n, v0)
+    z = np.zeros((nz + 1) * nr), np.zeros(nz * (nr + 1))
+    du = _kick(p, r0, z0, phi, v, z[0], z[1], h.ravel(), vr)
+    k = Q_E * p.ddt / M_E * mu0 * v0
+    du_r = du[0] * np.cos(phi) + du[1] * np.sin(phi)
+    if vel == "axial":
+        assert np.allclose(-du_r / k, expect, rtol=1e-8, atol=0)
+        assert np.allclose(du[2], 0.0, atol=1e-12 * abs(k) * h0)
+    else:
+        assert np.allclose(du[2] / k, expect, rtol=1e-8, atol=0)
+        assert np.allclose(du_r, 0.0, atol=1e-12 * abs(k) * h0)
+
+
+def test_em4_uniform_hphi_rotates_momentum_in_the_rz_plane():
+    """Uniform Hphi, E = 0: du = (q dt/m) mu0 H (vr z^ - vz r^), which is
+    perpendicular to u, so |u'|^2 = |u|^2 + (q dt mu0 H / m)^2 (vr^2 + vz^2)."""
+    p, r0, z0, phi = _gather_case(n=500, seed=6)
+    # away from the wedge faces and z ends: the drift after the kick must not
+    # wrap (which rotates the velocity by PHIp) or reflect
+    r0 = np.maximum(r0, 2 * p.dr)
+    z0 = np.clip(z0, 1e-6, p.nz * p.dz - 1e-6)
+    rng = np.random.default_rng(7)
+    n, H, mu0 = r0.size, 30.0, 4e-7 * math.pi
+    v = rng.standard_normal((3, n)) * 2e7
+    vr = v[0] * np.cos(phi) + v[1] * np.sin(phi)
+    er, ez, h = np.zeros((p.nz + 1) * p.nr), np.zeros(p.nz * (p.nr + 1)), np.full(p.nz * p.nr, H)
+    st = oracle.State(np.vstack([r0 * np.cos(phi), r0 * np.sin(phi), z0, v]))
+    c = 3e8
+    u0 = v / np.sqrt(1 - np.sum(v ** 2, axis=0) / c ** 2)
+    oracle.em4(p, st, r0.copy(), phi.copy(), vr.copy(), er, ez, h)
+    v1 = st.v[3:6, :n]
+    u1 = v1 / np.sqrt(1 - np.sum(v1 ** 2, axis=0) / c ** 2)
+    k = Q_E * p.ddt / M_E * mu0 * H
+    du = u1 - u0
+    rhat = np.vstack([np.cos(phi), np.sin(phi), np.zeros(n)])
+    expect = k * (vr * np.array([[0], [0], [1.0]]) - v[2] * rhat)
+    assert np.allclose(du, expect, rtol=1e-8, atol=1e-8 * np.abs(expect).max())
+    lhs = np.sum(u1 ** 2, axis=0)
+    rhs = np.sum(u0 ** 2, axis=0) + k ** 2 * (vr ** 2 + v[2] ** 2)
+    assert np.allclose(lhs, rhs, rtol=1e-12, atol=0)
+
+
+# ---- FDTD current coupling (EM3.cu:29, 35, 39) ----------------------------------------
+def _e_update(p, er, ez, h, jr, jz, tau, axis_sign=1):
+    er, ez = er.copy().ravel(), ez.copy().ravel()
+    oracle.lib().orc_fdtd_e(oracle.ptr(er), oracle.ptr(ez), oracle.ptr(oracle.a(h)), oracle.ptr(oracle.a(jr)),
+                            oracle.ptr(oracle.a(jz)), p.nr, p.nz, p.eps0, p.dr, p.dz, tau, p.ez_metric,
+                            axis_sign)
+    return er.reshape(p.nz + 1, p.nr), ez.reshape(p.nz, p.nr + 1)
+
+
+@pytest.mark.parametrize("axis_sign", [1, -1])
+def test_fdtd_uniform_current_follows_ampere(axis_sign):
+    """With H = 0 Ampere-Maxwell reads dE/dt = -J/eps0: a uniform J lowers every
+    E node the update owns by tau J / eps0 (interior Er rows, Ez columns 1..nr-1).
+    On the axis the reference adds +tau Jz / eps0 (EM3.cu:35, divergence D2):
+    the axis node follows Ampere only with axis_j_sign = -1."""
+    nr, nz = 12, 10
+    p = params(nr, nz)
+    rng = np.random.default_rng(8)
+    er0 = rng.standard_normal((nz + 1, nr)) * 1e3
+    ez0 = rng.standard_normal((nz, nr + 1)) * 1e3
+    h = np.zeros(nz * nr)
+    J, tau = 3e7, 1e-15
+    er, ez = _e_update(p, er0, ez0, h, np.full(nr * nz, J), np.full(nr * nz, -2 * J), tau, axis_sign)
+    d = tau * J / p.eps0
+    assert np.allclose(er[1:nz] - er0[1:nz], -d, rtol=1e-9, atol=0)
+    assert np.array_equal(er[0], er0[1]) and np.array_equal(er[nz], er0[nz - 1])   # EM3.cu:22,26
+    assert np.allclose(ez[:, 1:nr] - ez0[:, 1:nr], 2 * d, rtol=1e-9, atol=0)
+    assert np.array_equal(ez[:, nr], ez0[:, nr])                                     # PEC wall
+    ampere_axis = np.allclose(ez[:, 0] - ez0[:, 0], 2 * d, rtol=1e-9, atol=0)
+    assert ampere_axis == (axis_sign == -1)
+
+
+@pytest.mark.parametrize("comp,cell", [("r", (3, 4)), ("r", (9, 7)), ("z", (2, 5)), ("z", (10, 3))])
+def test_fdtd_current_energy_theorem(comp, cell):
+    """Poynting: the first-order energy change of one E update from a current
+    J in one cell is -tau J . E(cell centre) V(i) (EM3.cu:29 / :39 averaging of
+    J onto the E nodes).  E is uniform in r and linear in z, so the centre value
+    is exact; the energy is the reference's trapezoid tally
+    (electricmagneticfield5.m:167-177).  The linear part is isolated with +J and
+    -J (the quadratic part cancels); a sign, averaging or index mistake in the
+    current terms changes it by 2x, 0.5x or a cell's worth of E."""
+    nr, nz = 12, 10
+    p = params(nr, nz)
+    i0, k0 = cell
+    a0, g = 200.0, 4e8
+    er0 = np.tile((a0 + g * np.arange(nz + 1) * p.dz)[:, None], (1, nr))
+    ez0 = np.tile((a0 + g * (np.arange(nz) + 0.5) * p.dz)[:, None], (1, nr + 1))
+    h = np.zeros(nz * nr)
+    tau, J = 1e-15, 1e6
+    st = oracle.State(np.zeros((6, 1)))
+    st.dele[:] = 1
+
+    def energy(sign):
+        j = np.zeros(nr * nz)
+        j[i0 + k0 * nr] = sign * J
+        jr, jz = (j, np.zeros_like(j)) if comp == "r" else (np.zeros_like(j), j)
+        er, ez = _e_update(p, er0, ez0, h, jr, jz, tau)
+        return oracle.tallies(p, er, ez, h, st)[0]
+
+    lin = (energy(1) - energy(-1)) / 2
+    e_centre = a0 + g * (k0 + 0.5) * p.dz
+    V = p.phip / 2 * ((i0 + 1) ** 2 - i0 ** 2) * p.dr ** 2 * p.dz
+    assert lin == pytest.approx(-tau * J * e_centre * V, rel=1e-7, abs=0)
+
+
+# ---- excitation exponents (find_CS0.m:47-55, ParameterExcCS_Nitrogen.csv columns) -----
+@pytest.mark.parametrize("level", range(7))
+def test_excitation_level_exponents(raw, level):
+    """One Jackman-Green level, sigma ~ (W/T)^Omega (1 - (W/T)^gamma)^nu:
+    near threshold (1 - (W/T)^gamma)^nu ~ (gamma (T - W)/W)^nu, so the log-slope
+    in (T - W) is nu (column "v"); at high energy the log-slope in T is -Omega.
+    Swapping the "v" and "miu" columns changes the threshold slope of the
+    levels where they differ."""
+    row = raw["t_exc"][level]
+    W, om, nu = row[0], row[2], row[3]
+    one = dict(raw)
+    one["t_exc"] = raw["t_exc"][level:level + 1].copy()
+    t = oracle.build_tables(one)
+    eps = np.array([1e-7, 1e-6])
+    _, _, sx = oracle.cross_sections(t, W * (1 + eps) / 1000.0)
+    slope = math.log(sx[1] / sx[0]) / math.log(eps[1] / eps[0])
+    assert slope == pytest.approx(nu, abs=2e-4)
+    _, _, sx = oracle.cross_sections(t, np.array([1e6, 2e6]) * W / 1000.0)
+    assert math.log(sx[1] / sx[0]) / math.log(2.0) == pytest.approx(-om, abs=1e-4)
+
+
+def test_excitation_table_has_levels_that_tell_nu_from_gamma(raw):
+    t = raw["t_exc"]
+    assert np.any(t[:, 3] != t[:, 4])
+
+
+# ---- ionization side products (electron_transport.cu:167-204) --------------------------
+def _ionizations(otab, n=60000, seed=11):
+    import gasx
+    from gasx.workloads import speed_from_ev
+    p = oracle.params_from(gasx.make_params(20, 20))
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(40.0, 3000.0, n)
+    d = rng.standard_normal((3, n))
+    d /= np.linalg.norm(d, axis=0)
+    v0 = d * speed_from_ev(T)
+    st = oracle.State(np.vstack([np.full(n, 5e-5), np.full(n, 1e-6), np.full(n, 1e-4), v0]))
+    g = oracle.make_gas(np.full((1, 1), 3e26), 20 * p.dr, 20 * p.dz, t_scalar=450.0)
+    ev, sv, iv = oracle.collide(p, otab, g, st, 1e-13, seed=seed, step=1, ions=True)
+    m = ev == 2
+    return p, v0[:, m], st.v[3:6, :n][:, m], sv[m].T, iv[m].T
+
+
+def test_ion_velocity_is_maxwellian(otab):
+    """Each component ~ N(0, k_B T / m_ion) (electron_transport.cu:202-204)."""
+    from scipy import stats
+    p, _, _, _, iv = _ionizations(otab)
+    sig = math.sqrt(p.k_b * 450.0 / p.m_ion)
+    n = iv.shape[1]
+    assert n > 5000
+    for comp in iv:
+        assert abs(comp.mean()) < 5 * sig / math.sqrt(n)
+        chi = np.sum(comp ** 2) / sig ** 2                         # chi^2 with n degrees of freedom
+        assert stats.chi2.sf(chi, n) > 1e-3 and stats.chi2.cdf(chi, n) > 1e-3
+        assert stats.kstest(comp / sig, "norm").pvalue > 1e-3
+    # independent components: |v|^2 / sigma^2 follows chi^2 with 3 degrees of
+    # freedom (the Maxwell speed distribution), which dependent components with
+    # normal marginals would not
+    assert abs(np.corrcoef(iv)[0, 1]) < 5 / math.sqrt(n)
+    assert stats.kstest(np.sum(iv ** 2, axis=0) / sig ** 2, stats.chi2(3).cdf).pvalue > 1e-3
+
+
+def test_ionization_kinematics_are_moller_binary_encounters(otab):
+    """Free-electron (Moller) kinematics with the binding energy left out of the
+    angles: an electron of kinetic energy T giving W to the other leaves it at
+    cos^2 = W (T + 2mc^2) / (T (W + 2mc^2)) and turns by
+    cos^2 = (T - W)(T + 2mc^2) / (T (T - W + 2mc^2)).  The reference measures the
+    secondary's polar angle from +z of the lab (not from the primary's
+    direction) and puts it at azimuth phi_p - pi, opposite the primary's
+    scattering azimuth about its old direction (electron_transport.cu:167-200)."""
+    p, v0, v1, vs, _ = _ionizations(otab, n=20000, seed=12)
+    c = p.c
+    mc2_em = p.me_em * c * c / p.e_em
+    mc2 = p.me_mc * c * c / p.e_mc
+
+    def ev(v, m_c2):
+        b2 = np.sum(v ** 2, axis=0) / c ** 2
+        return m_c2 * (1 / np.sqrt(1 - b2) - 1)
+
+    T = ev(v0, mc2_em)
+    W = ev(vs, mc2)
+    cs = vs[2] / np.linalg.norm(vs, axis=0)
+    assert np.all(cs >= 0)                                        # lab frame, measured from +z
+    assert np.allclose(cs ** 2, W * (T + 2 * mc2) / (T * (W + 2 * mc2)), rtol=1e-9)
+    cp = np.sum(v0 * v1, axis=0) / (np.linalg.norm(v0, axis=0) * np.linalg.norm(v1, axis=0))
+    assert np.all(cp > 0)
+    assert np.allclose(cp ** 2, (T - W) * (T + 2 * mc2) / (T * (T - W + 2 * mc2)), rtol=1e-7)
+    # azimuths: the primary's about its old direction (rotate.m frame), the
+    # secondary's about +z
+    vp2 = v0[0] ** 2 + v0[1] ** 2
+    vn = np.linalg.norm(v0, axis=0)
+    e1 = np.vstack([v0[0] * v0[2], v0[1] * v0[2], -vp2]) / (vn * np.sqrt(vp2))
+    e2 = np.vstack([-v0[1], v0[0], np.zeros_like(vp2)]) / np.sqrt(vp2)
+    phi_p = np.arctan2(np.sum(v1 * e2, axis=0), np.sum(v1 * e1, axis=0))
+    phi_s = np.arctan2(vs[1], vs[0])
+    dphi = np.angle(np.exp(1j * (phi_s - (phi_p - math.pi))))
+    ok = vp2 > 1e-3 * vn ** 2
+    assert ok.sum() > 1000 and np.allclose(dphi[ok], 0.0, atol=1e-7)
