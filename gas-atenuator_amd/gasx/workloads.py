@@ -55,12 +55,12 @@ class Workload:
 # The five configurations of the measurement plan.
 WORKLOADS = {
     "plain_cpu_64x128": Workload("plain_cpu_64x128", 64, 128, 100_000, 200, 2.5e25),
-    # cell sort every 40 steps: the median electron moves ~1e-3 cell per step,
-    # and on this workload 5 / 10 / 20 / 40 measured 0.350 / 0.318 / 0.303 /
-    # 0.295 ms per step (eight current copies), 40 / 80 / never 0.287 / 0.284 /
-    # 0.293 (two copies): the sort's cost against the locality it restores
+    # cell sort every 10 steps as the configuration specifies; the median
+    # electron moves ~1e-3 cell per step, so a longer interval keeps most of
+    # the locality for less sorting (extra["sort_every_tuned"])
     "plain_gpu_256x512": Workload("plain_gpu_256x512", 256, 512, 4_000_000, 1000, 2.5e25,
-                                  capacity_factor=2.0, secondaries=True, sort_every=40),
+                                  capacity_factor=2.0, secondaries=True, sort_every=10,
+                                  extra={"sort_every_tuned": 40}),
     "batched_128x256": Workload("batched_128x256", 128, 256, 0, 500, 2.5e25,
                                 extra={"runs": 64, "n_lo": 1e4, "n_hi": 2e6,
                                        "dens_lo": 2.5e24, "dens_hi": 2.5e26}),
