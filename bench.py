@@ -6,16 +6,19 @@ Workload (default): the plain GPU step at scale -- axisymmetric r-z grid
 spot (sigma_r = 50 um) with the 70/30 400 +- 20 eV / 30 +- 10 eV energy mix,
 uniform N2 at 2.5e25 m^-3, one collision check per push (dt = 1e-14 s,
 10 field substeps), ionization secondaries on (capacity 2x), cell sort every
-40 steps, FP32 particles, FP64 field solver.
+10 steps, FP32 particles, FP64 field solver.
 
 A step = one push of every electron + its collision check + the field cycle.
 value = particle-steps/s over the K timed steps (inputs resident in HBM).
-roofline: the fused particle kernel, 48 algorithmic bytes per particle-step
-(FP32 x, y, z, vx, vy, vz read + write) over its average launch duration
-measured with HIP events on the library's stream (every launch of the timed
-region); peak 8 TB/s.
-cpu_baseline: the FP64 C oracle (one thread) on a bounded sample of the same
-workload on this host.
+roofline: the step against HBM, 48 algorithmic bytes per particle-step
+(FP32 x, y, z, vx, vy, vz read + write) at the measured particle-steps/s;
+peak 8 TB/s.  kernel_frac: the same bytes over the dominant kernel's
+average launch duration (HIP events on the library's stream, every launch
+of the timed region).  traffic: HBM bytes per step from the committed
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this same command.
+cpu_baseline: the FP64 C oracle (OpenMP) on this host, one thread and all
+the threads OMP_NUM_THREADS allows, on a slice of this workload and on the
+plain CPU-runnable configuration (64 x 128, 1e5 electrons, 200 steps).
 """
 from __future__ import annotations
 
@@ -48,7 +51,9 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-tuned-sort", action="store_true",
+                    help="skip the extra run at the tuned cell-sort interval")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (5 times) and print its size: "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -67,40 +72,90 @@ def copy_calibration(mib: int):
     print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
 
 
-def cpu_baseline(w, host, n_particles: int, n_steps: int, seed: int):
-    """FP64 oracle on the first n_particles of the same seeded electrons."""
+def cpu_threads() -> int:
+    """Threads of the all-core CPU leg: OMP_NUM_THREADS when set (the host's
+    share for one GPU), else the CPUs this process may run on."""
+    import os
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_model() -> str:
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_leg(w, host, n_particles: int, n_steps: int, seed: int, threads: int, secondaries: bool):
+    """The FP64 oracle loop on the first n_particles of host for n_steps with
+    `threads` OpenMP threads; returns (particle-steps/s, wall s, particle-steps)."""
     import numpy as np
 
     import gasx
     import oracle
+    from gasx.workloads import density_grid
 
     p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
     op = oracle.params_from(p)
     ot = oracle.build_tables(gasx.raw_tables())
     n = min(n_particles, host[0].size)
     st = np.stack([a[:n] for a in host]).astype(np.float64)
-    o = oracle.State(st, cap=2 * n)
-    from gasx.workloads import density_grid
+    o = oracle.State(st, cap=2 * n if secondaries else n)
     dens, gdr, gdz = density_grid(w)
     og = oracle.make_gas(dens.astype(np.float64), gdr, gdz)
     er = np.zeros((w.nz + 1) * w.nr)
     ez = np.zeros(w.nz * (w.nr + 1))
     h = np.zeros(w.nz * w.nr)
-    t0 = time.perf_counter()
-    _, _, _, cnt = oracle.run(op, ot, og, o, er, ez, h, n_steps, collide=True,
-                              secondaries=w.secondaries, seed=seed)
-    dt = time.perf_counter() - t0
-    # pushes performed: live particles each step (absorption is rare; count live at the end)
+    oracle.set_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        oracle.run(op, ot, og, o, er, ez, h, n_steps, collide=True, secondaries=secondaries, seed=seed)
+        dt = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(1)
+    # pushes performed: live particles each step (absorption is rare; the
+    # population grows about linearly with secondaries)
     live_end = int(np.sum(o.dele[: o.n] == 0))
     psteps = 0.5 * (n + live_end) * n_steps
+    return psteps / dt, dt, psteps
+
+
+def cpu_baseline(w, host, n_particles: int, n_steps: int, seed: int):
+    """The FP64 oracle on this host: a slice of the benchmarked workload (its
+    first n_particles electrons for n_steps) and the plain CPU-runnable
+    configuration exactly, each with one thread and with all threads."""
+    from gasx.workloads import WORKLOADS, seed_electrons
+
+    T = cpu_threads()
+    vN, tN, _ = oracle_leg(w, host, n_particles, n_steps, seed, T, w.secondaries)
+    v1, t1, _ = oracle_leg(w, host, n_particles, n_steps, seed, 1, w.secondaries)
+    c1 = WORKLOADS["plain_cpu_64x128"]
+    h1 = seed_electrons(c1.n_electrons, c1.rp, c1.lp, c1.phip, c1.sigma_r, seed=seed)
+    c1N, c1tN, c1ps = oracle_leg(c1, h1, c1.n_electrons, c1.n_steps, seed, T, False)
+    c11, c1t1, _ = oracle_leg(c1, h1, c1.n_electrons, c1.n_steps, seed, 1, False)
+    n = min(n_particles, host[0].size)
     return {
-        "value": psteps / dt,
+        "value": vN,
         "unit": "particle-steps/s",
-        "cores": 1,
+        "cores": T,
         "kind": "port",
-        "sample": (f"FP64 C oracle, 1 thread: first {n} of the seeded electrons on the "
+        "sample": (f"FP64 C oracle, {T} OpenMP threads: first {n} of the seeded electrons on the "
                    f"{w.nr}x{w.nz} grid, {n_steps} steps with collisions"
-                   f"{' and secondaries' if w.secondaries else ''} ({dt:.1f} s)"),
+                   f"{' and secondaries' if w.secondaries else ''} ({tN:.1f} s)"),
+        "single_thread": {"value": v1, "cores": 1, "wall_s": t1},
+        "nproc": len(__import__("os").sched_getaffinity(0)),
+        "cpu_model": cpu_model(),
+        "plain_cpu_64x128": {
+            "sample": (f"the whole configuration: {c1.nr}x{c1.nz}, {c1.n_electrons} electrons, "
+                       f"{c1.n_steps} steps, collisions on, secondaries off"),
+            "particle_steps": c1ps,
+            "value": c1N, "cores": T, "wall_s": c1tN,
+            "single_thread": {"value": c11, "cores": 1, "wall_s": c1t1},
+        },
     }
 
 
@@ -116,7 +171,7 @@ def cell_sorted(host, w):
 
 def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     """Batched ensemble (64 runs of 128x256, log-uniform density and electron
-    count, collisions on, secondaries off) in one captured graph with the
+    count, collisions and secondaries on with capacity 2x) in one captured graph with the
     autotuned launch shapes, against the same runs one after another with
     gasx_run.  Both legs start from the same cell-ordered electrons and produce
     identical events run by run (checked from the per-run counters)."""
@@ -129,11 +184,12 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     w = WORKLOADS["batched_128x256"]
     n_steps = steps or w.n_steps
     counts, dens = batched_runs(w, seed)
+    caps = [2 * int(n) for n in counts]               # secondaries on, growth capped at 2x
     p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
     ctx = gasx.Context(p, device=0)
     runs = [cell_sorted(seed_electrons(int(n), w.rp, w.lp, w.phip, w.sigma_r, seed=seed + 100 + r), w)
             for r, n in enumerate(counts)]
-    ens = gasx.Ensemble(ctx, [int(n) for n in counts], dens, w.rp, w.lp)
+    ens = gasx.Ensemble(ctx, caps, dens, w.rp, w.lp)
     for r, h in enumerate(runs):
         ens.load_run(r, *h)
     init = (ens.pool.clone(), ens.counts.clone())
@@ -146,7 +202,7 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
         ens.counters(reset=True)
         torch.cuda.synchronize()
 
-    kw = dict(collide=True, secondaries=False, seed=seed + 7, run_id=0)
+    kw = dict(collide=True, secondaries=True, seed=seed + 7, run_id=0)
     tuning = ens.autotune(trial_steps=5, **kw)
     reset()
     ens.run(n_steps, use_graph=True, **kw)            # capture + instantiate + first replay
@@ -158,7 +214,8 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     per_run = ens.counters()
     psteps_b = sum(c.n_pushed for c in per_run)
     out = {"value": psteps_b / wall_b, "unit": "particle-steps/s", "runs": len(counts),
-           "steps": n_steps, "electrons": int(counts.sum()), "wall_s": wall_b, "tuning": tuning,
+           "steps": n_steps, "electrons": int(counts.sum()), "capacity_factor": 2.0,
+           "secondaries": True, "wall_s": wall_b, "tuning": tuning,
            "graph": True, "grid": f"{w.nr}x{w.nz}"}
     del ens, init
     torch.cuda.empty_cache()
@@ -167,7 +224,7 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     wall_s, psteps_s, same = 0.0, 0, True
     gas_grids = [gasx.GasGrid(d, w.rp, w.lp, temperature=300.0) for d in dens]
     for r, h in enumerate(runs):
-        e = gasx.ParticleSet(int(counts[r]))
+        e = gasx.ParticleSet(caps[r])
         e.load(*h)
         f = gasx.FieldState(w.nr, w.nz)
         ctx.counters(reset=True)
@@ -343,20 +400,35 @@ def main():
     avg_ms = tm["particle_ms"] / n_timed
     n_launch = args.steps
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * psteps / n_launch
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    kernel_achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # the step: the same 48 bytes per particle-step at the measured rate
+    step_achieved = BYTES_PER_PARTICLE_STEP * value / 1e9
     # the drift/deposit kernel and the whole particle pass from the breakdown steps
     n_dd = max(1, tb["n_drift_launches"])
     dd_ms = tb["drift_ms"] / n_dd
     pass_ms = (tb["particle_ms"] + tb["drift_ms"]) / n_bd
-    # HBM bytes per launch from the committed FETCH_SIZE + WRITE_SIZE passes
-    # (calibrated against a plain copy), stored per particle-step and scaled
-    # to this run's particle-steps per launch.
-    traffic = None
-    tfile = REPO / "profiles" / f"k_particles_push_collide_{w.name}.traffic.txt"
+    # HBM bytes per step from the committed FETCH_SIZE + WRITE_SIZE passes of
+    # this same command (calibrated against a plain copy); none for other
+    # step counts
+    traffic, traffic_src = None, None
+    tfile = REPO / "profiles" / f"step_{w.name}_steps{args.steps}_warmup{args.warmup}.traffic.txt"
     if tfile.exists():
         for ln in tfile.read_text().splitlines():
-            if ln.startswith("hbm_bytes_per_particle_step:"):
-                traffic = float(ln.split(":", 1)[1]) * psteps / n_launch
+            if ln.startswith("hbm_bytes_per_step:"):
+                traffic = float(ln.split(":", 1)[1])
+                traffic_src = f"profiles/{tfile.name} (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, of this command)"
+    tuned = None
+    ts = w.extra.get("sort_every_tuned")
+    if ts and not args.no_tuned_sort:
+        # the same timed steps from the same start with the tuned sort interval
+        reset()
+        ctx.counters(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(e, f, gas, args.steps, **{**run_kw, "sort_every": ts})
+        torch.cuda.synchronize()
+        wt = time.perf_counter() - t0
+        tuned = {"sort_every": ts, "value": ctx.counters().n_pushed / wt, "ms_per_step": wt * 1e3 / args.steps}
     line = {
         "metric": METRIC,
         "value": value,
@@ -378,14 +450,19 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": step_achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": step_achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_particles (gather + push + collision Monte Carlo + secondary staging)",
-            "avg_launch_ms": avg_ms,
+            "traffic_per": "step",
+            "traffic_source": traffic_src,
             "bytes_per_particle_step": BYTES_PER_PARTICLE_STEP,
+            "scope": "whole step (48 B per particle-step at the measured particle-steps/s)",
+            "kernel": "k_particles (gather + push + collision Monte Carlo + secondary staging)",
+            "kernel_achieved": kernel_achieved,
+            "kernel_frac": kernel_achieved / HBM_PEAK_GBS,
+            "kernel_avg_launch_ms": avg_ms,
         },
         "deposit_kernel": {   # breakdown steps; 24 algorithmic bytes (the state, read)
             "avg_launch_ms": dd_ms,
@@ -404,6 +481,8 @@ def main():
         "fdtd_cell_updates_per_s": w.nr * w.nz * w.n_sub * n_bd / (tb["fdtd_ms"] * 1e-3)
         if tb["fdtd_ms"] > 0 else None,
     }
+    if tuned:
+        line["tuned_sort"] = tuned
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, host, args.cpu_particles, args.cpu_steps, args.seed + 1)
     if not args.no_batched:

@@ -13,6 +13,19 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the particle, field and collision loops (orc_set_threads).
+ * 1 by default: the checker's FP64 sums then have one fixed order.  With
+ * more threads every particle and every cell is still computed by the same
+ * arithmetic; only the deposit's per-thread partial currents are summed in
+ * thread order, which changes the last bits of J. */
+static int g_threads = 1;
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
+int orc_get_threads(void) { return g_threads; }
 
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -304,6 +317,7 @@ double orc_u01(uint32_t x)
 void orc_fdtd_h(double* h, const double* er, const double* ez, int nr, int nz,
                 double mu, double dr, double dz, double tau)
 {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int k = 0; k < nz; ++k)
         for (int i = 0; i < nr; ++i) {
             double dez = ez[(i + 1) + k * (nr + 1)] - ez[i + k * (nr + 1)];
@@ -323,6 +337,7 @@ void orc_fdtd_e(double* er, double* ez, const double* h, const double* jr, const
     double* ez0 = (double*)malloc(nez * sizeof(double));
     memcpy(er0, er, ner * sizeof(double));
     memcpy(ez0, ez, nez * sizeof(double));
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int k = 0; k <= nz; ++k)
         for (int i = 0; i < nr; ++i) {
             int n = i + k * nr;
@@ -331,6 +346,7 @@ void orc_fdtd_e(double* er, double* ez, const double* h, const double* jr, const
             else er[n] = er0[n] - tau / eps * (jr[n - nr] + jr[n]) / 2
                          - tau / (eps * dz) * (h[n] - h[n - nr]);   /* :29 */
         }
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int k = 0; k < nz; ++k)
         for (int i = 0; i < nr; ++i) {                        /* Ez(nr,k) is never written */
             int nzi = i + k * (nr + 1), n = i + k * nr;
@@ -399,6 +415,7 @@ static void wedge_bc(const orc_params* p, double* x, double* y, double* z,
 void orc_em2(const orc_params* p, double* x, double* y, double* z, double* vx, double* vy, double* vz,
              double* r, double* phi, double* vr, int32_t* del, int64_t n)
 {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
         if (del[i] == 1) continue;
         x[i] += 0.5 * vx[i] * p->ddt;
@@ -411,13 +428,13 @@ void orc_em2(const orc_params* p, double* x, double* y, double* z, double* vx, d
 
 /* EM7.cu:51-202: clamped area weighting onto the Hphi grid,
  * J += FN*b*q*v / V(i),  V(i) = PHIp/2*((i+1)^2-i^2) dr^2 dz. */
-void orc_em7(const orc_params* p, const double* r, const double* z, const double* q,
-             const double* vr, const double* vz, double* jr, double* jz,
-             const int32_t* del, int64_t n)
+static void em7_range(const orc_params* p, const double* r, const double* z, const double* q,
+                      const double* vr, const double* vz, double* jr, double* jz,
+                      const int32_t* del, int64_t m0, int64_t m1)
 {
     int nr = p->nr, nz = p->nz;
     double dr = p->dr, dz = p->dz;
-    for (int64_t m = 0; m < n; ++m) {
+    for (int64_t m = m0; m < m1; ++m) {
         if (del[m] == 1) continue;       /* EM7.cu:163-173 zero weights */
         double r0 = r[m], z0 = z[m];
         int ar = (int)floor(r0 / dr - 0.5), az = (int)floor(z0 / dz - 0.5);
@@ -442,6 +459,41 @@ void orc_em7(const orc_params* p, const double* r, const double* z, const double
     }
 }
 
+void orc_em7(const orc_params* p, const double* r, const double* z, const double* q,
+             const double* vr, const double* vz, double* jr, double* jz,
+             const int32_t* del, int64_t n)
+{
+    const int T = g_threads;
+    if (T <= 1 || n < 4096) {
+        em7_range(p, r, z, q, vr, vz, jr, jz, del, 0, n);
+        return;
+    }
+    /* per-thread partial currents over contiguous particle ranges, summed in
+     * thread order */
+    size_t nc = (size_t)p->nr * p->nz;
+    double* part = (double*)calloc((size_t)T * 2 * nc, sizeof(double));
+    if (!part) {
+        em7_range(p, r, z, q, vr, vz, jr, jz, del, 0, n);
+        return;
+    }
+#pragma omp parallel num_threads(T)
+    {
+        int t = 0, nt = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        nt = omp_get_num_threads();
+#endif
+        int64_t m0 = n * t / nt, m1 = n * (t + 1) / nt;
+        em7_range(p, r, z, q, vr, vz, part + (size_t)t * 2 * nc, part + ((size_t)t * 2 + 1) * nc, del, m0, m1);
+    }
+    for (int t = 0; t < T; ++t)
+        for (size_t c = 0; c < nc; ++c) {
+            jr[c] += part[(size_t)t * 2 * nc + c];
+            jz[c] += part[((size_t)t * 2 + 1) * nc + c];
+        }
+    free(part);
+}
+
 static int ifloor(double v) { return (int)floor(v); }
 
 /* EM4.cu:36-177: bilinear gather with the EM4 edge rules, Lorentz force with
@@ -452,6 +504,7 @@ void orc_em4(const orc_params* p, double* x, double* y, double* z, double* vx, d
 {
     int R = p->nr, Z = p->nz;
     double dr = p->dr, dz = p->dz, c = p->c, mu = p->mu0, dt = p->ddt;
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
         if (del[i] == 1) continue;
         double r0 = r[i], z0 = z[i];
@@ -642,6 +695,7 @@ void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
     const double c = p->c;
     const double e_e = p->me_mc * c * c / p->e_mc;      /* electron_transport.cu:52 */
     uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1024) if (g_threads > 1)
     for (int64_t m = 0; m < n; ++m) {
         ev[m] = 0;
         if (del[m] == 1) continue;

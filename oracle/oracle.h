@@ -55,6 +55,10 @@ typedef struct {
     double dcs_logv[ORC_MAXT * ORC_NANG];/* V_interp, [e*181 + angle] */
 } orc_tables;
 
+/* ---- threads of the loops (OpenMP; 1 by default) ---- */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* ---- tables (electron_collision.m:12-41, find_CS0.m:14-56) ---- */
 int orc_tables_build(orc_tables* t,
                      const double* t_el, int n_el,      /* 2 x n_el row-major */

@@ -99,6 +99,9 @@ def lib() -> C.CDLL:
         L.orc_tables_build.argtypes = [C.POINTER(OrcTables), _P, C.c_int, _P, C.c_int, _P, C.c_int,
                                        _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -128,6 +131,11 @@ def build_tables(raw: dict) -> OrcTables:
     if st != 0:
         raise ValueError(f"orc_tables_build failed ({st})")
     return t
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's loops (1 by default: one fixed FP64 summation order)."""
+    lib().orc_set_threads(int(n))
 
 
 def philox(ctr, key):

@@ -77,3 +77,25 @@ def test_fused_and_two_launch_append_scans_agree(gpu):
     assert np.array_equal(s0, s1, equal_nan=True)
     for a, b in zip(f0, f1):
         assert np.array_equal(a, b)
+
+
+def test_capacity_crossing_keeps_the_first_secondaries(gpu):
+    """With room for k secondaries, gasx_collide keeps exactly the first k of
+    what a run with room for all appends (parent slot order) and counts the
+    rest as dropped."""
+    import gasx
+    n = 700_000
+    part = wedge_particles(n, NR, NZ, 1e-5, 1e-5, 0.1, 79, vmax=2.5e7, edge_frac=0.0)
+    gas = gasx.GasGrid.uniform(1e26, NR * 1e-5, NZ * 1e-5)
+    params = gasx.make_params(NR, NZ, fn=1e3)
+    outs = []
+    for cap in (2 * n, n + 30_000):
+        log(f"capacity crossing n={n} cap={cap}")
+        ctx = gasx.Context(params)
+        e = gasx.ParticleSet(cap).load(*part)
+        ctx.collide(e, gas, 1e-14, seed=6, step=0)
+        outs.append((e.state(), ctx.counters()))
+    (big, cb), (small, cs) = outs
+    assert cb.n_ionize == cs.n_ionize > 30_000 and cb.n_dropped_sec == 0
+    assert small.shape[1] == n + 30_000 and cs.n_dropped_sec == cs.n_ionize - 30_000
+    assert np.array_equal(small, big[:, : n + 30_000])

@@ -713,3 +713,33 @@ def test_ionization_kinematics_are_moller_binary_encounters(otab):
     dphi = np.angle(np.exp(1j * (phi_s - (phi_p - math.pi))))
     ok = vp2 > 1e-3 * vn ** 2
     assert ok.sum() > 1000 and np.allclose(dphi[ok], 0.0, atol=1e-7)
+
+
+# ---- appends at a capacity crossing (electron_collision.m:68-75) ------------------------
+def test_capacity_crossing_keeps_the_first_parents_in_slot_order(otab):
+    """The secondaries of one collision step are appended in parent slot order
+    (logical-mask order of electron_collision.m:68-75); when the capacity cuts
+    the append, the kept ones are the first in that order: a run with room for
+    k secondaries holds exactly the first k secondaries of a run with room for
+    all, and the dropped count is the rest."""
+    import gasx
+    from gasx.workloads import seed_electrons
+    p = oracle.params_from(gasx.make_params(16, 32))
+    n = 6000
+    st = np.stack(seed_electrons(n, 16 * p.dr, 32 * p.dz, p.phip, seed=21))
+    g = oracle.make_gas(np.full((1, 1), 2e26), 16 * p.dr, 32 * p.dz)
+    z = np.zeros
+    outs = []
+    for cap in (3 * n, n + 200):
+        o = oracle.State(st, cap=cap)
+        _, _, _, c = oracle.run(p, otab, g, o, z((32 + 1) * 16), z(32 * 17), z(32 * 16), 1,
+                                collide=True, secondaries=True, seed=4)
+        outs.append((o, c))
+    (big, cb), (small, cs) = outs
+    assert cb.n_ionize == cs.n_ionize > 400 and cb.n_dropped == 0
+    assert small.n == n + 200 and cs.n_dropped == cs.n_ionize - 200
+    assert np.array_equal(small.v[:, n:n + 200], big.v[:, n:n + 200])
+    # the kept secondaries sit at their parents' positions, parents ascending
+    pos = {tuple(v): i for i, v in enumerate(big.v[:3, :n].T.tolist())}
+    parents = [pos[tuple(v)] for v in big.v[:3, n:big.n].T.tolist()]
+    assert np.all(np.diff(parents) > 0) and len(parents) == cb.n_ionize
