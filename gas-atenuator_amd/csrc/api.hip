@@ -642,7 +642,7 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
 // are the context's kJCopies-fold accumulators
 static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool drift, bool deposit,
                                      unsigned long long* qr, unsigned long long* qz, size_t jstride = 0,
-                                     int timed_kind = kTParticle, const AppendScan* scan = nullptr)
+                                     int timed_kind = kTParticle)
 {
     PartArgs pa = make_part(ctx, s);
     pa.qr = qr; pa.qz = qz;
@@ -652,8 +652,7 @@ static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool 
                                                                              (int64_t)ctx->cu_count * 16,
                                                                              kCounterSlots)));
     const KernelEvents ke = kernel_events(ctx, timed_kind);
-    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream, ke.start, ke.stop,
-                                scan),
+    HIPCHK(launch_drift_deposit(drift, deposit, false, ctx->geom, pa, blocks, ctx->stream, ke.start, ke.stop),
            "drift/deposit");
     return GASX_OK;
 }
@@ -701,37 +700,18 @@ extern "C" gasx_status gasx_cross_sections(gasx_ctx* ctx, const float* e_kev, in
     return GASX_OK;
 }
 
-// Appends what the last particle kernel over `src` staged.  Ions must be
-// appended before the electrons' own secondaries (the electron count fixes
-// the number of staged chunks).
-// The arguments of the electron secondaries' append scan (src: the species
-// whose particle launch staged them, appended to itself).
-static AppendScan secondary_scan(gasx_ctx* ctx, const gasx_species* src)
-{
-    AppendScan a;
-    a.src_count = src->count;
-    a.dst_count = src->count;
-    a.stage_cnt = ctx->stage_cnt;
-    a.scan_tmp = ctx->scan_tmp;
-    a.tmp2 = ctx->base_tmp;
-    a.max_segs = (int32_t)(ctx->stage_slots / kSeg);
-    return a;
-}
-
-// scanned: the step's deposit launch made the scan (secondary_scan, electrons)
+// Appends what the last particle launch over `src` staged into dst (parent
+// slot order).  Ions must be appended before the electrons' own secondaries:
+// the electron count fixes the number of staged segments.
 static gasx_status append_from_stage(gasx_ctx* ctx, const Staging& stg, const gasx_species* src,
-                                     gasx_species* dst, bool ion, unsigned long long* qr = nullptr,
-                                     unsigned long long* qz = nullptr, size_t jstride = 0,
-                                     bool scanned = false)
+                                     gasx_species* dst, bool ion)
 {
     float* d[6] = {dst->x, dst->y, dst->z, dst->vx, dst->vy, dst->vz};
-    const float kq = (float)(4294967296.0 / ctx->p.c) * dst->qnum;
     TimedScope ts(ctx, kTAppend);
     HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? stg.ion : stg.sec,
                          ion ? stg.ion_cnt : stg.sec_cnt, src->count, stg.scan_tmp,
                          stg.base_tmp + (ion ? 2 : 0), stg.max_segs,
-                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->geom, kq, qr, qz, jstride,
-                         ctx->stream, scanned),
+                         &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->stream),
            "append");
     return GASX_OK;
 }
@@ -892,15 +872,8 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                                k == sorted_k ? sorted : nullptr);
             if (st != GASX_OK) return st;
             if (k == sorted_k) sorted_k = -1;
-            // without ions (whose append shares the scan buffers) the deposit
-            // launch also makes the secondaries' append scan
-            const bool fuse_scan = !last && k == 0 && o.collide && o.secondaries && !o.ions;
-            if (!last) {
-                const AppendScan sc = secondary_scan(ctx, &sp[k]);
-                st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift,
-                                       fuse_scan ? &sc : nullptr);
-                if (st != GASX_OK) return st;
-            }
+            // appends first (ions, then the electrons' secondaries, already
+            // drifted), so the deposit that follows covers the appended slots too
             if (k == 0 && o.collide) {
                 const Staging stg = ctx_staging(ctx);
                 if (o.ions) {
@@ -908,10 +881,13 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                     if (st != GASX_OK) return st;
                 }
                 if (o.secondaries) {
-                    st = append_from_stage(ctx, stg, &sp[0], &sp[0], false, last ? nullptr : ctx->q[nxt][0],
-                                           last ? nullptr : ctx->q[nxt][1], nh, fuse_scan);
+                    st = append_from_stage(ctx, stg, &sp[0], &sp[0], false);
                     if (st != GASX_OK) return st;
                 }
+            }
+            if (!last) {
+                st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
+                if (st != GASX_OK) return st;
             }
         }
         if (o.sort_every > 0 && !last && ((s + 1) % o.sort_every) == 0) {

@@ -308,17 +308,16 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
             pp.qr = pp.qz = nullptr;
             HIPCHK(launch_particles(flags, ctx->geom, pp, cs, grid, ctx->stream), "particles");
         }
-        if (!last) {
-            TimedScope ts(ctx, kTDrift);
-            HIPCHK(launch_drift_deposit(false, true, true, ctx->geom, ps, grid, ctx->stream), "deposit");
-        }
-        if (sec) {
+        if (sec) {   // appended first: the deposit then covers the new slots
             TimedScope ts(ctx, kTAppend);
             HIPCHK(launch_append_batch(pool, b->counts, st.runs, n, st.stage, st.stage_cnt, st.chunk_start,
                                        st.offs, st.tmp_run, (int32_t)stage_segs, st.run_counters,
-                                       &ctx->d_counters[kCDropSec], ctx->geom, pa.kq, ps.qr, ps.qz,
-                                       jstride, ctx->stream),
+                                       &ctx->d_counters[kCDropSec], ctx->stream),
                    "append");
+        }
+        if (!last) {
+            TimedScope ts(ctx, kTDrift);
+            HIPCHK(launch_drift_deposit(false, true, true, ctx->geom, ps, grid, ctx->stream), "deposit");
         }
     }
     if (o->n_steps % 2 == 1) {

@@ -100,7 +100,7 @@ enum : uint32_t {
     kStageSec = 16u,   // stage ionization secondaries for append
     kStageIon = 32u,   // stage N2+ ions for append
     kBatch = 64u,      // chunks belong to the runs of a batched ensemble
-    kDriftSec = 128u,  // half drift + BC of staged secondaries (appended after the deposit has read the slots)
+    kDriftSec = 128u,  // half drift + BC of staged secondaries (they join at the step's end, x^{n+3/2})
     kDrift2 = 256u,    // the step's second half drift + BC of every electron, after its collision
                        // (the launch that follows then deposits without writing the particles)
 };
@@ -216,37 +216,22 @@ int particle_occupancy(uint32_t flags);
 // ev0 / ev1 (optional) receive the kernel's own start and end times.
 hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, const CollArgs& ca,
                             int max_blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-// The first half of an append (the exclusive scan of the staging segments'
-// counts in groups, launch_append's first launch), made by extra workgroups
-// of a deposit launch instead: same inputs and outputs as that launch.
-struct AppendScan {
-    const int32_t* src_count;     // fixes the number of staged segments
-    const int32_t* dst_count;     // tmp2[0] <- *dst_count
-    const int32_t* stage_cnt;
-    int32_t* scan_tmp;            // max_segs segment offsets, then the group totals
-    int32_t* tmp2;                // [old dst count, staged segments]
-    int32_t max_segs;
-};
 // EM2 half drift and / or EM7 deposit (batch: both); the grid is in kChunk
-// chunks.  scan (deposit alone, not batched): the launch also makes that scan.
+// chunks.
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr,
-                                const AppendScan* scan = nullptr);
-// scanned: the scan was made by the step's deposit launch (same arguments)
+                                int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+// Appends what a particle launch staged (scan, then copy into dst in parent
+// slot order); the appended slots are not deposited here.
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
-                         int32_t max_chunks, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s,
-                         bool scanned = false);
+                         int32_t max_segs, unsigned long long* dropped, hipStream_t s);
 // batched append: per-run scan over the chunks recorded in chunk_start, then copy
 hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,
                                float* const stage[6], const int32_t* stage_cnt,
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
-                               unsigned long long* dropped, const Geom& g, float kq,
-                               unsigned long long* qr, unsigned long long* qz, size_t jstride,
-                               hipStream_t s);
+                               unsigned long long* dropped, hipStream_t s);
 hipError_t fdtd_prepare();
 hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx, hipStream_t s);

@@ -66,17 +66,6 @@ __device__ __forceinline__ int dep_cell(const DepCells& d, int a, int nr)
     return ((a & 1) ? d.i1 : d.i0) + ((a & 2) ? d.k1 : d.k0) * nr;
 }
 
-__device__ __forceinline__ void deposit_global(const DepCells& d, int nr, unsigned long long* qr,
-                                               unsigned long long* qz)
-{
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int c = dep_cell(d, a, nr);
-        if (d.qr[a]) atomicAdd(&qr[c], (unsigned long long)d.qr[a]);
-        if (d.qz[a]) atomicAdd(&qz[c], (unsigned long long)d.qz[a]);
-    }
-}
-
 // ---- gather + push (EM4.cu:36-148) ------------------------------------------
 // In two parts so that a lane can have several particles' field reads in
 // flight together: gather_issue computes the stencils and reads the 12 field

@@ -866,66 +866,20 @@ __global__ __launch_bounds__(kBlock) void k_drift_deposit(Geom g, PartArgs pa0)
 constexpr int kScanGroup = 1024;         // staging segments per group of the append's scan
 constexpr int kMaxScanGroups = 1024;     // staging of up to 2^20 segments
 
-// One group of the append's scan by a kBlock workgroup (kScanGroup / kBlock
-// segments per thread): every segment's exclusive offset within its group,
-// the group's total, and (group 0) the destination's old count and the
-// number of staged segments.
-__device__ void append_scan_group(const AppendScan& sc, int grp)
-{
-    __shared__ int s_w[kBlock / kWaveSize];
-    constexpr int kT = kScanGroup / kBlock;
-    int nseg = (*sc.src_count + kSeg - 1) / kSeg;
-    if (nseg > sc.max_segs) nseg = sc.max_segs;
-    const int c0 = grp * kScanGroup + (int)threadIdx.x * kT;
-    int v[kT], sum = 0;
-#pragma unroll
-    for (int j = 0; j < kT; ++j) {
-        v[j] = c0 + j < nseg ? sc.stage_cnt[c0 + j] : 0;
-        sum += v[j];
-    }
-    int tot;
-    int off = block_excl_scan(sum, s_w, tot);
-#pragma unroll
-    for (int j = 0; j < kT; ++j) {
-        if (c0 + j < nseg) sc.scan_tmp[c0 + j] = off;
-        off += v[j];
-    }
-    if (threadIdx.x == 0) {
-        sc.scan_tmp[sc.max_segs + grp] = tot;
-        if (grp == 0) {
-            sc.tmp2[0] = *sc.dst_count;
-            sc.tmp2[1] = nseg;
-        }
-    }
-}
-
 // The deposit alone (every push of the production loop): six waves per SIMD
 // (80 registers, a 384-cell tile in four copies) hide more of its LDS and
-// memory latency than four.  Not batched, workgroups ndep.. (if any) make
-// the append's scan (one group each) instead.
+// memory latency than four.  It runs after the step's appends, so the
+// appended secondaries are deposited with the rest through the LDS tiles.
 template <bool kBat>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deposit(Geom g, PartArgs pa0,
-                                                                                           AppendScan sc, int ndep)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_deposit(Geom g, PartArgs pa0)
 {
-    if (!kBat && (int)blockIdx.x >= ndep) {
-        append_scan_group(sc, (int)blockIdx.x - ndep);
-        return;
-    }
-    drift_deposit_pass<false, true, kBat>(g, pa0, ndep);
+    drift_deposit_pass<false, true, kBat>(g, pa0, (int)gridDim.x);
 }
 
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
-                                int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, const AppendScan* scan)
+                                int blocks, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1)
 {
     blocks = blocks > 0 ? blocks : 1;
-    AppendScan sc{};
-    int nscan = 0;                                 // extra workgroups: one per scan group
-    if (scan) {
-        if (batch || drift || !deposit) return hipErrorInvalidValue;
-        nscan = (scan->max_segs + kScanGroup - 1) / kScanGroup;
-        if (nscan < 1 || nscan > kMaxScanGroups) return hipErrorInvalidValue;
-        sc = *scan;
-    }
     // the events (if any) take the kernel's own start and end times: no marker
     // packets between the launches of the step
     if (batch) {
@@ -933,15 +887,13 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
         if (drift)
             hipExtLaunchKernelGGL((k_drift_deposit<true, true, true>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
         else
-            hipExtLaunchKernelGGL(k_deposit<true>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa, sc,
-                                  blocks);
+            hipExtLaunchKernelGGL(k_deposit<true>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift && deposit) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, true, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (drift) {
         hipExtLaunchKernelGGL((k_drift_deposit<true, false, false>), dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else if (deposit) {
-        hipExtLaunchKernelGGL(k_deposit<false>, dim3(blocks + nscan), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa,
-                              sc, blocks);
+        hipExtLaunchKernelGGL(k_deposit<false>, dim3(blocks), dim3(kBlock), 0, s, ev0, ev1, 0, g, pa);
     } else {
         return hipErrorInvalidValue;
     }
@@ -997,10 +949,10 @@ __device__ int block_scan_counts(const int32_t* cnt, int32_t* offs, int n, int* 
 // ---- append -------------------------------------------------------------------
 // Two launches, both parallel over the staging segments: k_append_scan gives
 // every segment its offset within a group of kScanGroup segments and every
-// group its total; k_append_copy prefixes the group totals in LDS, places
-// each staged entry at count + offset (parent slot order), deposits it, and
-// updates the count.  src_count fixes the number of staged segments.  The
-// scan can instead be made by a deposit launch (k_deposit, AppendScan).
+// group its total; k_append_copy prefixes the group totals in LDS and places
+// the staged entries at count + offset (parent slot order), then updates the
+// count.  src_count fixes the number of staged segments.  The appended slots
+// are deposited afterwards, with the rest, by the step's deposit launch.
 
 __global__ __launch_bounds__(kScanGroup) void k_append_scan(const int32_t* src_count,
                                                             const int32_t* dst_count,
@@ -1035,6 +987,23 @@ __global__ __launch_bounds__(kScanGroup) void k_append_scan(const int32_t* src_c
     }
 }
 
+// The segment of entry e of a workgroup's batch: the largest j in [0, kBlock)
+// with pre[j] <= e (pre is the batch's exclusive entry prefix, ascending).
+__device__ __forceinline__ int batch_segment(const int* pre, int e)
+{
+    int lo = 0, hi = kBlock;                   // pre[lo] <= e < pre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Workgroup b copies the entries of staged segments [b*kBlock, (b+1)*kBlock):
+// a batch lies inside one scan group (kScanGroup is a multiple of kBlock), so
+// its destinations are contiguous, from the first segment's offset on; every
+// thread copies whole entries (six floats), consecutive threads consecutive
+// destinations.
 __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, float* d2, float* d3,
                                                         float* d4, float* d5, int32_t cap,
                                                         int32_t* dst_count,
@@ -1044,17 +1013,18 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
                                                         const int32_t* stage_cnt,
                                                         const int32_t* offs, const int32_t* totals,
                                                         int32_t ngroups, const int32_t* tmp2,
-                                                        unsigned long long* dropped,
-                                                        Geom g, float kq, unsigned long long* qr,
-                                                        unsigned long long* qz, size_t jstride)
+                                                        unsigned long long* dropped)
 {
+    static_assert(kScanGroup % kBlock == 0, "a batch of segments lies inside one scan group");
     __shared__ int s_pre[kMaxScanGroups + 1];
+    __shared__ int s_seg[kBlock + 1];
     __shared__ int s_wave[kBlock / 64];
     const int base = tmp2[0];
     const int nseg = tmp2[1];
     // full: everything staged is dropped; only workgroup 0 (the count and the
     // dropped tally) has work
     if (base >= cap && blockIdx.x != 0) return;
+    if ((int)blockIdx.x * kBlock >= nseg && blockIdx.x != 0) return;
     // exclusive prefix of the group totals (four per thread)
     {
         constexpr int kG = kMaxScanGroups / kBlock;
@@ -1084,33 +1054,27 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
         __syncthreads();
     }
     if (base >= cap) return;                       // full: everything staged was dropped
-    if (qr) {                                      // this workgroup's copy of the current
-        qr += (size_t)(blockIdx.x % kJCopies) * jstride;
-        qz += (size_t)(blockIdx.x % kJCopies) * jstride;
-    }
-    // a wave per staged segment (a segment holds a few entries on average)
-    const int lane = (int)threadIdx.x & 63;
-    const int nwaves = (int)gridDim.x * (kBlock / 64);
-    for (int c = (int)blockIdx.x * (kBlock / 64) + ((int)threadIdx.x >> 6); c < nseg; c += nwaves) {
-        const int cnt = stage_cnt[c];
-        if (cnt == 0) continue;
-        const int first = base + s_pre[c / kScanGroup] + offs[c];
-        if (first >= cap) continue;
-        for (int j = lane; j < cnt; j += 64) {
-            const int dst = first + j;
-            const int src = c * kSeg + j;
-            if (dst < cap) {
-                const float x = s0[src], y = s1[src], z = s2[src];
-                const float vx = s3[src], vy = s4[src], vz = s5[src];
-                d0[dst] = x; d1[dst] = y; d2[dst] = z;
-                d3[dst] = vx; d4[dst] = vy; d5[dst] = vz;
-                if (qr) {                                  // EM7 of the new slot
-                    DepCells dc;
-                    deposit_cells(g, x, y, z, vx, vy, vz, kq, dc);
-                    deposit_global(dc, g.nr, qr, qz);
-                }
+    for (int bt = blockIdx.x; bt * kBlock < nseg; bt += gridDim.x) {   // workgroup-uniform
+        const int c0 = bt * kBlock;
+        const int c = c0 + (int)threadIdx.x;
+        const int cnt = c < nseg ? stage_cnt[c] : 0;
+        int tot;
+        const int pre = block_excl_scan(cnt, s_wave, tot);
+        s_seg[threadIdx.x] = pre;
+        if (threadIdx.x == 0) s_seg[kBlock] = tot;
+        const int first = base + s_pre[c0 / kScanGroup] + offs[c0];
+        __syncthreads();
+        if (first < cap) {
+            for (int e = (int)threadIdx.x; e < tot; e += kBlock) {
+                const int dst = first + e;
+                if (dst >= cap) break;
+                const int j = batch_segment(s_seg, e);
+                const int src = (c0 + j) * kSeg + (e - s_seg[j]);
+                d0[dst] = s0[src]; d1[dst] = s1[src]; d2[dst] = s2[src];
+                d3[dst] = s3[src]; d4[dst] = s4[src]; d5[dst] = s5[src];
             }
         }
+        __syncthreads();                           // s_seg is rewritten by the next batch
     }
 }
 
@@ -1118,27 +1082,20 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
-                         int32_t max_segs, unsigned long long* dropped, const Geom& g, float kq,
-                         unsigned long long* qr, unsigned long long* qz, size_t jstride, hipStream_t s,
-                         bool scanned)
+                         int32_t max_segs, unsigned long long* dropped, hipStream_t s)
 {
     const int ngroups = (max_segs + kScanGroup - 1) / kScanGroup;
     if (ngroups < 1 || ngroups > kMaxScanGroups) return hipErrorInvalidValue;
     int32_t* totals = scan_tmp + max_segs;
-    if (!scanned) {
-        hipLaunchKernelGGL(k_append_scan, dim3(ngroups), dim3(kScanGroup), 0, s, src_count, dst_count,
-                           stage_cnt, scan_tmp, totals, tmp2, max_segs);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    // a wave per segment, at most 1024 workgroups (with a full population all
-    // but workgroup 0 return at once, and launching them is most of the cost)
-    const int wseg = (max_segs + kBlock / 64 - 1) / (kBlock / 64);
-    const int blocks = wseg < 1024 ? (wseg > 0 ? wseg : 1) : 1024;
-    hipLaunchKernelGGL(k_append_copy, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
+    hipLaunchKernelGGL(k_append_scan, dim3(ngroups), dim3(kScanGroup), 0, s, src_count, dst_count,
+                       stage_cnt, scan_tmp, totals, tmp2, max_segs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // a workgroup per kBlock staged segments (those past the staged count return at once)
+    const int blocks = (max_segs + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_append_copy, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped, g, kq,
-                       qr, qz, jstride);
+                       stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped);
     return hipGetLastError();
 }
 
@@ -1169,6 +1126,10 @@ __global__ __launch_bounds__(1024) void k_append_scan_batch(int32_t* counts, con
     }
 }
 
+// Workgroup b copies the entries of staged chunks [b*kBlock, (b+1)*kBlock),
+// which may belong to several runs: each entry finds its chunk (batch prefix)
+// and the chunk its run (chunk_start), and lands at the run's old count +
+// its offset in the run.
 __global__ __launch_bounds__(kBlock) void k_append_copy_batch(float* d0, float* d1, float* d2, float* d3,
                                                               float* d4, float* d5, const BatchRun* runs,
                                                               int nruns, const float* s0, const float* s1,
@@ -1176,45 +1137,41 @@ __global__ __launch_bounds__(kBlock) void k_append_copy_batch(float* d0, float* 
                                                               const float* s4, const float* s5,
                                                               const int32_t* stage_cnt,
                                                               const int32_t* chunk_start,
-                                                              const int32_t* offs, const int32_t* tmp_run,
-                                                              Geom g, float kq, unsigned long long* qr,
-                                                              unsigned long long* qz, size_t jstride)
+                                                              const int32_t* offs, const int32_t* tmp_run)
 {
     __shared__ int s_cs[kMaxRuns + 1];
-    if (qr) {                                      // this workgroup's copy of the current
-        qr += (size_t)(blockIdx.x % kJCopies) * jstride;
-        qz += (size_t)(blockIdx.x % kJCopies) * jstride;
-    }
+    __shared__ int s_seg[kBlock + 1];
+    __shared__ int s_wave[kBlock / 64];
     for (int r = threadIdx.x; r <= nruns; r += blockDim.x) s_cs[r] = chunk_start[r];
     __syncthreads();
     const int total = s_cs[nruns];
-    const size_t nh = (size_t)g.nr * g.nz;
-    for (int c = blockIdx.x; c < total; c += gridDim.x) {
-        const int cnt = stage_cnt[c];
-        if (cnt == 0) continue;
-        int lo = 0, hi = nruns;                    // s_cs[lo] <= c < s_cs[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_cs[mid] <= c) lo = mid; else hi = mid;
-        }
-        const BatchRun br = runs[lo];
-        const int first = tmp_run[2 * lo] + offs[c];
-        for (int j = threadIdx.x; j < cnt; j += blockDim.x) {
-            const int local = first + j;
+    for (int bt = blockIdx.x; bt * kBlock < total; bt += gridDim.x) {   // workgroup-uniform
+        const int c0 = bt * kBlock;
+        const int c = c0 + (int)threadIdx.x;
+        const int cnt = c < total ? stage_cnt[c] : 0;
+        int tot;
+        const int pre = block_excl_scan(cnt, s_wave, tot);
+        s_seg[threadIdx.x] = pre;
+        if (threadIdx.x == 0) s_seg[kBlock] = tot;
+        __syncthreads();
+        for (int e = (int)threadIdx.x; e < tot; e += kBlock) {
+            const int j = batch_segment(s_seg, e);
+            const int ch = c0 + j;
+            int lo = 0, hi = nruns;                // s_cs[lo] <= ch < s_cs[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_cs[mid] <= ch) lo = mid; else hi = mid;
+            }
+            const BatchRun br = runs[lo];
+            const int local = tmp_run[2 * lo] + offs[ch] + (e - s_seg[j]);
             if (local < br.cap) {
                 const int64_t dst = br.base + local;
-                const int src = c * kSeg + j;
-                const float x = s0[src], y = s1[src], z = s2[src];
-                const float vx = s3[src], vy = s4[src], vz = s5[src];
-                d0[dst] = x; d1[dst] = y; d2[dst] = z;
-                d3[dst] = vx; d4[dst] = vy; d5[dst] = vz;
-                if (qr) {
-                    DepCells dc;
-                    deposit_cells(g, x, y, z, vx, vy, vz, kq, dc);
-                    deposit_global(dc, g.nr, qr + (size_t)lo * nh, qz + (size_t)lo * nh);
-                }
+                const int src = ch * kSeg + (e - s_seg[j]);
+                d0[dst] = s0[src]; d1[dst] = s1[src]; d2[dst] = s2[src];
+                d3[dst] = s3[src]; d4[dst] = s4[src]; d5[dst] = s5[src];
             }
         }
+        __syncthreads();                           // s_seg is rewritten by the next batch
     }
 }
 
@@ -1222,18 +1179,17 @@ hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const Batc
                                float* const stage[6], const int32_t* stage_cnt,
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
-                               unsigned long long* dropped, const Geom& g, float kq,
-                               unsigned long long* qr, unsigned long long* qz, size_t jstride,
-                               hipStream_t s)
+                               unsigned long long* dropped, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append_scan_batch, dim3(nruns), dim3(1024), 0, s, counts, runs, stage_cnt,
                        chunk_start, offs, tmp_run, run_counters, dropped);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int blocks = total_chunks < 4096 ? (total_chunks > 0 ? total_chunks : 1) : 4096;
+    const int batches = (total_chunks + kBlock - 1) / kBlock;
+    const int blocks = batches < 4096 ? (batches > 0 ? batches : 1) : 4096;
     hipLaunchKernelGGL(k_append_copy_batch, dim3(blocks), dim3(kBlock), 0, s, pool[0], pool[1], pool[2],
                        pool[3], pool[4], pool[5], runs, nruns, stage[0], stage[1], stage[2], stage[3],
-                       stage[4], stage[5], stage_cnt, chunk_start, offs, tmp_run, g, kq, qr, qz, jstride);
+                       stage[4], stage[5], stage_cnt, chunk_start, offs, tmp_run);
     return hipGetLastError();
 }
 
