@@ -123,9 +123,8 @@ __device__ __forceinline__ void deposit_box(const Geom& g, float r, float z, int
     k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
 }
 
-// Block-wide bounding box of [ilo,ihi] x [klo,khi] (ilo > ihi: none).  Every
-// thread must call it.
-__device__ __forceinline__ TileBox tile_box(int ilo, int ihi, int klo, int khi, int* s_red)
+// Wave-wide bounding box of [ilo,ihi] x [klo,khi], left in every lane.
+__device__ __forceinline__ void wave_box(int& ilo, int& ihi, int& klo, int& khi)
 {
     for (int o = 32; o > 0; o >>= 1) {
         ilo = min(ilo, __shfl_xor(ilo, o, 64));
@@ -133,23 +132,39 @@ __device__ __forceinline__ TileBox tile_box(int ilo, int ihi, int klo, int khi, 
         klo = min(klo, __shfl_xor(klo, o, 64));
         khi = max(khi, __shfl_xor(khi, o, 64));
     }
+}
+
+// A box of at most `cells` cells, or none (nk == 0).
+__device__ __forceinline__ TileBox fit_box(int ilo, int ihi, int klo, int khi, int cells)
+{
+    TileBox b{0, 0, 0, 0};
+    if (ihi >= ilo && khi >= klo && (ihi - ilo + 1) * (khi - klo + 1) <= cells) {
+        b.ilo = ilo; b.klo = klo; b.wi = ihi - ilo + 1; b.nk = khi - klo + 1;
+    }
+    return b;
+}
+
+// Block-wide bounding box of [ilo,ihi] x [klo,khi] (ilo > ihi: none) if it
+// fits `cells`; the arguments are replaced by the wave's own box and the
+// block's box is left in blk[4].  Every thread must call it.
+__device__ __forceinline__ TileBox tile_box(int& ilo, int& ihi, int& klo, int& khi, int* s_red, int cells,
+                                            int (&blk)[4])
+{
+    wave_box(ilo, ihi, klo, khi);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (lane == 0) {
         s_red[wave] = ilo; s_red[nw + wave] = ihi;
         s_red[2 * nw + wave] = klo; s_red[3 * nw + wave] = khi;
     }
     __syncthreads();
-    ilo = 0x7fffffff; ihi = -1; klo = 0x7fffffff; khi = -1;
+    int bil = 0x7fffffff, bih = -1, bkl = 0x7fffffff, bkh = -1;
     for (int w = 0; w < nw; ++w) {
-        ilo = min(ilo, s_red[w]); ihi = max(ihi, s_red[nw + w]);
-        klo = min(klo, s_red[2 * nw + w]); khi = max(khi, s_red[3 * nw + w]);
+        bil = min(bil, s_red[w]); bih = max(bih, s_red[nw + w]);
+        bkl = min(bkl, s_red[2 * nw + w]); bkh = max(bkh, s_red[3 * nw + w]);
     }
     __syncthreads();
-    TileBox b{0, 0, 0, 0};
-    if (ihi >= ilo && (ihi - ilo + 1) * (khi - klo + 1) <= kTile) {
-        b.ilo = ilo; b.klo = klo; b.wi = ihi - ilo + 1; b.nk = khi - klo + 1;
-    }
-    return b;
+    blk[0] = bil; blk[1] = bih; blk[2] = bkl; blk[3] = bkh;
+    return fit_box(bil, bih, bkl, bkh, cells);
 }
 
 // The summed contributions of a run of particles with one stencil (first cell
@@ -813,13 +828,37 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
             }
         }
         if constexpr (kDe) {
-            const TileBox bx = tile_box(ilo, ihi, klo, khi, s_red);
-            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) {
-#pragma unroll
-                for (int c = 0; c < kTCopies; ++c) { s_tr[c * kTStride + t] = 0ull; s_tz[c * kTStride + t] = 0ull; }
+            // the workgroup's box in the tile's kTCopies copies (dense chunks:
+            // neighbouring lanes add into the same cells); a larger box in the
+            // whole tile as one copy (sparse chunks, as the appended
+            // secondaries of slowly growing runs are between cell sorts:
+            // few particles per cell, so little contention); else each wave
+            // its own box in one copy's worth; what lies outside goes to global
+            int blk[4];
+            TileBox bx = tile_box(ilo, ihi, klo, khi, s_red, kTile, blk);
+            int t0 = (int)threadIdx.x, tstep = (int)blockDim.x, tbase = 0, ncop = kTCopies;
+            unsigned long long *my_tr, *my_tz;
+            if (bx.nk != 0) {                      // workgroup-uniform
+                my_tr = s_tr + (threadIdx.x % kTCopies) * kTStride;
+                my_tz = s_tz + (threadIdx.x % kTCopies) * kTStride;
+            } else {
+                ncop = 1;
+                bx = fit_box(blk[0], blk[1], blk[2], blk[3], kTCopies * kTStride);
+                if (bx.nk == 0) {                  // workgroup-uniform
+                    bx = fit_box(ilo, ihi, klo, khi, kTStride);
+                    t0 = (int)(threadIdx.x & (kWaveSize - 1));
+                    tstep = kWaveSize;
+                    tbase = (int)(threadIdx.x / kWaveSize) * kTStride;
+                }
+                my_tr = s_tr + tbase;
+                my_tz = s_tz + tbase;
             }
-            unsigned long long* const my_tr = s_tr + (threadIdx.x % kTCopies) * kTStride;
-            unsigned long long* const my_tz = s_tz + (threadIdx.x % kTCopies) * kTStride;
+            for (int t = t0; t < bx.wi * bx.nk; t += tstep) {
+                for (int c = 0; c < ncop; ++c) {
+                    s_tr[c * kTStride + tbase + t] = 0ull;
+                    s_tz[c * kTStride + tbase + t] = 0ull;
+                }
+            }
             __syncthreads();
             DepRun run_ = {-1, 0, 0, 0, {0, 0, 0, 0, 0, 0, 0, 0}};
 #pragma unroll
@@ -841,10 +880,12 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
             }
             if (run_.i0 >= 0) run_add(bx, run_, nr_, my_tr, my_tz, pa.qr, pa.qz);
             __syncthreads();
-            for (int t = threadIdx.x; t < bx.wi * bx.nk; t += blockDim.x) {
-                unsigned long long a = s_tr[t], c = s_tz[t];
-#pragma unroll
-                for (int k = 1; k < kTCopies; ++k) { a += s_tr[k * kTStride + t]; c += s_tz[k * kTStride + t]; }
+            for (int t = t0; t < bx.wi * bx.nk; t += tstep) {
+                unsigned long long a = s_tr[tbase + t], c = s_tz[tbase + t];
+                for (int k = 1; k < ncop; ++k) {
+                    a += s_tr[k * kTStride + tbase + t];
+                    c += s_tz[k * kTStride + tbase + t];
+                }
                 const int cell = (bx.ilo + t % bx.wi) + (bx.klo + t / bx.wi) * nr_;
                 if (a) atomicAdd(&pa.qr[cell], a);
                 if (c) atomicAdd(&pa.qz[cell], c);
