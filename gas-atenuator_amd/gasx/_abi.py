@@ -127,6 +127,26 @@ class RunOpts(C.Structure):
     ]
 
 
+class PhotoSource(C.Structure):
+    _fields_ = [
+        ("photon_kev", C.c_double), ("n_photons", C.c_double), ("beam_radius", C.c_double),
+        ("binding_ev", C.POINTER(C.c_double)), ("n_shells", C.c_int32),
+        ("mac", C.POINTER(C.c_double)), ("n_mac", C.c_int32),
+        ("atomic_mass", C.c_double), ("auger_kev", C.c_double),
+        ("seed", C.c_uint64), ("pulse", C.c_uint32), ("run", C.c_uint32),
+        ("density_out", C.c_void_p), ("temperature_out", C.c_void_p),
+    ]
+
+
+class PhotoResult(C.Structure):
+    _fields_ = [
+        ("photons_left", C.c_double), ("n_events", C.c_int64), ("n_electrons", C.c_int64),
+        ("n_ions", C.c_int64), ("n_dropped_electrons", C.c_int64), ("n_dropped_ions", C.c_int64),
+        ("sigma_pe", C.c_double), ("electron_kev", C.c_double), ("auger", C.c_int32),
+        ("n_rings", C.c_int32),
+    ]
+
+
 # exported symbols and their prototypes (every declaration of include/gasx.h)
 _P = C.c_void_p
 PROTOTYPES = {
@@ -165,6 +185,8 @@ PROTOTYPES = {
     "gasx_batch_set_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "gasx_batch_get_tuning": (C.c_int, [_P, C.POINTER(BatchTuning)]),
     "gasx_batch_counters": (C.c_int, [_P, C.c_int, C.POINTER(Counters), C.c_int]),
+    "gasx_photoionize": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Gas),
+                                   C.POINTER(PhotoSource), C.POINTER(PhotoResult)]),
     "gasx_diff_record": (C.c_int, [_P, C.POINTER(Species), C.POINTER(DiffSpec)]),
     "gasx_diff_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]),
     "gasx_diff_backward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),

@@ -254,6 +254,37 @@ class Context:
         self._check(self.lib.gasx_collision_bound(self.ctx, C.byref(g), dt_coll, C.byref(out)))
         return out.value
 
+    # ---- photoionization source -----------------------------------------------------
+    def photoionize(self, e: ParticleSet, gas: GasGrid, photon_kev: float, n_photons: float,
+                    beam_radius: float, ions: ParticleSet | None = None, seed: int = 0, pulse: int = 0,
+                    run: int = 0, deplete: bool = False, auger_kev: float = 0.383,
+                    atomic_mass: float = 14.0, tables: dict | None = None) -> dict:
+        """INCIDENCE1.m: photoelectrons (+ Auger electrons) appended to e, ions
+        to `ions`; with deplete the gas grid loses the ionized molecules and
+        their thermal energy (in place; a scalar temperature becomes a grid).
+        Returns the gasx_photo_result fields; electrons or ions beyond the
+        capacities are dropped and counted (n_dropped_*)."""
+        from .tables import source_tables
+        t = tables if tables is not None else source_tables()
+        be = np.ascontiguousarray(t["binding_ev"], dtype=np.float64)
+        mac = np.ascontiguousarray(t["mac"], dtype=np.float64)
+        if deplete and gas.temperature is None:
+            gas.temperature = torch.full_like(gas.density, gas.t_scalar)
+        PD = C.POINTER(C.c_double)
+        src = _abi.PhotoSource(photon_kev, n_photons, beam_radius, be.ctypes.data_as(PD), be.size,
+                               mac.ctypes.data_as(PD), mac.shape[0], atomic_mass, auger_kev, seed, pulse, run,
+                               _ptr(gas.density) if deplete else None,
+                               _ptr(gas.temperature) if deplete else None)
+        res = _abi.PhotoResult()
+        s = e.struct()
+        si = ions.struct() if ions is not None else None
+        g = gas.struct()
+        st = self.lib.gasx_photoionize(self.ctx, C.byref(s), C.byref(si) if si is not None else None,
+                                       C.byref(g), C.byref(src), C.byref(res))
+        if st not in (_abi.OK, _abi.E_CAPACITY):
+            self._check(st)
+        return {n: getattr(res, n) for n, _ in _abi.PhotoResult._fields_}
+
     # ---- production loop ---------------------------------------------------------
     def run(self, e: ParticleSet, f: FieldState, gas: GasGrid | None, n_steps: int,
             collide: bool = True, secondaries: bool = False, sort_every: int = 0, seed: int = 0,

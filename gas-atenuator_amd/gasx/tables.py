@@ -55,6 +55,21 @@ def raw_tables(data_dir: Path | None = None) -> dict[str, np.ndarray]:
     return out
 
 
+SOURCE_FILES = {
+    "binding_ev": "EB_14.csv",   # N shell binding energies (eV), K then valence (findEShell.m:2)
+    "mac": "14_MAC.csv",         # photon energy (MeV) ; mass attenuation (cm^2/g) (findSigmaPE.m:9)
+}
+
+
+def source_tables(data_dir: Path | None = None) -> dict[str, np.ndarray]:
+    """The photoionization source's data files of the reference, unchanged."""
+    d = Path(data_dir) if data_dir else DATA_DIR
+    out = {"binding_ev": np.ascontiguousarray(read_csv(d / SOURCE_FILES["binding_ev"]).ravel()),
+           "mac": np.ascontiguousarray(read_csv(d / SOURCE_FILES["mac"]))}
+    assert out["mac"].shape[1] == 2
+    return out
+
+
 class RawTableHandle:
     """Keeps the numpy buffers alive while the C struct points at them."""
 

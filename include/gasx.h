@@ -279,6 +279,54 @@ gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, double dt_col
 /* Kernel times of the last timed gasx_run (synchronizes). */
 gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out);
 
+/* ---- photoionization source ---------------------------------------------------
+ * Replaces INCIDENCE1.m:27-158 (with GUASS.m:3-15, findEShell.m:1-12,
+ * findSigmaPE.m:8-22, findPEangle.m:10-33, findV.m:5-8) on the r-z gas grid
+ * g: a Gaussian X-ray beam along +z (sigma = beam_radius / 3, rings of width
+ * gdr up to round(beam_radius / gdr)) is absorbed slab by slab (Beer-Lambert,
+ * dN = n N sigma_pe gdz, N -= dN); each (ring, slab) cell emits
+ * round(dN / FN) events.  An event appends a photoelectron of energy
+ * Ex - E_shell with a Sauter polar angle about +z, an Auger electron
+ * (auger_kev, polar angle uniform in [0, pi]) when the shell is not the last
+ * one of binding_ev, and an ion (to `ions`, may be NULL) with the cell's
+ * thermal velocity plus the recoil of the photon and the electrons.
+ * Electrons are appended as all photoelectrons (event order: slab by slab,
+ * rings outward), then all Auger electrons in the same order.  With
+ * density_out / temperature_out (device, gnz x gnr, may alias the gas arrays)
+ * the gas loses the ionized molecules and their thermal energy
+ * (INCIDENCE1.m:87-92).  The reference rounds per (r, phi, z) DSMC cell; this
+ * grid has no phi cells and rounds per (r, z) cell.  Synchronizes; returns
+ * GASX_E_CAPACITY (with the result filled) when electrons or ions were dropped. */
+typedef struct gasx_photo_source {
+    double photon_kev;          /* Ex (main3.m:26) */
+    double n_photons;           /* photons entering the wedge (main3.m:28) */
+    double beam_radius;         /* laser_r (main3.m:25), m */
+    const double* binding_ev;   /* host: shell binding energies in file order (EB_14.csv), eV */
+    int32_t n_shells;
+    const double* mac;          /* host: n_mac x 2 row-major (14_MAC.csv: MeV ; cm^2/g), energies increasing */
+    int32_t n_mac;
+    double atomic_mass;         /* m_a (main3.m:27) */
+    double auger_kev;           /* 0.383 (INCIDENCE1.m:130) */
+    uint64_t seed;              /* Philox key */
+    uint32_t pulse, run;        /* Philox counter words 1 and 2 */
+    float* density_out;         /* device or NULL (no depletion) */
+    float* temperature_out;     /* device, with density_out */
+} gasx_photo_source;
+
+typedef struct gasx_photo_result {
+    double photons_left;        /* No_photon (INCIDENCE1.m:84) */
+    int64_t n_events;           /* photoionized molecules (excited_molecule_number) */
+    int64_t n_electrons, n_ions;                  /* appended */
+    int64_t n_dropped_electrons, n_dropped_ions;  /* beyond capacity */
+    double sigma_pe;            /* m^2 (findSigmaPE.m:22) */
+    double electron_kev;        /* photoelectron energy Ex - E_shell */
+    int32_t auger;              /* 1: inner shell, an Auger electron per event */
+    int32_t n_rings;            /* beam rings on the gas grid */
+} gasx_photo_result;
+
+gasx_status gasx_photoionize(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, const gasx_gas* g,
+                             const gasx_photo_source* src, gasx_photo_result* out);
+
 /* ---- Differentiable run and its adjoint ------------------------------------
  * No counterpart in the reference, which has no gradients: these entry points
  * serve callers that fit the gas profile or the source to an observed field

@@ -851,3 +851,188 @@ int orc_run(const orc_params* p, const orc_tables* t, const orc_gas* g,
     free(jr); free(jz); free(q); free(m); free(ev); free(sv);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Photoionization source: INCIDENCE1.m:27-158                           */
+/* ------------------------------------------------------------------ */
+#define ORC_SRC_BLOCK 0x40000000u
+
+static void src_block(uint64_t seed, uint32_t e, uint32_t pulse, uint32_t run, uint32_t b, double u[4])
+{
+    uint32_t key[2] = { (uint32_t)seed, (uint32_t)(seed >> 32) };
+    uint32_t ctr[4] = { e, pulse, run, ORC_SRC_BLOCK + b }, o[4];
+    orc_philox4x32_10(ctr, key, o);
+    for (int i = 0; i < 4; ++i) u[i] = orc_u01(o[i]);
+}
+
+/* findV.m:5-8 */
+static double find_v(double ke)
+{
+    double c = 3e8, m = 9.109e-31;
+    double gm = 1.0 + ke / (m * c * c);
+    return c * sqrt(1.0 - 1.0 / (gm * gm));
+}
+
+/* findPEangle.m:6-14 constants for electron kinetic energy dE (J), m_e = 9.11e-31 */
+static void sauter_consts(double dE, double* A, double* bg, double* g0)
+{
+    double erest = 9.11e-31 * 3e8 * 3e8;
+    double gm = 1.0 + dE / erest;
+    double beta = sqrt(dE * (dE + 2 * erest)) / (dE + erest);
+    *A = 1.0 / beta - 1.0;
+    *bg = 0.5 * beta * gm * (gm - 1.0) * (gm - 2.0);
+    *g0 = 2.0 * (1.0 / *A + *bg);
+}
+
+/* findPEangle.m:16-33: proposals r1 -> v (inverse of v / (A + v)^3 on [0, 2]),
+ * accepted when r2 g0 < g(v); the last of 128 proposals if none is */
+static double sauter_v(double A, double bg, double g0, uint64_t seed, uint32_t e, uint32_t pulse, uint32_t run)
+{
+    double v = 0.0;
+    for (int b = 0; b < 64; ++b) {
+        double u[4];
+        src_block(seed, e, pulse, run, 3u + (uint32_t)b, u);
+        for (int h = 0; h < 2; ++h) {
+            double r1 = u[2 * h], r2 = u[2 * h + 1];
+            double d = (A + 2.0) * (A + 2.0) - 4.0 * r1;
+            v = 2.0 * A / d * (2.0 * r1 + (A + 2.0) * sqrt(r1));
+            double gv = (2.0 - v) * (1.0 / (A + v) + bg);
+            if (r2 * g0 < gv) return v;
+        }
+    }
+    return v;
+}
+
+double orc_sauter_v(double dE, uint64_t seed, uint32_t event, uint32_t pulse, uint32_t run)
+{
+    double A, bg, g0;
+    sauter_consts(dE, &A, &bg, &g0);
+    return sauter_v(A, bg, g0, seed, event, pulse, run);
+}
+
+static void thermal(uint64_t seed, uint32_t e, uint32_t pulse, uint32_t run, double n[3])
+{
+    double u[4];
+    src_block(seed, e, pulse, run, 1u, u);
+    double twopi = 6.283185307179586;
+    double r1 = sqrt(-2.0 * log(u[0])), r2 = sqrt(-2.0 * log(u[2]));
+    n[0] = r1 * cos(twopi * u[1]);
+    n[1] = r1 * sin(twopi * u[1]);
+    n[2] = r2 * cos(twopi * u[3]);
+}
+
+int orc_photoionize(const orc_params* p, const orc_gas* g, double* dens, double* temp, int deplete,
+                    const orc_photo_source* s, int64_t max_events, double* pe, double* ae, double* ion,
+                    double* vth, int32_t* cell, orc_photo_info* info)
+{
+    /* findEShell.m:4-12 */
+    int shell = -1;
+    for (int i = 0; i < s->n_shells; ++i)
+        if (s->photon_kev >= s->binding_ev[i] / 1000.0) { shell = i; break; }
+    if (shell < 0) return -1;
+    double e_shell = s->binding_ev[shell] / 1000.0;
+    int auger = shell != s->n_shells - 1;                       /* INCIDENCE1.m:124 */
+    /* findSigmaPE.m:18-22 */
+    double xs[256], ys[256];
+    if (s->n_mac < 2 || s->n_mac > 256) return -1;
+    for (int i = 0; i < s->n_mac; ++i) { xs[i] = s->mac[2 * i]; ys[i] = s->mac[2 * i + 1]; }
+    double mac = orc_interp1(xs, ys, s->n_mac, s->photon_kev / 1000.0);
+    if (!(mac >= 0)) return -1;
+    double sigma = mac * s->atomic_mass / 6.02e23 * 1e-4;
+    /* GUASS.m:3-15 with the integral of x exp(-x^2/(2 s^2)) in closed form */
+    double sg = s->beam_radius / 3.0;
+    int nr = (int)lround(s->beam_radius / g->gdr);
+    if (nr > g->gnr) nr = g->gnr;
+    if (nr > 4096) nr = 4096;
+    if (nr < 1) nr = 1;
+    double* N = (double*)malloc((size_t)nr * sizeof(double));
+    double tot = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        double a0 = i * g->gdr, a1 = (i + 1) * g->gdr;
+        N[i] = sg * sg * (exp(-a0 * a0 / (2 * sg * sg)) - exp(-a1 * a1 / (2 * sg * sg)));
+        tot += N[i];
+    }
+    for (int i = 0; i < nr; ++i) N[i] = s->n_photons * (N[i] / tot);
+    double dE = (s->photon_kev - e_shell) * 1.6e-16;             /* INCIDENCE1.m:28 */
+    double A, bg, g0;
+    sauter_consts(dE, &A, &bg, &g0);
+    double v_pe = find_v(dE), v_ae = find_v(s->auger_kev * 1.6e-16);
+    double p_ph = s->photon_kev * 1.6e-16 / 3e8;                 /* :113 */
+    double m_e = 9.11e-31, m_mol = 2.0 * s->atomic_mass * 1.6726e-27, kb = 1.38064853e-23;
+    double twopi = 6.283185307179586, pi = 3.141592653589793;
+    int ncell = nr * g->gnz;
+    int32_t* cnt = (int32_t*)malloc((size_t)ncell * sizeof(int32_t));
+    /* INCIDENCE1.m:51-60: Beer-Lambert down the slabs, per ring */
+    for (int k = 0; k < g->gnz; ++k)
+        for (int i = 0; i < nr; ++i) {
+            double n = dens[k * g->gnr + i];
+            double dN = n * N[i];
+            dN = dN * sigma;
+            dN = dN * g->gdz;
+            N[i] = N[i] - dN;
+            cnt[k * nr + i] = (int32_t)round(dN / p->fn);
+        }
+    int64_t total = 0;
+    for (int c = 0; c < ncell; ++c) total += cnt[c];
+    if (total > max_events) { free(N); free(cnt); return -2; }
+    int64_t e = 0;
+    for (int c = 0; c < ncell; ++c) {
+        int ring = c % nr, slab = c / nr;
+        double T = g->temp ? temp[slab * g->gnr + ring] : g->t_scalar;
+        double sv = sqrt(kb * T / m_mol);
+        double s2 = 0.0;
+        for (int j = 0; j < cnt[c]; ++j, ++e) {
+            double u[4], nn[3];
+            src_block(s->seed, (uint32_t)e, s->pulse, s->run, 0u, u);
+            double r = ((double)ring + u[0]) * g->gdr;                  /* :67 */
+            double phi = u[1] * p->phip;                                /* :68 */
+            double z = ((double)slab + u[2]) * g->gdz;                  /* :62 */
+            double chi = twopi * u[3];                                  /* :102 */
+            double x = r * cos(phi), y = r * sin(phi);
+            double v = sauter_v(A, bg, g0, s->seed, (uint32_t)e, s->pulse, s->run);
+            double ct = 1.0 - v, st = sqrt(v * (2.0 - v));
+            double pvx = v_pe * st * cos(chi), pvy = v_pe * st * sin(chi), pvz = v_pe * ct;
+            thermal(s->seed, (uint32_t)e, s->pulse, s->run, nn);
+            double tx = sv * nn[0], ty = sv * nn[1], tz = sv * nn[2];
+            s2 = s2 + ((tx * tx + ty * ty) + tz * tz);
+            double mx = m_mol * tx - m_e * pvx;
+            double my = m_mol * ty - m_e * pvy;
+            double mz = p_ph + m_mol * tz - m_e * pvz;                  /* :126-128 */
+            if (auger) {                                                /* :130-154 */
+                double ua[4];
+                src_block(s->seed, (uint32_t)e, s->pulse, s->run, 2u, ua);
+                double th = pi * ua[0], ch = twopi * ua[1];
+                double ax = v_ae * sin(th) * cos(ch), ay = v_ae * sin(th) * sin(ch), az = v_ae * cos(th);
+                mx = mx - m_e * ax; my = my - m_e * ay; mz = mz - m_e * az;
+                ae[e] = x; ae[max_events + e] = y; ae[2 * max_events + e] = z;
+                ae[3 * max_events + e] = ax; ae[4 * max_events + e] = ay; ae[5 * max_events + e] = az;
+            }
+            pe[e] = x; pe[max_events + e] = y; pe[2 * max_events + e] = z;
+            pe[3 * max_events + e] = pvx; pe[4 * max_events + e] = pvy; pe[5 * max_events + e] = pvz;
+            ion[e] = x; ion[max_events + e] = y; ion[2 * max_events + e] = z;
+            ion[3 * max_events + e] = mx / m_mol; ion[4 * max_events + e] = my / m_mol;
+            ion[5 * max_events + e] = mz / m_mol;
+            vth[e] = tx; vth[max_events + e] = ty; vth[2 * max_events + e] = tz;
+            cell[e] = c;
+        }
+        if (deplete && cnt[c] > 0) {                                   /* :87-92 */
+            double V = p->phip / 2.0 * ((double)(ring + 1) * (ring + 1) - (double)ring * ring) * g->gdr * g->gdr * g->gdz;
+            int gi = slab * g->gnr + ring;
+            double Nc = dens[gi] * V;
+            double Nn = Nc - p->fn * (double)cnt[c];
+            double E = 1.5 * kb * Nc * T - 0.5 * p->fn * m_mol * s2;
+            if (Nn > 0.0) { dens[gi] = Nn / V; temp[gi] = E / (1.5 * kb * Nn); }
+            else { dens[gi] = 0.0; temp[gi] = T; }
+        }
+    }
+    double left = 0.0;
+    for (int i = 0; i < nr; ++i) left += N[i];
+    info->photons_left = left;
+    info->sigma_pe = sigma;
+    info->electron_kev = s->photon_kev - e_shell;
+    info->auger = auger;
+    info->n_rings = nr;
+    info->n_events = total;
+    free(N); free(cnt);
+    return 0;
+}

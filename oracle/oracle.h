@@ -128,6 +128,35 @@ void orc_collide(const orc_params* p, const orc_tables* t, const orc_gas* g,
                  int8_t* ev, double* sec_v /* 3*n: secondary velocity */,
                  double* ion_v /* 3*n or NULL */);
 
+/* ---- photoionization source (INCIDENCE1.m:27-158, GUASS.m:3-15, findEShell.m:1-12,
+ *      findSigmaPE.m:8-22, findPEangle.m:10-33, findV.m:5-8) ---- */
+typedef struct {
+    double photon_kev, n_photons, beam_radius;
+    const double* binding_ev; int32_t n_shells;   /* eV, file order */
+    const double* mac; int32_t n_mac;             /* n_mac x 2: MeV ; cm^2/g */
+    double atomic_mass, auger_kev;
+    uint64_t seed; uint32_t pulse, run;
+} orc_photo_source;
+
+typedef struct {
+    double photons_left, sigma_pe, electron_kev;
+    int32_t auger, n_rings;
+    int64_t n_events;
+} orc_photo_info;
+
+/* Events in emission order (slab by slab, rings outward).  pe, ae, ion: 6 x
+ * max_events (x, y, z, vx, vy, vz rows of length max_events); ae unused
+ * without an inner shell; cell[e] = slab * n_rings + ring; vth: 3 x max_events
+ * thermal velocities of the ions (before the recoil).  With deplete, dens and
+ * temp (gnz x gnr) are updated in place (INCIDENCE1.m:87-92).  Returns 0, or
+ * -1 (bad input) / -2 (more events than max_events). */
+int orc_photoionize(const orc_params* p, const orc_gas* g, double* dens, double* temp, int deplete,
+                    const orc_photo_source* s, int64_t max_events, double* pe, double* ae, double* ion,
+                    double* vth, int32_t* cell, orc_photo_info* info);
+/* Sauter sample v = 1 - cos(theta) of a photoelectron of kinetic energy dE (J)
+ * from the proposals (r1, r2) pairs of the source's blocks (findPEangle.m:16-33) */
+double orc_sauter_v(double dE_joule, uint64_t seed, uint32_t event, uint32_t pulse, uint32_t run);
+
 /* ---- tallies (electricmagneticfield5.m:167-177) ---- */
 void orc_tallies(const orc_params* p, const double* er, const double* ez, const double* h,
                  const double* vx, const double* vy, const double* vz, const double* m,

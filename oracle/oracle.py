@@ -53,6 +53,18 @@ class OrcCounts(C.Structure):
                 ("n_dropped", C.c_int64)]
 
 
+class OrcPhotoSource(C.Structure):
+    _fields_ = [("photon_kev", C.c_double), ("n_photons", C.c_double), ("beam_radius", C.c_double),
+                ("binding_ev", _D), ("n_shells", C.c_int32), ("mac", _D), ("n_mac", C.c_int32),
+                ("atomic_mass", C.c_double), ("auger_kev", C.c_double), ("seed", C.c_uint64),
+                ("pulse", C.c_uint32), ("run", C.c_uint32)]
+
+
+class OrcPhotoInfo(C.Structure):
+    _fields_ = [("photons_left", C.c_double), ("sigma_pe", C.c_double), ("electron_kev", C.c_double),
+                ("auger", C.c_int32), ("n_rings", C.c_int32), ("n_events", C.c_int64)]
+
+
 _lib = None
 
 
@@ -99,6 +111,11 @@ def lib() -> C.CDLL:
         L.orc_tables_build.argtypes = [C.POINTER(OrcTables), _P, C.c_int, _P, C.c_int, _P, C.c_int,
                                        _P, C.c_int, _P, C.c_int, _P, C.c_int, _P, C.c_int]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_photoionize.argtypes = [C.POINTER(OrcParams), C.POINTER(OrcGas), _P, _P, C.c_int,
+                                      C.POINTER(OrcPhotoSource), C.c_int64, _P, _P, _P, _P, _P,
+                                      C.POINTER(OrcPhotoInfo)]
+        L.orc_sauter_v.restype = C.c_double
+        L.orc_sauter_v.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = None
         L.orc_get_threads.restype = C.c_int
@@ -269,3 +286,31 @@ def tallies(p, er, ez, h, st: State):
     lib().orc_tallies(C.byref(p), ptr(a(er)), ptr(a(ez)), ptr(a(h)), ptr(st.v[3]), ptr(st.v[4]),
                       ptr(st.v[5]), ptr(st.m), ptr(st.dele), st.n, ptr(out))
     return out
+
+
+def photoionize(p, dens, gdr, gdz, photon_kev, n_photons, beam_radius, tables, temp=None, t_scalar=300.0,
+                deplete=False, seed=0, pulse=0, run=0, auger_kev=0.383, atomic_mass=14.0, max_events=None):
+    """INCIDENCE1.m restated (orc_photoionize).  dens / temp: (gnz, gnr) FP64
+    grids (copied; the depleted grids are returned).  Returns a dict with the
+    per-event arrays pe, ae, ion (6 x n), vth (3 x n), cell (n) and the info."""
+    d = np.array(np.atleast_2d(dens), dtype=np.float64)
+    t = np.array(np.atleast_2d(temp), dtype=np.float64) if temp is not None else np.full_like(d, t_scalar)
+    g = OrcGas(d.ctypes.data_as(_D), t.ctypes.data_as(_D) if (temp is not None or deplete) else None,
+               d.shape[1], d.shape[0], gdr, gdz, t_scalar)
+    be = a(tables["binding_ev"]).ravel()
+    mac = a(tables["mac"])
+    s = OrcPhotoSource(photon_kev, n_photons, beam_radius, be.ctypes.data_as(_D), be.size,
+                       mac.ctypes.data_as(_D), mac.shape[0], atomic_mass, auger_kev, seed, pulse, run)
+    m = int(max_events or 1 << 20)
+    pe, ae, ion = (np.zeros((6, m)) for _ in range(3))
+    vth = np.zeros((3, m))
+    cell = np.zeros(m, dtype=np.int32)
+    info = OrcPhotoInfo()
+    rc = lib().orc_photoionize(C.byref(p), C.byref(g), ptr(d), ptr(t), int(deplete), C.byref(s), m, ptr(pe),
+                               ptr(ae), ptr(ion), ptr(vth), ptr(cell), C.byref(info))
+    if rc != 0:
+        raise ValueError(f"orc_photoionize failed ({rc})")
+    n = info.n_events
+    return {"pe": pe[:, :n], "ae": ae[:, :n] if info.auger else None, "ion": ion[:, :n], "vth": vth[:, :n],
+            "cell": cell[:n], "dens": d, "temp": t,
+            "info": {k: getattr(info, k) for k, _ in OrcPhotoInfo._fields_}}
