@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-source", action="store_true", help="skip the photoionization-source leg")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-tuned-sort", action="store_true",
@@ -169,7 +170,7 @@ def cell_sorted(host, w):
     return tuple(a[order] for a in host)
 
 
-def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
+def bench_batched(seed: int, steps: int | None = None, sequential: bool = True, secondaries: bool = True):
     """Batched ensemble (64 runs of 128x256, log-uniform density and electron
     count, collisions and secondaries on with capacity 2x) in one captured graph with the
     autotuned launch shapes, against the same runs one after another with
@@ -184,7 +185,7 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     w = WORKLOADS["batched_128x256"]
     n_steps = steps or w.n_steps
     counts, dens = batched_runs(w, seed)
-    caps = [2 * int(n) for n in counts]               # secondaries on, growth capped at 2x
+    caps = [(2 if secondaries else 1) * int(n) for n in counts]   # growth capped at 2x
     p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
     ctx = gasx.Context(p, device=0)
     runs = [cell_sorted(seed_electrons(int(n), w.rp, w.lp, w.phip, w.sigma_r, seed=seed + 100 + r), w)
@@ -202,7 +203,7 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
         ens.counters(reset=True)
         torch.cuda.synchronize()
 
-    kw = dict(collide=True, secondaries=True, seed=seed + 7, run_id=0)
+    kw = dict(collide=True, secondaries=secondaries, seed=seed + 7, run_id=0)
     tuning = ens.autotune(trial_steps=5, **kw)
     reset()
     ens.run(n_steps, use_graph=True, **kw)            # capture + instantiate + first replay
@@ -214,8 +215,8 @@ def bench_batched(seed: int, steps: int | None = None, sequential: bool = True):
     per_run = ens.counters()
     psteps_b = sum(c.n_pushed for c in per_run)
     out = {"value": psteps_b / wall_b, "unit": "particle-steps/s", "runs": len(counts),
-           "steps": n_steps, "electrons": int(counts.sum()), "capacity_factor": 2.0,
-           "secondaries": True, "wall_s": wall_b, "tuning": tuning,
+           "steps": n_steps, "electrons": int(counts.sum()), "capacity_factor": 2.0 if secondaries else 1.0,
+           "secondaries": secondaries, "wall_s": wall_b, "tuning": tuning,
            "graph": True, "grid": f"{w.nr}x{w.nz}"}
     del ens, init
     torch.cuda.empty_cache()
@@ -305,6 +306,49 @@ def bench_adjoint(seed: int, steps: int | None = None):
             "production_forward_s": t_prod, "backward_over_production_forward": t_bwd / t_prod,
             "forward_plus_adjoint_over_production_forward": (t_fwd + t_bwd) / t_prod,
             "objective_J": wv, "grad_amplitude": ga, "grad_nodes_norm": float(np.linalg.norm(g))}
+
+
+def bench_photoionization(seed: int):
+    """The photoionization source (gasx_photoionize) on the 256x512 workload's
+    grid: a 1 keV pulse, Gaussian beam of sigma 50 um (beam radius 150 um),
+    uniform N2 at 2.5e25 m^-3 and 300 K, FN = 1, gas depletion on; the photon
+    number gives about 2e6 photoionizations (a photoelectron and an Auger
+    electron each, and an N2++ ion).  Timed over 5 pulses into fresh
+    populations."""
+    import numpy as np
+    import torch
+
+    import gasx
+    from gasx.core import M_ION
+    from gasx.tables import source_tables
+    from gasx.workloads import WORKLOADS
+
+    w = WORKLOADS["plain_gpu_256x512"]
+    p = gasx.make_params(w.nr, w.nz, w.dr, w.dz, w.phip, w.ddt, w.n_sub, fn=1.0)
+    ctx = gasx.Context(p, device=0)
+    tabs = source_tables()
+    n_photons = 2.45e6
+    e = gasx.ParticleSet(5_000_000)
+    ions = gasx.ParticleSet(2_500_000, qnum=2.0, mass=M_ION)
+    dens0 = np.full((w.nz, w.nr), w.density, np.float32)
+    gas = gasx.GasGrid(dens0, w.dr, w.dz, temperature=np.full_like(dens0, 300.0))
+    kw = dict(ions=ions, deplete=True, tables=tabs)
+    ctx.photoionize(e, gas, 1.0, n_photons, 150e-6, seed=seed, **kw)       # warm
+    walls, res = [], None
+    for k in range(5):
+        e.count_t.zero_()
+        ions.count_t.zero_()
+        gas.density.copy_(torch.from_numpy(dens0))
+        gas.temperature.fill_(300.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = ctx.photoionize(e, gas, 1.0, n_photons, 150e-6, seed=seed, pulse=k, **kw)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.median(walls))
+    return {"events": res["n_events"], "electrons": res["n_electrons"], "ions": res["n_ions"],
+            "photons_left_fraction": res["photons_left"] / n_photons, "sigma_pe_m2": res["sigma_pe"],
+            "wall_s": wall, "events_per_s": res["n_events"] / wall,
+            "note": "median of 5 pulses, host call to return (result read back included)"}
 
 
 def main():
@@ -489,9 +533,17 @@ def main():
         del e, f, host_t
         torch.cuda.empty_cache()
         line["batched"] = bench_batched(args.seed)
+        torch.cuda.empty_cache()
+        nb = bench_batched(args.seed, secondaries=False)
+        line["batched"]["without_secondaries"] = {
+            k: nb[k] for k in ("value", "wall_s", "sequential_value", "sequential_wall_s", "speedup_vs_sequential",
+                               "identical_events_to_sequential", "tuning")}
     if not args.no_adjoint:
         torch.cuda.empty_cache()
         line["adjoint"] = bench_adjoint(args.seed)
+    if not args.no_source:
+        torch.cuda.empty_cache()
+        line["photoionization"] = bench_photoionization(args.seed)
     print(json.dumps(line))
 
 
