@@ -64,6 +64,22 @@ def build_library(jobs: int = 4) -> Path:
     return LIB_PATH
 
 
+SHIM = ROOT / "shim"
+DRIVER_PATH = SHIM / "transport_driver"
+ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
+
+
+def build_shim() -> Path:
+    """The C driver of the ABI (shim/transport_driver.c over shim/gasx_shim.c),
+    plain C against libgasx.so and the HIP runtime.  The MEX gateway
+    (shim/gasx_mex.c) needs MATLAB's mex and is not built here."""
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{ROCM / 'include'}",
+          str(SHIM / "gasx_shim.c"), str(SHIM / "transport_driver.c"),
+          f"-L{PKG_DIR}", "-lgasx", f"-L{ROCM / 'lib'}", "-lamdhip64", "-lm",
+          "-Wl,-rpath,$ORIGIN/../gasx", f"-Wl,-rpath,{ROCM / 'lib'}", "-o", str(DRIVER_PATH)])
+    return DRIVER_PATH
+
+
 def build_oracle() -> Path:
     """Compile the FP64 CPU oracle (test infrastructure) with its Makefile."""
     oracle_dir = REPO / "oracle"
@@ -77,4 +93,5 @@ def clean() -> None:
 
 if __name__ == "__main__":
     print(build_library())
+    print(build_shim())
     print(build_oracle())

@@ -1,0 +1,84 @@
+/*
+ * gasx_shim.h -- host-array glue between the reference's MATLAB call surface
+ * and the C ABI of include/gasx.h.
+ *
+ * The reference's GPU work is reached from four MATLAB functions, each
+ * called once per transport step or pulse:
+ *   INCIDENCE1(...)             main3.m:134                     -> gasx_shim_photoionize
+ *   electricmagneticfield6(...) electron_transport_field3.m:68  -> gasx_shim_em_step
+ *   find_CS0(...)               electron_transport_field3.m:108 -> gasx_shim_find_cs
+ *   electron_collision(...)     electron_transport_field3.m:109 -> gasx_shim_collide
+ * Their arguments are host double arrays: particles as N x 3 column-major
+ * matrices (x, y, z columns: three arrays of N doubles), fields in the
+ * reference's own layout.  The shim keeps the three species (electrons,
+ * N2+ from electron-impact ionization, N2++ from photoionization;
+ * electricmagneticfield6.m:65-69) and the fields resident on the device
+ * between calls; the set / get calls move them to and from host arrays.
+ * gasx_mex.c maps MATLAB's mxArrays onto these calls; transport_driver.c
+ * replays main3.m / electron_transport_field3.m with them from C.
+ *
+ * Plain C, host only; every call returns a gasx_status.
+ */
+#ifndef GASX_SHIM_H
+#define GASX_SHIM_H
+
+#include <stdint.h>
+
+#include "gasx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gasx_shim gasx_shim;
+
+enum { GASX_SHIM_ELECTRONS = 0, GASX_SHIM_ION1 = 1, GASX_SHIM_ION2 = 2 };
+
+/* Loads the N2 tables from data_dir (the reference's CSV files, read as
+ * electron_transport_field3.m:8-37 reads them), creates the context and the
+ * device buffers: capacity[3] slots per species, a gnr x gnz gas grid of
+ * cells gdr x gdz. */
+gasx_status gasx_shim_create(int device, const gasx_params* p, const char* data_dir, const int64_t capacity[3],
+                             int32_t gnr, int32_t gnz, double gdr, double gdz, gasx_shim** out);
+void gasx_shim_destroy(gasx_shim* sh);
+gasx_ctx* gasx_shim_context(gasx_shim* sh);
+
+/* Host -> device: n particles of a species, Cartesian positions and
+ * velocities (doubles; stored as FP32). */
+gasx_status gasx_shim_set_particles(gasx_shim* sh, int species, const double* x, const double* y, const double* z,
+                                    const double* vx, const double* vy, const double* vz, int64_t n);
+/* Device -> host: the live particles of a species in slot order (absorbed
+ * ones removed, electricmagneticfield6.m:176-178; the device copy is
+ * compacted the same way).  The arrays stay valid until the next call. */
+gasx_status gasx_shim_get_particles(gasx_shim* sh, int species, const double** x, const double** y,
+                                    const double** z, const double** vx, const double** vy, const double** vz,
+                                    int64_t* n);
+/* Fields in the reference layout (Er nr x (nz+1), Ez (nr+1) x nz, Hphi nr x nz, r fastest). */
+gasx_status gasx_shim_set_fields(gasx_shim* sh, const double* er, const double* ez, const double* hphi);
+gasx_status gasx_shim_get_fields(gasx_shim* sh, double* er, double* ez, double* hphi);
+/* Gas density (m^-3) and temperature (K) per gas cell, gnz x gnr, iz * gnr + ir. */
+gasx_status gasx_shim_set_gas(gasx_shim* sh, const double* density, const double* temperature);
+gasx_status gasx_shim_get_gas(gasx_shim* sh, double* density, double* temperature);
+
+/* INCIDENCE1.m:27-158: photoelectrons and Auger electrons to the
+ * electrons, N2++ to ion2, the gas depleted.  ex_kev, n_photons and
+ * laser_r as main3.m:25-28; the binding energies and MAC table from data_dir. */
+gasx_status gasx_shim_photoionize(gasx_shim* sh, double ex_kev, double n_photons, double laser_r, uint64_t seed,
+                                  uint32_t pulse, gasx_photo_result* result);
+/* electricmagneticfield6.m:138-188: n_push pushes of all three species
+ * with the fields. */
+gasx_status gasx_shim_em_step(gasx_shim* sh, int n_push);
+/* find_CS0.m:1-61 on host energies (keV). */
+gasx_status gasx_shim_find_cs(gasx_shim* sh, const double* e_kev, int64_t n, double* sig_el, double* sig_ion,
+                              double* sig_exc);
+/* electron_collision.m:1-75: one collision check of duration dt for every
+ * electron; secondaries appended to the electrons, N2+ to ion1. */
+gasx_status gasx_shim_collide(gasx_shim* sh, double dt, uint64_t seed, uint32_t step);
+/* Kinetic energies (keV) of the live electrons in slot order, as
+ * electricmagneticfield6.m:172 returns them (the push's constants). */
+gasx_status gasx_shim_electron_kev(gasx_shim* sh, const double** e_kev, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GASX_SHIM_H */

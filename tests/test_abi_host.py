@@ -126,3 +126,29 @@ def test_ring_density_collapses_phi_cells():
     assert np.allclose(out, [[0, 1.5, 4], [6, 7.5, 10]])
     with pytest.raises(ValueError):
         ring_density(vals, sum_cell, 7, 2)
+
+
+DRIVER = Path(__file__).resolve().parents[1] / "gas-atenuator_amd" / "shim" / "transport_driver"
+
+
+def test_transport_driver_argument_and_table_errors(tmp_path):
+    """The C driver's failures that come before any HIP call: usage, a
+    missing data directory, a table of the wrong shape."""
+    import shutil
+    import subprocess
+    from gasx.tables import DATA_DIR
+    assert DRIVER.exists(), "build() makes the driver"
+    r = subprocess.run([str(DRIVER)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    args = ["64", "128", "3", "1e6", "2", "2", "1"]
+    r = subprocess.run([str(DRIVER), str(tmp_path / "none"), str(tmp_path / "o.bin")] + args,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "create: invalid argument" in r.stderr
+    bad = tmp_path / "n2"
+    shutil.copytree(DATA_DIR, bad)
+    lines = (bad / "ElasticCS_Nitrogen.csv").read_text().replace("\r\n", "\n").replace("\r", "\n").splitlines()
+    (bad / "ElasticCS_Nitrogen.csv").write_text("\n".join(lines[:-1]) + "\n")   # 181 rows, not 182
+    r = subprocess.run([str(DRIVER), str(bad), str(tmp_path / "o.bin")] + args, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "create: invalid argument" in r.stderr
+    assert not (tmp_path / "o.bin").exists()
