@@ -369,7 +369,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
         chk(dalloc(&ctx->base_tmp, 4), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
-        chk(hipHostMalloc((void**)&ctx->h_count, sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
+        chk(hipHostMalloc((void**)&ctx->h_count, kSortInfoWords * sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
         if (st == GASX_OK) {
             chk(hipMemcpy(ctx->d_tab, &dt, sizeof(dt), hipMemcpyHostToDevice), "upload tables");
             chk(hipMemset(ctx->d_counters, 0, (size_t)kNCounters * kCounterSlots * sizeof(unsigned long long)),

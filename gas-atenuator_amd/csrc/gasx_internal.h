@@ -248,9 +248,11 @@ hipError_t launch_tallies(const gasx_params& p, const double* er, const double* 
                           hipStream_t s);
 // sorted_out non-null: the permuted arrays are left in the scratch (their six
 // pointers returned) instead of being copied back into s
-// pinned_count (optional): a pinned host word the slot count is read through
+// pinned_info (optional): kSortInfoWords pinned host words the slot count
+// and the key descents are read through
+constexpr int kSortInfoWords = 2 + 4096;
 hipError_t launch_sort(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes,
                        size_t* need_bytes, hipStream_t st, float** sorted_out = nullptr,
-                       int32_t* pinned_count = nullptr);
+                       int32_t* pinned_info = nullptr);
 
 }  // namespace gasx

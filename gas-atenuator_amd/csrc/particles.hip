@@ -1040,11 +1040,16 @@ __device__ __forceinline__ int batch_segment(const int* pre, int e)
     return lo;
 }
 
-// Workgroup b copies the entries of staged segments [b*kBlock, (b+1)*kBlock):
-// a batch lies inside one scan group (kScanGroup is a multiple of kBlock), so
-// its destinations are contiguous, from the first segment's offset on; every
-// thread copies whole entries (six floats), consecutive threads consecutive
-// destinations.
+// kAppendSplit workgroups share a batch of kBlock staged segments: workgroup
+// b copies every kAppendSplit-th run of kBlock entries of batch b / kAppendSplit,
+// from run b % kAppendSplit on.  A batch lies inside one scan group
+// (kScanGroup is a multiple of kBlock), so its destinations are contiguous,
+// from the first segment's offset on; every thread copies whole entries (six
+// floats), consecutive threads consecutive destinations.  One workgroup per
+// batch left a few hundred workgroups, each waiting on its own loads, for a
+// copy of a few hundred thousand entries.
+constexpr int kAppendSplit = 8;
+
 __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, float* d2, float* d3,
                                                         float* d4, float* d5, int32_t cap,
                                                         int32_t* dst_count,
@@ -1064,8 +1069,9 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
     const int nseg = tmp2[1];
     // full: everything staged is dropped; only workgroup 0 (the count and the
     // dropped tally) has work
+    const int part = (int)blockIdx.x % kAppendSplit;
     if (base >= cap && blockIdx.x != 0) return;
-    if ((int)blockIdx.x * kBlock >= nseg && blockIdx.x != 0) return;
+    if ((int)blockIdx.x / kAppendSplit * kBlock >= nseg && blockIdx.x != 0) return;
     // exclusive prefix of the group totals (four per thread)
     {
         constexpr int kG = kMaxScanGroups / kBlock;
@@ -1095,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
         __syncthreads();
     }
     if (base >= cap) return;                       // full: everything staged was dropped
-    for (int bt = blockIdx.x; bt * kBlock < nseg; bt += gridDim.x) {   // workgroup-uniform
+    for (int bt = (int)blockIdx.x / kAppendSplit; bt * kBlock < nseg; bt += (int)gridDim.x / kAppendSplit) {
         const int c0 = bt * kBlock;
         const int c = c0 + (int)threadIdx.x;
         const int cnt = c < nseg ? stage_cnt[c] : 0;
@@ -1106,7 +1112,7 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
         const int first = base + s_pre[c0 / kScanGroup] + offs[c0];
         __syncthreads();
         if (first < cap) {
-            for (int e = (int)threadIdx.x; e < tot; e += kBlock) {
+            for (int e = part * kBlock + (int)threadIdx.x; e < tot; e += kBlock * kAppendSplit) {
                 const int dst = first + e;
                 if (dst >= cap) break;
                 const int j = batch_segment(s_seg, e);
@@ -1132,9 +1138,10 @@ hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                        stage_cnt, scan_tmp, totals, tmp2, max_segs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // a workgroup per kBlock staged segments (those past the staged count return at once)
-    const int blocks = (max_segs + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_append_copy, dim3(blocks > 0 ? blocks : 1), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
+    // kAppendSplit workgroups per kBlock staged segments (those past the
+    // staged count return at once)
+    const int blocks = (max_segs + kBlock - 1) / kBlock * kAppendSplit;
+    hipLaunchKernelGGL(k_append_copy, dim3(blocks > 0 ? blocks : kAppendSplit), dim3(kBlock), 0, s, dst[0], dst[1], dst[2],
                        dst[3], dst[4], dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3],
                        stage[4], stage[5], stage_cnt, scan_tmp, totals, ngroups, tmp2, dropped);
     return hipGetLastError();

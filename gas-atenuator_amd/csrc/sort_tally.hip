@@ -30,46 +30,86 @@ __device__ __forceinline__ uint32_t cell_key(const Geom& g, float x, float y, fl
 }
 
 // Four consecutive slots per thread: float4 loads where the arrays allow,
-// uint4 stores into the (256-byte aligned) sort scratch.
-__global__ void k_sort_keys(Geom g, const float* x, const float* y, const float* z, int32_t n,
-                            uint32_t* keys, uint32_t* vals)
+// uint4 stores into the (256-byte aligned) sort scratch.  The slot count is
+// read on the device (info[0] <- n).  Workgroup b writes to info[2 + b] the
+// descents of the key sequence among its slots (slot i in [1, n) with
+// key[i] < key[i-1]): their sum tells the host how far from cell order the
+// slots are.  No atomics: every workgroup writes its own word.
+constexpr int kSortKeyBlocks = kSortInfoWords - 2;
+
+__global__ __launch_bounds__(256) void k_sort_keys(Geom g, const float* x, const float* y, const float* z,
+                                                   const int32_t* count, int32_t cap, uint32_t* keys,
+                                                   uint32_t* vals, int32_t* info)
 {
+    const int n = *count < cap ? *count : cap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) info[0] = n;
     const bool aligned = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y) |
                            reinterpret_cast<uintptr_t>(z)) & 15) == 0;
+    const int lane = threadIdx.x & 63;
     const int ng = (n + 3) >> 2;
+    int desc = 0;
+    // a wave covers 64 consecutive groups of four slots, so all its lanes
+    // take the same number of iterations except in the last one, where the
+    // active lanes are a prefix of the wave
     for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x) {
         const int i0 = gi * 4;
+        uint32_t k[4];
         if (aligned && i0 + 4 <= n) {
             const float4 a = *reinterpret_cast<const float4*>(x + i0);
             const float4 b = *reinterpret_cast<const float4*>(y + i0);
             const float4 c = *reinterpret_cast<const float4*>(z + i0);
-            *reinterpret_cast<uint4*>(keys + i0) = make_uint4(cell_key(g, a.x, b.x, c.x), cell_key(g, a.y, b.y, c.y),
-                                                             cell_key(g, a.z, b.z, c.z), cell_key(g, a.w, b.w, c.w));
+            k[0] = cell_key(g, a.x, b.x, c.x); k[1] = cell_key(g, a.y, b.y, c.y);
+            k[2] = cell_key(g, a.z, b.z, c.z); k[3] = cell_key(g, a.w, b.w, c.w);
+            *reinterpret_cast<uint4*>(keys + i0) = make_uint4(k[0], k[1], k[2], k[3]);
             *reinterpret_cast<uint4*>(vals + i0) = make_uint4(i0, i0 + 1, i0 + 2, i0 + 3);
         } else {
-            for (int i = i0; i < i0 + 4 && i < n; ++i) {
-                keys[i] = cell_key(g, x[i], y[i], z[i]);
-                vals[i] = (uint32_t)i;
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + j < n ? i0 + j : n - 1;   // past n: the last slot again (no descent)
+                k[j] = cell_key(g, x[i], y[i], z[i]);
+                if (i0 + j < n) {
+                    keys[i0 + j] = k[j];
+                    vals[i0 + j] = (uint32_t)(i0 + j);
+                }
             }
         }
+        // the key before slot i0: the previous lane's last key, or for lane 0
+        // slot i0-1 itself
+        uint32_t prev = __shfl_up(k[3], 1, 64);
+        if (lane == 0) prev = i0 > 0 ? cell_key(g, x[i0 - 1], y[i0 - 1], z[i0 - 1]) : 0u;
+        desc += (int)(k[0] < prev) + (int)(k[1] < k[0]) + (int)(k[2] < k[1]) + (int)(k[3] < k[2]);
+    }
+    __shared__ int s_desc[4];
+    for (int o = 32; o > 0; o >>= 1) desc += __shfl_xor(desc, o, 64);
+    if (lane == 0) s_desc[threadIdx.x >> 6] = desc;
+    __syncthreads();
+    if (threadIdx.x == 0) info[2 + blockIdx.x] = s_desc[0] + s_desc[1] + s_desc[2] + s_desc[3];
+}
+
+// Sets the live count from the sorted keys: ascending, dead slots (key ==
+// dead) last.
+__device__ __forceinline__ void set_live_count(const uint32_t* keys, int i0, int n, uint32_t dead, int32_t* count)
+{
+    for (int i = i0; i < i0 + 4 && i < n; ++i) {
+        const uint32_t kk = keys[i];
+        if (kk < dead && (i + 1 == n || keys[i + 1] >= dead)) *count = i + 1;
+        if (i == 0 && kk >= dead) *count = 0;
     }
 }
 
 // Gathers the sorted order into the scratch copies (four slots per thread,
-// uint4 / float4 accesses of the scratch) and sets the live count: the keys
-// are ascending with the dead slots (key == dead) last.
-__global__ void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,
-                          const float* s4, const float* s5, float* d0, float* d1, float* d2,
-                          float* d3, float* d4, float* d5, const uint32_t* perm, const uint32_t* keys,
-                          uint32_t dead, int32_t n, int32_t* count)
+// uint4 / float4 accesses of the scratch) and sets the live count.  For slots
+// close to cell order: neighbouring destinations read neighbouring sources.
+__global__ __launch_bounds__(256) void k_permute(const float* s0, const float* s1, const float* s2, const float* s3,
+                                                 const float* s4, const float* s5, float* d0, float* d1, float* d2,
+                                                 float* d3, float* d4, float* d5, const uint32_t* perm,
+                                                 const uint32_t* keys, uint32_t dead, int32_t n, int32_t* count)
 {
     const int ng = (n + 3) >> 2;
     for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x) {
         const int i0 = gi * 4;
         if (i0 + 4 <= n) {
-            const uint4 k4 = *reinterpret_cast<const uint4*>(keys + i0);
             const uint4 p4 = *reinterpret_cast<const uint4*>(perm + i0);
-            const uint32_t k[4] = {k4.x, k4.y, k4.z, k4.w}, q[4] = {p4.x, p4.y, p4.z, p4.w};
+            const uint32_t q[4] = {p4.x, p4.y, p4.z, p4.w};
             // a permutation of [0, n): every source index is in range
             *reinterpret_cast<float4*>(d0 + i0) = make_float4(s0[q[0]], s0[q[1]], s0[q[2]], s0[q[3]]);
             *reinterpret_cast<float4*>(d1 + i0) = make_float4(s1[q[0]], s1[q[1]], s1[q[2]], s1[q[3]]);
@@ -77,22 +117,61 @@ __global__ void k_permute(const float* s0, const float* s1, const float* s2, con
             *reinterpret_cast<float4*>(d3 + i0) = make_float4(s3[q[0]], s3[q[1]], s3[q[2]], s3[q[3]]);
             *reinterpret_cast<float4*>(d4 + i0) = make_float4(s4[q[0]], s4[q[1]], s4[q[2]], s4[q[3]]);
             *reinterpret_cast<float4*>(d5 + i0) = make_float4(s5[q[0]], s5[q[1]], s5[q[2]], s5[q[3]]);
-            const uint32_t after = i0 + 4 < n ? keys[i0 + 4] : dead;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t nx = j < 3 ? k[j + 1] : after;
-                if (k[j] < dead && nx >= dead) *count = i0 + j + 1;
-            }
-            if (i0 == 0 && k[0] >= dead) *count = 0;
         } else {
             for (int i = i0; i < n; ++i) {
-                const uint32_t kk = keys[i], s = perm[i];
+                const uint32_t s = perm[i];
                 d0[i] = s0[s]; d1[i] = s1[s]; d2[i] = s2[s];
                 d3[i] = s3[s]; d4[i] = s4[s]; d5[i] = s5[s];
-                if (kk < dead && (i + 1 == n || keys[i + 1] >= dead)) *count = i + 1;
-                if (i == 0 && kk >= dead) *count = 0;
             }
         }
+        set_live_count(keys, i0, n, dead, count);
+    }
+}
+
+// Far from cell order, a gather of six 4-byte values per slot fetches six
+// cache lines per slot.  The disordered path first packs each slot into a
+// 32-byte record (x, y, z, vx | vy, vz, -, -; two 16-byte stores, coalesced),
+// then gathers one aligned record per destination: one line, not six.
+__global__ __launch_bounds__(256) void k_pack_records(const float* s0, const float* s1, const float* s2,
+                                                      const float* s3, const float* s4, const float* s5,
+                                                      float4* rec, int32_t n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        rec[2 * (size_t)i] = make_float4(s0[i], s1[i], s2[i], s3[i]);
+        rec[2 * (size_t)i + 1] = make_float4(s4[i], s5[i], 0.0f, 0.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_permute_records(const float4* rec, float* d0, float* d1, float* d2,
+                                                         float* d3, float* d4, float* d5, const uint32_t* perm,
+                                                         const uint32_t* keys, uint32_t dead, int32_t n,
+                                                         int32_t* count)
+{
+    const int ng = (n + 3) >> 2;
+    for (int gi = blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += gridDim.x * blockDim.x) {
+        const int i0 = gi * 4;
+        if (i0 + 4 <= n) {
+            const uint4 p4 = *reinterpret_cast<const uint4*>(perm + i0);
+            const uint32_t q[4] = {p4.x, p4.y, p4.z, p4.w};
+            float4 a[4], b[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {   // a permutation of [0, n): every record is in range
+                a[j] = rec[2 * (size_t)q[j]];
+                b[j] = rec[2 * (size_t)q[j] + 1];
+            }
+            *reinterpret_cast<float4*>(d0 + i0) = make_float4(a[0].x, a[1].x, a[2].x, a[3].x);
+            *reinterpret_cast<float4*>(d1 + i0) = make_float4(a[0].y, a[1].y, a[2].y, a[3].y);
+            *reinterpret_cast<float4*>(d2 + i0) = make_float4(a[0].z, a[1].z, a[2].z, a[3].z);
+            *reinterpret_cast<float4*>(d3 + i0) = make_float4(a[0].w, a[1].w, a[2].w, a[3].w);
+            *reinterpret_cast<float4*>(d4 + i0) = make_float4(b[0].x, b[1].x, b[2].x, b[3].x);
+            *reinterpret_cast<float4*>(d5 + i0) = make_float4(b[0].y, b[1].y, b[2].y, b[3].y);
+        } else {
+            for (int i = i0; i < n; ++i) {
+                const float4 a = rec[2 * (size_t)perm[i]], b = rec[2 * (size_t)perm[i] + 1];
+                d0[i] = a.x; d1[i] = a.y; d2[i] = a.z; d3[i] = a.w; d4[i] = b.x; d5[i] = b.y;
+            }
+        }
+        set_live_count(keys, i0, n, dead, count);
     }
 }
 
@@ -103,9 +182,13 @@ using SortConfig = rocprim::radix_sort_config<
     rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 16>, rocprim::kernel_config<1024, 16>, 9,
                                         rocprim::block_radix_rank_algorithm::match>>;
 
-// scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n][pad][rocprim tmp]
+// More descents than one slot in kDisorder: the record path.
+constexpr int kDisorder = 8;
+
+// scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n]
+// [records 8 x n][info][pad][rocprim tmp]
 hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st, float** sorted_out, int32_t* pinned_count)
+                       size_t* need_bytes, hipStream_t st, float** sorted_out, int32_t* pinned_info)
 {
     const size_t cap = (size_t)sp->capacity;
     const size_t a = (cap * 4 + 255) & ~(size_t)255;
@@ -117,25 +200,10 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
                                   (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)(cap > 0 ? cap : 1),
                                   0, bits, st);
     if (e != hipSuccess) return e;
-    const size_t need = 4 * a + 6 * a + 256 + ((tmp_bytes + 255) & ~(size_t)255);
+    const size_t info_bytes = ((2 + kSortKeyBlocks) * sizeof(int32_t) + 255) & ~(size_t)255;
+    const size_t need = 18 * a + info_bytes + ((tmp_bytes + 255) & ~(size_t)255);
     if (need_bytes) *need_bytes = need;
     if (!scratch || scratch_bytes < need) return hipErrorInvalidValue;
-    // the slot count (the one host synchronisation of the sort: rocPRIM takes
-    // the size on the host)
-    int32_t n_local = 0;
-    int32_t* hn = pinned_count ? pinned_count : &n_local;
-    e = hipMemcpyAsync(hn, sp->count, sizeof(int32_t), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    const int32_t n = *hn;
-    if (n <= 0) {
-        if (sorted_out) {   // nothing to permute: the particles stay where they are
-            float* own[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
-            for (int c = 0; c < 6; ++c) sorted_out[c] = own[c];
-        }
-        return hipSuccess;
-    }
     char* base = (char*)scratch;
     uint32_t* k_in = (uint32_t*)(base);
     uint32_t* k_out = (uint32_t*)(base + a);
@@ -143,20 +211,54 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
     uint32_t* v_out = (uint32_t*)(base + 3 * a);
     float* tmp[6];
     for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * a);
-    void* rtmp = (void*)(base + 10 * a + 256);
-    int blocks = ((n + 3) / 4 + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, n, k_in,
-                       v_in);
+    float4* rec = (float4*)(base + 10 * a);
+    int32_t* info = (int32_t*)(base + 18 * a);
+    void* rtmp = (void*)(base + 18 * a + info_bytes);
+    // keys and descents over the device's slot count, then the one host
+    // synchronisation of the sort (rocPRIM takes the size on the host)
+    int blocks = (int)(((cap + 3) / 4 + 255) / 256);
+    if (blocks > kSortKeyBlocks) blocks = kSortKeyBlocks;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, sp->count,
+                       (int32_t)cap, k_in, v_in, info);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    static thread_local int32_t local[2 + kSortKeyBlocks];
+    int32_t* hi = pinned_info ? pinned_info : local;
+    e = hipMemcpyAsync(hi, info, (2 + (size_t)blocks) * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    const int32_t n = hi[0];
+    int64_t descents = 0;
+    for (int b = 0; b < blocks; ++b) descents += hi[2 + b];
+    if (n <= 0) {
+        if (sorted_out) {   // nothing to permute: the particles stay where they are
+            float* own[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
+            for (int c = 0; c < 6; ++c) sorted_out[c] = own[c];
+        }
+        return hipSuccess;
+    }
     size_t tb = tmp_bytes;
     e = rocprim::radix_sort_pairs<SortConfig>(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
     if (e != hipSuccess) return e;
     const uint32_t dead = (uint32_t)(g.nr * g.nz);
-    hipLaunchKernelGGL(k_permute, dim3(blocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
-                       sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], v_out, k_out,
-                       dead, n, sp->count);
+    int pblocks = ((n + 3) / 4 + 255) / 256;
+    if (pblocks > 8192) pblocks = 8192;
+    if ((int64_t)descents * kDisorder > (int64_t)n) {
+        int rblocks = (n + 255) / 256;
+        if (rblocks > 8192) rblocks = 8192;
+        hipLaunchKernelGGL(k_pack_records, dim3(rblocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx, sp->vy,
+                           sp->vz, rec, n);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_permute_records, dim3(pblocks), dim3(256), 0, st, (const float4*)rec, tmp[0], tmp[1],
+                           tmp[2], tmp[3], tmp[4], tmp[5], v_out, k_out, dead, n, sp->count);
+    } else {
+        hipLaunchKernelGGL(k_permute, dim3(pblocks), dim3(256), 0, st, sp->x, sp->y, sp->z, sp->vx,
+                           sp->vy, sp->vz, tmp[0], tmp[1], tmp[2], tmp[3], tmp[4], tmp[5], v_out, k_out,
+                           dead, n, sp->count);
+    }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (sorted_out) {

@@ -352,6 +352,44 @@ def test_sort_drops_absorbed_and_orders_cells(gpu):
     assert np.array_equal(live[:, a], g[:, b])
 
 
+@pytest.mark.parametrize("order", ["random", "near_cell_order"])
+def test_sort_is_stable_on_both_permute_paths(gpu, order):
+    """The sort permutes far-from-cell-order slots through packed records and
+    near-cell-order slots directly (the choice follows the key descents); both
+    give the stable order: slots of one cell keep their relative order."""
+    import gasx
+    log(f"sort permutation ({order})")
+    nr, nz = 40, 70
+    p, ctx, op = mk(nr, nz)
+    n = 50001
+    st = wedge_particles(n, nr, nz, p.dr, p.dz, p.phip, 21, edge_frac=0.0)
+    half = np.float32(0.5)
+
+    def keys(a):
+        r = np.sqrt(a[0] * a[0] + a[1] * a[1])
+        kr = np.clip(np.floor(r * np.float32(1 / p.dr) - half), 0, nr - 1)
+        kz = np.clip(np.floor(a[2] * np.float32(1 / p.dz) - half), 0, nz - 1)
+        return (kz * nr + kr).astype(np.int64)
+
+    if order == "near_cell_order":
+        st = st[:, np.argsort(keys(st), kind="stable")]
+        sw = np.arange(0, n - 1, 97)                     # a few neighbours out of order
+        st[:, sw], st[:, sw + 1] = st[:, sw + 1].copy(), st[:, sw].copy()
+    st[2, 7::29] = np.nan                                # absorbed slots sort last and drop out
+    st[3] = np.arange(n, dtype=np.float32)               # slot ids (exact in FP32)
+    ps = gasx.ParticleSet(n + 3).load(*st)
+    ctx.sort(ps)
+    g = ps.state()
+    ids = g[3].astype(np.int64)
+    live = np.nonzero(~np.isnan(st[2]))[0]
+    assert np.array_equal(np.sort(ids), live)
+    k = keys(g)
+    assert np.all(np.diff(k) >= 0)
+    same = np.diff(k) == 0
+    assert np.all(np.diff(ids)[same] > 0)                # stable within a cell
+    assert np.array_equal(g, st[:, ids])
+
+
 def test_tallies_match_oracle(gpu):
     import gasx
     log("tallies")
