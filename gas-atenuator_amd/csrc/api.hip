@@ -307,8 +307,8 @@ static void fill_geometry(gasx_ctx* c)
     a.junit = p.fn * p.e_em * p.c / 4294967296.0;
     a.vcoef = p.phip / 2 * p.dr * p.dr * p.dz;
     // default tiles: 16 x 32 (r x z), shrunk for tiny grids
-    a.ti = std::min(16, p.nr);
-    a.tk = std::min(32, p.nz);
+    a.ti = std::min(GASX_FDTD_TI, p.nr);
+    a.tk = std::min(GASX_FDTD_TK, p.nz);
     a.ntr = (p.nr + a.ti - 1) / a.ti;
     a.ntk = (p.nz + a.tk - 1) / a.tk;
 }

@@ -145,8 +145,8 @@ bool tile_fits(const gasx_ctx* ctx, int ti, int tk)
     FdtdArgs a = ctx->fa;
     a.ti = std::min(ti, ctx->p.nr);
     a.tk = std::min(tk, ctx->p.nz);
-    const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
-    return fdtd_lds_bytes(a) <= 160 * 1024 && (W + 1) * (Hh + 1) <= 1024 * 8;
+    int k = 0, bs = 0;
+    return fdtd_fwd_lds_bytes(a) <= 160 * 1024 && fdtd_fwd_shape(a, &k, &bs);
 }
 
 // Everything upload() writes into the device descriptors depends on.

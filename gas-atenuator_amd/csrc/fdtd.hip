@@ -15,13 +15,38 @@
 
 namespace gasx {
 
-template <int BS, int MAXP>
+// Update kinds of a thread's positions, decoded once per launch.
+// Er: flags (update; write the Jacobi copy into the row below / above)
+enum : unsigned char { kErUpdate = 1, kErCopyBelow = 2, kErCopyAbove = 4 };
+enum : unsigned char { kEzKeep = 0, kEzAxis, kEzUpdate };
+
+// Workgroup b runs on XCD b mod 8 (round-robin dispatch); tile b' of the
+// grid is given to it such that each XCD takes a contiguous eighth of the
+// tiles, whose overlapping halos then meet in that XCD's L2.  A bijection of
+// [0, G): the last G mod 8 workgroups keep their own index.
+__device__ __forceinline__ int xcd_tile(int b, int G)
+{
+    const int per = G >> 3;
+    return b < (per << 3) ? (b & 7) * per + (b >> 3) : b;
+}
+
+// Thread layout: a thread owns a strip of K consecutive positions (li, lk) of
+// one column li (positions i in [ri0, ri1], k in [rk0, rk1]; a position
+// holds H, Er and Ez of that index where they exist).  It keeps their values
+// in registers and publishes every update to LDS for its neighbours, so a
+// half update reads from LDS only the neighbours outside the strip: Ez(i+1)
+// and the Er above the strip for H; the H below the strip and H(i-1) for E.
+// Strips start at multiples of K >= 2, so rows 0 and 1 are always one
+// thread's (the Jacobi copy Er(0) <- Er(1) stays in its registers).
+template <int BS, int K>
 __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdRun* runs, int fixed_j)
 {
+    static_assert(K >= 2, "rows 0 and 1 share a strip");
     extern __shared__ double lds[];
     const int tiles_per_run = a.ntr * a.ntk;
-    const int run = blockIdx.x / tiles_per_run;
-    const int t = blockIdx.x - run * tiles_per_run;
+    const int bt = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int run = bt / tiles_per_run;
+    const int t = bt - run * tiles_per_run;
     FdtdRun R = runs ? runs[run] : r0;
     // a cycle split over two launches: the first writes the mid buffers, the
     // second reads them (a.part: 0 whole cycle, 1 first, 2 second)
@@ -35,129 +60,161 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
     const int ri0 = max(0, i0 - a.halo), ri1 = min(nr, i1 + a.halo);
     const int rk0 = max(0, k0 - a.halo), rk1 = min(nz, k1 + a.halo);
     const int W = ri1 - ri0, Hh = rk1 - rk0;
-    // LDS: H  W x Hh ; Er W x (Hh+1) ; Ez (W+1) x Hh
+    // LDS: H  W x Hh ; Er W x (Hh+1) ; Ez (W+1) x Hh ; Jr, Jz  W x Hh (the
+    // current as read: fixed point converted to double, or FP64)
     double* sH = lds;
     double* sEr = sH + W * Hh;
     double* sEz = sEr + W * (Hh + 1);
+    double* sJr = sEz + (W + 1) * Hh;
+    double* sJz = sJr + W * Hh;
     const int ldz = W + 1;
-    const int npos = (W + 1) * (Hh + 1);   // positions (i,k), i in [ri0,ri1], k in [rk0,rk1]
+    const int ncol = W + 1, nstrip = (Hh + 1 + K - 1) / K;
+    const int tid = (int)threadIdx.x;
+    const bool active = tid < ncol * nstrip;
+    const int li = active ? tid % ncol : 0;
+    const int sk = active ? (tid / ncol) * K : 0;       // the strip's first row
+    const int gi = ri0 + li;                            // <= nr
+    const int ci = min(gi, nr - 1);
 
     // ---- load ------------------------------------------------------------------
     // Every global read of the thread is issued before any is used (indices
     // clamped into the arrays, so the reads need no guard; a value read for a
-    // position that does not use it is dropped below).
-    double vH[MAXP], vEr[MAXP], vEz[MAXP], vr0[MAXP], vr1[MAXP], vz0[MAXP], vz1[MAXP];
+    // position that does not use it is dropped below).  The current is read
+    // once per cell; the E updates take their neighbours' (k-1 for Er, i-1 for
+    // Ez, inside the tile for every updated position) from LDS.
+    double H[K], Er[K], Ez[K], vJr[K], vJz[K];
 #pragma unroll
-    for (int q = 0; q < MAXP; ++q) {
-        const int p = min((int)threadIdx.x + q * BS, npos - 1);
-        const int li = p % (W + 1), lk = p / (W + 1);
-        const int gi = ri0 + li, gk = rk0 + lk;              // gi <= nr, gk <= nz
-        const int ci = min(gi, nr - 1), ck = min(gk, nz - 1);
-        vH[q] = R.h_in[ci + ck * nr];
-        vEr[q] = R.er_in[ci + gk * nr];
-        vEz[q] = R.ez_in[gi + ck * (nr + 1)];
-        const int km = max(gk - 1, 0), im = max(gi - 1, 0);
+    for (int j = 0; j < K; ++j) {
+        const int lk = min(sk + j, Hh);
+        const int gk = rk0 + lk;                        // <= nz
+        const int ck = min(gk, nz - 1);
+        H[j] = R.h_in[ci + ck * nr];
+        Er[j] = R.er_in[ci + gk * nr];
+        Ez[j] = R.ez_in[gi + ck * (nr + 1)];
         if (fixed_j) {
-            vr0[q] = (double)(long long)R.qr[ci + km * nr];
-            vr1[q] = (double)(long long)R.qr[ci + ck * nr];
-            vz0[q] = (double)(long long)R.qz[min(im, nr - 1) + ck * nr];
-            vz1[q] = (double)(long long)R.qz[ci + ck * nr];
+            vJr[j] = (double)(long long)R.qr[ci + ck * nr];
+            vJz[j] = (double)(long long)R.qz[ci + ck * nr];
         } else {
-            vr0[q] = R.jr[ci + km * nr];
-            vr1[q] = R.jr[ci + ck * nr];
-            vz0[q] = R.jz[min(im, nr - 1) + ck * nr];
-            vz1[q] = R.jz[ci + ck * nr];
+            vJr[j] = R.jr[ci + ck * nr];
+            vJz[j] = R.jz[ci + ck * nr];
         }
     }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int lk = sk + j;
+        if (!active || lk > Hh) continue;
+        if (li < W && lk < Hh) {
+            sH[li + lk * W] = H[j];
+            sJr[li + lk * W] = vJr[j];
+            sJz[li + lk * W] = vJz[j];
+        }
+        if (li < W) sEr[li + lk * W] = Er[j];
+        if (lk < Hh) sEz[li + lk * ldz] = Ez[j];
+    }
+    __syncthreads();
 
-    // ---- LDS tile, per-position sources and coefficients (registers) ---------
+    // ---- kinds, sources and coefficients (registers) ----------------------
     // J = junit * Q / V(i) for the fixed-point current, as read for FP64 J
     auto jscale = [&](int i) -> double {
         return fixed_j ? a.junit / (a.vcoef * (double)((i + 1) * (i + 1) - i * i)) : 1.0;
     };
-    double srcR[MAXP], srcZ[MAXP], cA[MAXP], cB[MAXP];
+    // Ez: the axis column (EM3.cu:35) and the interior columns (:39) are
+    // column properties; the rest (halo edge columns, i = nr) keep their values
+    const unsigned char ezcol = (gi == 0) ? kEzAxis : (gi <= nr - 1 && li > 0 && li < W) ? kEzUpdate : kEzKeep;
+    double cA = 0.0, cB = 0.0;
+    if (ezcol == kEzUpdate) {
+        const double rs = (a.ez_metric ? (gi + 1) : gi) * a.dr;
+        cA = a.ce * (1 / (2 * rs) + 1 / a.dr);
+        cB = a.ce * (1 / (2 * rs) - 1 / a.dr);
+    }
+    // Er rows gk = 0 and nz take the Jacobi copies of rows 1 and nz-1
+    // (EM3.cu:22,26): the owner of rows 1 and nz-1 writes them, from the
+    // values before the update, so the E update reads no other thread's E
+    // and runs in place between two barriers.
+    bool hk[K];
+    unsigned char erk[K], ezk[K];
+    double srcR[K], srcZ[K];
 #pragma unroll
-    for (int q = 0; q < MAXP; ++q) {
-        const int p = threadIdx.x + q * BS;
-        srcR[q] = 0.0; srcZ[q] = 0.0; cA[q] = 0.0; cB[q] = 0.0;
-        if (p < npos) {
-            const int li = p % (W + 1), lk = p / (W + 1);
-            const int gi = ri0 + li, gk = rk0 + lk;
-            if (li < W && lk < Hh) sH[li + lk * W] = vH[q];
-            if (li < W) sEr[li + lk * W] = vEr[q];
-            if (lk < Hh) sEz[li + lk * ldz] = vEz[q];
-            if (li < W && gk >= 1 && gk <= nz - 1) {   // EM3.cu:29 (jr at k-1 and k)
-                const double js = jscale(gi);
-                srcR[q] = a.ce * (js * vr0[q] + js * vr1[q]) / 2;
-            }
-            if (lk < Hh) {
-                if (gi == 0)                           // EM3.cu:35
-                    srcZ[q] = ((double)a.axis_sign * a.ce) * (jscale(0) * vz1[q]);
-                else if (gi <= nr - 1) {               // EM3.cu:39
-                    srcZ[q] = a.ce * (jscale(gi - 1) * vz0[q] + jscale(gi) * vz1[q]) / 2;
-                    const double rs = (a.ez_metric ? (gi + 1) : gi) * a.dr;
-                    cA[q] = a.ce * (1 / (2 * rs) + 1 / a.dr);
-                    cB[q] = a.ce * (1 / (2 * rs) - 1 / a.dr);
-                }
-            }
+    for (int j = 0; j < K; ++j) {
+        const int lk = sk + j;
+        const int gk = rk0 + lk;
+        const bool valid = active && lk <= Hh;
+        hk[j] = valid && li < W && lk < Hh;
+        erk[j] = 0; ezk[j] = kEzKeep; srcR[j] = 0.0; srcZ[j] = 0.0;
+        if (!valid) continue;
+        if (li < W && gk >= 1 && gk <= nz - 1 && lk > 0 && lk < Hh) {   // EM3.cu:29 (jr at k-1 and k)
+            const double js = jscale(gi);
+            srcR[j] = a.ce * (js * sJr[li + (lk - 1) * W] + js * sJr[li + lk * W]) / 2;
+            erk[j] = kErUpdate | ((gk == 1 && lk == 1) ? kErCopyBelow : 0) |
+                     ((gk == nz - 1 && lk + 1 == Hh) ? kErCopyAbove : 0);
+        }
+        if (lk < Hh) {
+            ezk[j] = ezcol;
+            if (ezcol == kEzAxis)
+                srcZ[j] = ((double)a.axis_sign * a.ce) * (jscale(0) * sJz[li + lk * W]);
+            else if (ezcol == kEzUpdate)
+                srcZ[j] = a.ce * (jscale(gi - 1) * sJz[(li - 1) + lk * W] + jscale(gi) * sJz[li + lk * W]) / 2;
         }
     }
-    __syncthreads();
+    // (no barrier: the first H update reads only E, in place since the load)
 
     // ---- H update (EM3_1.cu:15), scaled by s in {-1/2, 1, 1/2} -------------
+    // reads E, writes the thread's own H cells.  Er(k+1) inside the strip is
+    // the thread's register: rows it does not update are halo edges (never
+    // changed) or the boundary row nz when the thread wrote its copy.
+    // Each half update issues all its LDS reads first (addresses clamped into
+    // the tile; a value read for a position that does not use it is
+    // dropped), then computes and writes: the reads of the strip's positions
+    // then wait on one LDS round trip, not one per position (the compiler
+    // cannot move a read above a write to the same LDS array).
     auto h_update = [&](double s) {
         const double cr = s * a.ch_r, cz = s * a.ch_z;
+        double ez1[K];
 #pragma unroll
-        for (int q = 0; q < MAXP; ++q) {
-            const int p = threadIdx.x + q * BS;
-            if (p < npos) {
-                const int li = p % (W + 1), lk = p / (W + 1);
-                if (li < W && lk < Hh) {
-                    const double dez = sEz[(li + 1) + lk * ldz] - sEz[li + lk * ldz];
-                    const double der = sEr[li + (lk + 1) * W] - sEr[li + lk * W];
-                    sH[li + lk * W] = sH[li + lk * W] + (cr * dez - cz * der);
-                }
+        for (int j = 0; j < K; ++j) ez1[j] = sEz[min(li + 1, W) + min(sk + j, Hh - 1) * ldz];
+        const double ertop = sEr[min(li, W - 1) + min(sk + K, Hh) * W];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            if (hk[j]) {
+                const double er1 = j + 1 < K ? Er[j + 1] : ertop;
+                const double dez = ez1[j] - Ez[j];
+                const double der = er1 - Er[j];
+                H[j] = H[j] + (cr * dez - cz * der);
+                sH[li + (sk + j) * W] = H[j];
             }
         }
         __syncthreads();
     };
 
-    // ---- E update (EM3.cu:14-42), Jacobi in the boundary copies -------------
+    // ---- E update (EM3.cu:14-42): reads H, writes the thread's own E --------
     auto e_update = [&]() {
-        double nEr[MAXP], nEz[MAXP];
+        double hl[K];
 #pragma unroll
-        for (int q = 0; q < MAXP; ++q) {
-            const int p = threadIdx.x + q * BS;
-            nEr[q] = 0.0; nEz[q] = 0.0;
-            if (p < npos) {
-                const int li = p % (W + 1), lk = p / (W + 1);
-                const int gi = ri0 + li, gk = rk0 + lk;
-                if (li < W) {
-                    double v = sEr[li + lk * W];
-                    if (gk == 0) v = sEr[li + 1 * W];                     // :22
-                    else if (gk == nz) v = sEr[li + (lk - 1) * W];        // :26
-                    else if (lk > 0 && lk < Hh)                            // :29
-                        v = v - srcR[q] - a.cez * (sH[li + lk * W] - sH[li + (lk - 1) * W]);
-                    nEr[q] = v;
+        for (int j = 0; j < K; ++j) hl[j] = sH[max(li - 1, 0) + min(sk + j, Hh - 1) * W];
+        const double hbot = sH[min(li, W - 1) + max(sk - 1, 0) * W];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const int lk = sk + j;
+            if (erk[j] & kErUpdate) {                                      // :29
+                const double hb = j > 0 ? H[j - 1] : hbot;
+                const double old = Er[j];
+                Er[j] = old - srcR[j] - a.cez * (H[j] - hb);
+                sEr[li + lk * W] = Er[j];
+                if (erk[j] & kErCopyBelow) {                               // :22 Er(0) <- Er(1)
+                    sEr[li + (lk - 1) * W] = old;
+                    if (j > 0) Er[j - 1] = old;
                 }
-                if (lk < Hh) {
-                    double v = sEz[li + lk * ldz];
-                    if (gi == 0)                                           // :35
-                        v = v + srcZ[q] + a.c4 * sH[0 + lk * W];
-                    else if (gi <= nr - 1 && li > 0 && li < W)            // :39
-                        v = v - srcZ[q] + cA[q] * sH[li + lk * W] + cB[q] * sH[(li - 1) + lk * W];
-                    nEz[q] = v;
+                if (erk[j] & kErCopyAbove) {                               // :26 Er(nz) <- Er(nz-1)
+                    sEr[li + (lk + 1) * W] = old;
+                    if (j + 1 < K) Er[j + 1] = old;
                 }
             }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < MAXP; ++q) {
-            const int p = threadIdx.x + q * BS;
-            if (p < npos) {
-                const int li = p % (W + 1), lk = p / (W + 1);
-                if (li < W) sEr[li + lk * W] = nEr[q];
-                if (lk < Hh) sEz[li + lk * ldz] = nEz[q];
+            if (ezk[j] == kEzAxis) {                                       // :35 (li = 0: H(0) is the thread's)
+                Ez[j] = Ez[j] + srcZ[j] + a.c4 * H[j];
+                sEz[li + lk * ldz] = Ez[j];
+            } else if (ezk[j] == kEzUpdate) {                              // :39
+                Ez[j] = Ez[j] - srcZ[j] + cA * H[j] + cB * hl[j];
+                sEz[li + lk * ldz] = Ez[j];
             }
         }
         __syncthreads();
@@ -171,10 +228,12 @@ __global__ __launch_bounds__(BS) void k_fdtd(FdtdArgs a, FdtdRun r0, const FdtdR
     }
     if (final_part) h_update(0.5);
 
-    // ---- write the tile interior ------------------------------------------
-    for (int p = threadIdx.x; p < npos; p += BS) {
-        const int li = p % (W + 1), lk = p / (W + 1);
-        const int gi = ri0 + li, gk = rk0 + lk;
+    // ---- write the tile interior (from LDS, where every update landed) ------
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int lk = sk + j;
+        if (!active || lk > Hh) continue;
+        const int gk = rk0 + lk;
         const bool in_i = gi >= i0 && gi < i1;
         const bool in_k = gk >= k0 && gk < k1;
         if (li < W && lk < Hh && in_i && in_k) {
@@ -205,13 +264,20 @@ size_t fdtd_lds_bytes(const FdtdArgs& a)
     return (W * Hh + W * (Hh + 1) + (W + 1) * Hh) * sizeof(double);
 }
 
-template <int BS, int MAXP>
+// the forward kernel also holds the tile's current
+size_t fdtd_fwd_lds_bytes(const FdtdArgs& a)
+{
+    const size_t W = (size_t)a.ti + 2 * a.halo, Hh = (size_t)a.tk + 2 * a.halo;
+    return fdtd_lds_bytes(a) + 2 * W * Hh * sizeof(double);
+}
+
+template <int BS, int K>
 static hipError_t launch_fdtd_t(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev,
                                 int nruns, int fixed_j, hipStream_t s)
 {
-    const size_t lds = fdtd_lds_bytes(a);
+    const size_t lds = fdtd_fwd_lds_bytes(a);
     const int blocks = nruns * a.ntr * a.ntk;
-    hipLaunchKernelGGL((k_fdtd<BS, MAXP>), dim3(blocks), dim3(BS), lds, s, a, r0, runs_dev, fixed_j);
+    hipLaunchKernelGGL((k_fdtd<BS, K>), dim3(blocks), dim3(BS), lds, s, a, r0, runs_dev, fixed_j);
     return hipGetLastError();
 }
 
@@ -437,8 +503,13 @@ hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s
 // must not wait for the first launch, which may be under graph capture).
 hipError_t fdtd_prepare()
 {
-    const void* fns[] = {(const void*)k_fdtd<512, 2>,     (const void*)k_fdtd<1024, 2>,
-                         (const void*)k_fdtd<1024, 4>,    (const void*)k_fdtd<1024, 8>,
+    const void* fns[] = {(const void*)k_fdtd<256, 2>,     (const void*)k_fdtd<512, 2>,
+                         (const void*)k_fdtd<768, 2>,     (const void*)k_fdtd<1024, 2>,
+                         (const void*)k_fdtd<256, 3>,     (const void*)k_fdtd<512, 3>,
+                         (const void*)k_fdtd<768, 3>,     (const void*)k_fdtd<1024, 3>,
+                         (const void*)k_fdtd<256, 4>,     (const void*)k_fdtd<512, 4>,
+                         (const void*)k_fdtd<768, 4>,     (const void*)k_fdtd<1024, 4>,
+                         (const void*)k_fdtd<1024, 6>,    (const void*)k_fdtd<1024, 8>,
                          (const void*)k_fdtd_adj<512, 2>, (const void*)k_fdtd_adj<1024, 2>,
                          (const void*)k_fdtd_adj<1024, 4>, (const void*)k_fdtd_adj<1024, 8>};
     for (const void* f : fns) {
@@ -448,18 +519,42 @@ hipError_t fdtd_prepare()
     return hipSuccess;
 }
 
+// Strip length K and workgroup size of the forward kernel for a's tiles:
+// false when no instantiation holds them.
+#ifndef GASX_FDTD_MIN_K
+#define GASX_FDTD_MIN_K 2
+#endif
+bool fdtd_fwd_shape(const FdtdArgs& a, int* K, int* BS)
+{
+    const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
+    static const int ks[] = {2, 3, 4, 6, 8};
+    for (int k : ks) {
+        if (k < GASX_FDTD_MIN_K) continue;
+        const int threads = (W + 1) * ((Hh + 1 + k - 1) / k);
+        if (threads > 1024) continue;
+        const int bs = k > 4 ? 1024 : threads <= 256 ? 256 : threads <= 512 ? 512 : threads <= 768 ? 768 : 1024;
+        *K = k; *BS = bs;
+        return true;
+    }
+    return false;
+}
+
 static hipError_t launch_fdtd_part(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
                                    int fixed_j, hipStream_t s)
 {
-    const int W = a.ti + 2 * a.halo, Hh = a.tk + 2 * a.halo;
-    const int npos = (W + 1) * (Hh + 1);
-    if (fdtd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
-    // the most threads a tile can use, a few positions each: one workgroup per
-    // CU on small grids, so resident waves are what hides the latency
-    if (npos <= 512 * 2) return launch_fdtd_t<512, 2>(a, r0, runs_dev, nruns, fixed_j, s);
-    if (npos <= 1024 * 2) return launch_fdtd_t<1024, 2>(a, r0, runs_dev, nruns, fixed_j, s);
-    if (npos <= 1024 * 4) return launch_fdtd_t<1024, 4>(a, r0, runs_dev, nruns, fixed_j, s);
-    if (npos <= 1024 * 8) return launch_fdtd_t<1024, 8>(a, r0, runs_dev, nruns, fixed_j, s);
+    if (fdtd_fwd_lds_bytes(a) > 160 * 1024) return hipErrorInvalidValue;
+    // the shortest strips whose threads fit a workgroup (the most threads: one
+    // workgroup per CU on small grids, so resident waves hide the latency),
+    // in the smallest workgroup size that holds them
+    int K = 0, BS = 0;
+    if (!fdtd_fwd_shape(a, &K, &BS)) return hipErrorInvalidValue;
+#define GASX_FDTD_CASE(bs, k) \
+    if (BS == bs && K == k) return launch_fdtd_t<bs, k>(a, r0, runs_dev, nruns, fixed_j, s);
+    GASX_FDTD_CASE(256, 2) GASX_FDTD_CASE(512, 2) GASX_FDTD_CASE(768, 2) GASX_FDTD_CASE(1024, 2)
+    GASX_FDTD_CASE(256, 3) GASX_FDTD_CASE(512, 3) GASX_FDTD_CASE(768, 3) GASX_FDTD_CASE(1024, 3)
+    GASX_FDTD_CASE(256, 4) GASX_FDTD_CASE(512, 4) GASX_FDTD_CASE(768, 4) GASX_FDTD_CASE(1024, 4)
+    GASX_FDTD_CASE(1024, 6) GASX_FDTD_CASE(1024, 8)
+#undef GASX_FDTD_CASE
     return hipErrorInvalidValue;
 }
 

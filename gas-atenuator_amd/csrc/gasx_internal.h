@@ -20,6 +20,12 @@ constexpr int kWaveSize = 64;
 constexpr int kPer = GASX_KPER;  // consecutive slots per lane
 constexpr int kSeg = kWaveSize * kPer;  // slots per wave segment: the unit of work and of staging
 constexpr int kChunk = kBlock * kPer;   // alignment unit of pooled batch runs (a multiple of kSeg)
+#ifndef GASX_FDTD_TI
+#define GASX_FDTD_TI 16        // field tile of one workgroup: r cells
+#endif
+#ifndef GASX_FDTD_TK
+#define GASX_FDTD_TK 32        // field tile of one workgroup: z cells
+#endif
 #ifndef GASX_KPER_DRIFT
 #define GASX_KPER_DRIFT 4
 #endif
@@ -207,7 +213,9 @@ struct AdjFdtdRun {
 // runs_dev == nullptr: single run described by r0
 hipError_t launch_fdtd(const FdtdArgs& a, const FdtdRun& r0, const FdtdRun* runs_dev, int nruns,
                        int fixed_j, hipStream_t s);
-size_t fdtd_lds_bytes(const FdtdArgs& a);
+size_t fdtd_lds_bytes(const FdtdArgs& a);        // the transposed cycle's tile
+size_t fdtd_fwd_lds_bytes(const FdtdArgs& a);    // the forward cycle's tile (fields and current)
+bool fdtd_fwd_shape(const FdtdArgs& a, int* K, int* BS);   // strip length and workgroup size, if any fits
 // transposed cycle (adjoint), single run, tiles of a
 hipError_t launch_fdtd_adj(const FdtdArgs& a, const AdjFdtdRun& r, hipStream_t s);
 // push / collide / stage (kPush, kCollide, kStageSec, kStageIon, kDriftSec, kBatch)

@@ -60,6 +60,9 @@ def rel(a, b):
     (37, 101, 2, {"ez_metric": 1, "axis_j_sign": -1}),
     (16, 32, 1, {"n_sub": 4}),
     (90, 33, 2, {"n_sub": 1}),
+    (6, 2, 2, {}),                  # nz = 2: row 1 is both Jacobi copy rows
+    (3, 5, 3, {"n_sub": 3}),
+    (300, 520, 1, {}),              # several tiles in r and z
 ])
 def test_fdtd_matches_oracle(gpu, nr, nz, n_push, kw):
     import gasx
