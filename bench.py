@@ -37,7 +37,13 @@ METRIC = "particle-steps/s + %HBM roofline; batched vs plain; adjoint/forward ti
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_STEP = 48
 DEPOSIT_BYTES_PER_PARTICLE_STEP = 24
-TIME_EVERY = 1          # the timed region times every k_particles launch
+# The timed region times every other k_particles launch (0, 2, 4, ...): a
+# launch timed by its own HIP events idles the GPU for about 4 us before and
+# after it (rocprofv3 kernel trace), so timing all of them would add some 9 us
+# to every step of the headline.  The even launches of the 20-step default
+# sample the population's growth evenly (the mean of 0, 2, .., 18 equals that
+# of 0, 1, .., 18, the launches of the pushing kernel variant).
+TIME_EVERY = 2
 
 
 def parse():
@@ -415,9 +421,7 @@ def main():
     ctx.counters(reset=True)
     torch.cuda.synchronize()
     # timed region: every TIME_EVERY-th k_particles launch (the roofline
-    # kernel) timed by its own dispatch; a timed launch adds 0.3-1.4 us of idle
-    # time around it (under 0.7% of a step), and the average over every launch reads
-    # closer to a kernel trace than a 1-in-8 sample did
+    # kernel) timed by HIP events of its own dispatch
     t0 = time.perf_counter()
     ctx.run(e, f, gas, args.steps, time_kernels=2, time_every=TIME_EVERY, **run_kw)
     torch.cuda.synchronize()

@@ -147,6 +147,21 @@ class PhotoResult(C.Structure):
     ]
 
 
+class EsParams(C.Structure):
+    _fields_ = [
+        ("nr", C.c_int32), ("nz", C.c_int32), ("delta", C.c_double), ("phip", C.c_double),
+        ("weight", C.c_double), ("t_e", C.c_double), ("dt", C.c_double), ("n_e0", C.c_double),
+        ("tol", C.c_double), ("max_iter", C.c_int32),
+    ]
+
+
+class EsResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("converged", C.c_int32), ("n_e0", C.c_double),
+        ("max_dphi", C.c_double), ("mean_phi", C.c_double), ("int_rho_i", C.c_double),
+    ]
+
+
 # exported symbols and their prototypes (every declaration of include/gasx.h)
 _P = C.c_void_p
 PROTOTYPES = {
@@ -187,6 +202,8 @@ PROTOTYPES = {
     "gasx_batch_counters": (C.c_int, [_P, C.c_int, C.POINTER(Counters), C.c_int]),
     "gasx_photoionize": (C.c_int, [_P, C.POINTER(Species), C.POINTER(Species), C.POINTER(Gas),
                                    C.POINTER(PhotoSource), C.POINTER(PhotoResult)]),
+    "gasx_ion_es_push": (C.c_int, [_P, C.POINTER(Species), C.c_int, C.POINTER(EsParams), _P, _P, _P, _P, _P,
+                                   C.POINTER(EsResult)]),
     "gasx_diff_record": (C.c_int, [_P, C.POINTER(Species), C.POINTER(DiffSpec)]),
     "gasx_diff_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]),
     "gasx_diff_backward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -195,7 +212,7 @@ PROTOTYPES = {
     "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _lib = None
 

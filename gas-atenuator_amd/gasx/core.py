@@ -285,6 +285,25 @@ class Context:
             self._check(st)
         return {n: getattr(res, n) for n, _ in _abi.PhotoResult._fields_}
 
+    def ion_es_push(self, ions: list[ParticleSet], nr: int, nz: int, delta: float, weight: float, t_e: float,
+                    dt: float, n_e0: float = 1e15, tol: float = 0.01, max_iter: int = 1_000_000,
+                    phip: float | None = None) -> dict:
+        """ion_staticelectrofield_push.m with Poisson.m: the ion species are
+        pushed in place; returns the gasx_es_result fields and the grids phi,
+        rho_i, rho_e, e_r, e_z (torch FP64, nz x nr)."""
+        dev = ions[0].data.device
+        grids = {k: torch.empty((nz, nr), dtype=torch.float64, device=dev)
+                 for k in ("phi", "rho_i", "rho_e", "e_r", "e_z")}
+        arr = (Species * len(ions))(*[s.struct() for s in ions])
+        p = _abi.EsParams(nr, nz, delta, self.params.phip if phip is None else phip, weight, t_e, dt, n_e0, tol,
+                          max_iter)
+        res = _abi.EsResult()
+        self._check(self.lib.gasx_ion_es_push(self.ctx, arr, len(ions), C.byref(p), *[_ptr(grids[k]) for k in
+                                              ("phi", "rho_i", "rho_e", "e_r", "e_z")], C.byref(res)))
+        out = {n: getattr(res, n) for n, _ in _abi.EsResult._fields_}
+        out.update(grids)
+        return out
+
     # ---- production loop ---------------------------------------------------------
     def run(self, e: ParticleSet, f: FieldState, gas: GasGrid | None, n_steps: int,
             collide: bool = True, secondaries: bool = False, sort_every: int = 0, seed: int = 0,

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 5
+#define GASX_ABI_VERSION 6
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -326,6 +326,47 @@ typedef struct gasx_photo_result {
 
 gasx_status gasx_photoionize(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, const gasx_gas* g,
                              const gasx_photo_source* src, gasx_photo_result* out);
+
+/* ---- electrostatic ion step ---------------------------------------------------
+ * Replaces ion_staticelectrofield_push.m:21-156 with Poisson.m:25-68: the ion
+ * species are half drifted (x += dt/2 v), counted in the cells of an r-z grid
+ * of nr x nz cells of size delta (cell centres at (i + 1/2) delta, the grid of
+ * Poisson.m:11-12; r and z cells clamped separately), their charge density
+ * rho_i = W e sum_s q_s n_s / V(i) is formed (V(i) = PHIp/2 ((i+1)^2 - i^2)
+ * delta^3), and the potential PHI of Poisson.m's iteration is found: Jacobi
+ * sweeps of its stencil with Boltzmann electrons rho_e = e n_e0
+ * exp(e PHI / kB Te), n_e0 renormalised each sweep so that the electrons'
+ * charge equals the ions', only positive updates kept, PHI(0,:) = PHI(1,:)
+ * and PHI = 0 on the other three edges, until max|dPHI| <= tol mean(PHI)
+ * (or max_iter sweeps).  Then E = -gradient(PHI) (MATLAB's rule) and every
+ * ion gets v += q e E(cell) / m dt along r-hat and z-hat of its half-drifted
+ * position and x = x_half + dt/2 v.  phi, rho_i, rho_e, e_r, e_z: device,
+ * nr*nz doubles each, index j*nr + i (outputs).  The reference repeats one
+ * identical solve 100 times (Poisson.m:25-27); it is done once.
+ * Synchronizes. */
+typedef struct gasx_es_params {
+    int32_t nr, nz;             /* cells in r and z, >= 3 each */
+    double delta;               /* cell size (m): delta_r = delta_z (Poisson.m:9-10) */
+    double phip;                /* wedge angle of the cell volumes */
+    double weight;              /* W: real ions per simulated ion */
+    double t_e;                 /* electron temperature (K) */
+    double dt;                  /* ion step (s) */
+    double n_e0;                /* starting electron density scale (Poisson.m:33: 1e15) */
+    double tol;                 /* stopping rule factor (Poisson.m:36: 1/100) */
+    int32_t max_iter;           /* sweep cap (the reference has none) */
+} gasx_es_params;
+
+typedef struct gasx_es_result {
+    int32_t iterations;         /* Jacobi sweeps done */
+    int32_t converged;          /* 1: stopped by the rule, 0: by max_iter */
+    double n_e0;                /* final electron density scale */
+    double max_dphi, mean_phi;  /* the stopping quantities of the last sweep */
+    double int_rho_i;           /* trapz(trapz(rho_i .* R)) (Poisson.m:22) */
+} gasx_es_result;
+
+gasx_status gasx_ion_es_push(gasx_ctx* ctx, gasx_species* ions, int n_species, const gasx_es_params* p,
+                             double* phi, double* rho_i, double* rho_e, double* e_r, double* e_z,
+                             gasx_es_result* out);
 
 /* ---- Differentiable run and its adjoint ------------------------------------
  * No counterpart in the reference, which has no gradients: these entry points

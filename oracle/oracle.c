@@ -1036,3 +1036,146 @@ int orc_photoionize(const orc_params* p, const orc_gas* g, double* dens, double*
     free(N); free(cnt);
     return 0;
 }
+
+
+/* ------------------------------------------------------------------ */
+/* Electrostatic ion step: ion_staticelectrofield_push.m:21-156 with
+ * Poisson.m:25-68 on Poisson.m's cell-centre grid (R = (i + 1/2) delta,
+ * Poisson.m:11-12).  Where the two files disagree (ion charge on the nodes
+ * 0:delta:R, a clamp of the linear cell index) the cell that holds the ion,
+ * clamped in r and z separately, is used; Poisson.m's outer loop repeats one
+ * identical solve, which is done once. */
+static const double ES_E = 1.6e-19, ES_KB = 1.38064853e-23, ES_EPS = 8.854187817e-12;   /* Poisson.m:2-8 */
+
+static int es_cell(const orc_es_params* p, double x, double y, double z)
+{
+    double r = sqrt(x * x + y * y);
+    int ir = (int)floor(r / p->delta), iz = (int)floor(z / p->delta);
+    if (ir < 0) ir = 0;
+    if (ir > p->nr - 1) ir = p->nr - 1;
+    if (iz < 0) iz = 0;
+    if (iz > p->nz - 1) iz = p->nz - 1;
+    return iz * p->nr + ir;
+}
+
+static double es_trap_w(int i, int n) { return (i == 0 || i == n - 1) ? 0.5 : 1.0; }
+
+void orc_es_poisson(const orc_es_params* p, const double* rho_i, double* phi, double* rho_e, orc_es_info* info)
+{
+    const int nr = p->nr, nz = p->nz, N = nr * nz;
+    const double d = p->delta, d2 = d * d, den = 2 / d2 + 2 / d2;
+    const double beta = ES_E / (ES_KB * p->t_e);
+    double int_rho_i = 0.0;                                 /* Poisson.m:22 */
+    for (int c = 0; c < N; ++c) {
+        int i = c % nr, j = c / nr;
+        int_rho_i += es_trap_w(i, nr) * es_trap_w(j, nz) * (rho_i[c] * ((i + 0.5) * d));
+    }
+    int_rho_i *= d2;
+    double* nxt = (double*)malloc((size_t)N * sizeof(double));
+    for (int c = 0; c < N; ++c) { phi[c] = 0.0; rho_e[c] = 0.0; }   /* :28-31 */
+    double n_e0 = p->n_e0, maxd = 0.0, mean = 0.0;
+    int k = 0;
+    for (;;) {                                               /* :36 */
+        ++k;
+        double se = 0.0;
+        for (int j = 0; j < nz; ++j)
+            for (int i = 1; i < nr; ++i) {                   /* :41-56 (row 0 below) */
+                int c = j * nr + i;
+                double nv = 0.0;                             /* PHI(lenr,:), PHI(:,1), PHI(:,lenz) = 0 */
+                if (i < nr - 1 && j > 0 && j < nz - 1) {
+                    double pc = phi[c];
+                    double re = p->boltzmann ? ES_E * n_e0 * exp(beta * pc) : 0.0;
+                    rho_e[c] = re;
+                    double RHO = rho_i[c] - re;
+                    double R = (i + 0.5) * d;
+                    double g = (RHO / ES_EPS + (phi[c + 1] + phi[c - 1]) / d2 +
+                                1 / (R + d) * (phi[c + 1] - phi[c - 1]) / (2 * d) +
+                                (phi[c - nr] + phi[c + nr]) / d2) / den;
+                    nv = g > 0 ? g : pc;                     /* :58 PHI(g>0) = g(g>0) */
+                    se += es_trap_w(i, nr) * es_trap_w(j, nz) * (re * R);
+                }
+                nxt[c] = nv;
+            }
+        for (int j = 0; j < nz; ++j)                         /* :59 PHI(1,:) = PHI(2,:); then :61-62 */
+            nxt[j * nr] = (j == 0 || j == nz - 1) ? 0.0 : nxt[j * nr + 1];
+        double md = 0.0, sp = 0.0;
+        for (int c = 0; c < N; ++c) {
+            double dd = fabs(phi[c] - nxt[c]);
+            if (dd > md) md = dd;
+            sp += nxt[c];
+        }
+        if (p->boltzmann) n_e0 = n_e0 * int_rho_i / (se * d2);   /* :57 */
+        maxd = md;
+        mean = sp / (double)N;
+        memcpy(phi, nxt, (size_t)N * sizeof(double));
+        if (!(maxd > p->tol * mean && k < p->max_iter)) break;
+    }
+    free(nxt);
+    info->iterations = k;
+    info->converged = maxd > p->tol * mean ? 0 : 1;
+    info->n_e0 = n_e0;
+    info->max_dphi = maxd;
+    info->mean_phi = mean;
+    info->int_rho_i = int_rho_i;
+}
+
+void orc_es_field(const orc_es_params* p, const double* phi, double* er, double* ez)
+{
+    const int nr = p->nr, nz = p->nz;
+    const double d = p->delta;
+    for (int j = 0; j < nz; ++j)
+        for (int i = 0; i < nr; ++i) {
+            int c = j * nr + i;
+            double gr = i == 0 ? (phi[c + 1] - phi[c]) / d
+                      : i == nr - 1 ? (phi[c] - phi[c - 1]) / d : (phi[c + 1] - phi[c - 1]) / (2 * d);
+            double gz = j == 0 ? (phi[c + nr] - phi[c]) / d
+                      : j == nz - 1 ? (phi[c] - phi[c - nr]) / d : (phi[c + nr] - phi[c - nr]) / (2 * d);
+            er[c] = -gr;
+            ez[c] = -gz;
+        }
+}
+
+void orc_es_push(const orc_es_params* p, int nsp, double* const* x, double* const* y, double* const* z,
+                 double* const* vx, double* const* vy, double* const* vz, const int64_t* n, const double* q,
+                 const double* m, double* phi, double* rho_i, double* rho_e, double* er, double* ez,
+                 orc_es_info* info)
+{
+    const int nr = p->nr, N = p->nr * p->nz;
+    const double d = p->delta, hdt = 0.5 * p->dt;
+    int32_t* cnt = (int32_t*)calloc((size_t)nsp * N, sizeof(int32_t));
+    for (int s = 0; s < nsp; ++s)                            /* :28-31 half drift, :48-66 cells */
+        for (int64_t a = 0; a < n[s]; ++a) {
+            if (z[s][a] != z[s][a]) continue;
+            x[s][a] = x[s][a] + hdt * vx[s][a];
+            y[s][a] = y[s][a] + hdt * vy[s][a];
+            z[s][a] = z[s][a] + hdt * vz[s][a];
+            cnt[(size_t)s * N + es_cell(p, x[s][a], y[s][a], z[s][a])]++;
+        }
+    for (int c = 0; c < N; ++c) {                            /* :68-72 */
+        int i = c % nr;
+        double qq = 0.0;
+        for (int s = 0; s < nsp; ++s) qq = qq + p->weight * ES_E * q[s] * (double)cnt[(size_t)s * N + c];
+        double V = 0.5 * p->phip * ((double)((i + 1) * (i + 1) - i * i) * d * d) * d;
+        rho_i[c] = qq / V;
+    }
+    free(cnt);
+    orc_es_poisson(p, rho_i, phi, rho_e, info);
+    orc_es_field(p, phi, er, ez);
+    for (int s = 0; s < nsp; ++s) {                          /* :112-147 */
+        double k = q[s] * ES_E / m[s];
+        for (int64_t a = 0; a < n[s]; ++a) {
+            if (z[s][a] != z[s][a]) continue;
+            double xh = x[s][a], yh = y[s][a], zh = z[s][a];
+            int c = es_cell(p, xh, yh, zh);
+            double r = sqrt(xh * xh + yh * yh);
+            double ct = r > 0 ? xh / r : 1.0, st = r > 0 ? yh / r : 0.0;
+            double fr = k * er[c], fz = k * ez[c];
+            vx[s][a] = vx[s][a] + fr * ct * p->dt;
+            vy[s][a] = vy[s][a] + fr * st * p->dt;
+            vz[s][a] = vz[s][a] + fz * p->dt;
+            x[s][a] = xh + hdt * vx[s][a];
+            y[s][a] = yh + hdt * vy[s][a];
+            z[s][a] = zh + hdt * vz[s][a];
+        }
+    }
+}

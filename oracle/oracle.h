@@ -157,6 +157,29 @@ int orc_photoionize(const orc_params* p, const orc_gas* g, double* dens, double*
  * from the proposals (r1, r2) pairs of the source's blocks (findPEangle.m:16-33) */
 double orc_sauter_v(double dE_joule, uint64_t seed, uint32_t event, uint32_t pulse, uint32_t run);
 
+/* ---- electrostatic ion step (ion_staticelectrofield_push.m:21-156, Poisson.m:25-68) ---- */
+typedef struct {
+    int32_t nr, nz;
+    double delta, phip, weight, t_e, dt, n_e0, tol;
+    int32_t max_iter;
+    int32_t boltzmann;      /* 1: Poisson.m's Boltzmann electrons; 0: none (rho_e = 0, n_e0 kept) */
+} orc_es_params;
+typedef struct {
+    int32_t iterations, converged;
+    double n_e0, max_dphi, mean_phi, int_rho_i;
+} orc_es_info;
+/* Poisson.m's iteration for the charge density rho_i (nr*nz, index j*nr + i):
+ * phi and rho_e (outputs, nr*nz) */
+void orc_es_poisson(const orc_es_params* p, const double* rho_i, double* phi, double* rho_e, orc_es_info* info);
+/* E = -gradient(PHI) with MATLAB's rule (Poisson.m:69-71) */
+void orc_es_field(const orc_es_params* p, const double* phi, double* er, double* ez);
+/* The whole step for nsp species (x[s] .. vz[s] arrays of n[s] ions, charge
+ * number q[s], mass m[s]; z = NaN marks an absorbed ion), in place. */
+void orc_es_push(const orc_es_params* p, int nsp, double* const* x, double* const* y, double* const* z,
+                 double* const* vx, double* const* vy, double* const* vz, const int64_t* n, const double* q,
+                 const double* m, double* phi, double* rho_i, double* rho_e, double* er, double* ez,
+                 orc_es_info* info);
+
 /* ---- tallies (electricmagneticfield5.m:167-177) ---- */
 void orc_tallies(const orc_params* p, const double* er, const double* ez, const double* h,
                  const double* vx, const double* vy, const double* vz, const double* m,

@@ -65,6 +65,17 @@ class OrcPhotoInfo(C.Structure):
                 ("auger", C.c_int32), ("n_rings", C.c_int32), ("n_events", C.c_int64)]
 
 
+class OrcEsParams(C.Structure):
+    _fields_ = [("nr", C.c_int32), ("nz", C.c_int32), ("delta", C.c_double), ("phip", C.c_double),
+                ("weight", C.c_double), ("t_e", C.c_double), ("dt", C.c_double), ("n_e0", C.c_double),
+                ("tol", C.c_double), ("max_iter", C.c_int32), ("boltzmann", C.c_int32)]
+
+
+class OrcEsInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("n_e0", C.c_double),
+                ("max_dphi", C.c_double), ("mean_phi", C.c_double), ("int_rho_i", C.c_double)]
+
+
 _lib = None
 
 
@@ -116,6 +127,10 @@ def lib() -> C.CDLL:
                                       C.POINTER(OrcPhotoInfo)]
         L.orc_sauter_v.restype = C.c_double
         L.orc_sauter_v.argtypes = [C.c_double, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.orc_es_poisson.argtypes = [C.POINTER(OrcEsParams), _P, _P, _P, C.POINTER(OrcEsInfo)]
+        L.orc_es_field.argtypes = [C.POINTER(OrcEsParams), _P, _P, _P]
+        L.orc_es_push.argtypes = [C.POINTER(OrcEsParams), C.c_int] + [_P] * 6 + [_P, _P, _P] + [_P] * 5 + \
+            [C.POINTER(OrcEsInfo)]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = None
         L.orc_get_threads.restype = C.c_int
@@ -314,3 +329,47 @@ def photoionize(p, dens, gdr, gdz, photon_kev, n_photons, beam_radius, tables, t
     return {"pe": pe[:, :n], "ae": ae[:, :n] if info.auger else None, "ion": ion[:, :n], "vth": vth[:, :n],
             "cell": cell[:n], "dens": d, "temp": t,
             "info": {k: getattr(info, k) for k, _ in OrcPhotoInfo._fields_}}
+
+
+def es_params(nr, nz, delta, phip, weight, t_e, dt, n_e0=1e15, tol=0.01, max_iter=1_000_000, boltzmann=True):
+    return OrcEsParams(nr, nz, delta, phip, weight, t_e, dt, n_e0, tol, max_iter, int(boltzmann))
+
+
+def es_poisson(p: OrcEsParams, rho_i):
+    """Poisson.m's iteration restated (orc_es_poisson); rho_i (nz, nr)."""
+    rho_i = np.ascontiguousarray(rho_i, dtype=np.float64)
+    phi = np.zeros_like(rho_i)
+    rho_e = np.zeros_like(rho_i)
+    info = OrcEsInfo()
+    lib().orc_es_poisson(C.byref(p), ptr(rho_i), ptr(phi), ptr(rho_e), C.byref(info))
+    return phi, rho_e, {k: getattr(info, k) for k, _ in OrcEsInfo._fields_}
+
+
+def es_field(p: OrcEsParams, phi):
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    er, ez = np.zeros_like(phi), np.zeros_like(phi)
+    lib().orc_es_field(C.byref(p), ptr(phi), ptr(er), ptr(ez))
+    return er, ez
+
+
+def es_push(p: OrcEsParams, species):
+    """ion_staticelectrofield_push.m restated (orc_es_push).  species: list
+    of (state (6, n) FP64, charge number, mass); states are pushed in place.
+    Returns the grids (nz, nr) and the info."""
+    nsp = len(species)
+    rows = [np.ascontiguousarray(st, dtype=np.float64) for st, _, _ in species]
+    for st, r in zip([s for s, _, _ in species], rows):
+        if r is not st:
+            raise ValueError("species states must be C-contiguous FP64 (6, n) arrays")
+    PA = C.c_void_p * nsp
+    cols = [PA(*[r[k].ctypes.data for r in rows]) for k in range(6)]
+    n = (C.c_int64 * nsp)(*[r.shape[1] for r in rows])
+    q = (C.c_double * nsp)(*[qq for _, qq, _ in species])
+    m = (C.c_double * nsp)(*[mm for _, _, mm in species])
+    shape = (p.nz, p.nr)
+    phi, rho_i, rho_e, er, ez = (np.zeros(shape) for _ in range(5))
+    info = OrcEsInfo()
+    lib().orc_es_push(C.byref(p), nsp, *cols, n, q, m, ptr(phi), ptr(rho_i), ptr(rho_e), ptr(er), ptr(ez),
+                      C.byref(info))
+    return {"phi": phi, "rho_i": rho_i, "rho_e": rho_e, "e_r": er, "e_z": ez,
+            "info": {k: getattr(info, k) for k, _ in OrcEsInfo._fields_}}
