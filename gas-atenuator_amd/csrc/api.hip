@@ -408,6 +408,7 @@ extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     batch_release(ctx);
     diff_release(ctx);
+    dsmc_release(ctx);
     if (ctx->own_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return GASX_OK;

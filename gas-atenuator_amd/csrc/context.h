@@ -36,6 +36,8 @@ struct BatchState {
 
 // Recorded events, checkpoints and buffers of the differentiable run (adjoint.hip).
 struct DiffState;
+// Sort buffers, cell map and counters of the DSMC neutral-gas step (dsmc.hip).
+struct DsmcState;
 
 struct gasx_ctx {
     int device = 0;
@@ -84,6 +86,8 @@ struct gasx_ctx {
     gasx_batch_tuning tuning{16, 32, 16};
     // differentiable run (at most one per context)
     DiffState* diff = nullptr;
+    // DSMC neutral gas
+    DsmcState* dsmc = nullptr;
 };
 
 namespace gasx_host {
@@ -180,6 +184,8 @@ int particle_grid(const gasx_ctx* ctx, int64_t segments, int per_cu);
 void batch_release(gasx_ctx* ctx);
 // Frees the differentiable run (adjoint.hip).
 void diff_release(gasx_ctx* ctx);
+// Frees the DSMC buffers (dsmc.hip).
+void dsmc_release(gasx_ctx* ctx);
 
 }  // namespace gasx_host
 

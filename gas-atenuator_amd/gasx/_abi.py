@@ -162,6 +162,29 @@ class EsResult(C.Structure):
     ]
 
 
+class CellMap(C.Structure):
+    _fields_ = [("n_rings", C.c_int32), ("n_layers", C.c_int32), ("rp", C.c_double), ("lp", C.c_double),
+                ("phip", C.c_double), ("delta", C.c_double)]
+
+
+class Molecules(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("vx", C.c_void_p),
+                ("vy", C.c_void_p), ("vz", C.c_void_p), ("ei", C.c_void_p), ("count", C.c_void_p),
+                ("capacity", C.c_int64)]
+
+
+class DsmcParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("fn", C.c_double), ("mass", C.c_double), ("diameter", C.c_double),
+                ("sc_max", C.c_double), ("omega", C.c_double), ("zeta", C.c_double), ("alpha", C.c_double),
+                ("min_attempts", C.c_int32), ("reorder", C.c_int32)]
+
+
+class DsmcCounters(C.Structure):
+    _fields_ = [("n_attempts", C.c_int64), ("n_collisions", C.c_int64), ("n_sc_exceed", C.c_int64),
+                ("n_boundary", C.c_int64), ("n_wall_r", C.c_int64), ("n_wall_z", C.c_int64),
+                ("n_wrap", C.c_int64)]
+
+
 # exported symbols and their prototypes (every declaration of include/gasx.h)
 _P = C.c_void_p
 PROTOTYPES = {
@@ -204,6 +227,16 @@ PROTOTYPES = {
                                    C.POINTER(PhotoSource), C.POINTER(PhotoResult)]),
     "gasx_ion_es_push": (C.c_int, [_P, C.POINTER(Species), C.c_int, C.POINTER(EsParams), _P, _P, _P, _P, _P,
                                    C.POINTER(EsResult)]),
+    "gasx_cell_map_info": (C.c_int, [C.POINTER(CellMap), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "gasx_dsmc_cells": (C.c_int, [_P, C.POINTER(Molecules), C.POINTER(CellMap), _P]),
+    "gasx_dsmc_move": (C.c_int, [_P, C.POINTER(Molecules), C.POINTER(CellMap), C.c_double]),
+    "gasx_dsmc_collide": (C.c_int, [_P, C.POINTER(Molecules), C.POINTER(CellMap), C.POINTER(DsmcParams),
+                                    C.c_uint64, C.c_uint32]),
+    "gasx_dsmc_step": (C.c_int, [_P, C.POINTER(Molecules), C.POINTER(CellMap), C.POINTER(DsmcParams),
+                                 C.c_uint64, C.c_uint32]),
+    "gasx_dsmc_sample": (C.c_int, [_P, C.POINTER(Molecules), C.POINTER(CellMap), C.c_double, C.c_double,
+                                   _P, _P, _P, _P]),
+    "gasx_dsmc_get_counters": (C.c_int, [_P, C.POINTER(DsmcCounters), C.c_int]),
     "gasx_diff_record": (C.c_int, [_P, C.POINTER(Species), C.POINTER(DiffSpec)]),
     "gasx_diff_forward": (C.c_int, [_P, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_double)]),
     "gasx_diff_backward": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -212,7 +245,7 @@ PROTOTYPES = {
     "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 _lib = None
 

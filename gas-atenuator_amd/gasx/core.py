@@ -345,6 +345,13 @@ class Context:
     def synchronize(self):
         self._check(self.lib.gasx_synchronize(self.ctx))
 
+    # ---- DSMC neutral gas ------------------------------------------------------------
+    @property
+    def dsmc(self):
+        """The DSMC entry points (gasx.dsmc.Dsmc) of this context."""
+        from .dsmc import Dsmc
+        return Dsmc(self)
+
 
 class Ensemble:
     """Independent runs on the context's grid, advanced together by

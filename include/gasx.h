@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 6
+#define GASX_ABI_VERSION 7
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -367,6 +367,82 @@ typedef struct gasx_es_result {
 gasx_status gasx_ion_es_push(gasx_ctx* ctx, gasx_species* ions, int n_species, const gasx_es_params* p,
                              double* phi, double* rho_i, double* rho_e, double* e_r, double* e_z,
                              gasx_es_result* out);
+
+/* ---- DSMC neutral gas -----------------------------------------------------------
+ * Replaces the molecule step of main2.m:247-271: free_motion.m:3-7,
+ * boundary_conditions_specular.m:6-64 (specular_scattering_xy.m,
+ * boundary_phi_0.m / boundary_phi_phimax.m, specular_scattering_z.m),
+ * put_into_cells_symmetry.m:4-16 and calculate_VC.m:5-17 on the cell map of
+ * cell_structure.m:6-22, and collision_HS_internal.m:11-158 (no-time-counter
+ * hard-sphere pairs, Larsen-Borgnakke exchange of internal energy, VHS or VSS
+ * scattering).  Molecules are FP32 structure-of-arrays (x, y, z, vx, vy, vz in
+ * the wedge's Cartesian frame, ei internal energy in J); the arithmetic is
+ * FP64.  A molecule's cell is that of its stored position: ring
+ * ceil(r / dr), phi cell ceil(phi / dphi(ring)), layer ceil(z / dz), clamped
+ * into the map; its molecules are taken in ascending slot order (the
+ * reference's `find(INDEX == indexnum)`).  Each cell runs
+ * max(Npair, min_attempts) attempts, Npair = floor(SCmax FN N (N-1) / VC
+ * 0.5 dt), when it holds more than 2 molecules; every attempt draws its
+ * random numbers from Philox4x32-10 with key = seed and counter = (attempt,
+ * cell, step, block), so the result does not depend on how the GPU schedules
+ * the attempts.  VC is the volume of the cell's ring cell (the reference's
+ * VC holds one layer; every layer has the same cells). */
+typedef struct gasx_cell_map {
+    int32_t n_rings, n_layers;  /* rmax (r_divide) and zmax (z_divide) */
+    double rp, lp, phip;        /* Rp, Lp, PHIp */
+    double delta;               /* target arc length of cell_structure.m (main2.m:42 lambda) */
+} gasx_cell_map;
+
+typedef struct gasx_molecules {
+    float *x, *y, *z, *vx, *vy, *vz, *ei;   /* device, capacity each */
+    int32_t* count;                          /* device int32: slots in use; z = NaN marks an unused slot */
+    int64_t capacity;
+} gasx_molecules;
+
+typedef struct gasx_dsmc_params {
+    double dt;                  /* main2.m:44 */
+    double fn;                  /* real molecules per simulated one (main2.m:47) */
+    double mass;                /* m (main2.m:15) */
+    double diameter;            /* d of the hard-sphere cross section (main2.m:14) */
+    double sc_max;              /* SCmax = max of cr sigma (main2.m:35) */
+    double omega;               /* w (main2.m:16), < 1.5 */
+    double zeta;                /* internal degrees of freedom (main2.m:17), >= 2 */
+    double alpha;               /* alpha_ (main2.m:36): 1 = VHS isotropic scattering, else VSS */
+    int32_t min_attempts;       /* 1: the reference's loop, which makes one attempt even when Npair = 0; 0: Npair */
+    int32_t reorder;            /* 1: permute the molecules into cell order (stable) before colliding;
+                                   unused slots are dropped and count reduced */
+} gasx_dsmc_params;
+
+typedef struct gasx_dsmc_counters {
+    int64_t n_attempts, n_collisions;
+    int64_t n_sc_exceed;        /* attempts with cr sigma > SCmax (n_SC, collision_HS_internal.m:48-51) */
+    int64_t n_boundary;         /* N_boundary of boundary_conditions_specular.m */
+    int64_t n_wall_r, n_wall_z, n_wrap;
+} gasx_dsmc_counters;
+
+/* Cells per layer (cellnumxy) and phi cells per ring (n_rings entries, may be NULL). */
+gasx_status gasx_cell_map_info(const gasx_cell_map* m, int32_t* phi_divide, int64_t* cells_per_layer);
+/* 0-based cell of every slot below the count (device int32; n_cells for an unused slot). */
+gasx_status gasx_dsmc_cells(gasx_ctx* ctx, const gasx_molecules* mol, const gasx_cell_map* map, int32_t* cell);
+/* free_motion.m + boundary_conditions_specular.m: one move of duration dt. */
+gasx_status gasx_dsmc_move(gasx_ctx* ctx, gasx_molecules* mol, const gasx_cell_map* map, double dt);
+/* collision_HS_internal.m on the current positions (synchronizes once: the sort takes the count on the host). */
+gasx_status gasx_dsmc_collide(gasx_ctx* ctx, gasx_molecules* mol, const gasx_cell_map* map,
+                              const gasx_dsmc_params* p, uint64_t seed, uint32_t step);
+/* main2.m:250-264: move, cells, collisions (synchronizes once). */
+gasx_status gasx_dsmc_step(gasx_ctx* ctx, gasx_molecules* mol, const gasx_cell_map* map,
+                           const gasx_dsmc_params* p, uint64_t seed, uint32_t step);
+/* The gas the electron step reads (particle_density0.m:24-36, count_T_numbensity.m:11-13):
+ * per cell of the map density FN N / VC and temperature m / (3 kB) mean(v^2)
+ * (cell_density, cell_temperature: device, n_cells, may be NULL), and the same
+ * per (ring, layer) cell of the map (rz_density, rz_temperature: device,
+ * n_layers x n_rings, index iz * n_rings + ir, the gasx_gas layout with
+ * gdr = Rp / n_rings, gdz = Lp / n_layers; may be NULL).  Empty cells: 0. */
+gasx_status gasx_dsmc_sample(gasx_ctx* ctx, const gasx_molecules* mol, const gasx_cell_map* map, double fn,
+                             double mass, float* cell_density, float* cell_temperature, float* rz_density,
+                             float* rz_temperature);
+/* Counters accumulated by the DSMC calls (synchronizes). */
+gasx_status gasx_dsmc_get_counters(gasx_ctx* ctx, gasx_dsmc_counters* out, int reset);
 
 /* ---- Differentiable run and its adjoint ------------------------------------
  * No counterpart in the reference, which has no gradients: these entry points

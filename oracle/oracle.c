@@ -1179,3 +1179,343 @@ void orc_es_push(const orc_es_params* p, int nsp, double* const* x, double* cons
         }
     }
 }
+
+
+/* ------------------------------------------------------------------ */
+/* DSMC neutral-gas step (main2.m:247-264): free motion, specular walls,
+ * the (r, phi, z) cell map, and the no-time-counter hard-sphere collisions
+ * with Larsen-Borgnakke internal energy of collision_HS_internal.m.
+ *
+ * round32 = 1 rounds every stored position, velocity and internal energy to
+ * FP32 (the HIP path's storage; its arithmetic is FP64 like this file's), so
+ * the two agree bit for bit away from the last ulp of atan2/sin/cos/pow. */
+/* ------------------------------------------------------------------ */
+
+static double st32(double v, int round32) { return round32 ? (double)(float)v : v; }
+
+/* cell_structure.m:6-22 (phi_divide, dphi, sum_cell) and calculate_VC.m:5-17
+ * (one cell's volume per ring; every z layer has the same cells).  sum0 is
+ * the exclusive prefix (sum0[0] = 0, sum0[n_rings] = cellnumxy). */
+int64_t orc_cell_map_build(const orc_cell_map* m, int32_t* phi_divide, int32_t* sum0, double* dphi,
+                           double* vc_ring)
+{
+    const int rmax = m->n_rings;
+    const double dr = 1.0 * m->rp / rmax, dz = 1.0 * m->lp / m->n_layers;   /* calculate_VC.m:6-7 */
+    int64_t acc = 0;
+    for (int i = 1; i <= rmax; ++i) {
+        const double l = (double)i / rmax * m->rp * m->phip;                /* cell_structure.m:10 */
+        const int32_t pd = (int32_t)ceil(l / m->delta);                     /* :11 */
+        const double dp = m->phip / pd;                                     /* :12 */
+        const double a = (double)i * dr, b = (double)(i - 1) * dr;
+        if (phi_divide) phi_divide[i - 1] = pd;
+        if (sum0) sum0[i - 1] = (int32_t)acc;
+        if (dphi) dphi[i - 1] = dp;
+        if (vc_ring) vc_ring[i - 1] = 0.5 * (a * a - b * b) * dp * dz;      /* calculate_VC.m:15 */
+        acc += pd;
+    }
+    if (sum0) sum0[rmax] = (int32_t)acc;
+    return acc;
+}
+
+/* put_into_cells_symmetry.m:4-16 with r = sqrt(x^2 + y^2), phi = atan2(y, x)
+ * of the stored position; 1-based ceil indices, clamped into the map (the
+ * reference's index is 0 on the exact lower edges).  0-based result:
+ * (iz - 1) cellnumxy + sum0[ir - 1] + iphi - 1; -1 for z = NaN (absorbed). */
+int32_t orc_dsmc_cell(const orc_cell_map* m, const int32_t* pd, const int32_t* sum0, const double* dphi,
+                      double x, double y, double z)
+{
+    if (z != z) return -1;
+    const double dz = m->lp / m->n_layers, dr = m->rp / m->n_rings;
+    const double r = sqrt(x * x + y * y), phi = atan2(y, x);
+    double fz = ceil(z / dz), fr = ceil(r / dr);
+    int iz = fz < 1.0 ? 1 : (fz > m->n_layers ? m->n_layers : (int)fz);
+    int ir = fr < 1.0 ? 1 : (fr > m->n_rings ? m->n_rings : (int)fr);
+    double fp = ceil(phi / dphi[ir - 1]);
+    int ip = fp < 1.0 ? 1 : (fp > pd[ir - 1] ? pd[ir - 1] : (int)fp);
+    return (int32_t)((int64_t)(iz - 1) * sum0[m->n_rings] + sum0[ir - 1] + ip - 1);
+}
+
+void orc_dsmc_cells(const orc_cell_map* m, int64_t n, const double* x, const double* y, const double* z,
+                    int32_t* cell)
+{
+    int32_t* pd = (int32_t*)malloc(sizeof(int32_t) * m->n_rings);
+    int32_t* s0 = (int32_t*)malloc(sizeof(int32_t) * (m->n_rings + 1));
+    double* dp = (double*)malloc(sizeof(double) * m->n_rings);
+    orc_cell_map_build(m, pd, s0, dp, NULL);
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+    for (int64_t i = 0; i < n; ++i) cell[i] = orc_dsmc_cell(m, pd, s0, dp, x[i], y[i], z[i]);
+    free(pd); free(s0); free(dp);
+}
+
+/* free_motion.m:3-7, then boundary_conditions_specular.m:6-64 for one
+ * molecule: the r = Rp wall (specular_scattering_xy.m:5-40, once), the
+ * periodic phi faces (boundary_phi_0.m / boundary_phi_phimax.m as intended:
+ * position and velocity rotated by +-phimax; the reference's boundary_phi_0
+ * names undefined x, y and takes one argument fewer than its caller
+ * passes), then the z walls (specular_scattering_z.m:5-32).  counts:
+ * [boundary events (N_boundary), r-wall, z-wall, phi wraps]. */
+static void dsmc_move_one(const orc_cell_map* m, double dt, double* px, double* py, double* pz, double* pvx,
+                          double* pvy, double* pvz, int64_t c[4])
+{
+    const double Rp = m->rp, Lp = m->lp, pm = m->phip;
+    double vx = *pvx, vy = *pvy, vz = *pvz;
+    double x = *px + vx * dt, y = *py + vy * dt, z = *pz + vz * dt;     /* free_motion.m:3-5 */
+    double r = sqrt(x * x + y * y), phi = atan2(y, x);                   /* :6-7 */
+    if (r > Rp) {                                                        /* specular_scattering_xy.m */
+        c[0]++; c[1]++;
+        const double x0 = x - vx * dt, y0 = y - vy * dt;
+        const double r0 = sqrt(x0 * x0 + y0 * y0);
+        const double t1 = ((r - Rp) / (r - r0)) * dt;                    /* :15 */
+        const double xi = x - vx * t1, yi = y - vy * t1, zi = z - vz * t1;
+        const double th = atan2(yi, xi);                                 /* :23 */
+        const double ct = cos(th), s = sin(th);
+        double vr = vx * ct + vy * s;
+        const double vt = -vx * s + vy * ct;
+        vr = -vr;                                                        /* :30 */
+        vx = vr * ct - vt * s;
+        vy = vr * s + vt * ct;
+        x = xi + vx * t1; y = yi + vy * t1; z = zi + vz * t1;            /* :38-40 */
+        r = sqrt(x * x + y * y);
+        phi = atan2(y, x);                                               /* boundary_conditions_specular.m:12-13 */
+    }
+    const double cp = cos(pm), sp = sin(pm);
+    while (phi < 0) {                                                    /* :18-29 */
+        c[0]++; c[3]++;
+        const double rxy = sqrt(x * x + y * y);
+        phi = phi + pm;
+        x = rxy * cos(phi); y = rxy * sin(phi);
+        const double nvx = vx * cp - vy * sp, nvy = vx * sp + vy * cp;
+        vx = nvx; vy = nvy;
+    }
+    while (phi > pm) {                                                   /* :31-41 */
+        c[0]++; c[3]++;
+        const double rxy = sqrt(x * x + y * y);
+        phi = phi - pm;
+        x = rxy * cos(phi); y = rxy * sin(phi);
+        const double nvx = vx * cp + vy * sp, nvy = -vx * sp + vy * cp;
+        vx = nvx; vy = nvy;
+    }
+    if (z > Lp) {                                                        /* :44-54, specular_scattering_z.m:15 */
+        c[0]++; c[2]++;
+        const double z0 = z - vz * dt;
+        const double t1 = (z - Lp) / (z - z0) * dt;
+        const double xi = x - vx * t1, yi = y - vy * t1, zi = z - vz * t1;
+        vz = -vz;
+        x = xi + vx * t1; y = yi + vy * t1; z = zi + vz * t1;
+    }
+    if (z < 0) {                                                         /* :56-64, specular_scattering_z.m:17 */
+        c[0]++; c[2]++;
+        const double z0 = z - vz * dt;
+        const double t1 = (0 - z) / (z0 - z) * dt;
+        const double xi = x - vx * t1, yi = y - vy * t1, zi = z - vz * t1;
+        vz = -vz;
+        x = xi + vx * t1; y = yi + vy * t1; z = zi + vz * t1;
+    }
+    *px = x; *py = y; *pz = z; *pvx = vx; *pvy = vy; *pvz = vz;
+}
+
+void orc_dsmc_move(const orc_cell_map* m, double dt, int64_t n, double* x, double* y, double* z, double* vx,
+                   double* vy, double* vz, int round32, int64_t counts[4])
+{
+    int64_t c[4] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        if (z[i] != z[i]) continue;
+        dsmc_move_one(m, dt, &x[i], &y[i], &z[i], &vx[i], &vy[i], &vz[i], c);
+        x[i] = st32(x[i], round32); y[i] = st32(y[i], round32); z[i] = st32(z[i], round32);
+        vx[i] = st32(vx[i], round32); vy[i] = st32(vy[i], round32); vz[i] = st32(vz[i], round32);
+    }
+    for (int k = 0; k < 4; ++k) counts[k] += c[k];
+}
+
+/* Philox counter blocks of an attempt: counter = (attempt, cell, step, block). */
+#define DSMC_BLK0 0x50000000u      /* sub1, sub2, acceptance, scattering cosine */
+#define DSMC_BLK1 0x50000001u      /* scattering azimuth */
+#define DSMC_REDRAW 0x50000100u    /* + r: the r-th redraw of an equal pair (sub1, sub2) */
+#define DSMC_ET 0x51000000u        /* + j: translational-energy rounds 2j, 2j+1 */
+#define DSMC_ETRY 0x52000000u      /* + j: internal-energy split rounds 2j, 2j+1 */
+#define DSMC_MAX_REDRAW 4096
+#define DSMC_MAX_ROUNDS 65536
+
+double orc_dsmc_u(uint32_t w) { return ((double)w + 0.5) * (1.0 / 4294967296.0); }
+
+static void dsmc_draw(uint64_t seed, uint32_t a, uint32_t cell, uint32_t step, uint32_t blk, uint32_t o[4])
+{
+    const uint32_t ctr[4] = {a, cell, step, blk};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    orc_philox4x32_10(ctr, key, o);
+}
+
+/* x^e, with the exponents 0 and 1 of zeta = 2 exact */
+static double dsmc_pow(double x, double e) { return e == 1.0 ? x : (e == 0.0 ? 1.0 : pow(x, e)); }
+
+/* collision_HS_internal.m:26-135 for one cell: n_att attempts in order on
+ * the cell's molecules list[0..N) (ascending slot).  c: [attempts,
+ * collisions, SC > SCmax (n_SC)]. */
+static void dsmc_cell(const orc_dsmc_params* p, const int64_t* list, int64_t N, double vc, uint64_t seed,
+                      uint32_t cell, uint32_t step, double* vx, double* vy, double* vz, double* ei, int round32,
+                      int64_t c[3])
+{
+    if (N <= 2) return;                                                  /* :26 */
+    const double npair = floor(p->sc_max * p->fn * (double)N * (double)(N - 1) / vc * 0.5 * p->dt);   /* :27 */
+    int64_t n_att = (int64_t)npair;
+    if (n_att < p->min_attempts) n_att = p->min_attempts;               /* the do-while of :31-135 */
+    const double d2 = p->diameter * p->diameter, m = p->mass, w = p->omega, zeta = p->zeta;
+    const double k1 = (zeta + 0.5 - w) / (1.5 - w), a1 = 1.5 - w;      /* :65 */
+    const double k2 = (zeta + 0.5 - w) / (zeta - 1), b1 = zeta - 1;
+    const double c2 = pow(2.0, zeta - 2), h = zeta / 2 - 1;             /* :77 */
+    for (int64_t a = 0; a < n_att; ++a) {
+        uint32_t o[4], q[4];
+        dsmc_draw(seed, (uint32_t)a, cell, step, DSMC_BLK0, o);
+        int64_t s1 = (int64_t)(orc_dsmc_u(o[0]) * (double)N), s2 = (int64_t)(orc_dsmc_u(o[1]) * (double)N);
+        int r = 0;
+        while (s1 == s2 && r < DSMC_MAX_REDRAW) {                        /* :32-36 */
+            ++r;
+            dsmc_draw(seed, (uint32_t)a, cell, step, DSMC_REDRAW + (uint32_t)r, q);
+            s1 = (int64_t)(orc_dsmc_u(q[0]) * (double)N);
+            s2 = (int64_t)(orc_dsmc_u(q[1]) * (double)N);
+        }
+        c[0]++;
+        if (s1 == s2) continue;
+        const int64_t i1 = list[s1], i2 = list[s2];
+        double ur = vx[i1] - vx[i2], vr = vy[i1] - vy[i2], wr = vz[i1] - vz[i2];   /* :38-40 */
+        double cr = sqrt(ur * ur + vr * vr + wr * wr);
+        const double SC = cr * M_PI * d2;                                /* :42 */
+        if (SC > p->sc_max) c[2]++;                                      /* :48-51 */
+        if (!(orc_dsmc_u(o[2]) < SC / p->sc_max)) continue;              /* :52-54 */
+        const double um = (vx[i1] + vx[i2]) / 2, vm = (vy[i1] + vy[i2]) / 2, wm = (vz[i1] + vz[i2]) / 2;
+        const double Ec = 0.25 * m * (cr * cr) + ei[i1] + ei[i2];        /* :59 */
+        double Et = 0.0;
+        for (int j = 0; j < DSMC_MAX_ROUNDS; ++j) {                       /* :63-70 */
+            if ((j & 1) == 0) dsmc_draw(seed, (uint32_t)a, cell, step, DSMC_ET + (uint32_t)(j >> 1), q);
+            Et = Ec * orc_dsmc_u(q[2 * (j & 1)]);
+            const double pr = dsmc_pow(k1 * (Et / Ec), a1) * dsmc_pow(k2 * (1 - Et / Ec), b1);
+            if (orc_dsmc_u(q[2 * (j & 1) + 1]) < pr) break;
+        }
+        const double Ei = Ec - Et;                                       /* :71 */
+        double etry = 0.0;
+        for (int j = 0; j < DSMC_MAX_ROUNDS; ++j) {                       /* :74-83 */
+            if ((j & 1) == 0) dsmc_draw(seed, (uint32_t)a, cell, step, DSMC_ETRY + (uint32_t)(j >> 1), q);
+            etry = Ei * orc_dsmc_u(q[2 * (j & 1)]);
+            const double pr = c2 * dsmc_pow(etry / Ei, h) * dsmc_pow(1 - etry / Ei, h);
+            if (orc_dsmc_u(q[2 * (j & 1) + 1]) < pr) break;
+        }
+        ei[i1] = st32(etry, round32);                                    /* :84-85 */
+        ei[i2] = st32(Ei - etry, round32);
+        const double cri = 2 * sqrt(Et / m);                             /* :86 */
+        ur = ur * cri / cr; vr = vr * cri / cr; wr = wr * cri / cr;
+        cr = cri;
+        double urf, vrf, wrf;
+        uint32_t o1[4];
+        dsmc_draw(seed, (uint32_t)a, cell, step, DSMC_BLK1, o1);
+        if (fabs(p->alpha - 1) < 1e-6) {                                 /* VHS, :91-97 */
+            const double B = 2 * orc_dsmc_u(o[3]) - 1;
+            const double A = sqrt(1 - B * B);
+            const double C = 2 * M_PI * orc_dsmc_u(o1[0]);
+            urf = cr * B;
+            vrf = cr * A * cos(C);
+            wrf = cr * A * sin(C);
+        } else {   /* VSS, :98-113, with the w' term of Bird's rotation (+ u w sin e; :108 has -, which
+                      changes |g'| and breaks energy conservation) */
+            const double B = 2 * pow(orc_dsmc_u(o[3]), 1 / p->alpha) - 1;
+            const double A = sqrt(1 - B * B);
+            const double C = 2 * M_PI * orc_dsmc_u(o1[0]);
+            const double S = sin(C), CC = cos(C);
+            const double D = sqrt(vr * vr + wr * wr);
+            if (D > 1e-6) {
+                urf = B * ur + A * S * D;
+                vrf = B * vr + A * (cr * wr * CC - ur * vr * S) / D;
+                wrf = B * wr - A * (cr * vr * CC + ur * wr * S) / D;   /* Bird's sign; the reference's - loses energy */
+            } else {
+                urf = B * ur;
+                vrf = A * CC * ur;
+                wrf = A * S * ur;
+            }
+        }
+        vx[i1] = st32(um + 0.5 * urf, round32); vx[i2] = st32(um - 0.5 * urf, round32);   /* :115-120 */
+        vy[i1] = st32(vm + 0.5 * vrf, round32); vy[i2] = st32(vm - 0.5 * vrf, round32);
+        vz[i1] = st32(wm + 0.5 * wrf, round32); vz[i2] = st32(wm - 0.5 * wrf, round32);
+        c[1]++;
+    }
+}
+
+/* collision_HS_internal.m:11-158: every cell's molecules (ascending slot,
+ * `a = find(INDEX == indexnum)`) collide in their own sequence of attempts;
+ * cells are independent (the reference's parfor).  cell[i] from
+ * orc_dsmc_cells (-1 skipped).  counts: [attempts, collisions, n_SC]. */
+void orc_dsmc_collide(const orc_cell_map* m, const orc_dsmc_params* p, int64_t n, const int32_t* cell,
+                      double* vx, double* vy, double* vz, double* ei, uint64_t seed, uint32_t step, int round32,
+                      int64_t counts[3])
+{
+    int32_t* pd = (int32_t*)malloc(sizeof(int32_t) * m->n_rings);
+    int32_t* s0 = (int32_t*)malloc(sizeof(int32_t) * (m->n_rings + 1));
+    double* vcr = (double*)malloc(sizeof(double) * m->n_rings);
+    const int64_t nxy = orc_cell_map_build(m, pd, s0, NULL, vcr);
+    const int64_t ncell = nxy * m->n_layers;
+    int64_t* start = (int64_t*)calloc((size_t)ncell + 1, sizeof(int64_t));
+    int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* ring = (int32_t*)malloc(sizeof(int32_t) * (size_t)nxy);
+    for (int i = 0; i < m->n_rings; ++i)
+        for (int k = s0[i]; k < s0[i + 1]; ++k) ring[k] = i;
+    for (int64_t i = 0; i < n; ++i)
+        if (cell[i] >= 0) start[cell[i] + 1]++;
+    for (int64_t c = 0; c < ncell; ++c) start[c + 1] += start[c];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ncell > 0 ? ncell : 1));
+    for (int64_t c = 0; c < ncell; ++c) fill[c] = start[c];
+    for (int64_t i = 0; i < n; ++i)
+        if (cell[i] >= 0) list[fill[cell[i]]++] = i;
+    int64_t tot[3] = {0, 0, 0};
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 4) if (g_threads > 1)
+    for (int64_t c = 0; c < ncell; ++c) {
+        int64_t cc[3] = {0, 0, 0};
+        dsmc_cell(p, list + start[c], start[c + 1] - start[c], vcr[ring[c % nxy]], seed, (uint32_t)c, step, vx, vy,
+                  vz, ei, round32, cc);
+#pragma omp atomic
+        tot[0] += cc[0];
+#pragma omp atomic
+        tot[1] += cc[1];
+#pragma omp atomic
+        tot[2] += cc[2];
+    }
+    for (int k = 0; k < 3; ++k) counts[k] += tot[k];
+    free(pd); free(s0); free(vcr); free(start); free(list); free(ring); free(fill);
+}
+
+/* particle_density0.m:24-36 (FN count / VC per cell) and
+ * count_T_numbensity.m:11-13 (T = m / 3 / kB mean(v.^2) of the cell's
+ * molecules, v the speed) on the cell map; and the same on the (ring,
+ * layer) grid of the map (rz arrays n_layers x n_rings, index iz n_rings +
+ * ir: the gas grid of the electron step).  Empty cells: density 0, T 0. */
+void orc_dsmc_sample(const orc_cell_map* m, int64_t n, const int32_t* cell, const double* vx, const double* vy,
+                     const double* vz, double fn, double mass, double kb, double* cdens, double* ctemp,
+                     double* rdens, double* rtemp)
+{
+    int32_t* pd = (int32_t*)malloc(sizeof(int32_t) * m->n_rings);
+    int32_t* s0 = (int32_t*)malloc(sizeof(int32_t) * (m->n_rings + 1));
+    double* vcr = (double*)malloc(sizeof(double) * m->n_rings);
+    const int64_t nxy = orc_cell_map_build(m, pd, s0, NULL, vcr);
+    const int64_t ncell = nxy * m->n_layers;
+    int64_t* cnt = (int64_t*)calloc((size_t)ncell, sizeof(int64_t));
+    double* s2 = (double*)calloc((size_t)ncell, sizeof(double));
+    for (int64_t i = 0; i < n; ++i) {
+        if (cell[i] < 0) continue;
+        const double v = sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        cnt[cell[i]]++;
+        s2[cell[i]] += v * v;
+    }
+    for (int iz = 0; iz < m->n_layers; ++iz)
+        for (int ir = 0; ir < m->n_rings; ++ir) {
+            int64_t rc = 0;
+            double rs = 0.0;
+            for (int k = s0[ir]; k < s0[ir + 1]; ++k) {
+                const int64_t c = (int64_t)iz * nxy + k;
+                if (cdens) cdens[c] = fn * (double)cnt[c] / vcr[ir];
+                if (ctemp) ctemp[c] = cnt[c] ? mass / 3 / kb * (s2[c] / (double)cnt[c]) : 0.0;
+                rc += cnt[c];
+                rs += s2[c];
+            }
+            const int64_t g = (int64_t)iz * m->n_rings + ir;
+            if (rdens) rdens[g] = fn * (double)rc / (vcr[ir] * pd[ir]);
+            if (rtemp) rtemp[g] = rc ? mass / 3 / kb * (rs / (double)rc) : 0.0;
+        }
+    free(pd); free(s0); free(vcr); free(cnt); free(s2);
+}

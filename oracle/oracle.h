@@ -180,6 +180,36 @@ void orc_es_push(const orc_es_params* p, int nsp, double* const* x, double* cons
                  const double* m, double* phi, double* rho_i, double* rho_e, double* er, double* ez,
                  orc_es_info* info);
 
+/* ---- DSMC neutral-gas step (main2.m:247-264, collision_HS_internal.m) ---- */
+typedef struct {
+    int32_t n_rings, n_layers;   /* rmax (r_divide), zmax (z_divide) */
+    double rp, lp, phip;         /* Rp, Lp, PHIp */
+    double delta;                /* target arc length of cell_structure.m */
+} orc_cell_map;
+typedef struct {
+    double dt, fn, mass, diameter, sc_max, omega, zeta, alpha;
+    int32_t min_attempts;        /* 1: the reference's do-while (one attempt when Npair = 0) */
+} orc_dsmc_params;
+/* cell_structure.m + calculate_VC.m: returns cellnumxy; any output may be NULL
+ * (phi_divide, dphi, vc_ring: n_rings; sum0: n_rings + 1, exclusive prefix) */
+int64_t orc_cell_map_build(const orc_cell_map* m, int32_t* phi_divide, int32_t* sum0, double* dphi,
+                           double* vc_ring);
+int32_t orc_dsmc_cell(const orc_cell_map* m, const int32_t* pd, const int32_t* sum0, const double* dphi,
+                      double x, double y, double z);
+void orc_dsmc_cells(const orc_cell_map* m, int64_t n, const double* x, const double* y, const double* z,
+                    int32_t* cell);
+/* counts += [boundary events, r wall, z walls, phi wraps] */
+void orc_dsmc_move(const orc_cell_map* m, double dt, int64_t n, double* x, double* y, double* z, double* vx,
+                   double* vy, double* vz, int round32, int64_t counts[4]);
+double orc_dsmc_u(uint32_t w);
+/* counts += [attempts, collisions, SC > SCmax] */
+void orc_dsmc_collide(const orc_cell_map* m, const orc_dsmc_params* p, int64_t n, const int32_t* cell,
+                      double* vx, double* vy, double* vz, double* ei, uint64_t seed, uint32_t step, int round32,
+                      int64_t counts[3]);
+void orc_dsmc_sample(const orc_cell_map* m, int64_t n, const int32_t* cell, const double* vx, const double* vy,
+                     const double* vz, double fn, double mass, double kb, double* cdens, double* ctemp,
+                     double* rdens, double* rtemp);
+
 /* ---- tallies (electricmagneticfield5.m:167-177) ---- */
 void orc_tallies(const orc_params* p, const double* er, const double* ez, const double* h,
                  const double* vx, const double* vy, const double* vz, const double* m,

@@ -76,6 +76,17 @@ class OrcEsInfo(C.Structure):
                 ("max_dphi", C.c_double), ("mean_phi", C.c_double), ("int_rho_i", C.c_double)]
 
 
+class OrcCellMap(C.Structure):
+    _fields_ = [("n_rings", C.c_int32), ("n_layers", C.c_int32), ("rp", C.c_double), ("lp", C.c_double),
+                ("phip", C.c_double), ("delta", C.c_double)]
+
+
+class OrcDsmcParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("fn", C.c_double), ("mass", C.c_double), ("diameter", C.c_double),
+                ("sc_max", C.c_double), ("omega", C.c_double), ("zeta", C.c_double), ("alpha", C.c_double),
+                ("min_attempts", C.c_int32)]
+
+
 _lib = None
 
 
@@ -132,6 +143,16 @@ def lib() -> C.CDLL:
         L.orc_es_push.argtypes = [C.POINTER(OrcEsParams), C.c_int] + [_P] * 6 + [_P, _P, _P] + [_P] * 5 + \
             [C.POINTER(OrcEsInfo)]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_cell_map_build.restype = C.c_int64
+        L.orc_cell_map_build.argtypes = [C.POINTER(OrcCellMap), _P, _P, _P, _P]
+        L.orc_dsmc_cells.argtypes = [C.POINTER(OrcCellMap), C.c_int64, _P, _P, _P, _P]
+        L.orc_dsmc_move.argtypes = [C.POINTER(OrcCellMap), C.c_double, C.c_int64] + [_P] * 6 + [C.c_int, _P]
+        L.orc_dsmc_u.restype = C.c_double
+        L.orc_dsmc_u.argtypes = [C.c_uint32]
+        L.orc_dsmc_collide.argtypes = [C.POINTER(OrcCellMap), C.POINTER(OrcDsmcParams), C.c_int64, _P] + \
+            [_P] * 4 + [C.c_uint64, C.c_uint32, C.c_int, _P]
+        L.orc_dsmc_sample.argtypes = [C.POINTER(OrcCellMap), C.c_int64, _P, _P, _P, _P, C.c_double, C.c_double,
+                                      C.c_double, _P, _P, _P, _P]
         L.orc_set_threads.restype = None
         L.orc_get_threads.restype = C.c_int
         _lib = L
@@ -373,3 +394,81 @@ def es_push(p: OrcEsParams, species):
                       C.byref(info))
     return {"phi": phi, "rho_i": rho_i, "rho_e": rho_e, "e_r": er, "e_z": ez,
             "info": {k: getattr(info, k) for k, _ in OrcEsInfo._fields_}}
+
+
+# ---- DSMC neutral gas (main2.m:247-264, collision_HS_internal.m) ----------------
+def cell_map(n_rings, n_layers, rp, lp, phip, delta) -> OrcCellMap:
+    return OrcCellMap(n_rings, n_layers, rp, lp, phip, delta)
+
+
+def cell_map_build(m: OrcCellMap):
+    """cell_structure.m + calculate_VC.m: (phi_divide, sum0, dphi, vc_ring, cellnumxy)."""
+    nxy = lib().orc_cell_map_build(C.byref(m), None, None, None, None)
+    pd = np.zeros(m.n_rings, dtype=np.int32)
+    s0 = np.zeros(m.n_rings + 1, dtype=np.int32)
+    dp = np.zeros(m.n_rings)
+    vc = np.zeros(m.n_rings)
+    lib().orc_cell_map_build(C.byref(m), ptr(pd), ptr(s0), ptr(dp), ptr(vc))
+    return pd, s0, dp, vc, nxy
+
+
+def dsmc_params(dt, fn, mass, diameter, sc_max, omega=0.74, zeta=2.0, alpha=1.0, min_attempts=1) -> OrcDsmcParams:
+    return OrcDsmcParams(dt, fn, mass, diameter, sc_max, omega, zeta, alpha, min_attempts)
+
+
+def _mol(st):
+    if st.dtype != np.float64 or not st.flags.c_contiguous or st.shape[0] != 7:
+        raise ValueError("molecule state must be a C-contiguous FP64 (7, n) array")
+    return [st[k].ctypes.data for k in range(7)]
+
+
+def dsmc_cells(m: OrcCellMap, st: np.ndarray) -> np.ndarray:
+    """put_into_cells_symmetry.m: 0-based cell of each molecule (-1: z NaN)."""
+    n = st.shape[1]
+    cell = np.zeros(n, dtype=np.int32)
+    x, y, z = (np.ascontiguousarray(st[k], dtype=np.float64) for k in range(3))
+    lib().orc_dsmc_cells(C.byref(m), n, ptr(x), ptr(y), ptr(z), ptr(cell))
+    return cell
+
+
+def dsmc_move(m: OrcCellMap, dt: float, st: np.ndarray, round32: bool = True) -> np.ndarray:
+    """free_motion.m + boundary_conditions_specular.m in place; returns
+    [boundary events, r wall, z walls, phi wraps]."""
+    c = np.zeros(4, dtype=np.int64)
+    a = _mol(st)
+    lib().orc_dsmc_move(C.byref(m), dt, st.shape[1], *a[:6], int(round32), ptr(c))
+    return c
+
+
+def dsmc_collide(m: OrcCellMap, p: OrcDsmcParams, st: np.ndarray, seed: int, step: int, cell=None,
+                 round32: bool = True) -> np.ndarray:
+    """collision_HS_internal.m in place; returns [attempts, collisions, n_SC]."""
+    if cell is None:
+        cell = dsmc_cells(m, st)
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    c = np.zeros(3, dtype=np.int64)
+    a = _mol(st)
+    lib().orc_dsmc_collide(C.byref(m), C.byref(p), st.shape[1], ptr(cell), a[3], a[4], a[5], a[6], seed, step,
+                           int(round32), ptr(c))
+    return c
+
+
+def dsmc_reorder(m: OrcCellMap, st: np.ndarray) -> np.ndarray:
+    """Stable sort of the molecules by cell, unused (z NaN) slots dropped."""
+    cell = dsmc_cells(m, st)
+    keep = cell >= 0
+    order = np.argsort(np.where(keep, cell, np.iinfo(np.int32).max), kind="stable")[: int(keep.sum())]
+    return np.ascontiguousarray(st[:, order])
+
+
+def dsmc_sample(m: OrcCellMap, st: np.ndarray, fn: float, mass: float, kb: float = 1.38064853e-23):
+    """particle_density0.m + count_T_numbensity.m per cell and per (ring, layer)."""
+    pd, s0, dp, vc, nxy = cell_map_build(m)
+    cell = dsmc_cells(m, st)
+    n = st.shape[1]
+    cd, ct = np.zeros(nxy * m.n_layers), np.zeros(nxy * m.n_layers)
+    rd, rt = np.zeros((m.n_layers, m.n_rings)), np.zeros((m.n_layers, m.n_rings))
+    vx, vy, vz = (np.ascontiguousarray(st[k], dtype=np.float64) for k in range(3, 6))
+    lib().orc_dsmc_sample(C.byref(m), n, ptr(cell), ptr(vx), ptr(vy), ptr(vz), fn, mass, kb, ptr(cd), ptr(ct),
+                          ptr(rd), ptr(rt))
+    return {"cell_density": cd, "cell_temperature": ct, "rz_density": rd, "rz_temperature": rt}
