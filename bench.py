@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-source", action="store_true", help="skip the photoionization-source leg")
+    ap.add_argument("--no-dsmc", action="store_true", help="skip the DSMC neutral-gas leg")
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-tuned-sort", action="store_true",
@@ -357,6 +358,105 @@ def bench_photoionization(seed: int):
             "note": "median of 5 pulses, host call to return (result read back included)"}
 
 
+DSMC_BYTES_PER_MOLECULE_STEP = 56      # FP32 x, y, z, vx, vy, vz, ei read + written
+
+
+def dsmc_main2_case():
+    """main2.m:5-47: N2 at 10 Pa and 300 K in a wedge of Rp = 1 cm, Lp = 3 cm,
+    phimax = 0.5; cells of the mean free path (15 rings x 45 layers, 2880
+    cells), 5000 molecules per cell, dt = 0.2 us."""
+    import math
+
+    from gasx.dsmc import CellMap, K_B_DSMC, N2_DIAMETER
+
+    n_gas = 10.0 / K_B_DSMC / 300.0
+    lam = 1 / (math.sqrt(2) * math.pi * n_gas * N2_DIAMETER ** 2)
+    cm = CellMap(int(1e-2 / lam) + 1, int(0.03 / lam) + 1, 1e-2, 0.03, 0.5, lam)
+    return cm, n_gas
+
+
+def dsmc_cpu_baseline(cm, n_gas, seed: int):
+    """The oracle's DSMC step (move, cells, collisions; OpenMP over molecules
+    and cells) on the whole workload (2880 cells, 1.44e7 molecules) for 2
+    steps."""
+    import oracle as O
+    from gasx.dsmc import CellMap, seed_molecules
+
+    import numpy as np
+
+    layers = cm.n_layers
+    sl = CellMap(cm.n_rings, layers, cm.rp, cm.lp * layers / cm.n_layers, cm.phip, cm.delta)
+    om = O.cell_map(sl.n_rings, sl.n_layers, sl.rp, sl.lp, sl.phip, sl.delta)
+    nxy = O.cell_map_build(om)[4]
+    n = 5000 * nxy * layers
+    fn = n_gas * 0.5 * sl.rp ** 2 * sl.phip * sl.lp / n
+    from gasx.dsmc import dsmc_params
+    p = dsmc_params(2e-7, fn)
+    op = O.dsmc_params(p.dt, p.fn, p.mass, p.diameter, p.sc_max, p.omega, p.zeta, p.alpha, p.min_attempts)
+    st0 = np.ascontiguousarray(seed_molecules(n, sl, seed=seed).astype(np.float64))
+    out = {}
+    for threads in (cpu_threads(), 1):
+        O.set_threads(threads)
+        st = st0.copy()
+        t0 = time.perf_counter()
+        for k in range(2):
+            O.dsmc_move(om, p.dt, st, round32=True)
+            O.dsmc_collide(om, op, st, seed=seed, step=k, round32=True)
+        wall = time.perf_counter() - t0
+        out[threads] = (2 * n / wall, wall)
+    O.set_threads(1)
+    t_all, t_one = out[cpu_threads()], out[1]
+    return {"value": t_all[0], "unit": "molecule-steps/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"FP64 C oracle: the whole workload ({nxy * layers} cells, {n} molecules), "
+                      f"2 steps ({t_all[1]:.1f} s on {cpu_threads()} threads)",
+            "single_thread": {"value": t_one[0], "cores": 1, "wall_s": t_one[1]}}
+
+
+def bench_dsmc(seed: int, steps: int = 20, cpu: bool = True):
+    """The DSMC neutral-gas step (gasx_dsmc_step) on main2.m's case: 1.44e7
+    molecules, one move + cell sort + collision step per 0.2 us, molecules
+    permuted into cell order every step.  value = molecule-steps/s over the
+    timed steps (molecules resident in HBM)."""
+    import torch
+
+    import gasx
+    from gasx.dsmc import MoleculeSet, dsmc_params, seed_molecules
+
+    cm, n_gas = dsmc_main2_case()
+    ctx = gasx.Context(gasx.make_params(16, 16), device=0)
+    _, nxy = cm.info(ctx.lib)
+    ncell = nxy * cm.n_layers
+    n = 5000 * ncell
+    fn = n_gas * 0.5 * cm.rp ** 2 * cm.phip * cm.lp / n
+    mol = MoleculeSet(n).load(seed_molecules(n, cm, seed=seed))
+    p = dsmc_params(2e-7, fn, reorder=True)
+    for k in range(3):
+        ctx.dsmc.step(mol, cm, p, seed=seed, step=k)
+    torch.cuda.synchronize()
+    ctx.dsmc.counters(reset=True)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ctx.dsmc.step(mol, cm, p, seed=seed, step=3 + k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = ctx.dsmc.counters(reset=True)
+    value = n * steps / wall
+    achieved = DSMC_BYTES_PER_MOLECULE_STEP * value / 1e9
+    out = {"workload": "dsmc_main2", "molecules": n, "cells": ncell, "rings": cm.n_rings, "layers": cm.n_layers,
+           "dt_s": p.dt, "steps": steps, "warmup": 3, "ms_per_step": wall * 1e3 / steps, "value": value,
+           "unit": "molecule-steps/s",
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "bytes_per_molecule_step": DSMC_BYTES_PER_MOLECULE_STEP,
+                        "scope": "whole step (move, sort, reorder, collisions)"},
+           "attempts_per_step": c.n_attempts / steps, "collisions_per_step": c.n_collisions / steps,
+           "sc_exceed": c.n_sc_exceed, "wall_events_per_step": c.n_boundary / steps}
+    del mol
+    torch.cuda.empty_cache()
+    if cpu:
+        out["cpu_baseline"] = dsmc_cpu_baseline(cm, n_gas, seed)
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -374,6 +474,15 @@ def main():
     import gasx
     from gasx.workloads import WORKLOADS, density_grid, seed_electrons
 
+    if args.workload == "dsmc_main2":
+        d = bench_dsmc(args.seed, steps=args.steps if args.steps != 1000 else 20, cpu=not args.no_cpu_baseline)
+        print(json.dumps({
+            "metric": METRIC, "value": d["value"], "unit": d["unit"], "n_gpus": 1, "steps": d["steps"],
+            "warmup": d["warmup"], "ms_per_step": d["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp64 arithmetic, fp32 state", "data": "synthetic",
+            "config": {"workload": d["workload"], "molecules": d["molecules"], "cells": d["cells"]},
+            "roofline": d["roofline"], "cpu_baseline": d.get("cpu_baseline"), "dsmc": d}))
+        return
     w = WORKLOADS[args.workload]
     if w.name == "batched_128x256":
         b = bench_batched(args.seed, steps=args.steps if args.steps != 1000 else None)
@@ -548,6 +657,9 @@ def main():
     if not args.no_source:
         torch.cuda.empty_cache()
         line["photoionization"] = bench_photoionization(args.seed)
+    if not args.no_dsmc:
+        torch.cuda.empty_cache()
+        line["dsmc"] = bench_dsmc(args.seed, cpu=not args.no_cpu_baseline)
     print(json.dumps(line))
 
 

@@ -1317,14 +1317,17 @@ static void dsmc_move_one(const orc_cell_map* m, double dt, double* px, double* 
 void orc_dsmc_move(const orc_cell_map* m, double dt, int64_t n, double* x, double* y, double* z, double* vx,
                    double* vy, double* vz, int round32, int64_t counts[4])
 {
-    int64_t c[4] = {0, 0, 0, 0};
+    int64_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+#pragma omp parallel for num_threads(g_threads) schedule(static) reduction(+ : c0, c1, c2, c3) if (g_threads > 1)
     for (int64_t i = 0; i < n; ++i) {
         if (z[i] != z[i]) continue;
+        int64_t c[4] = {0, 0, 0, 0};
         dsmc_move_one(m, dt, &x[i], &y[i], &z[i], &vx[i], &vy[i], &vz[i], c);
+        c0 += c[0]; c1 += c[1]; c2 += c[2]; c3 += c[3];
         x[i] = st32(x[i], round32); y[i] = st32(y[i], round32); z[i] = st32(z[i], round32);
         vx[i] = st32(vx[i], round32); vy[i] = st32(vy[i], round32); vz[i] = st32(vz[i], round32);
     }
-    for (int k = 0; k < 4; ++k) counts[k] += c[k];
+    counts[0] += c0; counts[1] += c1; counts[2] += c2; counts[3] += c3;
 }
 
 /* Philox counter blocks of an attempt: counter = (attempt, cell, step, block). */
