@@ -370,6 +370,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->base_tmp, 4), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
         chk(hipHostMalloc((void**)&ctx->h_count, kSortInfoWords * sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
+        chk(hipEventCreateWithFlags(&ctx->sort_ev, hipEventDisableTiming), "sort event");
         if (st == GASX_OK) {
             chk(hipMemcpy(ctx->d_tab, &dt, sizeof(dt), hipMemcpyHostToDevice), "upload tables");
             chk(hipMemset(ctx->d_counters, 0, (size_t)kNCounters * kCounterSlots * sizeof(unsigned long long)),
@@ -405,6 +406,7 @@ extern "C" gasx_status gasx_ctx_destroy(gasx_ctx* ctx)
         if (ctx->istage[c]) hipFree(ctx->istage[c]);
     }
     if (ctx->h_count) hipHostFree(ctx->h_count);
+    if (ctx->sort_ev) hipEventDestroy(ctx->sort_ev);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     batch_release(ctx);
     diff_release(ctx);
@@ -643,9 +645,11 @@ static gasx_status run_particles(gasx_ctx* ctx, uint32_t flags, const gasx_speci
 // are the context's kJCopies-fold accumulators
 static gasx_status run_drift_deposit(gasx_ctx* ctx, const gasx_species* s, bool drift, bool deposit,
                                      unsigned long long* qr, unsigned long long* qz, size_t jstride = 0,
-                                     int timed_kind = kTParticle)
+                                     int timed_kind = kTParticle, float* const* in = nullptr)
 {
     PartArgs pa = make_part(ctx, s);
+    // in: read the particles from a sort's permuted copies (the drift writes all six values back)
+    if (in) { pa.xi = in[0]; pa.yi = in[1]; pa.zi = in[2]; pa.vxi = in[3]; pa.vyi = in[4]; pa.vzi = in[5]; }
     pa.qr = qr; pa.qz = qz;
     pa.jstride = qr ? jstride : 0;
     const int64_t chunks = (s->capacity + kChunkD - 1) / kChunkD;
@@ -791,6 +795,8 @@ extern "C" gasx_status gasx_collision_bound(gasx_ctx* ctx, const gasx_gas* g, do
 // Cell sort of one species; sorted_out non-null leaves the permuted particles
 // in the sort scratch (their pointers returned) instead of copying them back.
 static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_out);
+static gasx_status sort_species_begin(gasx_ctx* ctx, gasx_species* s);
+static gasx_status sort_species_finish(gasx_ctx* ctx, gasx_species* s, float** sorted_out);
 
 struct LoopOpts {
     int n_steps = 0;
@@ -815,10 +821,15 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
         gasx_status st = ensure_staging(ctx, sp[0].capacity, o.ions);
         if (st != GASX_OK) return st;
     }
-    // cell-sorted slots keep each chunk's current within one LDS tile
-    if (o.sort_every > 0) {
+    // cell-sorted slots keep each chunk's current within one LDS tile; the
+    // last species' permuted copies feed the prologue's drift, which writes
+    // them back in place of the sort's own copy back
+    float* sorted[6] = {};
+    int sorted_k = -1;                 // species whose next particle launch reads the sorted copies
+    const bool initial_sort = o.sort_every > 0;
+    if (initial_sort) {
         for (int k = 0; k < nsp; ++k) {
-            gasx_status st = gasx_sort(ctx, &sp[k]);
+            gasx_status st = sort_species(ctx, &sp[k], k == nsp - 1 ? sorted : nullptr);
             if (st != GASX_OK) return st;
         }
     }
@@ -829,32 +840,38 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
             HIPCHK(launch_fill_u64(ctx->q[b][k], (int64_t)(nh * kJCopies), ctx->stream), "zero q");
     // prologue: EM2 half drift + EM7 deposit of every species (electricmagneticfield6.m:141-144)
     for (int k = 0; k < nsp; ++k) {
-        gasx_status st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[0][0], ctx->q[0][1], nh, kTDrift);
+        gasx_status st = run_drift_deposit(ctx, &sp[k], true, true, ctx->q[0][0], ctx->q[0][1], nh, kTDrift,
+                                           initial_sort && k == nsp - 1 ? sorted : nullptr);
         if (st != GASX_OK) return st;
     }
-    // after an in-loop sort the last species sorted is read from the sort's
-    // permuted copies by its next particle launch, which writes it back in
-    // place of the sort's own copy back
-    float* sorted[6] = {};
-    int sorted_k = -1;
-    for (int s = 0; s < o.n_steps; ++s) {
+    // the field cycle of step s: E, H from t^s to t^(s+1) with the current deposited before it
+    auto field_cycle = [&](int s) -> gasx_status {
         const int cur = s & 1, nxt = cur ^ 1;
         const bool last = s == o.n_steps - 1;
-        gasx_status st;
         if (kJCopies > 1) {   // fold the current copies into copy 0
             TimedScope ts(ctx, kTFdtd);
             HIPCHK(launch_j_reduce(ctx->q[cur][0], ctx->q[cur][1], (int64_t)nh, nh, ctx->stream), "j reduce");
         }
         if ((s % 2) == 0)
-            st = fdtd_cycle(ctx, f->er, f->ez, f->hphi, ctx->er_b, ctx->ez_b, ctx->h_b,
-                            ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
-                            last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
-        else
-            st = fdtd_cycle(ctx, ctx->er_b, ctx->ez_b, ctx->h_b, f->er, f->ez, f->hphi,
-                            ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
-                            last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
-        if (st != GASX_OK) return st;
+            return fdtd_cycle(ctx, f->er, f->ez, f->hphi, ctx->er_b, ctx->ez_b, ctx->h_b,
+                              ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
+                              last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
+        return fdtd_cycle(ctx, ctx->er_b, ctx->ez_b, ctx->h_b, f->er, f->ez, f->hphi,
+                          ctx->q[cur][0], ctx->q[cur][1], nullptr, nullptr,
+                          last ? nullptr : ctx->q[nxt][0], last ? nullptr : ctx->q[nxt][1]);
+    };
+    bool cycle_ahead = false;          // this step's field cycle was enqueued before a sort
+    for (int s = 0; s < o.n_steps; ++s) {
+        const int cur = s & 1, nxt = cur ^ 1;
+        const bool last = s == o.n_steps - 1;
+        gasx_status st;
+        if (!cycle_ahead) {
+            st = field_cycle(s);
+            if (st != GASX_OK) return st;
+        }
+        cycle_ahead = false;
         ca.step = o.step0 + (uint32_t)s;
+        const bool sort_now = o.sort_every > 0 && !last && ((s + 1) % o.sort_every) == 0;
         for (int k = 0; k < nsp; ++k) {
             // two launches: push (+ collide, staged secondaries already
             // drifted), then drift + deposit of the existing slots; the
@@ -886,15 +903,32 @@ static gasx_status fused_loop(gasx_ctx* ctx, gasx_species* sp, int nsp, gasx_spe
                     if (st != GASX_OK) return st;
                 }
             }
+            // one species: the sort's key pass as soon as the positions are
+            // final, before the deposit, which does not move them
+            if (sort_now && nsp == 1) {
+                st = sort_species_begin(ctx, &sp[0]);
+                if (st != GASX_OK) return st;
+            }
             if (!last) {
                 st = run_drift_deposit(ctx, &sp[k], false, true, ctx->q[nxt][0], ctx->q[nxt][1], nh, kTDrift);
                 if (st != GASX_OK) return st;
             }
         }
-        if (o.sort_every > 0 && !last && ((s + 1) % o.sort_every) == 0) {
-            for (int k = 0; k < nsp; ++k) {
-                st = sort_species(ctx, &sp[k], k == nsp - 1 ? sorted : nullptr);
+        if (sort_now) {
+            if (nsp == 1) {
+                // the next step's field cycle does not depend on the particles'
+                // order: enqueued ahead, it keeps the GPU busy while the host
+                // waits for the slot count
+                st = field_cycle(s + 1);
                 if (st != GASX_OK) return st;
+                cycle_ahead = true;
+                st = sort_species_finish(ctx, &sp[0], sorted);
+                if (st != GASX_OK) return st;
+            } else {
+                for (int k = 0; k < nsp; ++k) {
+                    st = sort_species(ctx, &sp[k], k == nsp - 1 ? sorted : nullptr);
+                    if (st != GASX_OK) return st;
+                }
             }
             sorted_k = nsp - 1;
         }
@@ -949,12 +983,11 @@ extern "C" gasx_status gasx_sort(gasx_ctx* ctx, gasx_species* s)
     return sort_species(ctx, s, nullptr);
 }
 
-// sorted_out non-null: the permuted particles stay in the sort scratch (for
-// the next particle launch to read) instead of being copied back
-static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_out)
+// The key pass of a cell sort (sort_begin); the scratch is sized first.
+static gasx_status sort_species_begin(gasx_ctx* ctx, gasx_species* s)
 {
     size_t need = 0;
-    hipError_t e = launch_sort(ctx->geom, s, nullptr, 0, &need, ctx->stream);
+    hipError_t e = sort_begin(ctx->geom, s, nullptr, 0, &need, ctx->stream, ctx->h_count, ctx->sort_ev);
     if (e != hipSuccess && e != hipErrorInvalidValue) return hip_fail(ctx, e, "sort size");
     if (need > ctx->sort_bytes) {
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
@@ -963,11 +996,27 @@ static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_o
         HIPCHK(hipMalloc(&ctx->sort_tmp, need), "alloc sort");
         ctx->sort_bytes = need;
     }
-    TimedScope ts(ctx, kTSort);
-    HIPCHK(launch_sort(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream, sorted_out,
-                       ctx->h_count),
-           "sort");
+    HIPCHK(sort_begin(ctx->geom, s, ctx->sort_tmp, ctx->sort_bytes, nullptr, ctx->stream, ctx->h_count,
+                      ctx->sort_ev),
+           "sort keys");
     return GASX_OK;
+}
+
+// The rest of the sort (sort_finish).  sorted_out non-null: the permuted
+// particles stay in the sort scratch (for the next particle launch to read)
+// instead of being copied back.
+static gasx_status sort_species_finish(gasx_ctx* ctx, gasx_species* s, float** sorted_out)
+{
+    TimedScope ts(ctx, kTSort);
+    HIPCHK(sort_finish(ctx->geom, s, ctx->sort_tmp, ctx->stream, sorted_out, ctx->h_count, ctx->sort_ev), "sort");
+    return GASX_OK;
+}
+
+static gasx_status sort_species(gasx_ctx* ctx, gasx_species* s, float** sorted_out)
+{
+    gasx_status st = sort_species_begin(ctx, s);
+    if (st != GASX_OK) return st;
+    return sort_species_finish(ctx, s, sorted_out);
 }
 
 extern "C" gasx_status gasx_get_timing(gasx_ctx* ctx, gasx_timing* out)

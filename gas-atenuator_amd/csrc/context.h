@@ -69,6 +69,7 @@ struct gasx_ctx {
     void* sort_tmp = nullptr;
     size_t sort_bytes = 0;
     int32_t* h_count = nullptr;   // kSortInfoWords pinned host words: the sort reads the slot count and the key descents through them
+    hipEvent_t sort_ev = nullptr; // recorded after the sort's key pass and count copy
     double* d_tally = nullptr;
     int cu_count = 256;
     double max_rate = 0.0;   // max over the collision window of v(E) sigma_T(E) (m^3/s)

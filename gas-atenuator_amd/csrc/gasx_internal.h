@@ -256,11 +256,18 @@ hipError_t launch_tallies(const gasx_params& p, const double* er, const double* 
                           hipStream_t s);
 // sorted_out non-null: the permuted arrays are left in the scratch (their six
 // pointers returned) instead of being copied back into s
-// pinned_info (optional): kSortInfoWords pinned host words the slot count
-// and the key descents are read through
+// pinned_info: kSortInfoWords pinned host words the slot count and the key
+// descents are read through
 constexpr int kSortInfoWords = 2 + 4096;
-hipError_t launch_sort(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st, float** sorted_out = nullptr,
-                       int32_t* pinned_info = nullptr);
+// The sort in two parts.  sort_begin (scratch null or too small: only the
+// size in need_bytes, hipErrorInvalidValue) enqueues the key pass, the copy
+// of the slot count and the key descents to pinned_info and records `ready`;
+// sort_finish waits for `ready` on the host (not for the whole stream),
+// then enqueues the radix sort and the permutation.  Work enqueued between
+// the two keeps the GPU busy while the host waits.
+hipError_t sort_begin(const Geom& g, gasx_species* s, void* scratch, size_t scratch_bytes, size_t* need_bytes,
+                      hipStream_t st, int32_t* pinned_info, hipEvent_t ready);
+hipError_t sort_finish(const Geom& g, gasx_species* s, void* scratch, hipStream_t st, float** sorted_out,
+                       const int32_t* pinned_info, hipEvent_t ready);
 
 }  // namespace gasx

@@ -720,9 +720,15 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
     }
     const int nr_ = g.nr, nz_ = g.nz;
     constexpr uintptr_t kAlign = sizeof(float4) - 1;
+    // a single run may read its particles from a sort's permuted copies (xi..vzi)
+    // and write them back into x..vz; a batch reads and writes x..vz
+    const bool copy_in = !kBat && pa0.xi != pa0.x;
     const bool aligned = ((reinterpret_cast<uintptr_t>(pa0.x) | reinterpret_cast<uintptr_t>(pa0.y) |
                            reinterpret_cast<uintptr_t>(pa0.z) | reinterpret_cast<uintptr_t>(pa0.vx) |
-                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz)) &
+                           reinterpret_cast<uintptr_t>(pa0.vy) | reinterpret_cast<uintptr_t>(pa0.vz) |
+                           (kBat ? 0 : reinterpret_cast<uintptr_t>(pa0.xi) | reinterpret_cast<uintptr_t>(pa0.yi) |
+                                       reinterpret_cast<uintptr_t>(pa0.zi) | reinterpret_cast<uintptr_t>(pa0.vxi) |
+                                       reinterpret_cast<uintptr_t>(pa0.vyi) | reinterpret_cast<uintptr_t>(pa0.vzi))) &
                           kAlign) == 0;
     unsigned cnt[kNCounters] = {};
     int cnt_run = -1;
@@ -764,9 +770,11 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
         const bool vec = aligned && slot0 + kPerD <= n;
         float x[kPerD], y[kPerD], z[kPerD], vx[kPerD], vy[kPerD], vz[kPerD];
         bool alive[kPerD], was_alive[kPerD];
+        const float *sx = pa.x, *sy = pa.y, *sz = pa.z, *svx = pa.vx, *svy = pa.vy, *svz = pa.vz;
+        if (copy_in) { sx = pa.xi; sy = pa.yi; sz = pa.zi; svx = pa.vxi; svy = pa.vyi; svz = pa.vzi; }
         if (vec) {
-            vload4n(pa.x + slot0, x); vload4n(pa.y + slot0, y); vload4n(pa.z + slot0, z);
-            vload4n(pa.vx + slot0, vx); vload4n(pa.vy + slot0, vy); vload4n(pa.vz + slot0, vz);
+            vload4n(sx + slot0, x); vload4n(sy + slot0, y); vload4n(sz + slot0, z);
+            vload4n(svx + slot0, vx); vload4n(svy + slot0, vy); vload4n(svz + slot0, vz);
         } else {
 #pragma unroll
             for (int p = 0; p < kPerD; ++p) {
@@ -774,8 +782,8 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
                 x[p] = y[p] = vx[p] = vy[p] = vz[p] = 0.f;
                 z[p] = __int_as_float(0x7fc00000);
                 if (slot < n) {
-                    x[p] = pa.x[slot]; y[p] = pa.y[slot]; z[p] = pa.z[slot];
-                    vx[p] = pa.vx[slot]; vy[p] = pa.vy[slot]; vz[p] = pa.vz[slot];
+                    x[p] = sx[slot]; y[p] = sy[slot]; z[p] = sz[slot];
+                    vx[p] = svx[slot]; vy[p] = svy[slot]; vz[p] = svz[slot];
                 }
             }
         }
@@ -816,8 +824,8 @@ __device__ __forceinline__ void drift_deposit_pass(const Geom& g, const PartArgs
 #pragma unroll
                 for (int p = 0; p < kPerD; ++p) {
                     const int slot = slot0 + p;
-                    if (slot < n && was_alive[p]) {
-                        if (alive[p]) {
+                    if (slot < n && (was_alive[p] || copy_in)) {
+                        if (alive[p] || copy_in) {
                             pa.x[slot] = x[p]; pa.y[slot] = y[p]; pa.z[slot] = z[p];
                             pa.vx[slot] = vx[p]; pa.vy[slot] = vy[p]; pa.vz[slot] = vz[p];
                         } else {

@@ -187,51 +187,83 @@ constexpr int kDisorder = 8;
 
 // scratch layout: [keys_in n][keys_out n][vals_in n][vals_out n][6 x float n]
 // [records 8 x n][info][pad][rocprim tmp]
-hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes,
-                       size_t* need_bytes, hipStream_t st, float** sorted_out, int32_t* pinned_info)
-{
-    const size_t cap = (size_t)sp->capacity;
-    const size_t a = (cap * 4 + 255) & ~(size_t)255;
-    uint32_t nkeys = (uint32_t)(g.nr * g.nz);
+namespace {
+struct SortLayout {
+    size_t a = 0, info_bytes = 0, tmp_bytes = 0, need = 0;
     int bits = 1;
-    while ((1u << bits) <= nkeys && bits < 32) ++bits;
-    size_t tmp_bytes = 0;
-    hipError_t e = rocprim::radix_sort_pairs<SortConfig>((void*)nullptr, tmp_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                  (uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)(cap > 0 ? cap : 1),
-                                  0, bits, st);
+};
+
+hipError_t sort_layout(const Geom& g, size_t cap, hipStream_t st, SortLayout& L)
+{
+    L.a = (cap * 4 + 255) & ~(size_t)255;
+    const uint32_t nkeys = (uint32_t)(g.nr * g.nz);
+    L.bits = 1;
+    while ((1u << L.bits) <= nkeys && L.bits < 32) ++L.bits;
+    hipError_t e = rocprim::radix_sort_pairs<SortConfig>((void*)nullptr, L.tmp_bytes, (uint32_t*)nullptr,
+                                                         (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                                         (size_t)(cap > 0 ? cap : 1), 0, L.bits, st);
     if (e != hipSuccess) return e;
-    const size_t info_bytes = ((2 + kSortKeyBlocks) * sizeof(int32_t) + 255) & ~(size_t)255;
-    const size_t need = 18 * a + info_bytes + ((tmp_bytes + 255) & ~(size_t)255);
-    if (need_bytes) *need_bytes = need;
-    if (!scratch || scratch_bytes < need) return hipErrorInvalidValue;
-    char* base = (char*)scratch;
-    uint32_t* k_in = (uint32_t*)(base);
-    uint32_t* k_out = (uint32_t*)(base + a);
-    uint32_t* v_in = (uint32_t*)(base + 2 * a);
-    uint32_t* v_out = (uint32_t*)(base + 3 * a);
-    float* tmp[6];
-    for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * a);
-    float4* rec = (float4*)(base + 10 * a);
-    int32_t* info = (int32_t*)(base + 18 * a);
-    void* rtmp = (void*)(base + 18 * a + info_bytes);
-    // keys and descents over the device's slot count, then the one host
-    // synchronisation of the sort (rocPRIM takes the size on the host)
+    L.info_bytes = ((2 + kSortKeyBlocks) * sizeof(int32_t) + 255) & ~(size_t)255;
+    L.need = 18 * L.a + L.info_bytes + ((L.tmp_bytes + 255) & ~(size_t)255);
+    return hipSuccess;
+}
+
+int key_blocks(size_t cap)
+{
     int blocks = (int)(((cap + 3) / 4 + 255) / 256);
     if (blocks > kSortKeyBlocks) blocks = kSortKeyBlocks;
-    if (blocks < 1) blocks = 1;
+    return blocks < 1 ? 1 : blocks;
+}
+}  // namespace
+
+hipError_t sort_begin(const Geom& g, gasx_species* sp, void* scratch, size_t scratch_bytes, size_t* need_bytes,
+                      hipStream_t st, int32_t* pinned_info, hipEvent_t ready)
+{
+    const size_t cap = (size_t)sp->capacity;
+    SortLayout L;
+    hipError_t e = sort_layout(g, cap, st, L);
+    if (e != hipSuccess) return e;
+    if (need_bytes) *need_bytes = L.need;
+    if (!scratch || scratch_bytes < L.need || !pinned_info || !ready) return hipErrorInvalidValue;
+    char* base = (char*)scratch;
+    uint32_t* k_in = (uint32_t*)(base);
+    uint32_t* v_in = (uint32_t*)(base + 2 * L.a);
+    int32_t* info = (int32_t*)(base + 18 * L.a);
+    const int blocks = key_blocks(cap);
     hipLaunchKernelGGL(k_sort_keys, dim3(blocks), dim3(256), 0, st, g, sp->x, sp->y, sp->z, sp->count,
                        (int32_t)cap, k_in, v_in, info);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    static thread_local int32_t local[2 + kSortKeyBlocks];
-    int32_t* hi = pinned_info ? pinned_info : local;
-    e = hipMemcpyAsync(hi, info, (2 + (size_t)blocks) * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(pinned_info, info, (2 + (size_t)blocks) * sizeof(int32_t), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
+    return hipEventRecord(ready, st);
+}
+
+hipError_t sort_finish(const Geom& g, gasx_species* sp, void* scratch, hipStream_t st, float** sorted_out,
+                       const int32_t* pinned_info, hipEvent_t ready)
+{
+    const size_t cap = (size_t)sp->capacity;
+    SortLayout L;
+    hipError_t e = sort_layout(g, cap, st, L);
     if (e != hipSuccess) return e;
-    const int32_t n = hi[0];
+    char* base = (char*)scratch;
+    uint32_t* k_in = (uint32_t*)(base);
+    uint32_t* k_out = (uint32_t*)(base + L.a);
+    uint32_t* v_in = (uint32_t*)(base + 2 * L.a);
+    uint32_t* v_out = (uint32_t*)(base + 3 * L.a);
+    float* tmp[6];
+    for (int c = 0; c < 6; ++c) tmp[c] = (float*)(base + (4 + c) * L.a);
+    float4* rec = (float4*)(base + 10 * L.a);
+    void* rtmp = (void*)(base + 18 * L.a + L.info_bytes);
+    // the one host synchronisation of the sort (rocPRIM takes the size on the
+    // host): on the key pass's event only, so work queued after it keeps the
+    // GPU busy meanwhile
+    e = hipEventSynchronize(ready);
+    if (e != hipSuccess) return e;
+    const int blocks = key_blocks(cap);
+    const int32_t n = pinned_info[0];
     int64_t descents = 0;
-    for (int b = 0; b < blocks; ++b) descents += hi[2 + b];
+    for (int b = 0; b < blocks; ++b) descents += pinned_info[2 + b];
     if (n <= 0) {
         if (sorted_out) {   // nothing to permute: the particles stay where they are
             float* own[6] = {sp->x, sp->y, sp->z, sp->vx, sp->vy, sp->vz};
@@ -239,8 +271,8 @@ hipError_t launch_sort(const Geom& g, gasx_species* sp, void* scratch, size_t sc
         }
         return hipSuccess;
     }
-    size_t tb = tmp_bytes;
-    e = rocprim::radix_sort_pairs<SortConfig>(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, bits, st);
+    size_t tb = L.tmp_bytes;
+    e = rocprim::radix_sort_pairs<SortConfig>(rtmp, tb, k_in, k_out, v_in, v_out, (size_t)n, 0, L.bits, st);
     if (e != hipSuccess) return e;
     const uint32_t dead = (uint32_t)(g.nr * g.nz);
     int pblocks = ((n + 3) / 4 + 255) / 256;
