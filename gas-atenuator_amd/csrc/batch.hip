@@ -33,7 +33,8 @@ void free_run_buffers(BatchState& b)
     dfree(b.er_m); dfree(b.ez_m); dfree(b.h_m);
     dfree(b.er32); dfree(b.ez32); dfree(b.h32);
     for (auto& qq : b.q) { dfree(qq[0]); dfree(qq[1]); }
-    dfree(b.runs); dfree(b.fdtd_runs); dfree(b.chunk_start); dfree(b.tmp_run); dfree(b.run_counters);
+    dfree(b.runs); dfree(b.ion_runs); dfree(b.fdtd_runs); dfree(b.chunk_start); dfree(b.tmp_run);
+    dfree(b.run_counters);
     b.nruns_cap = 0;
     b.uploaded_key.clear();
 }
@@ -43,6 +44,9 @@ void free_stage_buffers(BatchState& b)
     for (auto& s : b.stage) dfree(s);
     dfree(b.offs); dfree(b.stage_cnt);
     b.stage_segs = 0;
+    for (auto& s : b.istage) dfree(s);
+    dfree(b.istage_cnt);
+    b.istage_segs = 0;
 }
 
 void drop_graph(BatchState& b)
@@ -62,7 +66,7 @@ Sizes sizes(const gasx_ctx* ctx)
     return Sizes{nr * nz, nr * (nz + 1), (nr + 1) * nz};
 }
 
-gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs)
+gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs, bool ions)
 {
     BatchState& b = ctx->batch;
     const Sizes z = sizes(ctx);
@@ -83,6 +87,7 @@ gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs)
         for (auto& qq : b.q)
             for (auto& q : qq) HIPCHK(dalloc(&q, n * z.nh * kJCopies), "alloc batch current");
         HIPCHK(dalloc(&b.runs, n), "alloc batch runs");
+        HIPCHK(dalloc(&b.ion_runs, n), "alloc batch ion runs");
         HIPCHK(dalloc(&b.fdtd_runs, 4 * n), "alloc batch field runs");
         HIPCHK(dalloc(&b.chunk_start, n + 1), "alloc batch chunks");
         HIPCHK(dalloc(&b.tmp_run, 2 * n), "alloc batch tmp");
@@ -99,11 +104,38 @@ gasx_status ensure_batch(gasx_ctx* ctx, int nruns, int64_t stage_segs)
         HIPCHK(dalloc(&b.stage_cnt, (size_t)stage_segs), "alloc batch stage counts");
         b.stage_segs = stage_segs;
     }
+    if (ions && stage_segs > b.istage_segs) {
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+        drop_graph(b);
+        for (auto& s : b.istage) dfree(s);
+        dfree(b.istage_cnt);
+        for (auto& s : b.istage) HIPCHK(dalloc(&s, (size_t)stage_segs * kSeg), "alloc batch ion stage");
+        HIPCHK(dalloc(&b.istage_cnt, (size_t)stage_segs), "alloc batch ion stage counts");
+        b.istage_segs = stage_segs;
+    }
     return GASX_OK;
 }
 
+bool has_ions(const gasx_batch* b, const gasx_run_opts* o) { return o->collide && b->ix != nullptr; }
+
+// Ranges [base[r], base[r] + capacity[r]) of nruns runs: each valid and none
+// overlapping; *slots = the end of the last range.
+bool ranges_ok(const int64_t* base, const int64_t* cap, int nruns, int64_t* slots)
+{
+    std::vector<int> idx(nruns);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int r = 0; r < nruns; ++r)
+        if (base[r] < 0 || base[r] % kChunk != 0 || cap[r] < 1 || cap[r] > (int64_t)INT32_MAX - kChunk)
+            return false;
+    std::sort(idx.begin(), idx.end(), [&](int a, int c) { return base[a] < base[c]; });
+    for (int k = 1; k < nruns; ++k)
+        if (base[idx[k - 1]] + cap[idx[k - 1]] > base[idx[k]]) return false;
+    *slots = base[idx.back()] + cap[idx.back()];
+    return true;
+}
+
 gasx_status validate(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o, int64_t* stage_segs,
-                     int64_t* pool_slots)
+                     int64_t* pool_slots, int64_t* ion_slots)
 {
     if (!b || !o || b->nruns < 1 || b->nruns > kMaxRuns || !b->x || !b->y || !b->z || !b->vx || !b->vy ||
         !b->vz || !b->base || !b->capacity || !b->counts || !b->er || !b->ez || !b->hphi)
@@ -112,21 +144,23 @@ gasx_status validate(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o,
         return fail(ctx, GASX_E_ARG, "batched runs need n_steps >= 0 and sort_every = 0");
     if (o->collide && (!b->density || !b->n_max || b->gnr < 1 || b->gnz < 1 || !(b->gdr > 0) || !(b->gdz > 0)))
         return fail(ctx, GASX_E_ARG, "batched collisions need a gas description");
-    std::vector<int> idx(b->nruns);
-    std::iota(idx.begin(), idx.end(), 0);
     int64_t chunks = 0;
     for (int r = 0; r < b->nruns; ++r) {
-        if (b->base[r] < 0 || b->base[r] % kChunk != 0 || b->capacity[r] < 1 ||
-            b->capacity[r] > (int64_t)INT32_MAX - kChunk || (o->collide && !(b->n_max[r] >= 0)))
-            return fail(ctx, GASX_E_ARG, "run " + std::to_string(r) + ": bad base, capacity or n_max");
-        chunks += (b->capacity[r] + kSeg - 1) / kSeg;
+        if (o->collide && !(b->n_max[r] >= 0))
+            return fail(ctx, GASX_E_ARG, "run " + std::to_string(r) + ": bad n_max");
+        if (b->capacity[r] >= 1) chunks += (b->capacity[r] + kSeg - 1) / kSeg;
     }
-    std::sort(idx.begin(), idx.end(), [&](int a, int c) { return b->base[a] < b->base[c]; });
-    for (int k = 1; k < b->nruns; ++k)
-        if (b->base[idx[k - 1]] + b->capacity[idx[k - 1]] > b->base[idx[k]])
-            return fail(ctx, GASX_E_ARG, "batch runs overlap in the pool");
+    if (!ranges_ok(b->base, b->capacity, b->nruns, pool_slots))
+        return fail(ctx, GASX_E_ARG, "batch runs: a bad base or capacity, or ranges that overlap in the pool");
+    *ion_slots = 0;
+    if (b->ix) {
+        if (!b->iy || !b->iz || !b->ivx || !b->ivy || !b->ivz || !b->ion_base || !b->ion_capacity ||
+            !b->ion_counts)
+            return fail(ctx, GASX_E_ARG, "invalid batch ion pool");
+        if (!ranges_ok(b->ion_base, b->ion_capacity, b->nruns, ion_slots))
+            return fail(ctx, GASX_E_ARG, "batch ions: a bad base or capacity, or ranges that overlap in the pool");
+    }
     *stage_segs = chunks;
-    *pool_slots = b->base[idx.back()] + b->capacity[idx.back()];
     return GASX_OK;
 }
 
@@ -166,6 +200,12 @@ std::vector<unsigned char> upload_key(const gasx_batch* b, const gasx_run_opts* 
     put(&o->collide, sizeof(o->collide));
     if (o->collide) put(b->n_max, b->nruns * sizeof(double));
     put(&o->run_id, sizeof(o->run_id));
+    const bool ions = b->ix != nullptr;
+    put(&ions, sizeof(ions));
+    if (ions) {
+        put(b->ion_base, b->nruns * sizeof(int64_t));
+        put(b->ion_capacity, b->nruns * sizeof(int64_t));
+    }
     return k;
 }
 
@@ -176,7 +216,7 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
     BatchState& st = ctx->batch;
     const Sizes z = sizes(ctx);
     const int n = b->nruns;
-    std::vector<BatchRun> runs(n);
+    std::vector<BatchRun> runs(n), iruns;
     for (int r = 0; r < n; ++r) {
         BatchRun& br = runs[r];
         br.base = b->base[r];
@@ -188,6 +228,13 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
         }
         br.run = o->run_id + (uint32_t)r;
         br.pad_ = 0;
+    }
+    if (b->ix) {   // the same runs over their ranges of the ion pool
+        iruns = runs;
+        for (int r = 0; r < n; ++r) {
+            iruns[r].base = b->ion_base[r];
+            iruns[r].cap = (int32_t)b->ion_capacity[r];
+        }
     }
     std::vector<FdtdRun> fr(4 * (size_t)n);
     for (int parity = 0; parity < 2; ++parity)
@@ -224,6 +271,9 @@ gasx_status upload(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opts* o)
             }
     HIPCHK(hipMemcpyAsync(st.runs, runs.data(), n * sizeof(BatchRun), hipMemcpyHostToDevice, ctx->stream),
            "upload runs");
+    if (b->ix)
+        HIPCHK(hipMemcpyAsync(st.ion_runs, iruns.data(), n * sizeof(BatchRun), hipMemcpyHostToDevice, ctx->stream),
+               "upload ion runs");
     HIPCHK(hipMemcpyAsync(st.fdtd_runs, fr.data(), fr.size() * sizeof(FdtdRun), hipMemcpyHostToDevice,
                           ctx->stream),
            "upload field runs");
@@ -270,7 +320,13 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
     pa.jstride = jstride;
     const int grid = particle_grid(ctx, stage_segs, tune.particle_blocks_per_cu);
     float* pool[6] = {b->x, b->y, b->z, b->vx, b->vy, b->vz};
+    float* ipool[6] = {b->ix, b->iy, b->iz, b->ivx, b->ivy, b->ivz};
     const bool sec = o->collide && o->secondaries;
+    const bool ions = has_ions(b, o);
+    if (ions) {
+        for (int c = 0; c < 6; ++c) pa.ion[c] = st.istage[c];
+        pa.ion_cnt = st.istage_cnt;
+    }
 
     for (int bb = 0; bb < 2; ++bb)
         for (int k = 0; k < 2; ++k)
@@ -297,6 +353,7 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
         // the single-run loop does
         uint32_t flags = kBatch | kPush | (last ? 0u : kDrift2);
         if (o->collide) flags |= kCollide | (sec ? (kStageSec | (last ? 0u : kDriftSec)) : 0u);
+        if (ions) flags |= kStageIon;
         PartArgs ps = pa;
         ps.qr = last ? nullptr : st.q[nxt][0];
         ps.qz = last ? nullptr : st.q[nxt][1];
@@ -308,12 +365,19 @@ gasx_status enqueue_steps(gasx_ctx* ctx, const gasx_batch* b, const gasx_run_opt
             pp.qr = pp.qz = nullptr;
             HIPCHK(launch_particles(flags, ctx->geom, pp, cs, grid, ctx->stream), "particles");
         }
-        if (sec) {   // appended first: the deposit then covers the new slots
+        if (sec || ions) {   // appended first (ions, then secondaries): the deposit then covers the new slots
             TimedScope ts(ctx, kTAppend);
-            HIPCHK(launch_append_batch(pool, b->counts, st.runs, n, st.stage, st.stage_cnt, st.chunk_start,
-                                       st.offs, st.tmp_run, (int32_t)stage_segs, st.run_counters,
-                                       &ctx->d_counters[kCDropSec], ctx->stream),
-                   "append");
+            if (ions)
+                HIPCHK(launch_append_batch(ipool, b->ion_counts, st.ion_runs, n, st.istage, st.istage_cnt,
+                                           st.chunk_start, st.offs, st.tmp_run, (int32_t)stage_segs,
+                                           st.run_counters, &ctx->d_counters[kCDropIon], kCDropIon,
+                                           ctx->stream),
+                       "append ions");
+            if (sec)
+                HIPCHK(launch_append_batch(pool, b->counts, st.runs, n, st.stage, st.stage_cnt, st.chunk_start,
+                                           st.offs, st.tmp_run, (int32_t)stage_segs, st.run_counters,
+                                           &ctx->d_counters[kCDropSec], kCDropSec, ctx->stream),
+                       "append");
         }
         if (!last) {
             TimedScope ts(ctx, kTDrift);
@@ -342,6 +406,10 @@ std::vector<unsigned char> graph_key(const gasx_batch* b, const gasx_run_opts* o
     put(b->base, b->nruns * sizeof(int64_t));
     put(b->capacity, b->nruns * sizeof(int64_t));
     if (o->collide) put(b->n_max, b->nruns * sizeof(double));
+    if (b->ix) {
+        put(b->ion_base, b->nruns * sizeof(int64_t));
+        put(b->ion_capacity, b->nruns * sizeof(int64_t));
+    }
     gasx_run_opts oo = *o;
     oo.time_kernels = 0;
     oo.time_every = 0;
@@ -352,11 +420,11 @@ std::vector<unsigned char> graph_key(const gasx_batch* b, const gasx_run_opts* o
 
 gasx_status run_batch(gasx_ctx* ctx, gasx_batch* b, const gasx_run_opts* o, const gasx_batch_tuning& tune)
 {
-    int64_t stage_segs = 0, pool = 0;
-    gasx_status s = validate(ctx, b, o, &stage_segs, &pool);
+    int64_t stage_segs = 0, pool = 0, ipool = 0;
+    gasx_status s = validate(ctx, b, o, &stage_segs, &pool, &ipool);
     if (s != GASX_OK) return s;
     if (o->n_steps == 0) return GASX_OK;
-    s = ensure_batch(ctx, b->nruns, stage_segs);
+    s = ensure_batch(ctx, b->nruns, stage_segs, has_ions(b, o));
     if (s != GASX_OK) return s;
     BatchState& st = ctx->batch;
     if (!o->use_graph) {
@@ -455,21 +523,24 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
 {
     if (!ctx || !b || !o || trial_steps < 1) return GASX_E_ARG;
     HIPCHK(hipSetDevice(ctx->device), "hipSetDevice");
-    int64_t stage_segs = 0, pool = 0;
-    gasx_status s = validate(ctx, b, o, &stage_segs, &pool);
+    int64_t stage_segs = 0, pool = 0, ipool = 0;
+    gasx_status s = validate(ctx, b, o, &stage_segs, &pool, &ipool);
     if (s != GASX_OK) return s;
-    s = ensure_batch(ctx, b->nruns, stage_segs);
+    const bool ions = has_ions(b, o);
+    s = ensure_batch(ctx, b->nruns, stage_segs, ions);
     if (s != GASX_OK) return s;
     const Sizes z = sizes(ctx);
     const size_t n = (size_t)b->nruns;
     // snapshot of everything a trial changes
     const size_t fbytes = n * (z.ner + z.nez + z.nh) * sizeof(double);
     const size_t pbytes = (size_t)pool * sizeof(float);
+    const size_t ibytes = ions ? (size_t)ipool * sizeof(float) : 0;
     char* snap = nullptr;
-    HIPCHK(hipMalloc((void**)&snap, 6 * pbytes + fbytes + n * sizeof(int32_t) +
+    HIPCHK(hipMalloc((void**)&snap, 6 * pbytes + 6 * ibytes + fbytes + 2 * n * sizeof(int32_t) +
                                         (size_t)kNCounters * kCounterSlots * 8 + n * kNCounters * 8),
            "alloc snapshot");
     float* pools[6] = {b->x, b->y, b->z, b->vx, b->vy, b->vz};
+    float* ipools[6] = {b->ix, b->iy, b->iz, b->ivx, b->ivy, b->ivz};
     auto copy_state = [&](bool save) -> hipError_t {
         char* p = snap;
         auto cp = [&](void* live, size_t bytes) -> hipError_t {
@@ -484,6 +555,8 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
         if (e == hipSuccess) e = cp(b->ez, n * z.nez * sizeof(double));
         if (e == hipSuccess) e = cp(b->hphi, n * z.nh * sizeof(double));
         if (e == hipSuccess) e = cp(b->counts, n * sizeof(int32_t));
+        for (int c = 0; c < 6 && ions && e == hipSuccess; ++c) e = cp(ipools[c], ibytes);
+        if (ions && e == hipSuccess) e = cp(b->ion_counts, n * sizeof(int32_t));
         if (e == hipSuccess) e = cp(ctx->d_counters, (size_t)kNCounters * kCounterSlots * 8);
         if (e == hipSuccess) e = cp(ctx->batch.run_counters, n * kNCounters * 8);
         return e;
@@ -527,7 +600,8 @@ extern "C" gasx_status gasx_batch_autotune(gasx_ctx* ctx, gasx_batch* b, const g
     }
     // one or two generations of resident workgroups, or a fixed oversubscription
     const uint32_t step_flags = kBatch | kPush | kDrift2 |
-                                (o->collide ? kCollide | (o->secondaries ? kStageSec | kDriftSec : 0u) : 0u);
+                                (o->collide ? kCollide | (o->secondaries ? kStageSec | kDriftSec : 0u) : 0u) |
+                                (ions ? kStageIon : 0u);
     const int occ = particle_occupancy(step_flags);
     const int per_cu[] = {occ, 2 * occ, 4, 8, 16, 32};
     for (int k = 0; k < (int)(sizeof(per_cu) / sizeof(per_cu[0])); ++k) {

@@ -17,12 +17,16 @@ struct BatchState {
     float *er32 = nullptr, *ez32 = nullptr, *h32 = nullptr;      // FP32 gather copies, run-major
     unsigned long long* q[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};   // int64 current, run-major
     gasx::BatchRun* runs = nullptr;             // device descriptors
+    gasx::BatchRun* ion_runs = nullptr;         // the runs' ranges of the ion pool
     gasx::FdtdRun* fdtd_runs = nullptr;         // 4 x nruns: [even|odd] x [clear|last]
     int32_t* chunk_start = nullptr;             // nruns + 1 segment prefix
     int32_t* tmp_run = nullptr;                 // 2 x nruns
     int32_t* offs = nullptr;                    // per staging segment
     int32_t* stage_cnt = nullptr;               // per staging segment
     float* stage[6] = {};
+    float* istage[6] = {};                      // ion staging (allocated with the first ion batch)
+    int32_t* istage_cnt = nullptr;
+    int64_t istage_segs = 0;
     unsigned long long* run_counters = nullptr; // nruns x kNCounters
     // captured step sequence (use_graph), replayed while the key matches
     hipGraphExec_t exec = nullptr;

@@ -239,7 +239,7 @@ hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const Batc
                                float* const stage[6], const int32_t* stage_cnt,
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
-                               unsigned long long* dropped, hipStream_t s);
+                               unsigned long long* dropped, int drop_idx, hipStream_t s);
 hipError_t fdtd_prepare();
 hipError_t launch_cross_sections(const DevTables* t, const float* e_kev, int64_t n, float* se,
                                  float* si, float* sx, hipStream_t s);

@@ -1163,7 +1163,7 @@ __global__ __launch_bounds__(1024) void k_append_scan_batch(int32_t* counts, con
                                                             const int32_t* chunk_start, int32_t* offs,
                                                             int32_t* tmp_run,
                                                             unsigned long long* run_counters,
-                                                            unsigned long long* dropped)
+                                                            unsigned long long* dropped, int drop_idx)
 {
     __shared__ int s_wtot[16];
     const int r = blockIdx.x;
@@ -1176,7 +1176,7 @@ __global__ __launch_bounds__(1024) void k_append_scan_batch(int32_t* counts, con
         tmp_run[2 * r] = n;
         counts[r] = n + app;
         if (total > app) {
-            atomicAdd(&run_counters[(size_t)r * kNCounters + kCDropSec], (unsigned long long)(total - app));
+            atomicAdd(&run_counters[(size_t)r * kNCounters + drop_idx], (unsigned long long)(total - app));
             atomicAdd(dropped, (unsigned long long)(total - app));
         }
     }
@@ -1235,10 +1235,10 @@ hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const Batc
                                float* const stage[6], const int32_t* stage_cnt,
                                const int32_t* chunk_start, int32_t* offs, int32_t* tmp_run,
                                int32_t total_chunks, unsigned long long* run_counters,
-                               unsigned long long* dropped, hipStream_t s)
+                               unsigned long long* dropped, int drop_idx, hipStream_t s)
 {
     hipLaunchKernelGGL(k_append_scan_batch, dim3(nruns), dim3(1024), 0, s, counts, runs, stage_cnt,
-                       chunk_start, offs, tmp_run, run_counters, dropped);
+                       chunk_start, offs, tmp_run, run_counters, dropped, drop_idx);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int batches = (total_chunks + kBlock - 1) / kBlock;
@@ -1339,6 +1339,10 @@ int particle_occupancy(uint32_t flags)
     case B | P | D2: return occupancy_t<B | P | D2>();
     case B | P | Co | D2: return occupancy_t<B | P | Co | D2>();
     case B | P | Co | S | DS | D2: return occupancy_t<B | P | Co | S | DS | D2>();
+    case B | P | Co | I: return occupancy_t<B | P | Co | I>();
+    case B | P | Co | S | I: return occupancy_t<B | P | Co | S | I>();
+    case B | P | Co | I | D2: return occupancy_t<B | P | Co | I | D2>();
+    case B | P | Co | S | I | DS | D2: return occupancy_t<B | P | Co | S | I | DS | D2>();
     default: return 1;
     }
 }
@@ -1370,6 +1374,10 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
     case B | P | D2: return launch_t<B | P | D2>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | D2: return launch_t<B | P | Co | D2>(g, pa, ca, blocks, s, ev0, ev1);
     case B | P | Co | S | DS | D2: return launch_t<B | P | Co | S | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | I: return launch_t<B | P | Co | I>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | S | I: return launch_t<B | P | Co | S | I>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | I | D2: return launch_t<B | P | Co | I | D2>(g, pa, ca, blocks, s, ev0, ev1);
+    case B | P | Co | S | I | DS | D2: return launch_t<B | P | Co | S | I | DS | D2>(g, pa, ca, blocks, s, ev0, ev1);
     default: return hipErrorInvalidValue;
     }
 }

@@ -103,6 +103,10 @@ class Batch(C.Structure):
         ("density", C.c_void_p), ("n_max", C.POINTER(C.c_double)),
         ("gnr", C.c_int32), ("gnz", C.c_int32),
         ("gdr", C.c_double), ("gdz", C.c_double), ("t_scalar", C.c_double),
+        ("ix", C.c_void_p), ("iy", C.c_void_p), ("iz", C.c_void_p),
+        ("ivx", C.c_void_p), ("ivy", C.c_void_p), ("ivz", C.c_void_p),
+        ("ion_base", C.POINTER(C.c_int64)), ("ion_capacity", C.POINTER(C.c_int64)),
+        ("ion_counts", C.c_void_p),
     ]
 
 
@@ -245,7 +249,7 @@ PROTOTYPES = {
     "gasx_fdtd_transpose": (C.c_int, [_P, C.POINTER(Fields), _P, _P]),
 }
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 _lib = None
 

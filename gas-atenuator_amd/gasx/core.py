@@ -357,10 +357,13 @@ class Ensemble:
     """Independent runs on the context's grid, advanced together by
     gasx_batch_run: one particle launch and one field launch per step cover
     every run.  Run r owns pooled slots [base[r], base[r] + capacity[r]), its
-    own fields, gas density (gnz x gnr) and Philox run id (run_id + r)."""
+    own fields, gas density (gnz x gnr) and Philox run id (run_id + r).  With
+    ion_capacities, run r also owns ion slots [ion_base[r], ion_base[r] +
+    ion_capacity[r]) of an ion pool, where the N2+ ions of its ionizations are
+    appended as gasx_run appends them to its ion species."""
 
     def __init__(self, ctx: Context, capacities, densities, gdr: float, gdz: float,
-                 t_scalar: float = 300.0, device="cuda"):
+                 t_scalar: float = 300.0, device="cuda", ion_capacities=None):
         self.ctx = ctx
         lib = ctx.lib
         align = lib.gasx_batch_align()
@@ -387,6 +390,15 @@ class Ensemble:
         self.n_max = np.asarray([float(x.max()) for x in d], dtype=np.float64)
         self.gnz, self.gnr = d[0].shape
         self.gdr, self.gdz, self.t_scalar = float(gdr), float(gdz), float(t_scalar)
+        self.ion_pool = self.ion_counts = None
+        if ion_capacities is not None:
+            self.ion_capacity = np.asarray([int(c) for c in ion_capacities], dtype=np.int64)
+            if self.ion_capacity.shape != (self.nruns,) or np.any(self.ion_capacity < 1):
+                raise ValueError("one positive ion capacity per run")
+            ipad = (self.ion_capacity + align - 1) // align * align
+            self.ion_base = np.concatenate([[0], np.cumsum(ipad)[:-1]]).astype(np.int64)
+            self.ion_pool = torch.zeros((6, int(ipad.sum())), dtype=torch.float32, device=device)
+            self.ion_counts = torch.zeros(self.nruns, dtype=torch.int32, device=device)
 
     def load_run(self, r: int, x, y, z, vx, vy, vz):
         n = len(x)
@@ -403,18 +415,32 @@ class Ensemble:
         b = int(self.base[r])
         return self.pool[:, b: b + n].cpu().numpy()
 
+    def run_ions(self, r: int) -> np.ndarray:
+        """(6, n) float32 host copy of run r's ions."""
+        if self.ion_pool is None:
+            raise ValueError("the ensemble has no ion pool")
+        n = int(self.ion_counts[r].item())
+        b = int(self.ion_base[r])
+        return self.ion_pool[:, b: b + n].cpu().numpy()
+
     def struct(self) -> "_abi.Batch":
         d = self.pool
+        ions = self.ion_pool is not None
         self._keep = (np.ascontiguousarray(self.base), np.ascontiguousarray(self.capacity),
-                      np.ascontiguousarray(self.n_max))
+                      np.ascontiguousarray(self.n_max),
+                      np.ascontiguousarray(self.ion_base) if ions else None,
+                      np.ascontiguousarray(self.ion_capacity) if ions else None)
         P64, PD = C.POINTER(C.c_int64), C.POINTER(C.c_double)
+        ion_args = ([self.ion_pool[k].data_ptr() for k in range(6)] +
+                    [self._keep[3].ctypes.data_as(P64), self._keep[4].ctypes.data_as(P64),
+                     self.ion_counts.data_ptr()]) if ions else [None] * 6 + [P64(), P64(), None]
         return _abi.Batch(self.nruns, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(),
                           d[3].data_ptr(), d[4].data_ptr(), d[5].data_ptr(),
                           self._keep[0].ctypes.data_as(P64), self._keep[1].ctypes.data_as(P64),
                           self.counts.data_ptr(), self.er.data_ptr(), self.ez.data_ptr(),
                           self.hphi.data_ptr(), self.density.data_ptr(),
                           self._keep[2].ctypes.data_as(PD), self.gnr, self.gnz, self.gdr, self.gdz,
-                          self.t_scalar)
+                          self.t_scalar, *ion_args)
 
     @staticmethod
     def _opts(n_steps, collide, secondaries, seed, run_id, step0, use_graph, time_kernels):

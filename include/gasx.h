@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define GASX_ABI_VERSION 7
+#define GASX_ABI_VERSION 8
 
 typedef enum gasx_status {
     GASX_OK = 0,
@@ -223,10 +223,14 @@ gasx_status gasx_run(gasx_ctx* ctx, gasx_species* e, gasx_species* ions, gasx_fi
  * Many independent runs of the production loop on the context's grid, advanced
  * together: one particle launch and one field launch per step cover every run.
  * The reference has no batched form; each run is exactly what gasx_run with
- * the same seed, run id (opts.run_id + r) and gas would produce (sort_every
- * must be 0; ions are not supported).  Particle slots of all runs live in one
- * pooled SoA: run r owns [base[r], base[r] + capacity[r]), base[r] a multiple
- * of gasx_batch_align(). */
+ * the same seed, run id (opts.run_id + r), gas and ions would produce
+ * (sort_every must be 0).  Particle slots of all runs live in one pooled SoA:
+ * run r owns [base[r], base[r] + capacity[r]), base[r] a multiple of
+ * gasx_batch_align().  With an ion pool (ix non-NULL), the N2+ ions of
+ * ionizations are appended to it as gasx_run appends them to its ions
+ * (electron_collision.m:63-66): run r owns [ion_base[r], ion_base[r] +
+ * ion_capacity[r]) of the ion pool, ion_base[r] a multiple of
+ * gasx_batch_align(). */
 typedef struct gasx_batch {
     int32_t nruns;                       /* 1 .. 1024 */
     float *x, *y, *z, *vx, *vy, *vz;     /* pooled SoA, device */
@@ -239,6 +243,10 @@ typedef struct gasx_batch {
     int32_t gnr, gnz;
     double gdr, gdz;
     double t_scalar;                     /* gas temperature (K) */
+    float *ix, *iy, *iz, *ivx, *ivy, *ivz;   /* pooled ion SoA, device, or ix = NULL: no ions */
+    const int64_t* ion_base;             /* host, nruns */
+    const int64_t* ion_capacity;         /* host, nruns */
+    int32_t* ion_counts;                 /* device int32[nruns]: ion slots in use per run */
 } gasx_batch;
 
 /* Launch shapes of the batched loop (picked by gasx_batch_autotune). */

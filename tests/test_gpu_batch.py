@@ -78,6 +78,51 @@ def test_batch_equals_sequential_runs(gpu, secondaries, n_steps):
     assert sum(c.n_ionize for c in per_run) > 0
 
 
+@pytest.mark.parametrize("secondaries,use_graph", [(False, False), (True, False), (True, True)])
+def test_batch_with_ions_equals_sequential_runs(gpu, secondaries, use_graph):
+    """Runs with an ion pool: each run's ions (the N2+ of its ionizations,
+    appended in parent slot order) and electrons equal gasx_run's with an ion
+    species; one run's ion capacity is too small, so its drops are counted."""
+    import torch
+    import gasx
+    log(f"batch with ions secondaries={secondaries} graph={use_graph}")
+    params = gasx.make_params(NR, NZ, fn=2e3)
+    sizes, dens, parts, caps, _ = ensemble_case(4)
+    icaps = [max(1, 3 * n) for n in sizes]
+    icaps[3] = 5                                          # fills within the first step
+    ctx = gasx.Context(params)
+    ens = gasx.Ensemble(ctx, caps, dens, 4e-5, 4e-5, ion_capacities=icaps)
+    for r, p in enumerate(parts):
+        if p.shape[1]:
+            ens.load_run(r, *p)
+    ens.run(6, collide=True, secondaries=secondaries, seed=21, run_id=7, use_graph=use_graph)
+    torch.cuda.synchronize()
+    per_run = ens.counters()
+    total_ions = 0
+    for r in range(len(sizes)):
+        c1 = gasx.Context(params)
+        e = gasx.ParticleSet(caps[r])
+        if parts[r].shape[1]:
+            e.load(*parts[r])
+        ions = gasx.ParticleSet(icaps[r])
+        f = gasx.FieldState(NR, NZ)
+        gas = gasx.GasGrid(dens[r], 4e-5, 4e-5, temperature=300.0)
+        c1.run(e, f, gas, 6, collide=True, secondaries=secondaries, sort_every=0, seed=21, run_id=7 + r,
+               ions=ions)
+        c = c1.counters()
+        assert_same(ens, r, e.state(), f.numpy())
+        gi = ens.run_ions(r)
+        ri = ions.state()
+        assert gi.shape == ri.shape, (r, gi.shape, ri.shape)
+        assert np.array_equal(gi, ri, equal_nan=True), r
+        got = per_run[r]
+        assert (got.n_ionize, got.n_dropped_ion, got.n_dropped_sec) == \
+               (c.n_ionize, c.n_dropped_ion, c.n_dropped_sec), r
+        total_ions += gi.shape[1]
+    assert total_ions > 0
+    assert per_run[3].n_dropped_ion > 0 and ens.run_ions(3).shape[1] == 5
+
+
 def test_batch_graph_replays_match_eager(gpu):
     """One captured graph of the whole step sequence, replayed twice with
     eager work and synchronizations between the replays."""
@@ -181,4 +226,10 @@ def test_batch_rejects_bad_descriptions(gpu):
     bad_base[1] = bad_base[0] + 1                         # misaligned and overlapping
     b.base = bad_base.ctypes.data_as(C.POINTER(C.c_int64))
     o = _abi.RunOpts(2, 1, 0, 0, 0, 0, 0, 0, 0)
+    assert ctx.lib.gasx_batch_run(ctx.ctx, C.byref(b), C.byref(o)) == _abi.E_ARG
+    # ion pool ranges that overlap
+    ens_i = gasx.Ensemble(ctx, caps, dens, 4e-5, 4e-5, ion_capacities=[64] * len(caps))
+    b = ens_i.struct()
+    bad_ib = np.zeros(len(caps), dtype=np.int64)
+    b.ion_base = bad_ib.ctypes.data_as(C.POINTER(C.c_int64))
     assert ctx.lib.gasx_batch_run(ctx.ctx, C.byref(b), C.byref(o)) == _abi.E_ARG
