@@ -144,6 +144,13 @@ __device__ __forceinline__ void half_axis(float s, int n, int& i0, int& i1, floa
     w1 = (lo || hi) ? 0.0f : f;
 }
 
+// p[i] for an element index below 2^30: the byte offset zero-extended onto
+// the base, so a uniform base stays in scalar registers (one vector add).
+__device__ __forceinline__ float ld_off(const float* p, uint32_t i)
+{
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + (uint64_t)(i << 2));
+}
+
 // Linear weights on a node axis (nodes at i*d, i in [0,n]): EM4.cu:41-80.
 __device__ __forceinline__ void node_axis(float s, int n, int& i0, int& i1, float& w0, float& w1)
 {

@@ -98,12 +98,17 @@ __device__ __forceinline__ void gather_issue(const Geom& g, const PartArgs& pa, 
     half_axis(szh, g.nz, kh0, kh1, G.c0, G.c1);
     // Ez: nodes in r, cell-centred in z (EM4.cu:61-80)
     node_axis(srn, g.nr, j0, j1, G.d0, G.d1);
-    G.er[0] = pa.er[i0 + k0 * nr]; G.er[1] = pa.er[i1 + k0 * nr];
-    G.er[2] = pa.er[i0 + k1 * nr]; G.er[3] = pa.er[i1 + k1 * nr];
-    G.h[0] = pa.h[i0 + kh0 * nr]; G.h[1] = pa.h[i1 + kh0 * nr];
-    G.h[2] = pa.h[i0 + kh1 * nr]; G.h[3] = pa.h[i1 + kh1 * nr];
-    G.ez[0] = pa.ez[j0 + kh0 * ld]; G.ez[1] = pa.ez[j1 + kh0 * ld];
-    G.ez[2] = pa.ez[j0 + kh1 * ld]; G.ez[3] = pa.ez[j1 + kh1 * ld];
+    // every index is clamped into its array: unsigned 32-bit element offsets
+    // (24-bit row products) keep the address arithmetic to one add per read
+    const uint32_t r0 = __umul24((uint32_t)k0, (uint32_t)nr), r1 = __umul24((uint32_t)k1, (uint32_t)nr);
+    const uint32_t h0 = __umul24((uint32_t)kh0, (uint32_t)nr), h1 = __umul24((uint32_t)kh1, (uint32_t)nr);
+    const uint32_t e0 = __umul24((uint32_t)kh0, (uint32_t)ld), e1 = __umul24((uint32_t)kh1, (uint32_t)ld);
+    G.er[0] = ld_off(pa.er, (uint32_t)i0 + r0); G.er[1] = ld_off(pa.er, (uint32_t)i1 + r0);
+    G.er[2] = ld_off(pa.er, (uint32_t)i0 + r1); G.er[3] = ld_off(pa.er, (uint32_t)i1 + r1);
+    G.h[0] = ld_off(pa.h, (uint32_t)i0 + h0); G.h[1] = ld_off(pa.h, (uint32_t)i1 + h0);
+    G.h[2] = ld_off(pa.h, (uint32_t)i0 + h1); G.h[3] = ld_off(pa.h, (uint32_t)i1 + h1);
+    G.ez[0] = ld_off(pa.ez, (uint32_t)j0 + e0); G.ez[1] = ld_off(pa.ez, (uint32_t)j1 + e0);
+    G.ez[2] = ld_off(pa.ez, (uint32_t)j0 + e1); G.ez[3] = ld_off(pa.ez, (uint32_t)j1 + e1);
 }
 
 __device__ __forceinline__ void push_apply(const Geom& g, const PartArgs& pa, const FieldGather& G, float& vx,
@@ -197,27 +202,31 @@ __device__ inline float elastic_theta(const DevTables* t, const SmallTabs* st, c
     }
     const float w = (lx - st->dcs_loge[i]) / (st->dcs_loge[i + 1] - st->dcs_loge[i]);
     const float wa = (1.0f - w) * st->dcs_z[i], wb = w * st->dcs_z[i + 1];
-    const float* v0 = &t->dcs_logv[i * kNAng];
-    const float* v1 = &t->dcs_logv[(i + 1) * kNAng];
-    const float* p0 = &t->dcs_p[i * kNAng];
-    const float* p1 = &t->dcs_p[(i + 1) * kNAng];
+    // element offsets of nodes i and i+1 in the angle tables (unsigned 32-bit:
+    // a uniform table base stays in scalar registers)
+    const uint32_t o0 = (uint32_t)i * kNAng, o1 = o0 + kNAng;
 #pragma unroll 1
     for (uint32_t it = 0; it < 64; ++it) {
         const U4 R = philox4x32_10(slot, ca.step, ca.run, 2u + it, ca.key0, ca.key1);
         const float a = u01(R.a), b = u01(R.b), c = u01(R.c), d = u01(R.d);
-        const int j = (a * (wa + wb) < wa) ? i : i + 1;
-        const float* C = &t->dcs_cdf[j * kNAng];
+        const bool first = a * (wa + wb) < wa;
+        const int j = first ? i : i + 1;
+        const uint32_t oc = first ? o0 : o1;
         // largest k with C[k] <= b lies in [guide[g], guide[g+1]] (binary search)
         const int gi = (int)(b * (float)kGuide);
         int k = st->guide[j][gi], khi = st->guide[j][gi + 1] + 1;
         if (khi > kNAng - 1) khi = kNAng - 1;
         while (khi - k > 1) {
             const int mid = (k + khi) >> 1;
-            if (C[mid] <= b) k = mid; else khi = mid;
+            if (ld_off(t->dcs_cdf, oc + (uint32_t)mid) <= b) k = mid; else khi = mid;
         }
-        const float M = (1.0f - w) * (p0[k] + p0[k + 1]) + w * (p1[k] + p1[k + 1]);
-        const float m = st->sind[k] * __expf((1.0f - w) * v0[k] + w * v1[k]) +
-                        st->sind[k + 1] * __expf((1.0f - w) * v0[k + 1] + w * v1[k + 1]);
+        const uint32_t uk = (uint32_t)k;
+        const float M = (1.0f - w) * (ld_off(t->dcs_p, o0 + uk) + ld_off(t->dcs_p, o0 + uk + 1)) +
+                        w * (ld_off(t->dcs_p, o1 + uk) + ld_off(t->dcs_p, o1 + uk + 1));
+        const float m = st->sind[k] * __expf((1.0f - w) * ld_off(t->dcs_logv, o0 + uk) +
+                                             w * ld_off(t->dcs_logv, o1 + uk)) +
+                        st->sind[k + 1] * __expf((1.0f - w) * ld_off(t->dcs_logv, o0 + uk + 1) +
+                                                 w * ld_off(t->dcs_logv, o1 + uk + 1));
         if (c * M < m) return ((float)k + d) * (3.14159265358979f / 180.0f);
     }
     const U4 R = philox4x32_10(slot, ca.step, ca.run, 66u, ca.key0, ca.key1);
@@ -234,89 +243,159 @@ struct CollOut {
     float pt;              // p_T of the check (0 outside the tables' energy window)
 };
 
-// find_CS0.m cross sections, interp1 NaN gating (collide only for
-// 15.6 eV <= T <= 20 keV), linear p_T, branch choice by u2, and the elastic /
-// ionization / excitation kinematics of electron_transport.cu with the
-// MATLAB choices.  Contraction is off in the collision math (here and in
-// its callees): the kernels that inline it (single run, batched ensemble)
-// then cannot fuse differently, and a batched run stays bit-identical to the
-// same run alone.
-__device__ inline void collide_candidate(const Geom& g, const CollArgs& ca, const SmallTabs* stab,
-                                  uint32_t slot, float x, float y, float z, float& vx, float& vy,
-                                  float& vz, float u1, float u2, float u3, float u4, bool want_ion,
-                                  CollOut& o)
+// Gas cell of a position (the density / temperature grid, clamped).
+__device__ __forceinline__ void gas_cell(const CollArgs& ca, float x, float y, float z, int& ir, int& iz)
 {
 #pragma clang fp contract(off)
-    o.ev = 0;
-    o.pt = 0.0f;
+    const float r = fsqrt(x * x + y * y);
+    ir = (int)floorf(r * ca.inv_gdr);
+    iz = (int)floorf(z * ca.inv_gdz);
+    ir = ir < 0 ? 0 : (ir > ca.gnr - 1 ? ca.gnr - 1 : ir);
+    iz = iz < 0 ? 0 : (iz > ca.gnz - 1 ? ca.gnz - 1 : iz);
+}
+
+// The collision check of one candidate (electron_transport.cu:64-85 with the
+// find_CS0.m cross sections and interp1 NaN gating: only 15.6 eV <= T <=
+// 20 keV collides): linear p_T = v dt n sigma_T, the decision u1 < p_T and
+// the branch by u2 (typ 1 elastic, 2 ionization, 3 excitation, 0 none);
+// ev carries kEvViol (p_T above the rejection bound) and kEvSat (p_T > 1).
+// Contraction is off in the collision math (here and in the kinematics
+// below): the kernels that inline it (single run, batched ensemble) then
+// cannot fuse differently, and a batched run stays bit-identical to the
+// same run alone.
+struct CollDecision {
+    int typ;
+    int ev;
+    float pt;              // p_T of the check (0 outside the tables' energy window)
+};
+
+__device__ inline void collide_decide(const Geom& g, const CollArgs& ca, const SmallTabs* stab, float x, float y,
+                                      float z, float vx, float vy, float vz, float u1, float u2, CollDecision& d)
+{
+#pragma clang fp contract(off)
+    d.typ = 0;
+    d.ev = 0;
+    d.pt = 0.0f;
     const float v2 = vx * vx + vy * vy + vz * vz;
     const float T = ev_from_speed2(v2, g.inv_c2, ca.ev_per_mc2_em);
     if (!(T >= 15.6f && T <= 20000.0f)) return;
-    const DevTables* t = ca.tab;
     float se, si, sx;
     cross_sections(stab, T, se, si, sx);
-    const float r = fsqrt(x * x + y * y);
-    int ir = (int)floorf(r * ca.inv_gdr), iz = (int)floorf(z * ca.inv_gdz);
-    ir = ir < 0 ? 0 : (ir > ca.gnr - 1 ? ca.gnr - 1 : ir);
-    iz = iz < 0 ? 0 : (iz > ca.gnz - 1 ? ca.gnz - 1 : iz);
+    int ir, iz;
+    gas_cell(ca, x, y, z, ir, iz);
     const float dens = ca.dens[iz * ca.gnr + ir];
     const float st = se + si + sx;
     const float ve = fsqrt(v2);
     const float pt = ve * ca.dt_coll * dens * st;      // electron_transport.cu:74-77
-    o.pt = pt;
-    if (pt > ca.pmax && ca.pmax < 1.0f) o.ev |= kEvViol;
+    d.pt = pt;
+    if (pt > ca.pmax && ca.pmax < 1.0f) d.ev |= kEvViol;
     if (!(u1 < pt)) return;                             // :85
-    if (pt > 1.0f) o.ev |= kEvSat;
+    if (pt > 1.0f) d.ev |= kEvSat;
+    if (u2 * st < se) d.typ = 1;                        // elastic (:86-137)
+    else if (u2 * st < se + si && T > 15.6f) d.typ = 2; // ionization (:139-205)
+    else d.typ = 3;                                     // excitation (:206-226)
+}
+
+// The energy (eV) and speed of the push's electron, as collide_decide
+// computes them.
+__device__ __forceinline__ void candidate_energy(const Geom& g, const CollArgs& ca, float vx, float vy, float vz,
+                                                 float& T, float& ve)
+{
+#pragma clang fp contract(off)
+    const float v2 = vx * vx + vy * vy + vz * vz;
+    T = ev_from_speed2(v2, g.inv_c2, ca.ev_per_mc2_em);
+    ve = fsqrt(v2);
+}
+
+// Elastic scattering (electron_transport.cu:86-137): polar angle from the
+// merged DCS tables, azimuth 2 pi u4.
+__device__ inline void collide_elastic(const CollArgs& ca, const SmallTabs* stab, uint32_t slot, float T, float u4,
+                                       float& vx, float& vy, float& vz)
+{
+#pragma clang fp contract(off)
     const float twopi = 6.28318530717959f;
-    if (u2 * st < se) {                                 // elastic (:86-137)
-        const float th = elastic_theta(t, stab, ca, slot, T * 1e-3f);
-        rotate_v(vx, vy, vz, th, twopi * u4, ca.rot_eps);
-        o.ev |= 1;
-    } else if (u2 * st < se + si && T > 15.6f) {        // ionization (:139-205)
-        const float I = 15.6f;
-        const float T0 = 4.17f - 1000.0f / (T + 2.0f * I);
-        const float tt = 13.8f * T / (T + I);
-        const float Tm = (T - I) * 0.5f;
-        const float k1 = atanf((Tm - T0) / tt);
-        const float k2 = atanf(T0 / tt);
-        float k0 = T0 + tt * tanf(u3 * (k1 + k2) - k2);
-        k0 = fminf(fmaxf(k0, 0.0f), Tm);
-        const float E1 = k0, E2 = T - E1 - I;
-        const float Ep = E2 > E1 ? E2 : E1;
-        const float Es = E2 > E1 ? E1 : E2;
-        const float Ee = ca.mc2_mc_ev;
-        const float kt = k0 / T;
-        const float sp = sqrtf(kt / ((1.0f - kt) * T / (2.0f * Ee) + 1.0f));
-        const float ss = sqrtf((1.0f - kt) / (1.0f + k0 / (2.0f * Ee)));
-        const float thp = asinf(fminf(sp, 1.0f)), ths = asinf(fminf(ss, 1.0f));
-        const float php = twopi * u4, phs = php - 3.14159265358979f;
-        const float ratio = speed_from_ev(Ep, Ee, g.c) / ve;
-        vx *= ratio; vy *= ratio; vz *= ratio;
-        rotate_v(vx, vy, vz, thp, php, ca.rot_eps);
-        const float vs = speed_from_ev(Es, Ee, g.c);
-        float sts, cts, sps, cps;
-        __sincosf(ths, &sts, &cts);   // |angles| <= pi: no range reduction needed
-        __sincosf(phs, &sps, &cps);
-        o.ev |= 2 | kEvSec;
-        o.svx = vs * sts * cps; o.svy = vs * sts * sps; o.svz = vs * cts;   // :198-200
-        if (want_ion) {                                  // :202-204, Box-Muller normals
-            const U4 Q = philox4x32_10(slot, ca.step, ca.run, 1u, ca.key0, ca.key1);
-            const float a1 = u01(Q.a), a2 = u01(Q.b), a3 = u01(Q.c), a4 = u01(Q.d);
-            const float Tg = ca.temp ? ca.temp[iz * ca.gnr + ir] : ca.t_scalar;
-            const float sg = ca.ion_vth_k * sqrtf(Tg);
-            const float rr1 = sqrtf(-2.0f * logf(a1)), rr2 = sqrtf(-2.0f * logf(a3));
-            float s1, c1, s2, c2;
-            __sincosf(twopi * a2, &s1, &c1);
-            __sincosf(twopi * a4, &s2, &c2);
-            o.ev |= kEvIon;
-            o.ivx = sg * rr1 * c1; o.ivy = sg * rr1 * s1; o.ivz = sg * rr2 * c2;
-        }
-    } else {                                             // excitation (:206-226)
-        const float edep = exc_edep(stab, T, ca.exc_form);
-        const float ratio = speed_from_ev(T - edep, ca.mc2_mc_ev, g.c) / ve;
-        vx *= ratio; vy *= ratio; vz *= ratio;
-        o.ev |= 3;
+    const float th = elastic_theta(ca.tab, stab, ca, slot, T * 1e-3f);
+    rotate_v(vx, vy, vz, th, twopi * u4, ca.rot_eps);
+}
+
+// Ionization (electron_transport.cu:139-205): secondary energy by u3, the
+// primary and the secondary directions, and with want_ion the N2+ Maxwellian
+// velocity at the gas temperature of the site (:202-204).
+__device__ inline void collide_ionize(const Geom& g, const CollArgs& ca, uint32_t slot, float x, float y, float z,
+                                      float T, float ve, float u3, float u4, bool want_ion, float& vx, float& vy,
+                                      float& vz, CollOut& o)
+{
+#pragma clang fp contract(off)
+    const float twopi = 6.28318530717959f;
+    const float I = 15.6f;
+    const float T0 = 4.17f - 1000.0f / (T + 2.0f * I);
+    const float tt = 13.8f * T / (T + I);
+    const float Tm = (T - I) * 0.5f;
+    const float k1 = atanf((Tm - T0) / tt);
+    const float k2 = atanf(T0 / tt);
+    float k0 = T0 + tt * tanf(u3 * (k1 + k2) - k2);
+    k0 = fminf(fmaxf(k0, 0.0f), Tm);
+    const float E1 = k0, E2 = T - E1 - I;
+    const float Ep = E2 > E1 ? E2 : E1;
+    const float Es = E2 > E1 ? E1 : E2;
+    const float Ee = ca.mc2_mc_ev;
+    const float kt = k0 / T;
+    const float sp = sqrtf(kt / ((1.0f - kt) * T / (2.0f * Ee) + 1.0f));
+    const float ss = sqrtf((1.0f - kt) / (1.0f + k0 / (2.0f * Ee)));
+    const float thp = asinf(fminf(sp, 1.0f)), ths = asinf(fminf(ss, 1.0f));
+    const float php = twopi * u4, phs = php - 3.14159265358979f;
+    const float ratio = speed_from_ev(Ep, Ee, g.c) / ve;
+    vx *= ratio; vy *= ratio; vz *= ratio;
+    rotate_v(vx, vy, vz, thp, php, ca.rot_eps);
+    const float vs = speed_from_ev(Es, Ee, g.c);
+    float sts, cts, sps, cps;
+    __sincosf(ths, &sts, &cts);   // |angles| <= pi: no range reduction needed
+    __sincosf(phs, &sps, &cps);
+    o.ev |= 2 | kEvSec;
+    o.svx = vs * sts * cps; o.svy = vs * sts * sps; o.svz = vs * cts;   // :198-200
+    if (want_ion) {                                  // :202-204, Box-Muller normals
+        const U4 Q = philox4x32_10(slot, ca.step, ca.run, 1u, ca.key0, ca.key1);
+        const float a1 = u01(Q.a), a2 = u01(Q.b), a3 = u01(Q.c), a4 = u01(Q.d);
+        int ir, iz;
+        gas_cell(ca, x, y, z, ir, iz);
+        const float Tg = ca.temp ? ca.temp[iz * ca.gnr + ir] : ca.t_scalar;
+        const float sg = ca.ion_vth_k * sqrtf(Tg);
+        const float rr1 = sqrtf(-2.0f * logf(a1)), rr2 = sqrtf(-2.0f * logf(a3));
+        float s1, c1, s2, c2;
+        __sincosf(twopi * a2, &s1, &c1);
+        __sincosf(twopi * a4, &s2, &c2);
+        o.ev |= kEvIon;
+        o.ivx = sg * rr1 * c1; o.ivy = sg * rr1 * s1; o.ivz = sg * rr2 * c2;
     }
+}
+
+// Excitation (electron_transport.cu:206-226): the mean loss of the levels.
+__device__ inline void collide_excite(const Geom& g, const CollArgs& ca, const SmallTabs* stab, float T, float ve,
+                                      float& vx, float& vy, float& vz)
+{
+#pragma clang fp contract(off)
+    const float edep = exc_edep(stab, T, ca.exc_form);
+    const float ratio = speed_from_ev(T - edep, ca.mc2_mc_ev, g.c) / ve;
+    vx *= ratio; vy *= ratio; vz *= ratio;
+}
+
+// One candidate from decision to kinematics (the collision step of a lane
+// on its own; the particle kernel runs the parts in passes per type).
+__device__ inline void collide_candidate(const Geom& g, const CollArgs& ca, const SmallTabs* stab,
+                                         uint32_t slot, float x, float y, float z, float& vx, float& vy,
+                                         float& vz, float u1, float u2, float u3, float u4, bool want_ion,
+                                         CollOut& o)
+{
+    CollDecision d;
+    collide_decide(g, ca, stab, x, y, z, vx, vy, vz, u1, u2, d);
+    o.ev = d.ev | d.typ;
+    o.pt = d.pt;
+    if (d.typ == 0) return;
+    float T, ve;
+    candidate_energy(g, ca, vx, vy, vz, T, ve);
+    if (d.typ == 1) collide_elastic(ca, stab, slot, T, u4, vx, vy, vz);
+    else if (d.typ == 2) collide_ionize(g, ca, slot, x, y, z, T, ve, u3, u4, want_ion, vx, vy, vz, o);
+    else collide_excite(g, ca, stab, T, ve, vx, vy, vz);
 }
 
 }  // namespace gasx

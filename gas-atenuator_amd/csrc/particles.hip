@@ -204,8 +204,11 @@ static_assert(kPer % kGatherBatch == 0, "gather batch divides the slots per lane
 
 static_assert(kPer == 4, "one decision draw covers a lane's four slots");
 
-constexpr int kQW = 11;     // LDS queue words per candidate
-constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes 64)
+// LDS ring words per candidate: x y z vx vy vz, u1 (u4 after the decision),
+// u3 (after the decision), slot
+constexpr int kQW = 9;
+constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes them all; a power of two)
+static_assert((kQCap & (kQCap - 1)) == 0 && kQCap == 2 * kWaveSize, "a round decides two entries per lane");
 
 // Event counts of a wave added to the block's slot and, for an ensemble, to
 // the run's slot (lane 0 adds; several waves share the block slot).
@@ -316,6 +319,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
     constexpr int kWaves = kBlock / kWaveSize;
     __shared__ int s_wave[kWaves];
     __shared__ float s_q[kCol ? kWaves * kQW * kQCap : 1];   // per wave: kQW rows of kQCap
+    __shared__ uint8_t s_list[kCol ? kWaves * kQCap : 1];    // per wave: a round's entries by type
     __shared__ int s_cs[kBat ? kMaxRuns + 1 : 1];
     static_assert(sizeof(SmallTabs) % 4 == 0, "SmallTabs is copied in 32-bit words");
     __shared__ uint32_t s_tab_raw[kCol ? sizeof(SmallTabs) / 4 : 1];
@@ -373,93 +377,170 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
 
     // ---- the wave's candidate ring (wave-uniform state) ---------------------
     float (*Q)[kQCap] = reinterpret_cast<float (*)[kQCap]>(s_q + (kCol ? wave * kQW * kQCap : 0));
+    uint8_t* L = s_list + (kCol ? wave * kQCap : 0);
     int q_head = 0, q_n = 0;
-    int c_seg = -1, c_sec = 0, c_ion = 0;   // staging counts carried for the last round's last segment
+    int c_seg = -1, c_sec = 0, c_ion = 0;   // staging counts carried for the last staged segment
     PartArgs qpa = pa0;                       // the queued entries' run (one run at a time)
     CollArgs qca = ca0;
     int q_cs = 0;                             // first staging segment of that run
-    // Processes the m (1..64) oldest queued candidates, one per lane.
-    auto round = [&](int m) {
-        // the segments' stores (by other lanes) complete before this round rewrites velocities
-        __threadfence_block();
-        int qi = q_head + lane;
-        if (qi >= kQCap) qi -= kQCap;
-        const bool act = lane < m;
-        int ev = 0, slot = 0, gs = 0;
-        bool seg_last = false;   // the last candidate of its segment: records the segment's counts
-        float px = 0.f, py = 0.f, pz = 0.f;
-        CollOut co{};
-        if (act) {
-            px = Q[0][qi]; py = Q[1][qi]; pz = Q[2][qi];
-            float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
-            const int raw = __float_as_int(Q[10][qi]);
-            slot = raw & 0x7fffffff;
-            seg_last = raw < 0;
-            const U4 R = philox4x32_10((uint32_t)slot, qca.step, qca.run, 0u, qca.key0, qca.key1);
-            collide_candidate(g, qca, s_tab, (uint32_t)slot, px, py, pz, vx, vy, vz, Q[6][qi], u01(R.b),
-                              u01(R.c), u01(R.d), (F & kStageIon) != 0, co);
-            ev = co.ev;
-            if constexpr ((F & kDrift2) != 0) {
-                // the candidate's second half drift from its mid-step position,
-                // with its velocity after the collision (null or not)
-                float x2 = px, y2 = py, z2 = pz;
-                half_drift(g, x2, y2, z2, vx, vy, vz);
-                if (!wedge_bc(g, x2, y2, z2, vx, vy, vz)) {
-                    z2 = __int_as_float(0x7fc00000);   // absorbed: z = NaN
-                    lc.sa += 1u << 16;
-                }
-                qpa.x[slot] = x2; qpa.y[slot] = y2; qpa.z[slot] = z2;
-                qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz;
-            } else {
-                if (ev & kEvMask) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
+    const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+    // A candidate's last part: with kDrift2 its second half drift from the
+    // mid-step position with its velocity after the collision (null or not),
+    // then its six values; otherwise its new velocity, if it changed.
+    auto finish = [&](uint32_t slot, float px, float py, float pz, float vx, float vy, float vz, bool changed) {
+        if constexpr ((F & kDrift2) != 0) {
+            float x2 = px, y2 = py, z2 = pz;
+            half_drift(g, x2, y2, z2, vx, vy, vz);
+            if (!wedge_bc(g, x2, y2, z2, vx, vy, vz)) {
+                z2 = __int_as_float(0x7fc00000);   // absorbed: z = NaN
+                lc.sa += 1u << 16;
             }
-            gs = q_cs + slot / kSeg;
+            qpa.x[slot] = x2; qpa.y[slot] = y2; qpa.z[slot] = z2;
+            qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz;
+        } else {
+            if (changed) { qpa.vx[slot] = vx; qpa.vy[slot] = vy; qpa.vz[slot] = vz; }
         }
-        wave_sync();   // the ring entries are free once read
-        bool has_sec = (F & kStageSec) && act && (ev & kEvSec);
-        const bool has_ion = (F & kStageIon) && act && (ev & kEvIon);
-        float sx = px, sy = py, sz = pz, svx = co.svx, svy = co.svy, svz = co.svz;
-        if ((F & kDriftSec) && has_sec) {
-            half_drift(g, sx, sy, sz, svx, svy, svz);
-            if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; lc.sa += 1u << 16; }
+    };
+    // Processes the m (1..kQCap) oldest queued candidates: a decision pass
+    // over two entries per lane (null ones finish there), then one pass per
+    // collision type over that type's entries in queue order, compacted onto
+    // the lanes, so each type's kinematics run once for up to 64 collisions
+    // instead of once per 64 candidates of all types.
+    auto round = [&](int m) {
+        // the segments' stores (by other lanes) complete before this round rewrites them
+        __threadfence_block();
+        int typ0 = 0, typ1 = 0;
+        auto decide = [&](int e) -> int {
+            if (e >= m) return 0;
+            const int qi = (q_head + e) & (kQCap - 1);
+            const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
+            const float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
+            const uint32_t slot = __float_as_uint(Q[8][qi]);
+            const U4 R = philox4x32_10(slot, qca.step, qca.run, 0u, qca.key0, qca.key1);
+            CollDecision d;
+            collide_decide(g, qca, s_tab, px, py, pz, vx, vy, vz, Q[6][qi], u01(R.b), d);
+            lc.xv += (unsigned)((d.ev & kEvViol) != 0) << 16;
+            lc.sa += (unsigned)((d.ev & kEvSat) != 0);
+            if (d.typ == 0) {
+                finish(slot, px, py, pz, vx, vy, vz, false);
+            } else {   // the branch uniforms the kinematics pass reads
+                Q[6][qi] = u01(R.d);
+                Q[7][qi] = u01(R.c);
+            }
+            return d.typ;
+        };
+        typ0 = decide(lane);
+        typ1 = decide(kWaveSize + lane);
+        // per-type lists of entry indices in queue order: [0, n1) elastic,
+        // [n1, n2) ionization, [n2, n3) excitation
+        int lim[4];
+        lim[0] = 0;
+#pragma unroll
+        for (int t = 1; t <= 3; ++t) {
+            const unsigned long long ma = __ballot(typ0 == t), mb = __ballot(typ1 == t);
+            const int na = __popcll(ma);
+            if (typ0 == t) L[lim[t - 1] + __popcll(ma & below)] = (uint8_t)lane;
+            if (typ1 == t) L[lim[t - 1] + na + __popcll(mb & below)] = (uint8_t)(kWaveSize + lane);
+            lim[t] = lim[t - 1] + na + __popcll(mb);
         }
-        // rank of each staged entry inside its segment: entries of earlier rounds
-        // (carried) plus the earlier lanes of this round in the same segment
-        const int gprev = __shfl_up(gs, 1, 64);
-        const unsigned long long heads = __ballot(act && (lane == 0 || gs != gprev));
-        const unsigned long long below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-        const int head = 63 - __clzll(heads & (below | (1ull << lane)));
-        const unsigned long long span = below & ~((head == 0) ? 0ull : ((~0ull) >> (64 - head)));
-        int rs = __popcll(__ballot(has_sec) & span), ri = __popcll(__ballot(has_ion) & span);
-        if (gs == c_seg) { rs += c_sec; ri += c_ion; }
-        if (has_sec) {
-            const int d = gs * kSeg + rs;
-            qpa.sec[0][d] = sx; qpa.sec[1][d] = sy; qpa.sec[2][d] = sz;
-            qpa.sec[3][d] = svx; qpa.sec[4][d] = svy; qpa.sec[5][d] = svz;
+        wave_sync();
+        // elastic
+        for (int k0 = lim[0]; k0 < lim[1]; k0 += kWaveSize) {
+            const int k = k0 + lane;
+            if (k < lim[1]) {
+                const int qi = (q_head + L[k]) & (kQCap - 1);
+                const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
+                float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
+                const uint32_t slot = __float_as_uint(Q[8][qi]);
+                float T, ve;
+                candidate_energy(g, qca, vx, vy, vz, T, ve);
+                collide_elastic(qca, s_tab, slot, T, Q[6][qi], vx, vy, vz);
+                finish(slot, px, py, pz, vx, vy, vz, true);
+                lc.ei += 1u;
+            }
         }
-        if (has_ion) {   // ion at the ionization site (electron_collision.m:63)
-            const int d = gs * kSeg + ri;
-            qpa.ion[0][d] = px; qpa.ion[1][d] = py; qpa.ion[2][d] = pz;
-            qpa.ion[3][d] = co.ivx; qpa.ion[4][d] = co.ivy; qpa.ion[5][d] = co.ivz;
+        // excitation
+        for (int k0 = lim[2]; k0 < lim[3]; k0 += kWaveSize) {
+            const int k = k0 + lane;
+            if (k < lim[3]) {
+                const int qi = (q_head + L[k]) & (kQCap - 1);
+                const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
+                float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
+                const uint32_t slot = __float_as_uint(Q[8][qi]);
+                float T, ve;
+                candidate_energy(g, qca, vx, vy, vz, T, ve);
+                collide_excite(g, qca, s_tab, T, ve, vx, vy, vz);
+                finish(slot, px, py, pz, vx, vy, vz, true);
+                lc.xv += 1u;
+            }
         }
-        const int rs1 = rs + has_sec, ri1 = ri + has_ion;
-        if (seg_last) {
-            if (F & kStageSec) qpa.sec_cnt[gs] = rs1;
-            if (F & kStageIon) qpa.ion_cnt[gs] = ri1;
+        // ionization: secondaries and ions staged at their rank in the
+        // segment (the passes take the ionizations in slot order); the last
+        // lane of each segment records the segment's counts so far (a later
+        // pass of the same segment overwrites them with the larger totals)
+        for (int k0 = lim[1]; k0 < lim[2]; k0 += kWaveSize) {
+            const int k = k0 + lane;
+            const int cnt = min(lim[2] - k0, kWaveSize);
+            const bool act = lane < cnt;
+            int gs = 0;
+            float px = 0.f, py = 0.f, pz = 0.f;
+            CollOut co{};
+            if (act) {
+                const int qi = (q_head + L[k]) & (kQCap - 1);
+                px = Q[0][qi]; py = Q[1][qi]; pz = Q[2][qi];
+                float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
+                const uint32_t slot = __float_as_uint(Q[8][qi]);
+                float T, ve;
+                candidate_energy(g, qca, vx, vy, vz, T, ve);
+                collide_ionize(g, qca, slot, px, py, pz, T, ve, Q[7][qi], Q[6][qi], (F & kStageIon) != 0, vx, vy,
+                               vz, co);
+                finish(slot, px, py, pz, vx, vy, vz, true);
+                lc.ei += 1u << 16;
+                gs = q_cs + (int)(slot / kSeg);
+            }
+            bool has_sec = (F & kStageSec) && act;
+            const bool has_ion = (F & kStageIon) && act;
+            float sx = px, sy = py, sz = pz, svx = co.svx, svy = co.svy, svz = co.svz;
+            if ((F & kDriftSec) && has_sec) {
+                half_drift(g, sx, sy, sz, svx, svy, svz);
+                if (!wedge_bc(g, sx, sy, sz, svx, svy, svz)) { has_sec = false; lc.sa += 1u << 16; }
+            }
+            if constexpr ((F & (kStageSec | kStageIon)) != 0) {
+                // rank of each staged entry inside its segment: entries of earlier
+                // passes (carried) plus the earlier lanes of this pass in the segment
+                const int gprev = __shfl_up(gs, 1, 64);
+                const int gnext = __shfl_down(gs, 1, 64);
+                const unsigned long long heads = __ballot(act && (lane == 0 || gs != gprev));
+                const int head = 63 - __clzll(heads & (below | (1ull << lane)));
+                const unsigned long long span = below & ~((head == 0) ? 0ull : ((~0ull) >> (64 - head)));
+                int rs = __popcll(__ballot(has_sec) & span), ri = __popcll(__ballot(has_ion) & span);
+                if (gs == c_seg) { rs += c_sec; ri += c_ion; }
+                if (has_sec) {
+                    const int d = gs * kSeg + rs;
+                    qpa.sec[0][d] = sx; qpa.sec[1][d] = sy; qpa.sec[2][d] = sz;
+                    qpa.sec[3][d] = svx; qpa.sec[4][d] = svy; qpa.sec[5][d] = svz;
+                }
+                if (has_ion) {   // ion at the ionization site (electron_collision.m:63)
+                    const int d = gs * kSeg + ri;
+                    qpa.ion[0][d] = px; qpa.ion[1][d] = py; qpa.ion[2][d] = pz;
+                    qpa.ion[3][d] = co.ivx; qpa.ion[4][d] = co.ivy; qpa.ion[5][d] = co.ivz;
+                }
+                const int rs1 = rs + has_sec, ri1 = ri + has_ion;
+                if (act && (lane == cnt - 1 || gnext != gs)) {
+                    if (F & kStageSec) qpa.sec_cnt[gs] = rs1;
+                    if (F & kStageIon) qpa.ion_cnt[gs] = ri1;
+                }
+                c_seg = __shfl(gs, cnt - 1, 64);
+                c_sec = __shfl(rs1, cnt - 1, 64);
+                c_ion = __shfl(ri1, cnt - 1, 64);
+            }
         }
-        c_seg = __shfl(gs, m - 1, 64);
-        c_sec = __shfl(rs1, m - 1, 64);
-        c_ion = __shfl(ri1, m - 1, 64);
-        const int typ = ev & kEvMask;
-        lc.ei += (unsigned)(typ == 1) | (unsigned)(typ == 2) << 16;
-        lc.xv += (unsigned)(typ == 3) | (unsigned)((ev & kEvViol) != 0) << 16;
-        lc.sa += (unsigned)((ev & kEvSat) != 0);
-        q_head += m;
-        if (q_head >= kQCap) q_head -= kQCap;
+        wave_sync();   // the ring entries and the lists are free
+        q_head = (q_head + m) & (kQCap - 1);
         q_n -= m;
     };
     auto drain = [&]() {
-        while (q_n > 0) round(q_n < kWaveSize ? q_n : kWaveSize);
+        while (q_n > 0) round(q_n < kQCap ? q_n : kQCap);
     };
 
     const int n_waves = gridDim.x * kWaves;
@@ -625,7 +706,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 for (int p = 0; p < kPer; ++p) mine += cand[p];
                 int total;
                 const int off = wave_prefix_small<kPerBits>(mine, total);
-                if (lane == 0 && total == 0) {   // otherwise the round of its last candidate writes them
+                if (lane == 0) {   // a round's ionization pass overwrites them with the segment's counts
                     if (F & kStageSec) pa.sec_cnt[gseg] = 0;
                     if (F & kStageIon) pa.ion_cnt[gseg] = 0;
                 }
@@ -643,8 +724,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                                 Q[0][qi] = x[p]; Q[1][qi] = y[p]; Q[2][qi] = z[p];
                                 Q[3][qi] = vx[p]; Q[4][qi] = vy[p]; Q[5][qi] = vz[p];
                                 Q[6][qi] = u1[p];
-                                // slot, and in the sign bit: the segment's last candidate
-                                Q[10][qi] = __int_as_float((slot0 + p) | (pos == total - 1 ? (int)0x80000000 : 0));
+                                Q[8][qi] = __uint_as_float((uint32_t)(slot0 + p));
                             }
                             ++pos;
                         }
@@ -652,7 +732,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                     q_n += take;
                     done += take;
                     wave_sync();
-                    while (q_n >= kWaveSize) round(kWaveSize);
+                    if (q_n == kQCap) round(kQCap);
                 }
             }
         }
