@@ -178,6 +178,17 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
                                         unsigned long long* s_tz, unsigned long long* qr,
                                         unsigned long long* qz)
 {
+    // the usual case: the whole stencil inside the box, eight tile adds without branches
+    const int ti0 = d.i0 - b.ilo, tk0 = d.k0 - b.klo;
+    if ((unsigned)ti0 < (unsigned)b.wi && (unsigned)(ti0 + d.di) < (unsigned)b.wi && (unsigned)tk0 < (unsigned)b.nk &&
+        (unsigned)(tk0 + d.dk) < (unsigned)b.nk) {
+        const int t0 = ti0 + tk0 * b.wi, t1 = t0 + d.di, t2 = t0 + d.dk * b.wi, t3 = t2 + d.di;
+        atomicAdd(&s_tr[t0], (unsigned long long)d.q[0]); atomicAdd(&s_tz[t0], (unsigned long long)d.q[4]);
+        atomicAdd(&s_tr[t1], (unsigned long long)d.q[1]); atomicAdd(&s_tz[t1], (unsigned long long)d.q[5]);
+        atomicAdd(&s_tr[t2], (unsigned long long)d.q[2]); atomicAdd(&s_tz[t2], (unsigned long long)d.q[6]);
+        atomicAdd(&s_tr[t3], (unsigned long long)d.q[3]); atomicAdd(&s_tz[t3], (unsigned long long)d.q[7]);
+        return;
+    }
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const int i = d.i0 + ((a & 1) ? d.di : 0), k = d.k0 + ((a & 2) ? d.dk : 0);
