@@ -255,53 +255,72 @@ template <bool kMove>
 __global__ __launch_bounds__(256) void k_dsmc_move(Mol mo, DMap m, double dt, MoveOut o)
 {
 #pragma clang fp contract(off)
+    constexpr int kU = 4;                  // molecules per lane per pass: their loads are issued together
     __shared__ int s_q[4][128];
     const int n = *mo.count;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int* q = s_q[wv];
     int qn = 0;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long base = (long long)blockIdx.x * blockDim.x + wv * 64; base < n; base += stride) {
-        const int i = (int)base + lane;
-        bool rare = false;
-        if (i < n) {
-            float fx = mo.x[i], fy = mo.y[i], fz = mo.z[i];
-            const float fvx = mo.vx[i], fvy = mo.vy[i], fvz = mo.vz[i];
-            if (kMove && fz == fz) {
-                const double x = (double)fx + (double)fvx * dt;
-                const double y = (double)fy + (double)fvy * dt;
-                const double z = (double)fz + (double)fvz * dt;
-                const bool clear = m.fast && x * x + y * y < m.rp2_lo && y > 0.0 &&
-                                   y * m.cp - x * m.sp < -1e-9 * (fabs(x) + fabs(y)) && z > 0.0 && z < m.lp;
-                if (clear) {
-                    fx = (float)x; fy = (float)y; fz = (float)z;
-                } else {
-                    rare = true;
-                }
-            }
-            if (!rare) {
-                if (kMove || o.rec) put_molecule(mo, o, i, fx, fy, fz, fvx, fvy, fvz, false);
-                o.keys[i] = (uint32_t)dsmc_cell_fast(m, fx, fy, fz);
-                o.vals[i] = (uint32_t)i;
+    const long long stride = (long long)gridDim.x * blockDim.x * kU;
+    for (long long base = ((long long)blockIdx.x * blockDim.x + wv * 64) * kU; base < n; base += stride) {
+        float fx[kU], fy[kU], fz[kU], fvx[kU], fvy[kU], fvz[kU], fe[kU];
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+            const long long i = base + j * 64 + lane;
+            fx[j] = fy[j] = fz[j] = fvx[j] = fvy[j] = fvz[j] = fe[j] = 0.0f;
+            if (i < n) {
+                fx[j] = mo.x[i]; fy[j] = mo.y[i]; fz[j] = mo.z[i];
+                fvx[j] = mo.vx[i]; fvy[j] = mo.vy[i]; fvz[j] = mo.vz[i];
+                if (o.rec) fe[j] = mo.ei[i];
             }
         }
-        if (kMove) {
-            const unsigned long long mask = __ballot(rare);
-            if (mask) {
-                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-                if (rare) q[qn + rank] = i;
-                qn += __popcll(mask);
-                if (qn >= 64) {
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    int b = 0;
-                    if (lane == 0) b = atomicAdd(o.rq_n, 64);
-                    b = __shfl(b, 0, 64);
-                    o.rq[b + lane] = q[lane];
-                    const int rest = qn - 64;
-                    const int carry = lane < rest ? q[lane + 64] : 0;
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    if (lane < rest) q[lane] = carry;
-                    qn = rest;
+#pragma unroll
+        for (int j = 0; j < kU; ++j) {
+            const int i = (int)(base + j * 64 + lane);
+            bool rare = false;
+            if (base + j * 64 + lane < n) {
+                float x1 = fx[j], y1 = fy[j], z1 = fz[j];
+                if (kMove && z1 == z1) {
+                    const double x = (double)x1 + (double)fvx[j] * dt;
+                    const double y = (double)y1 + (double)fvy[j] * dt;
+                    const double z = (double)z1 + (double)fvz[j] * dt;
+                    const bool clear = m.fast && x * x + y * y < m.rp2_lo && y > 0.0 &&
+                                       y * m.cp - x * m.sp < -1e-9 * (fabs(x) + fabs(y)) && z > 0.0 && z < m.lp;
+                    if (clear) {
+                        x1 = (float)x; y1 = (float)y; z1 = (float)z;
+                    } else {
+                        rare = true;
+                    }
+                }
+                if (!rare) {
+                    if (o.rec) {
+                        o.rec[2 * (size_t)i] = make_float4(x1, y1, z1, fvx[j]);
+                        o.rec[2 * (size_t)i + 1] = make_float4(fvy[j], fvz[j], fe[j], 0.0f);
+                    } else if (kMove) {
+                        mo.x[i] = x1; mo.y[i] = y1; mo.z[i] = z1;
+                    }
+                    o.keys[i] = (uint32_t)dsmc_cell_fast(m, x1, y1, z1);
+                    o.vals[i] = (uint32_t)i;
+                }
+            }
+            if (kMove) {
+                const unsigned long long mask = __ballot(rare);
+                if (mask) {
+                    const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+                    if (rare) q[qn + rank] = i;
+                    qn += __popcll(mask);
+                    if (qn >= 64) {
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        int b = 0;
+                        if (lane == 0) b = atomicAdd(o.rq_n, 64);
+                        b = __shfl(b, 0, 64);
+                        o.rq[b + lane] = q[lane];
+                        const int rest = qn - 64;
+                        const int carry = lane < rest ? q[lane + 64] : 0;
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                        if (lane < rest) q[lane] = carry;
+                        qn = rest;
+                    }
                 }
             }
         }
