@@ -152,3 +152,46 @@ def test_transport_driver_argument_and_table_errors(tmp_path):
                        timeout=60)
     assert r.returncode == 1 and "create: invalid argument" in r.stderr
     assert not (tmp_path / "o.bin").exists()
+
+
+STRUCTS = {   # ctypes mirror -> C typedef of include/gasx.h
+    "Params": "gasx_params", "RawTables": "gasx_raw_tables", "Species": "gasx_species", "Fields": "gasx_fields",
+    "Gas": "gasx_gas", "Counters": "gasx_counters", "Timing": "gasx_timing", "Batch": "gasx_batch",
+    "BatchTuning": "gasx_batch_tuning", "DiffSpec": "gasx_diff_spec", "RunOpts": "gasx_run_opts",
+    "PhotoSource": "gasx_photo_source", "PhotoResult": "gasx_photo_result", "EsParams": "gasx_es_params",
+    "EsResult": "gasx_es_result", "CellMap": "gasx_cell_map", "Molecules": "gasx_molecules",
+    "DsmcParams": "gasx_dsmc_params", "DsmcCounters": "gasx_dsmc_counters",
+}
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every ctypes structure has the size, field names and field offsets of
+    its C struct (compiled against include/gasx.h with the host compiler)."""
+    import shutil
+    import subprocess
+    from gasx import _abi
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "gasx.h"', "int main(void) {"]
+    for py, c in STRUCTS.items():
+        cls = getattr(_abi, py)
+        lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
+        for name, _ in cls._fields_:
+            lines.append(f'printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", f"-I{HEADER.parent}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            py, key, val = line.split()
+            got[(py, key)] = int(val)
+    for py in STRUCTS:
+        cls = getattr(_abi, py)
+        assert got[(py, "size")] == C.sizeof(cls), py
+        for name, _ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, (py, name)
