@@ -1,6 +1,7 @@
 /*
  * gasx_mex.c -- MATLAB MEX gateway: the reference's four GPU-touching
- * functions on top of gasx_shim.c, with their argument conventions.  Build
+ * functions and main2.m's molecule step on top of gasx_shim.c, with their
+ * argument conventions.  Build
  * where MATLAB is installed (its mex.h is needed; this repository's image
  * has no MATLAB):
  *
@@ -16,6 +17,10 @@
  *       gasx_mex('em_step', h, ep, ev, i1p, i1v, i2p, i2v, Er, Ez, Hphi, cycle_time);        % electricmagneticfield6
  *   [s_el, s_ion, s_exc] = gasx_mex('find_cs', h, E_keV);                                    % find_CS0
  *   [ep, ev, i1p, i1v] = gasx_mex('collide', h, ep, ev, i1p, i1v, dens, temp, dt, seed, step); % electron_collision
+ *   gasx_mex('dsmc_load', h, [n_rings n_layers Rp Lp PHIp delta], position, velocity, ei, capacity); % main2.m:40-106
+ *   gasx_mex('dsmc_step', h, dt, FN, seed, step, reorder);                                  % main2.m:250-264
+ *   [rho_cell, T_cell, rho_rz, T_rz] = gasx_mex('dsmc_sample', h, FN, to_gas);               % particle_density0
+ *   [position, velocity, ei] = gasx_mex('dsmc_get', h);
  *   gasx_mex('destroy', h);
  *
  * Particles are N x 3 doubles (Cartesian positions in the wedge, velocities),
@@ -25,6 +30,7 @@
  */
 #include "mex.h"
 
+#include <math.h>
 #include <stdint.h>
 #include <string.h>
 
@@ -156,6 +162,67 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[])
         check(s, sh);
         get_species(sh, GASX_SHIM_ELECTRONS, &plhs[0], &plhs[1]);
         get_species(sh, GASX_SHIM_ION1, &plhs[2], &plhs[3]);
+    } else if (!strcmp(cmd, "dsmc_load")) {
+        /* main2.m:40-106: molecules (N x 3 positions and velocities, N x 1
+         * internal energies) on the cell map of cell_structure.m
+         * (map = [n_rings n_layers Rp Lp PHIp delta]), room for capacity */
+        if (nrhs != 7) mexErrMsgIdAndTxt("gasx:args", "dsmc_load: 6 arguments");
+        gasx_shim* sh = handle_of(prhs[1]);
+        if (mxGetNumberOfElements(prhs[2]) != 6) mexErrMsgIdAndTxt("gasx:shape", "map = [n_rings n_layers Rp Lp PHIp delta]");
+        const double* mp = mxGetPr(prhs[2]);
+        gasx_cell_map map;
+        map.n_rings = (int32_t)mp[0]; map.n_layers = (int32_t)mp[1];
+        map.rp = mp[2]; map.lp = mp[3]; map.phip = mp[4]; map.delta = mp[5];
+        const int64_t n = (int64_t)mxGetM(prhs[3]);
+        if (mxGetN(prhs[3]) != 3 || mxGetN(prhs[4]) != 3 || (int64_t)mxGetM(prhs[4]) != n ||
+            (int64_t)mxGetNumberOfElements(prhs[5]) != n)
+            mexErrMsgIdAndTxt("gasx:shape", "molecules: N x 3 positions, N x 3 velocities, N internal energies");
+        const double *x = mxGetPr(prhs[3]), *v = mxGetPr(prhs[4]);
+        check(gasx_shim_dsmc_load(sh, &map, x, x + n, x + 2 * n, v, v + n, v + 2 * n, mxGetPr(prhs[5]), n,
+                                  (int64_t)scalar(prhs[6])),
+              sh);
+    } else if (!strcmp(cmd, "dsmc_step")) {
+        /* main2.m:250-264 for the resident molecules:
+         * gasx_mex('dsmc_step', h, dt, FN, seed, step, reorder) with N2's
+         * constants and SCmax of main2.m:5-17,35 */
+        if (nrhs != 7) mexErrMsgIdAndTxt("gasx:args", "dsmc_step: 6 arguments");
+        gasx_shim* sh = handle_of(prhs[1]);
+        const double kb = 1.38064853e-23, m = 4.65e-26;
+        gasx_dsmc_params dp;
+        memset(&dp, 0, sizeof dp);
+        dp.dt = scalar(prhs[2]); dp.fn = scalar(prhs[3]); dp.mass = m; dp.diameter = 3.7e-10;
+        dp.sc_max = 15.0 * (3.14159265358979323846 * 4.17e-10 * 4.17e-10) * sqrt(kb * 300.0 / m);
+        dp.omega = 0.74; dp.zeta = 2.0; dp.alpha = 1.0; dp.min_attempts = 1; dp.reorder = (int32_t)scalar(prhs[6]);
+        check(gasx_shim_dsmc_step(sh, &dp, (uint64_t)scalar(prhs[4]), (uint32_t)scalar(prhs[5])), sh);
+    } else if (!strcmp(cmd, "dsmc_sample")) {
+        /* [rho_cell, T_cell, rho_rz, T_rz] = gasx_mex('dsmc_sample', h, FN, to_gas):
+         * particle_density0.m / count_T_numbensity.m; with to_gas the (ring,
+         * layer) values become the electron step's gas */
+        if (nrhs != 4 || nlhs != 4) mexErrMsgIdAndTxt("gasx:args", "dsmc_sample: 3 arguments, 4 outputs");
+        gasx_shim* sh = handle_of(prhs[1]);
+        int64_t ncell = 0, nrz = 0;
+        check(gasx_shim_dsmc_sizes(sh, &ncell, &nrz), sh);
+        plhs[0] = mxCreateDoubleMatrix((mwSize)ncell, 1, mxREAL);
+        plhs[1] = mxCreateDoubleMatrix((mwSize)ncell, 1, mxREAL);
+        plhs[2] = mxCreateDoubleMatrix((mwSize)nrz, 1, mxREAL);
+        plhs[3] = mxCreateDoubleMatrix((mwSize)nrz, 1, mxREAL);
+        check(gasx_shim_dsmc_sample(sh, scalar(prhs[2]), 4.65e-26, mxGetPr(plhs[0]), mxGetPr(plhs[1]),
+                                    mxGetPr(plhs[2]), mxGetPr(plhs[3]), (int)scalar(prhs[3])),
+              sh);
+    } else if (!strcmp(cmd, "dsmc_get")) {
+        /* [position, velocity, ei] = gasx_mex('dsmc_get', h) */
+        if (nrhs != 2 || nlhs != 3) mexErrMsgIdAndTxt("gasx:args", "dsmc_get: 1 argument, 3 outputs");
+        gasx_shim* sh = handle_of(prhs[1]);
+        const double *x, *y, *z, *vx, *vy, *vz, *ei;
+        int64_t n;
+        check(gasx_shim_dsmc_get(sh, &x, &y, &z, &vx, &vy, &vz, &ei, &n), sh);
+        plhs[0] = mxCreateDoubleMatrix((mwSize)n, 3, mxREAL);
+        plhs[1] = mxCreateDoubleMatrix((mwSize)n, 3, mxREAL);
+        plhs[2] = mxCreateDoubleMatrix((mwSize)n, 1, mxREAL);
+        double *p = mxGetPr(plhs[0]), *v = mxGetPr(plhs[1]);
+        memcpy(p, x, n * sizeof(double)); memcpy(p + n, y, n * sizeof(double)); memcpy(p + 2 * n, z, n * sizeof(double));
+        memcpy(v, vx, n * sizeof(double)); memcpy(v + n, vy, n * sizeof(double)); memcpy(v + 2 * n, vz, n * sizeof(double));
+        memcpy(mxGetPr(plhs[2]), ei, n * sizeof(double));
     } else if (!strcmp(cmd, "destroy")) {
         if (nrhs != 2) mexErrMsgIdAndTxt("gasx:args", "destroy: 1 argument");
         gasx_shim_destroy(handle_of(prhs[1]));

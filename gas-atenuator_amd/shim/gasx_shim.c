@@ -43,6 +43,16 @@ struct gasx_shim {
     float* h_cs;
     int64_t cs_cap;
     double* h_kev;
+    /* DSMC molecules of main2.m (x, y, z, vx, vy, vz, ei) and their cell map */
+    float* dm[7];
+    int32_t* dm_count;
+    int64_t dm_cap;
+    float* h_mol;                       /* host staging, 7 x dm_cap */
+    double* h_mol64;
+    gasx_cell_map map;
+    int64_t n_cells;
+    float* d_samp;                      /* sample outputs: 2 x n_cells + 2 x (rings x layers) */
+    float* h_samp;
 };
 
 /* ---- CSV (the reference's data files, any line endings, optional header) ---- */
@@ -229,6 +239,13 @@ void gasx_shim_destroy(gasx_shim* sh)
         free(sh->sp[k].h32);
         free(sh->sp[k].h64);
     }
+    for (int c = 0; c < 7; ++c)
+        if (sh->dm[c]) hipFree(sh->dm[c]);
+    if (sh->dm_count) hipFree(sh->dm_count);
+    if (sh->d_samp) hipFree(sh->d_samp);
+    free(sh->h_mol);
+    free(sh->h_mol64);
+    free(sh->h_samp);
     void* dev[] = {sh->d_er, sh->d_ez, sh->d_h, sh->d_dens, sh->d_temp, sh->d_cs};
     for (size_t i = 0; i < sizeof dev / sizeof dev[0]; ++i)
         if (dev[i]) hipFree(dev[i]);
@@ -466,5 +483,152 @@ gasx_status gasx_shim_electron_kev(gasx_shim* sh, const double** e_kev, int64_t*
     }
     *e_kev = sh->h_kev;
     *n = m;
+    return GASX_OK;
+}
+
+/* ---- DSMC neutral gas (main2.m) --------------------------------------------- */
+
+static gasx_molecules molecules_view(gasx_shim* sh)
+{
+    gasx_molecules m;
+    m.x = sh->dm[0]; m.y = sh->dm[1]; m.z = sh->dm[2];
+    m.vx = sh->dm[3]; m.vy = sh->dm[4]; m.vz = sh->dm[5]; m.ei = sh->dm[6];
+    m.count = sh->dm_count;
+    m.capacity = sh->dm_cap;
+    return m;
+}
+
+static void free_molecules(gasx_shim* sh)
+{
+    for (int c = 0; c < 7; ++c) {
+        if (sh->dm[c]) hipFree(sh->dm[c]);
+        sh->dm[c] = NULL;
+    }
+    if (sh->dm_count) hipFree(sh->dm_count);
+    if (sh->d_samp) hipFree(sh->d_samp);
+    sh->dm_count = NULL;
+    sh->d_samp = NULL;
+    free(sh->h_mol);
+    free(sh->h_mol64);
+    free(sh->h_samp);
+    sh->h_mol = NULL;
+    sh->h_mol64 = NULL;
+    sh->h_samp = NULL;
+    sh->dm_cap = 0;
+}
+
+gasx_status gasx_shim_dsmc_load(gasx_shim* sh, const gasx_cell_map* map, const double* x, const double* y,
+                                const double* z, const double* vx, const double* vy, const double* vz,
+                                const double* ei, int64_t n, int64_t capacity)
+{
+    if (!sh || !map || n < 0 || capacity < 1 || n > capacity || capacity > INT32_MAX ||
+        (n > 0 && (!x || !y || !z || !vx || !vy || !vz || !ei)))
+        return GASX_E_ARG;
+    int64_t nxy = 0;
+    gasx_status st = gasx_cell_map_info(map, NULL, &nxy);
+    if (st != GASX_OK) return st;
+    st = gasx_synchronize(sh->ctx);
+    if (st != GASX_OK) return st;
+    const int64_t ncell = nxy * map->n_layers, nrz = (int64_t)map->n_rings * map->n_layers;
+    if (capacity != sh->dm_cap || ncell != sh->n_cells || nrz != (int64_t)sh->map.n_rings * sh->map.n_layers) {
+        free_molecules(sh);
+        int ok = 1;
+        for (int c = 0; c < 7 && ok; ++c) ok = hipMalloc((void**)&sh->dm[c], (size_t)capacity * sizeof(float)) == hipSuccess;
+        ok = ok && hipMalloc((void**)&sh->dm_count, sizeof(int32_t)) == hipSuccess;
+        ok = ok && hipMalloc((void**)&sh->d_samp, (size_t)(2 * ncell + 2 * nrz) * sizeof(float)) == hipSuccess;
+        sh->h_mol = (float*)malloc((size_t)capacity * 7 * sizeof(float));
+        sh->h_mol64 = (double*)malloc((size_t)capacity * 7 * sizeof(double));
+        sh->h_samp = (float*)malloc((size_t)(2 * ncell + 2 * nrz) * sizeof(float));
+        sh->dm_cap = capacity;
+        if (!ok || !sh->h_mol || !sh->h_mol64 || !sh->h_samp) {
+            free_molecules(sh);
+            return GASX_E_NOMEM;
+        }
+    }
+    sh->map = *map;
+    sh->n_cells = ncell;
+    const double* src[7] = {x, y, z, vx, vy, vz, ei};
+    for (int c = 0; c < 7; ++c) {
+        float* h = sh->h_mol + (size_t)c * capacity;
+        for (int64_t i = 0; i < n; ++i) h[i] = (float)src[c][i];
+        if (n) HCHK(hipMemcpy(sh->dm[c], h, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+    }
+    const int32_t cnt = (int32_t)n;
+    HCHK(hipMemcpy(sh->dm_count, &cnt, sizeof cnt, hipMemcpyHostToDevice));
+    return GASX_OK;
+}
+
+gasx_status gasx_shim_dsmc_step(gasx_shim* sh, const gasx_dsmc_params* p, uint64_t seed, uint32_t step)
+{
+    if (!sh || !p || !sh->dm_count) return GASX_E_ARG;
+    gasx_molecules m = molecules_view(sh);
+    return gasx_dsmc_step(sh->ctx, &m, &sh->map, p, seed, step);
+}
+
+gasx_status gasx_shim_dsmc_sample(gasx_shim* sh, double fn, double mass, double* cell_density,
+                                  double* cell_temperature, double* rz_density, double* rz_temperature, int to_gas)
+{
+    if (!sh || !sh->dm_count || !(fn > 0) || !(mass > 0)) return GASX_E_ARG;
+    const int64_t nc = sh->n_cells, nrz = (int64_t)sh->map.n_rings * sh->map.n_layers;
+    if (to_gas && (sh->gnr != sh->map.n_rings || sh->gnz != sh->map.n_layers)) return GASX_E_ARG;
+    gasx_molecules m = molecules_view(sh);
+    float* d = sh->d_samp;
+    gasx_status st = gasx_dsmc_sample(sh->ctx, &m, &sh->map, fn, mass, d, d + nc, d + 2 * nc, d + 2 * nc + nrz);
+    if (st == GASX_OK) st = gasx_synchronize(sh->ctx);
+    if (st != GASX_OK) return st;
+    HCHK(hipMemcpy(sh->h_samp, d, (size_t)(2 * nc + 2 * nrz) * sizeof(float), hipMemcpyDeviceToHost));
+    const float* h = sh->h_samp;
+    for (int64_t i = 0; i < nc; ++i) {
+        if (cell_density) cell_density[i] = h[i];
+        if (cell_temperature) cell_temperature[i] = h[nc + i];
+    }
+    for (int64_t i = 0; i < nrz; ++i) {
+        if (rz_density) rz_density[i] = h[2 * nc + i];
+        if (rz_temperature) rz_temperature[i] = h[2 * nc + nrz + i];
+    }
+    if (to_gas) {   /* the (ring, layer) gas becomes the electron step's gas (particle_density0.m) */
+        double nmax = 0.0;
+        for (int64_t i = 0; i < nrz; ++i)
+            if ((double)h[2 * nc + i] > nmax) nmax = (double)h[2 * nc + i];
+        HCHK(hipMemcpy(sh->d_dens, d + 2 * nc, (size_t)nrz * sizeof(float), hipMemcpyDeviceToDevice));
+        HCHK(hipMemcpy(sh->d_temp, d + 2 * nc + nrz, (size_t)nrz * sizeof(float), hipMemcpyDeviceToDevice));
+        sh->n_max = nmax;
+    }
+    return GASX_OK;
+}
+
+gasx_status gasx_shim_dsmc_get(gasx_shim* sh, const double** x, const double** y, const double** z,
+                               const double** vx, const double** vy, const double** vz, const double** ei,
+                               int64_t* n)
+{
+    if (!sh || !sh->dm_count || !n) return GASX_E_ARG;
+    gasx_status st = gasx_synchronize(sh->ctx);
+    if (st != GASX_OK) return st;
+    int32_t cnt = 0;
+    HCHK(hipMemcpy(&cnt, sh->dm_count, sizeof cnt, hipMemcpyDeviceToHost));
+    const size_t cp = (size_t)sh->dm_cap;
+    for (int c = 0; c < 7; ++c) {
+        if (cnt) HCHK(hipMemcpy(sh->h_mol + c * cp, sh->dm[c], (size_t)cnt * sizeof(float), hipMemcpyDeviceToHost));
+        for (int32_t i = 0; i < cnt; ++i) sh->h_mol64[c * cp + i] = (double)sh->h_mol[c * cp + i];
+    }
+    const double* h = sh->h_mol64;
+    const double** out[7] = {x, y, z, vx, vy, vz, ei};
+    for (int c = 0; c < 7; ++c)
+        if (out[c]) *out[c] = h + c * cp;
+    *n = cnt;
+    return GASX_OK;
+}
+
+gasx_status gasx_shim_dsmc_counters(gasx_shim* sh, gasx_dsmc_counters* out, int reset)
+{
+    if (!sh || !out) return GASX_E_ARG;
+    return gasx_dsmc_get_counters(sh->ctx, out, reset);
+}
+
+gasx_status gasx_shim_dsmc_sizes(gasx_shim* sh, int64_t* n_cells, int64_t* n_rz)
+{
+    if (!sh || !sh->dm_count || !n_cells || !n_rz) return GASX_E_ARG;
+    *n_cells = sh->n_cells;
+    *n_rz = (int64_t)sh->map.n_rings * sh->map.n_layers;
     return GASX_OK;
 }

@@ -15,7 +15,8 @@
  * electricmagneticfield6.m:65-69) and the fields resident on the device
  * between calls; the set / get calls move them to and from host arrays.
  * gasx_mex.c maps MATLAB's mxArrays onto these calls; transport_driver.c
- * replays main3.m / electron_transport_field3.m with them from C.
+ * replays main3.m / electron_transport_field3.m with them from C, and
+ * dsmc_driver.c main2.m's molecule loop.
  *
  * Plain C, host only; every call returns a gasx_status.
  */
@@ -77,6 +78,30 @@ gasx_status gasx_shim_collide(gasx_shim* sh, double dt, uint64_t seed, uint32_t 
 /* Kinetic energies (keV) of the live electrons in slot order, as
  * electricmagneticfield6.m:172 returns them (the push's constants). */
 gasx_status gasx_shim_electron_kev(gasx_shim* sh, const double** e_kev, int64_t* n);
+
+/* ---- DSMC neutral gas: main2.m's molecule loop (main2.m:247-271) ----------
+ * main2.m:59-106's molecules (positions, velocities, internal energy, host
+ * doubles; stored as FP32) and cell_structure.m's map onto the device, with
+ * room for `capacity` molecules. */
+gasx_status gasx_shim_dsmc_load(gasx_shim* sh, const gasx_cell_map* map, const double* x, const double* y,
+                                const double* z, const double* vx, const double* vy, const double* vz,
+                                const double* ei, int64_t n, int64_t capacity);
+/* main2.m:250-264: free flight, specular walls, cells and collision_HS_internal.m. */
+gasx_status gasx_shim_dsmc_step(gasx_shim* sh, const gasx_dsmc_params* p, uint64_t seed, uint32_t step);
+/* particle_density0.m / count_T_numbensity.m: density (m^-3) and temperature
+ * (K) per cell of the map and per (ring, layer), n_layers x n_rings with the
+ * ring fastest; any output may be NULL.  With to_gas the (ring, layer) values
+ * also become the electron step's gas (the gas grid given to create must then
+ * be n_rings x n_layers). */
+gasx_status gasx_shim_dsmc_sample(gasx_shim* sh, double fn, double mass, double* cell_density,
+                                  double* cell_temperature, double* rz_density, double* rz_temperature, int to_gas);
+/* The molecules back to host in slot order (valid until the next call). */
+gasx_status gasx_shim_dsmc_get(gasx_shim* sh, const double** x, const double** y, const double** z,
+                               const double** vx, const double** vy, const double** vz, const double** ei,
+                               int64_t* n);
+gasx_status gasx_shim_dsmc_counters(gasx_shim* sh, gasx_dsmc_counters* out, int reset);
+/* Cells of the loaded map and its (ring, layer) cells: the sample's output sizes. */
+gasx_status gasx_shim_dsmc_sizes(gasx_shim* sh, int64_t* n_cells, int64_t* n_rz);
 
 #ifdef __cplusplus
 }

@@ -170,3 +170,64 @@ def test_driver_replays_the_transport_loop(gpu, otab, tmp_path):
         err = np.abs(g[name] - ref[name]).max() / scale
         print(f"{name}: max err / max |ref| {err:.2e}")
         assert err < 5e-4
+
+
+DSMC_DRIVER = REPO / "gas-atenuator_amd" / "shim" / "dsmc_driver"
+
+
+def test_dsmc_driver_matches_the_library_step(tmp_path):
+    """main2.m's molecule loop from C through the shim (shim/dsmc_driver.c):
+    load, DSMC steps with the cell reorder, sample into the electron step's gas.
+    The same steps through the Python binding give the same molecules bit for
+    bit, and the gas the shim hands to the electron step is the sample's
+    (ring, layer) grid."""
+    import math
+    import torch
+    import gasx
+    from gasx.dsmc import CellMap, MoleculeSet, dsmc_params, seed_molecules, N2_MASS
+    log("dsmc driver")
+    kb = 1.38064853e-23
+    n_gas = 10.0 / kb / 300.0                                     # main2.m:12,18-19
+    lam = 1 / (math.sqrt(2) * math.pi * n_gas * 3.7e-10 ** 2)
+    cm = CellMap(int(1e-2 / lam) + 1, int(0.004 / lam) + 1, 1e-2, 0.004, 0.5, lam)
+    n = 150_000
+    st = seed_molecules(n, cm, seed=4)
+    inp = tmp_path / "mol.bin"
+    with open(inp, "wb") as f:
+        f.write(np.int64(n).tobytes())
+        f.write(np.ascontiguousarray(st.astype(np.float64)).tobytes())
+    fn, dt, steps, seed = 5e8, 2e-7, 3, 9
+    out = tmp_path / "out.bin"
+    r = subprocess.run([str(DSMC_DRIVER), str(DATA), str(inp), str(out), str(cm.n_rings), str(cm.n_layers),
+                        repr(cm.rp), repr(cm.lp), repr(cm.phip), repr(cm.delta), repr(dt), repr(fn), str(steps),
+                        str(seed), "1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    b = out.read_bytes()
+    assert b[:8] == b"GASXDSM1"
+    nm = int(np.frombuffer(b, np.int64, 1, 8)[0])
+    mol = np.frombuffer(b, np.float64, 7 * nm, 16).reshape(7, nm)
+    off = 16 + 56 * nm
+    ctr = np.frombuffer(b, np.int64, 7, off)
+    nrz = cm.n_rings * cm.n_layers
+    rz = np.frombuffer(b, np.float64, 4 * nrz, off + 56).reshape(4, cm.n_layers, cm.n_rings)
+    assert off + 56 + 32 * nrz == len(b)
+    # the same steps through the binding
+    ctx = gasx.Context(gasx.make_params(16, 16), device=0)
+    m = MoleculeSet(n).load(st)
+    p = dsmc_params(dt, fn, reorder=True)
+    ctx.dsmc.counters(reset=True)
+    for k in range(steps):
+        ctx.dsmc.step(m, cm, p, seed=seed, step=k)
+    torch.cuda.synchronize()
+    c = ctx.dsmc.counters(reset=True)
+    assert nm == m.n == n
+    assert np.array_equal(mol, m.state().astype(np.float64))
+    assert list(ctr) == [c.n_attempts, c.n_collisions, c.n_sc_exceed, c.n_boundary, c.n_wall_r, c.n_wall_z,
+                         c.n_wrap]
+    assert c.n_collisions > 1000
+    s = ctx.dsmc.sample(m, cm, fn, N2_MASS)
+    assert np.array_equal(rz[0], s["rz_density"].cpu().numpy().astype(np.float64))
+    assert np.array_equal(rz[1], s["rz_temperature"].cpu().numpy().astype(np.float64))
+    # the electron step's gas is the sample's (ring, layer) grid
+    assert np.array_equal(rz[2], rz[0]) and np.array_equal(rz[3], rz[1])
+    assert rz[0].mean() == pytest.approx(n * fn / (0.5 * cm.rp ** 2 * cm.phip * cm.lp), rel=0.02)
