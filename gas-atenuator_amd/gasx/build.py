@@ -67,14 +67,17 @@ def build_library(jobs: int = 4) -> Path:
 SHIM = ROOT / "shim"
 DRIVER_PATH = SHIM / "transport_driver"
 DSMC_DRIVER_PATH = SHIM / "dsmc_driver"
+ES_DRIVER_PATH = SHIM / "es_driver"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
 
 def build_shim() -> Path:
-    """The C drivers of the ABI (shim/transport_driver.c and shim/dsmc_driver.c
-    over shim/gasx_shim.c), plain C against libgasx.so and the HIP runtime.
+    """The C drivers of the ABI (shim/transport_driver.c, shim/dsmc_driver.c and
+    shim/es_driver.c over shim/gasx_shim.c), plain C against libgasx.so and the
+    HIP runtime.
     The MEX gateway (shim/gasx_mex.c) needs MATLAB's mex and is not built here."""
-    for main, out in (("transport_driver.c", DRIVER_PATH), ("dsmc_driver.c", DSMC_DRIVER_PATH)):
+    for main, out in (("transport_driver.c", DRIVER_PATH), ("dsmc_driver.c", DSMC_DRIVER_PATH),
+                      ("es_driver.c", ES_DRIVER_PATH)):
         _run(["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{ROCM / 'include'}",
               str(SHIM / "gasx_shim.c"), str(SHIM / main),
               f"-L{PKG_DIR}", "-lgasx", f"-L{ROCM / 'lib'}", "-lamdhip64", "-lm",

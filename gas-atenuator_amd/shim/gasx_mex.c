@@ -17,6 +17,8 @@
  *       gasx_mex('em_step', h, ep, ev, i1p, i1v, i2p, i2v, Er, Ez, Hphi, cycle_time);        % electricmagneticfield6
  *   [s_el, s_ion, s_exc] = gasx_mex('find_cs', h, E_keV);                                    % find_CS0
  *   [ep, ev, i1p, i1v] = gasx_mex('collide', h, ep, ev, i1p, i1v, dens, temp, dt, seed, step); % electron_collision
+ *   [i1p, i1v, i2p, i2v, RHO, rho_e, PHI, T_e, E_r] = gasx_mex('es_push', h, i1p, i1v, i2p, i2v, ...
+ *       delta, delta_t, Rp, Lp, phimax, FN, T_e);                                           % ion_staticelectrofield_push
  *   gasx_mex('dsmc_load', h, [n_rings n_layers Rp Lp PHIp delta], position, velocity, ei, capacity); % main2.m:40-106
  *   gasx_mex('dsmc_step', h, dt, FN, seed, step, reorder);                                  % main2.m:250-264
  *   [rho_cell, T_cell, rho_rz, T_rz] = gasx_mex('dsmc_sample', h, FN, to_gas);               % particle_density0
@@ -162,6 +164,31 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[])
         check(s, sh);
         get_species(sh, GASX_SHIM_ELECTRONS, &plhs[0], &plhs[1]);
         get_species(sh, GASX_SHIM_ION1, &plhs[2], &plhs[3]);
+    } else if (!strcmp(cmd, "es_push")) {
+        /* [i1p, i1v, i2p, i2v, RHO, rho_e, PHI, T_e, gridE_r] = gasx_mex('es_push', h, i1p, i1v, i2p, i2v,
+         *     delta, delta_t, Rp, Lp, phimax, FN, T_e): ion_staticelectrofield_push.m's signature (its
+         *     PHI and RHO inputs only feed an energy printout there) */
+        if (nrhs != 13 || nlhs != 9) mexErrMsgIdAndTxt("gasx:args", "es_push: 12 arguments, 9 outputs");
+        gasx_shim* sh = handle_of(prhs[1]);
+        put_species(sh, GASX_SHIM_ION1, prhs[2], prhs[3]);
+        put_species(sh, GASX_SHIM_ION2, prhs[4], prhs[5]);
+        const double delta = scalar(prhs[6]);
+        const int32_t nr = (int32_t)(scalar(prhs[8]) / delta + 0.5), nz = (int32_t)(scalar(prhs[9]) / delta + 0.5);
+        const double t_e = scalar(prhs[12]);
+        plhs[4] = mxCreateDoubleMatrix((mwSize)nr, (mwSize)nz, mxREAL);
+        plhs[5] = mxCreateDoubleMatrix((mwSize)nr, (mwSize)nz, mxREAL);
+        plhs[6] = mxCreateDoubleMatrix((mwSize)nr, (mwSize)nz, mxREAL);
+        plhs[8] = mxCreateDoubleMatrix((mwSize)nr, (mwSize)nz, mxREAL);
+        double* rho = mxGetPr(plhs[4]);
+        gasx_es_result r;
+        check(gasx_shim_es_push(sh, nr, nz, delta, scalar(prhs[7]), scalar(prhs[11]), t_e, mxGetPr(plhs[6]), rho,
+                                mxGetPr(plhs[5]), mxGetPr(plhs[8]), NULL, &r),
+              sh);
+        const double* re = mxGetPr(plhs[5]);
+        for (size_t i = 0; i < (size_t)nr * nz; ++i) rho[i] -= re[i];   /* RHO = rho_i - rho_e (Poisson.m:43) */
+        get_species(sh, GASX_SHIM_ION1, &plhs[0], &plhs[1]);
+        get_species(sh, GASX_SHIM_ION2, &plhs[2], &plhs[3]);
+        plhs[7] = mxCreateDoubleScalar(t_e);
     } else if (!strcmp(cmd, "dsmc_load")) {
         /* main2.m:40-106: molecules (N x 3 positions and velocities, N x 1
          * internal energies) on the cell map of cell_structure.m

@@ -632,3 +632,30 @@ gasx_status gasx_shim_dsmc_sizes(gasx_shim* sh, int64_t* n_cells, int64_t* n_rz)
     *n_rz = (int64_t)sh->map.n_rings * sh->map.n_layers;
     return GASX_OK;
 }
+
+/* ---- electrostatic ion step (ion_staticelectrofield_push.m) ----------------- */
+
+gasx_status gasx_shim_es_push(gasx_shim* sh, int32_t nr, int32_t nz, double delta, double dt, double fn, double t_e,
+                              double* phi, double* rho_i, double* rho_e, double* e_r, double* e_z,
+                              gasx_es_result* res)
+{
+    if (!sh || !res || nr < 3 || nz < 3 || !(delta > 0) || !(dt >= 0) || !(fn > 0) || !(t_e > 0)) return GASX_E_ARG;
+    const size_t ng = (size_t)nr * nz;
+    double* d = NULL;
+    HCHK(hipMalloc((void**)&d, 5 * ng * sizeof(double)));
+    gasx_species sp[2] = {species_view(sh, GASX_SHIM_ION1), species_view(sh, GASX_SHIM_ION2)};
+    gasx_es_params p;
+    memset(&p, 0, sizeof p);
+    p.nr = nr; p.nz = nz; p.delta = delta; p.phip = sh->p.phip; p.weight = fn; p.t_e = t_e; p.dt = dt;
+    p.n_e0 = 1e15;                 /* Poisson.m:33 */
+    p.tol = 0.01;                  /* Poisson.m:36 */
+    p.max_iter = 1000000;
+    gasx_status st = gasx_ion_es_push(sh->ctx, sp, 2, &p, d, d + ng, d + 2 * ng, d + 3 * ng, d + 4 * ng, res);
+    if (st == GASX_OK) st = gasx_synchronize(sh->ctx);
+    double* out[5] = {phi, rho_i, rho_e, e_r, e_z};
+    for (int k = 0; k < 5 && st == GASX_OK; ++k)
+        if (out[k] && hipMemcpy(out[k], d + k * ng, ng * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            st = GASX_E_HIP;
+    hipFree(d);
+    return st;
+}

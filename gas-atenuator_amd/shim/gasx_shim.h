@@ -79,6 +79,16 @@ gasx_status gasx_shim_collide(gasx_shim* sh, double dt, uint64_t seed, uint32_t 
  * electricmagneticfield6.m:172 returns them (the push's constants). */
 gasx_status gasx_shim_electron_kev(gasx_shim* sh, const double** e_kev, int64_t* n);
 
+/* ion_staticelectrofield_push.m:21-156 with Poisson.m:25-68 on the two ion
+ * species (N2+ q = +1, N2++ q = +2) of an nr x nz grid of cells delta (the
+ * wedge angle of create's parameters): the ions are pushed in place; PHI,
+ * rho_i, rho_e, E_r and E_z come back as nr x nz host doubles, r fastest
+ * (MATLAB's lenr x lenz layout; any may be NULL).  The reference's RHO is
+ * rho_i - rho_e. */
+gasx_status gasx_shim_es_push(gasx_shim* sh, int32_t nr, int32_t nz, double delta, double dt, double fn, double t_e,
+                              double* phi, double* rho_i, double* rho_e, double* e_r, double* e_z,
+                              gasx_es_result* res);
+
 /* ---- DSMC neutral gas: main2.m's molecule loop (main2.m:247-271) ----------
  * main2.m:59-106's molecules (positions, velocities, internal energy, host
  * doubles; stored as FP32) and cell_structure.m's map onto the device, with

@@ -231,3 +231,49 @@ def test_dsmc_driver_matches_the_library_step(tmp_path):
     # the electron step's gas is the sample's (ring, layer) grid
     assert np.array_equal(rz[2], rz[0]) and np.array_equal(rz[3], rz[1])
     assert rz[0].mean() == pytest.approx(n * fn / (0.5 * cm.rp ** 2 * cm.phip * cm.lp), rel=0.02)
+
+
+ES_DRIVER = REPO / "gas-atenuator_amd" / "shim" / "es_driver"
+
+
+def test_es_driver_matches_the_library_push(tmp_path):
+    """ion_staticelectrofield_push.m's call (electron_transport_field2.m:270)
+    from C through the shim (shim/es_driver.c): the pushed ions, PHI, rho_i,
+    rho_e and E equal the Python binding's for the same ions bit for bit."""
+    import gasx
+    from test_gpu_es import ions, M_ION
+    log("es driver")
+    nr, nz, d, w, te, dt = 30, 45, 1e-4, 1e4, 1e6, 2e-9
+    a = ions(4000, nr, nz, d, 3)
+    b = ions(1500, nr, nz, d, 4)
+    inp = tmp_path / "ions.bin"
+    with open(inp, "wb") as f:
+        for s in (a, b):
+            f.write(np.int64(s.shape[1]).tobytes())
+            f.write(np.ascontiguousarray(s.astype(np.float64)).tobytes())
+    out = tmp_path / "out.bin"
+    r = subprocess.run([str(ES_DRIVER), str(DATA), str(inp), str(out), str(nr), str(nz), repr(d), repr(dt),
+                        repr(w), repr(te), "0.5"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    buf = out.read_bytes()
+    assert buf[:8] == b"GASXESD1"
+    off = 8
+    got = []
+    for _ in range(2):
+        n = int(np.frombuffer(buf, np.int64, 1, off)[0])
+        got.append(np.frombuffer(buf, np.float64, 6 * n, off + 8).reshape(6, n))
+        off += 8 + 48 * n
+    it, conv = (int(v) for v in np.frombuffer(buf, np.int64, 2, off))
+    n_e0 = float(np.frombuffer(buf, np.float64, 1, off + 16)[0])
+    grids = np.frombuffer(buf, np.float64, 5 * nr * nz, off + 24).reshape(5, nz, nr)
+    assert off + 24 + 40 * nr * nz == len(buf)
+    ctx = gasx.Context(gasx.make_params(nr, nz, phip=0.5))
+    sa = gasx.ParticleSet(a.shape[1], qnum=1.0, mass=M_ION).load(*a)
+    sb = gasx.ParticleSet(b.shape[1], qnum=2.0, mass=M_ION).load(*b)
+    res = ctx.ion_es_push([sa, sb], nr, nz, d, w, te, dt)
+    assert (it, conv) == (res["iterations"], res["converged"]) and conv == 1
+    assert n_e0 == res["n_e0"]
+    for k, name in enumerate(("phi", "rho_i", "rho_e", "e_r", "e_z")):
+        assert np.array_equal(grids[k], res[name].cpu().numpy()), name
+    assert np.array_equal(got[0], sa.state(live_only=True).astype(np.float64))
+    assert np.array_equal(got[1], sb.state(live_only=True).astype(np.float64))
