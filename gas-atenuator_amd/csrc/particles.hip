@@ -15,11 +15,12 @@
 //   A  every lane: load, push, boundary conditions, the decision draw; slots
 //      with u1 < pmax (the exact rejection bound on p_T) inside the tables'
 //      energy window are candidates; the others make their second half drift
-//   B  the segment is stored; its candidates go into the wave's LDS ring and
-//      are processed 64 at a time, one per lane (cross sections, decision,
-//      elastic / ionization / excitation, second half drift), so the
-//      collision work is dense instead of divergent; secondaries and ions are
-//      staged
+//   B  the segment is stored; its candidates go into the wave's LDS ring; a
+//      full ring is one round: a decision pass (cross sections, the test
+//      against p_T, the branch) over kQEnt entries per lane, then one pass per
+//      collision type (elastic / ionization / excitation, second half drift)
+//      over that type's entries compacted onto the lanes, so the collision
+//      work is dense instead of divergent; secondaries and ions are staged
 //
 // k_drift_deposit<kDr, kDe, kBat> (k_deposit<kBat>: the deposit alone):
 //   kDrift    EM2.cu:23-57    half drift + BC (the prologue of a call)
@@ -216,10 +217,21 @@ static_assert(kPer % kGatherBatch == 0, "gather batch divides the slots per lane
 static_assert(kPer == 4, "one decision draw covers a lane's four slots");
 
 // LDS ring words per candidate: x y z vx vy vz, u1 (u4 after the decision),
-// u3 (after the decision), slot
-constexpr int kQW = 9;
-constexpr int kQCap = 128;  // candidates a wave's ring holds (a round takes them all; a power of two)
-static_assert((kQCap & (kQCap - 1)) == 0 && kQCap == 2 * kWaveSize, "a round decides two entries per lane");
+// with 9 words u3 (after the decision; with 8 the ionization pass draws it
+// again), slot
+#ifndef GASX_QW
+#define GASX_QW 8
+#endif
+#ifndef GASX_QCAP
+#define GASX_QCAP 256
+#endif
+constexpr int kQW = GASX_QW;
+constexpr int kQSlotRow = kQW - 1;
+constexpr int kQCap = GASX_QCAP;  // candidates a wave's ring holds (a round takes them all; a power of two)
+constexpr int kQEnt = kQCap / kWaveSize;   // entries a lane decides per round
+static_assert(kQW == 8 || kQW == 9, "ring rows");
+static_assert((kQCap & (kQCap - 1)) == 0 && kQCap % kWaveSize == 0 && kQCap <= 256,
+              "a power of two, whole entries per lane, byte list indices");
 
 // Event counts of a wave added to the block's slot and, for an ensemble, to
 // the run's slot (lane 0 adds; several waves share the block slot).
@@ -309,8 +321,8 @@ __device__ __forceinline__ void vstore4n(float* p, const float (&v)[N])
 // grid-stride over segments; no workgroup barrier inside the loop.
 //
 // Collision candidates (u1 < pmax, energy inside the tables' window) are
-// queued in the wave's LDS ring across segments and processed 64 at a time,
-// one per lane, so a round is full however few candidates a segment has.
+// queued in the wave's LDS ring across segments and processed kQCap at a
+// time, so a round is full however few candidates a segment has.
 // A segment's slots are stored when it is pushed; a round then rewrites the
 // particles that were candidates (after a fence on the wave's earlier stores:
 // their velocity after the collision and, with kDrift2, their second half
@@ -413,20 +425,19 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
         }
     };
     // Processes the m (1..kQCap) oldest queued candidates: a decision pass
-    // over two entries per lane (null ones finish there), then one pass per
+    // over kQEnt entries per lane (null ones finish there), then one pass per
     // collision type over that type's entries in queue order, compacted onto
     // the lanes, so each type's kinematics run once for up to 64 collisions
     // instead of once per 64 candidates of all types.
     auto round = [&](int m) {
         // the segments' stores (by other lanes) complete before this round rewrites them
         __threadfence_block();
-        int typ0 = 0, typ1 = 0;
         auto decide = [&](int e) -> int {
             if (e >= m) return 0;
             const int qi = (q_head + e) & (kQCap - 1);
             const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
             const float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
-            const uint32_t slot = __float_as_uint(Q[8][qi]);
+            const uint32_t slot = __float_as_uint(Q[kQSlotRow][qi]);
             const U4 R = philox4x32_10(slot, qca.step, qca.run, 0u, qca.key0, qca.key1);
             CollDecision d;
             collide_decide(g, qca, s_tab, px, py, pz, vx, vy, vz, Q[6][qi], u01(R.b), d);
@@ -436,23 +447,27 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 finish(slot, px, py, pz, vx, vy, vz, false);
             } else {   // the branch uniforms the kinematics pass reads
                 Q[6][qi] = u01(R.d);
-                Q[7][qi] = u01(R.c);
+                if constexpr (kQW == 9) Q[7][qi] = u01(R.c);
             }
             return d.typ;
         };
-        typ0 = decide(lane);
-        typ1 = decide(kWaveSize + lane);
+        int typ[kQEnt];
+#pragma unroll
+        for (int j = 0; j < kQEnt; ++j) typ[j] = decide(j * kWaveSize + lane);
         // per-type lists of entry indices in queue order: [0, n1) elastic,
         // [n1, n2) ionization, [n2, n3) excitation
         int lim[4];
         lim[0] = 0;
 #pragma unroll
         for (int t = 1; t <= 3; ++t) {
-            const unsigned long long ma = __ballot(typ0 == t), mb = __ballot(typ1 == t);
-            const int na = __popcll(ma);
-            if (typ0 == t) L[lim[t - 1] + __popcll(ma & below)] = (uint8_t)lane;
-            if (typ1 == t) L[lim[t - 1] + na + __popcll(mb & below)] = (uint8_t)(kWaveSize + lane);
-            lim[t] = lim[t - 1] + na + __popcll(mb);
+            int acc = lim[t - 1];
+#pragma unroll
+            for (int j = 0; j < kQEnt; ++j) {
+                const unsigned long long mj = __ballot(typ[j] == t);
+                if (typ[j] == t) L[acc + __popcll(mj & below)] = (uint8_t)(j * kWaveSize + lane);
+                acc += __popcll(mj);
+            }
+            lim[t] = acc;
         }
         wave_sync();
         // elastic
@@ -462,7 +477,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 const int qi = (q_head + L[k]) & (kQCap - 1);
                 const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
                 float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
-                const uint32_t slot = __float_as_uint(Q[8][qi]);
+                const uint32_t slot = __float_as_uint(Q[kQSlotRow][qi]);
                 float T, ve;
                 candidate_energy(g, qca, vx, vy, vz, T, ve);
                 collide_elastic(qca, s_tab, slot, T, Q[6][qi], vx, vy, vz);
@@ -477,7 +492,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 const int qi = (q_head + L[k]) & (kQCap - 1);
                 const float px = Q[0][qi], py = Q[1][qi], pz = Q[2][qi];
                 float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
-                const uint32_t slot = __float_as_uint(Q[8][qi]);
+                const uint32_t slot = __float_as_uint(Q[kQSlotRow][qi]);
                 float T, ve;
                 candidate_energy(g, qca, vx, vy, vz, T, ve);
                 collide_excite(g, qca, s_tab, T, ve, vx, vy, vz);
@@ -500,10 +515,16 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                 const int qi = (q_head + L[k]) & (kQCap - 1);
                 px = Q[0][qi]; py = Q[1][qi]; pz = Q[2][qi];
                 float vx = Q[3][qi], vy = Q[4][qi], vz = Q[5][qi];
-                const uint32_t slot = __float_as_uint(Q[8][qi]);
-                float T, ve;
+                const uint32_t slot = __float_as_uint(Q[kQSlotRow][qi]);
+                float T, ve, u3;
+                if constexpr (kQW == 9) {
+                    u3 = Q[7][qi];
+                } else {   // the decision's draw again
+                    const U4 R = philox4x32_10(slot, qca.step, qca.run, 0u, qca.key0, qca.key1);
+                    u3 = u01(R.c);
+                }
                 candidate_energy(g, qca, vx, vy, vz, T, ve);
-                collide_ionize(g, qca, slot, px, py, pz, T, ve, Q[7][qi], Q[6][qi], (F & kStageIon) != 0, vx, vy,
+                collide_ionize(g, qca, slot, px, py, pz, T, ve, u3, Q[6][qi], (F & kStageIon) != 0, vx, vy,
                                vz, co);
                 finish(slot, px, py, pz, vx, vy, vz, true);
                 lc.ei += 1u << 16;
@@ -735,7 +756,7 @@ __device__ __forceinline__ void particles_pass(const Geom& g, const PartArgs& pa
                                 Q[0][qi] = x[p]; Q[1][qi] = y[p]; Q[2][qi] = z[p];
                                 Q[3][qi] = vx[p]; Q[4][qi] = vy[p]; Q[5][qi] = vz[p];
                                 Q[6][qi] = u1[p];
-                                Q[8][qi] = __uint_as_float((uint32_t)(slot0 + p));
+                                Q[kQSlotRow][qi] = __uint_as_float((uint32_t)(slot0 + p));
                             }
                             ++pos;
                         }

@@ -8,9 +8,9 @@ tests here use populations above 3 * 2^18 slots, so its multi-group path
   velocity the collision gave it, and the appended block must list the
   parents in increasing slot order (electron_collision.m:68-75 appends the
   secondaries of the logical mask in index order);
-* gasx_run: the scan fused into the deposit launch (no ions) and the
-  two-launch scan (with ions, whose append shares the scan buffers) must
-  give the same electrons and fields bit for bit.
+* gasx_run with and without an ion species: the ions' append runs first and
+  shares the scan buffers with the electrons' own append, which must give
+  the same electrons and fields bit for bit.
 """
 import numpy as np
 import pytest
@@ -52,11 +52,11 @@ def test_collide_appends_in_parent_slot_order_across_scan_groups(gpu):
     assert parents.max() > 3 * 1024 * 256
 
 
-def test_fused_and_two_launch_append_scans_agree(gpu):
+def test_ion_append_leaves_the_electron_append_unchanged(gpu):
     import gasx
     from gasx.core import M_ION
     n, cap = 800_000, 1_000_000                           # 3907 segments: 4 scan groups
-    log(f"fused vs two-launch append scan n={n} cap={cap}")
+    log(f"run with and without ions n={n} cap={cap}")
     params = gasx.make_params(NR, NZ, fn=1e2)
     part = wedge_particles(n, NR, NZ, 1e-5, 1e-5, 0.1, 78, vmax=2.5e7, edge_frac=0.05)
     gas_d = np.full((1, 1), 1e26, np.float32)

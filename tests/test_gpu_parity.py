@@ -83,6 +83,35 @@ def test_fdtd_matches_oracle(gpu, nr, nz, n_push, kw):
     assert rel(gh, oh.reshape(gh.shape)) < 1e-12
 
 
+@pytest.mark.parametrize("nr,nz", [(64, 128), (256, 512)])
+def test_fdtd_steady_current_1000_pushes(gpu, nr, nz):
+    """1000 field cycles from rest under a steady Gaussian current, as the
+    field state is driven between two collision steps of the configurations:
+    each of Er, Ez and Hphi within 1e-5 (relative to its largest magnitude)
+    of the FP64 oracle, so rounding does not accumulate over a long run."""
+    import gasx
+    import torch
+    log(f"fdtd steady current nr={nr} nz={nz} 1000 pushes")
+    p, ctx, op = mk(nr, nz)
+    rc = (np.arange(nr) + 0.5) * p.dr
+    zc = (np.arange(nz) + 0.5) * p.dz
+    sig_r, sig_z = 0.15 * nr * p.dr, 0.1 * nz * p.dz
+    g = np.exp(-(rc[None, :] / sig_r) ** 2 - ((zc[:, None] - 0.4 * nz * p.dz) / sig_z) ** 2)
+    jz = 1e9 * g
+    jr = 2e8 * g * (rc[None, :] / sig_r)                  # zero on the axis
+    zero = [np.zeros((nz + 1, nr)), np.zeros((nz, nr + 1)), np.zeros((nz, nr))]
+    fs = gasx.FieldState(nr, nz)
+    load_fields(fs, *zero)
+    ctx.fdtd(fs, torch.from_numpy(jr).cuda(), torch.from_numpy(jz).cuda(), 1000)
+    got = fs.numpy()
+    want = oracle.fdtd_push(op, *zero, jr, jz, 1000)
+    errs = [rel(a, b.reshape(a.shape)) for a, b in zip(got, want)]
+    log(f"  relative error Er {errs[0]:.2e} Ez {errs[1]:.2e} Hphi {errs[2]:.2e}")
+    for b in want:
+        assert np.max(np.abs(b)) > 0
+    assert max(errs) < 1e-5
+
+
 def test_half_drift_matches_oracle(gpu):
     import gasx
     log("half drift")
