@@ -124,15 +124,31 @@ __device__ __forceinline__ void deposit_box(const Geom& g, float r, float z, int
     k0 = max(min(b - 1, g.nz - 1), 0); k1 = max(min(b + 2, g.nz - 1), 0);
 }
 
+// Minimum (kMin) or maximum of v over the wave, wave-uniform; every lane
+// active.  Within each row of 16 lanes by DPP (lane ^ 1, lane ^ 2, the
+// mirror in 8, the mirror in 16: each step pairs every lane with one of the
+// other half of its group), then across the four rows by reading lanes 0, 16,
+// 32 and 48 (no LDS round trips, as the cross-lane shuffles would take).
+template <bool kMin>
+__device__ __forceinline__ int wave_reduce(int v)
+{
+    auto op = [](int a, int b) { return kMin ? min(a, b) : max(a, b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));    // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));    // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));   // row_mirror
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
 // Wave-wide bounding box of [ilo,ihi] x [klo,khi], left in every lane.
 __device__ __forceinline__ void wave_box(int& ilo, int& ihi, int& klo, int& khi)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        ilo = min(ilo, __shfl_xor(ilo, o, 64));
-        ihi = max(ihi, __shfl_xor(ihi, o, 64));
-        klo = min(klo, __shfl_xor(klo, o, 64));
-        khi = max(khi, __shfl_xor(khi, o, 64));
-    }
+    ilo = wave_reduce<true>(ilo);
+    ihi = wave_reduce<false>(ihi);
+    klo = wave_reduce<true>(klo);
+    khi = wave_reduce<false>(khi);
 }
 
 // A box of at most `cells` cells, or none (nk == 0).
