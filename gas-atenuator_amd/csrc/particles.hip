@@ -67,16 +67,28 @@ __device__ __forceinline__ int wave_prefix_small(int v, int& total)
 
 constexpr int kPerBits = kPer <= 1 ? 1 : (kPer <= 3 ? 2 : (kPer <= 7 ? 3 : 4));
 
+// Inclusive prefix sum over the wave (lane order); every lane active.  DPP
+// within rows of 16 lanes (shifts by 1, 2, 4 and 8; lanes without a source
+// add 0), then the row totals: lane 15 of each row into the next row (rows 1
+// and 3), lane 31 into rows 2 and 3 -- no LDS round trips, as the cross-lane
+// shuffles would take.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);   // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);   // row_bcast:31, rows 2 and 3
+    return v;
+}
+
 // Exclusive prefix over the block of per-thread counts (thread order) and the
 // block total.  Every thread must call it (used once, before the main loop).
 __device__ __forceinline__ int block_excl_scan(int v, int* s_wave, int& total)
 {
     const int lane = lane_id(), wave = threadIdx.x >> 6;
-    int incl = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
+    const int incl = wave_incl_scan(v);
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     int off = 0, tot = 0;
@@ -1098,11 +1110,7 @@ __device__ int block_scan_counts(const int32_t* cnt, int32_t* offs, int n, int* 
             v[j] = b0 + j < n ? cnt[b0 + j] : 0;
             sum += v[j];
         }
-        int incl = sum;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int t = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += t;
-        }
+        const int incl = wave_incl_scan(sum);
         if (lane == 63) s_wtot[wave] = incl;
         __syncthreads();
         int woff = 0, tot = 0;
@@ -1143,11 +1151,7 @@ __global__ __launch_bounds__(kScanGroup) void k_append_scan(const int32_t* src_c
     const int c = blockIdx.x * kScanGroup + (int)threadIdx.x;
     const int v = c < nseg ? stage_cnt[c] : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int incl = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
+    const int incl = wave_incl_scan(v);
     if (lane == 63) s_wtot[wave] = incl;
     __syncthreads();
     int woff = 0, tot = 0;
