@@ -20,10 +20,11 @@ using namespace gasx_host;
 // object, held until the process ends.  Layout in 32-bit words: six FP32
 // secondary staging arrays and six ion staging arrays of kNaSlots each, the
 // per-segment secondary and ion counts, the append scan (segment offsets,
-// group totals) and the four append bases.  Every word a call reads is
-// written by that call first (each segment below the slot count gets its
-// count, the scan writes its offsets, totals and bases), so nothing needs
-// clearing between calls.
+// group totals) and the append's words.  Every word a call reads is written
+// by that call first (each segment below the slot count gets its count, the
+// scan writes its offsets, totals and bases), except the one-launch append's
+// four sync words, which are zero when the code object loads and which every
+// append leaves zero, so nothing needs clearing between calls.
 // ---------------------------------------------------------------------------
 namespace {
 constexpr int64_t kNaSlots = GASX_NOALLOC_MAX_SLOTS;
@@ -367,7 +368,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->jz_tmp, nh), "alloc jz");
         chk(dalloc(&ctx->d_counters, (size_t)kNCounters * kCounterSlots), "alloc counters");
         chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
-        chk(dalloc(&ctx->base_tmp, 4), "alloc base");
+        chk(dalloc(&ctx->base_tmp, 8), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
         chk(hipHostMalloc((void**)&ctx->h_count, kSortInfoWords * sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
         chk(hipEventCreateWithFlags(&ctx->sort_ev, hipEventDisableTiming), "sort event");
@@ -378,6 +379,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
             chk(hipMemset(ctx->er32, 0, ner * sizeof(float)), "zero er32");
             chk(hipMemset(ctx->ez32, 0, nez * sizeof(float)), "zero ez32");
             chk(hipMemset(ctx->h32, 0, nh * sizeof(float)), "zero h32");
+            chk(hipMemset(ctx->base_tmp, 0, 8 * sizeof(int32_t)), "zero append words");
         }
     }
     if (st != GASX_OK) {
@@ -474,7 +476,8 @@ struct Staging {
     float* ion[6];
     int32_t* ion_cnt;
     int32_t* scan_tmp;    // max_segs segment offsets, then the group totals
-    int32_t* base_tmp;    // [0..1] secondary append, [2..3] ion append
+    int32_t* base_tmp;    // [0..1] secondary append, [2..3] ion append (two-launch path);
+                          // [4..5], [6..7] their one-launch sync words (zero between calls)
     int32_t max_segs;
 };
 
@@ -715,7 +718,8 @@ static gasx_status append_from_stage(gasx_ctx* ctx, const Staging& stg, const ga
     TimedScope ts(ctx, kTAppend);
     HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? stg.ion : stg.sec,
                          ion ? stg.ion_cnt : stg.sec_cnt, src->count, stg.scan_tmp,
-                         stg.base_tmp + (ion ? 2 : 0), stg.max_segs,
+                         stg.base_tmp + (ion ? 2 : 0),
+                         reinterpret_cast<uint32_t*>(stg.base_tmp + 4 + (ion ? 2 : 0)), stg.max_segs,
                          &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->stream),
            "append");
     return GASX_OK;

@@ -228,11 +228,14 @@ hipError_t launch_particles(uint32_t flags, const Geom& g, const PartArgs& pa, c
 // chunks.
 hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom& g, const PartArgs& pa,
                                 int blocks, hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-// Appends what a particle launch staged (scan, then copy into dst in parent
-// slot order); the appended slots are not deposited here.
+// Appends what a particle launch staged (copy into dst in parent slot order:
+// one launch up to 32768 staging segments when sync is given, else a scan
+// launch and a copy launch); the appended slots are not deposited here.
+// tmp2: two words of the two-launch path; sync: two words of the one-launch
+// path, zero before the first call (every one-launch call leaves them zero).
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
-                         const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
+                         const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2, uint32_t* sync,
                          int32_t max_segs, unsigned long long* dropped, hipStream_t s);
 // batched append: per-run scan over the chunks recorded in chunk_start, then copy
 hipError_t launch_append_batch(float* const pool[6], int32_t* counts, const BatchRun* runs, int nruns,

@@ -1265,12 +1265,115 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
     }
 }
 
-// scan_tmp: max_segs segment offsets followed by the group totals
+// The append in one launch, for at most kOneMaxSegs staging segments: the
+// kOneSplit workgroups of a batch of kBlock segments read the counts of all
+// earlier segments themselves (at most kOneMaxSegs words, from L2) for the
+// batch's first destination, so no scan launch precedes the copy.  The
+// destination count is updated by the workgroup that finishes last: each
+// workgroup takes a ticket (sync[0]) after every one of its threads has used
+// the old count, and the batch of the last staged segment publishes the total
+// (sync[1]) before its ticket.  The sync words are shared only through
+// device-scope atomics (no fence: an agent-scope fence writes the L2 back);
+// the copies and the count reach the next launch at the kernel boundary.
+// The last workgroup resets both words, so the launch leaves them as it found
+// them (zero before the first launch).
+constexpr int kOneMaxSegs = 32768;
+#ifndef GASX_ONE_SPLIT
+#define GASX_ONE_SPLIT 4
+#endif
+constexpr int kOneSplit = GASX_ONE_SPLIT;
+
+__global__ __launch_bounds__(kBlock) void k_append_one(float* d0, float* d1, float* d2, float* d3, float* d4,
+                                                       float* d5, int32_t cap, int32_t* dst_count,
+                                                       const float* s0, const float* s1, const float* s2,
+                                                       const float* s3, const float* s4, const float* s5,
+                                                       const int32_t* stage_cnt, const int32_t* src_count,
+                                                       int32_t max_segs, uint32_t* sync,
+                                                       unsigned long long* dropped)
+{
+    __shared__ int s_seg[kBlock + 1];
+    __shared__ int s_wave[kBlock / 64];
+    const int tid = (int)threadIdx.x;
+    const int base = *dst_count;
+    const int nseg = (int)min((int64_t)max_segs, ((int64_t)*src_count + kSeg - 1) / kSeg);
+    const int bt = (int)blockIdx.x / kOneSplit, part = (int)blockIdx.x % kOneSplit;
+    const int c0 = bt * kBlock;
+    const bool live = c0 < nseg;                   // workgroup-uniform
+    // staged entries of the segments before the batch (all below nseg),
+    // sixteen loads in flight per thread (two rounds at kOneMaxSegs)
+    int before = 0;
+    if (live) {
+        if ((reinterpret_cast<uintptr_t>(stage_cnt) & 15) == 0) {
+            constexpr int kU = 16;
+            const int4* c4 = reinterpret_cast<const int4*>(stage_cnt);
+            const int nw = c0 / 4;
+            for (int i0 = 0; i0 < nw; i0 += kU * kBlock) {
+                int4 v[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const int i = i0 + u * kBlock + tid;
+                    v[u] = i < nw ? c4[i] : make_int4(0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) before += v[u].x + v[u].y + v[u].z + v[u].w;
+            }
+        } else {
+            for (int i = tid; i < c0; i += kBlock) before += stage_cnt[i];
+        }
+    }
+    const int cnt = live && c0 + tid < nseg ? stage_cnt[c0 + tid] : 0;
+    int first;
+    block_excl_scan(before, s_wave, first);
+    int tot;
+    const int pre = block_excl_scan(cnt, s_wave, tot);
+    s_seg[tid] = pre;
+    if (tid == 0) s_seg[kBlock] = tot;
+    __syncthreads();
+    first += base;
+    if (live && first < cap) {
+        for (int e = part * kBlock + tid; e < tot; e += kBlock * kOneSplit) {
+            const int dst = first + e;
+            if (dst >= cap) break;
+            const int j = batch_segment(s_seg, e);
+            const int src = (c0 + j) * kSeg + (e - s_seg[j]);
+            d0[dst] = s0[src]; d1[dst] = s1[src]; d2[dst] = s2[src];
+            d3[dst] = s3[src]; d4[dst] = s4[src]; d5[dst] = s5[src];
+        }
+    }
+    __syncthreads();                               // every thread has used the old count
+    if (tid == 0) {
+        if (live && part == 0 && c0 + kBlock >= nseg) {
+            const uint32_t prev = atomicExch(&sync[1], (uint32_t)(first - base + tot));
+            asm volatile("" ::"v"(prev));          // performed before the ticket is taken
+        }
+        const uint32_t ticket = atomicAdd(&sync[0], 1u);
+        if (ticket == gridDim.x - 1) {             // every workgroup has read the old count
+            const int total = nseg > 0 ? (int)atomicOr(&sync[1], 0u) : 0;
+            const int room = cap > base ? cap - base : 0;
+            const int app = total < room ? total : room;
+            *dst_count = base + app;
+            if (total > app && dropped) atomicAdd(dropped, (unsigned long long)(total - app));
+            atomicExch(&sync[1], 0u);
+            atomicExch(&sync[0], 0u);
+        }
+    }
+}
+
+// scan_tmp: max_segs segment offsets followed by the group totals; tmp2: two
+// words of the two-launch path; sync: two words of the one-launch path, zero
+// before the first append and left zero by every one-launch append
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
-                         const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2,
+                         const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2, uint32_t* sync,
                          int32_t max_segs, unsigned long long* dropped, hipStream_t s)
 {
+    if (sync && max_segs >= 1 && max_segs <= kOneMaxSegs) {
+        const int blocks = (max_segs + kBlock - 1) / kBlock * kOneSplit;
+        hipLaunchKernelGGL(k_append_one, dim3(blocks), dim3(kBlock), 0, s, dst[0], dst[1], dst[2], dst[3], dst[4],
+                           dst[5], cap, dst_count, stage[0], stage[1], stage[2], stage[3], stage[4], stage[5],
+                           stage_cnt, src_count, max_segs, sync, dropped);
+        return hipGetLastError();
+    }
     const int ngroups = (max_segs + kScanGroup - 1) / kScanGroup;
     if (ngroups < 1 || ngroups > kMaxScanGroups) return hipErrorInvalidValue;
     int32_t* totals = scan_tmp + max_segs;
