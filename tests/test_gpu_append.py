@@ -1,8 +1,9 @@
 """Secondary append at production-like capacities (GPU).
 
-The append scan works in groups of 1024 staging segments (2^18 slots); the
-tests here use populations above 3 * 2^18 slots, so its multi-group path
-(group totals prefixed across groups) runs:
+Up to 32768 staging segments (8.4e6 slots) the append is one launch whose
+workgroups sum the counts of all earlier segments; above, a scan launch in
+groups of 1024 segments (2^18 slots) and a copy launch.  The tests here use
+populations above 3 * 2^18 slots, and the order test runs both paths:
 
 * gasx_collide: every secondary is placed at its parent's position with the
   velocity the collision gave it, and the appended block must list the
@@ -23,9 +24,10 @@ pytestmark = pytest.mark.gpu
 NR, NZ = 32, 64
 
 
-def test_collide_appends_in_parent_slot_order_across_scan_groups(gpu):
+@pytest.mark.parametrize("cap", [1_200_000, 9_000_000])   # one launch; scan + copy (35157 segments)
+def test_collide_appends_in_parent_slot_order_across_scan_groups(gpu, cap):
     import gasx
-    n, cap = 900_000, 1_200_000                            # 3516 staging segments: 4 groups
+    n = 900_000                                             # 3516 staging segments: 4 groups
     log(f"append order n={n} cap={cap}")
     params = gasx.make_params(NR, NZ, fn=1e3)
     ctx = gasx.Context(params)
