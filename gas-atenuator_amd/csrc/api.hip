@@ -35,7 +35,7 @@ constexpr int64_t kNaStageCnt = 12 * kNaSlots;            // word offsets
 constexpr int64_t kNaIonCnt = kNaStageCnt + kNaSegs;
 constexpr int64_t kNaScan = kNaIonCnt + kNaSegs;
 constexpr int64_t kNaBase = kNaScan + kNaScanWords;
-constexpr int64_t kNaWords = kNaBase + 64;
+constexpr int64_t kNaWords = kNaBase + 4 + 2 * kAppendSyncWords;
 constexpr int kNaMaxDevices = 64;
 }  // namespace
 
@@ -368,7 +368,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
         chk(dalloc(&ctx->jz_tmp, nh), "alloc jz");
         chk(dalloc(&ctx->d_counters, (size_t)kNCounters * kCounterSlots), "alloc counters");
         chk(dalloc(&ctx->scan_tmp, 1), "alloc scan");
-        chk(dalloc(&ctx->base_tmp, 8), "alloc base");
+        chk(dalloc(&ctx->base_tmp, 4 + 2 * kAppendSyncWords), "alloc base");
         chk(dalloc(&ctx->d_tally, 4 * 256 + 8), "alloc tally");
         chk(hipHostMalloc((void**)&ctx->h_count, kSortInfoWords * sizeof(int32_t), hipHostMallocDefault), "alloc pinned count");
         chk(hipEventCreateWithFlags(&ctx->sort_ev, hipEventDisableTiming), "sort event");
@@ -379,7 +379,7 @@ extern "C" gasx_status gasx_ctx_create(int device, const gasx_params* p, const g
             chk(hipMemset(ctx->er32, 0, ner * sizeof(float)), "zero er32");
             chk(hipMemset(ctx->ez32, 0, nez * sizeof(float)), "zero ez32");
             chk(hipMemset(ctx->h32, 0, nh * sizeof(float)), "zero h32");
-            chk(hipMemset(ctx->base_tmp, 0, 8 * sizeof(int32_t)), "zero append words");
+            chk(hipMemset(ctx->base_tmp, 0, (4 + 2 * kAppendSyncWords) * sizeof(int32_t)), "zero append words");
         }
     }
     if (st != GASX_OK) {
@@ -477,7 +477,8 @@ struct Staging {
     int32_t* ion_cnt;
     int32_t* scan_tmp;    // max_segs segment offsets, then the group totals
     int32_t* base_tmp;    // [0..1] secondary append, [2..3] ion append (two-launch path);
-                          // [4..5], [6..7] their one-launch sync words (zero between calls)
+                          // then their one-launch sync words, kAppendSyncWords each (zero
+                          // between calls)
     int32_t max_segs;
 };
 
@@ -719,7 +720,7 @@ static gasx_status append_from_stage(gasx_ctx* ctx, const Staging& stg, const ga
     HIPCHK(launch_append(d, dst->count, (int32_t)dst->capacity, ion ? stg.ion : stg.sec,
                          ion ? stg.ion_cnt : stg.sec_cnt, src->count, stg.scan_tmp,
                          stg.base_tmp + (ion ? 2 : 0),
-                         reinterpret_cast<uint32_t*>(stg.base_tmp + 4 + (ion ? 2 : 0)), stg.max_segs,
+                         reinterpret_cast<uint32_t*>(stg.base_tmp + 4 + (ion ? kAppendSyncWords : 0)), stg.max_segs,
                          &ctx->d_counters[ion ? kCDropIon : kCDropSec], ctx->stream),
            "append");
     return GASX_OK;

@@ -231,8 +231,10 @@ hipError_t launch_drift_deposit(bool drift, bool deposit, bool batch, const Geom
 // Appends what a particle launch staged (copy into dst in parent slot order:
 // one launch up to 32768 staging segments when sync is given, else a scan
 // launch and a copy launch); the appended slots are not deposited here.
-// tmp2: two words of the two-launch path; sync: two words of the one-launch
-// path, zero before the first call (every one-launch call leaves them zero).
+// tmp2: two words of the two-launch path; sync: kAppendSyncWords words of the
+// one-launch path, zero before the first call (every one-launch call leaves
+// them zero).
+constexpr int kAppendSyncWords = 2 + 32768 / 256;
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2, uint32_t* sync,

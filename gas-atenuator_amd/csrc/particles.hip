@@ -1269,15 +1269,18 @@ __global__ __launch_bounds__(kBlock) void k_append_copy(float* d0, float* d1, fl
 // kOneSplit workgroups of a batch of kBlock segments read the counts of all
 // earlier segments themselves (at most kOneMaxSegs words, from L2) for the
 // batch's first destination, so no scan launch precedes the copy.  The
-// destination count is updated by the workgroup that finishes last: each
-// workgroup takes a ticket (sync[0]) after every one of its threads has used
-// the old count, and the batch of the last staged segment publishes the total
-// (sync[1]) before its ticket.  The sync words are shared only through
-// device-scope atomics (no fence: an agent-scope fence writes the L2 back);
-// the copies and the count reach the next launch at the kernel boundary.
-// The last workgroup resets both words, so the launch leaves them as it found
-// them (zero before the first launch).
+// destination count is updated by the workgroup that finishes last, found by
+// tickets in two levels: every workgroup takes its batch's ticket (sync[2 +
+// batch]) after all its threads have used the old count, the batch's last
+// takes the launch's ticket (sync[0]), and the batch of the last staged
+// segment publishes the total (sync[1]) before its ticket.  The sync words are
+// shared only through device-scope atomics (no fence: an agent-scope fence
+// writes the L2 back); the copies and the count reach the next launch at the
+// kernel boundary.  Each word is reset by the last to take it, so the launch
+// leaves them as it found them (zero before the first launch).
 constexpr int kOneMaxSegs = 32768;
+constexpr int kOneSyncWords = 2 + kOneMaxSegs / kBlock;
+static_assert(kOneSyncWords == kAppendSyncWords, "the callers' sync words");
 #ifndef GASX_ONE_SPLIT
 #define GASX_ONE_SPLIT 4
 #endif
@@ -1299,6 +1302,7 @@ __global__ __launch_bounds__(kBlock) void k_append_one(float* d0, float* d1, flo
     const int bt = (int)blockIdx.x / kOneSplit, part = (int)blockIdx.x % kOneSplit;
     const int c0 = bt * kBlock;
     const bool live = c0 < nseg;                   // workgroup-uniform
+    const bool last_batch = live && c0 + kBlock >= nseg;
     // staged entries of the segments before the batch (all below nseg),
     // sixteen loads in flight per thread (two rounds at kOneMaxSegs)
     int before = 0;
@@ -1322,9 +1326,8 @@ __global__ __launch_bounds__(kBlock) void k_append_one(float* d0, float* d1, flo
         }
     }
     const int cnt = live && c0 + tid < nseg ? stage_cnt[c0 + tid] : 0;
-    int first;
+    int first, tot;
     block_excl_scan(before, s_wave, first);
-    int tot;
     const int pre = block_excl_scan(cnt, s_wave, tot);
     s_seg[tid] = pre;
     if (tid == 0) s_seg[kBlock] = tot;
@@ -1342,12 +1345,14 @@ __global__ __launch_bounds__(kBlock) void k_append_one(float* d0, float* d1, flo
     }
     __syncthreads();                               // every thread has used the old count
     if (tid == 0) {
-        if (live && part == 0 && c0 + kBlock >= nseg) {
+        if (part == 0 && last_batch) {
             const uint32_t prev = atomicExch(&sync[1], (uint32_t)(first - base + tot));
             asm volatile("" ::"v"(prev));          // performed before the ticket is taken
         }
+        if (atomicAdd(&sync[2 + bt], 1u) != kOneSplit - 1) return;   // not the batch's last
+        atomicExch(&sync[2 + bt], 0u);
         const uint32_t ticket = atomicAdd(&sync[0], 1u);
-        if (ticket == gridDim.x - 1) {             // every workgroup has read the old count
+        if (ticket == gridDim.x / kOneSplit - 1) {  // every workgroup has read the old count
             const int total = nseg > 0 ? (int)atomicOr(&sync[1], 0u) : 0;
             const int room = cap > base ? cap - base : 0;
             const int app = total < room ? total : room;
@@ -1360,8 +1365,8 @@ __global__ __launch_bounds__(kBlock) void k_append_one(float* d0, float* d1, flo
 }
 
 // scan_tmp: max_segs segment offsets followed by the group totals; tmp2: two
-// words of the two-launch path; sync: two words of the one-launch path, zero
-// before the first append and left zero by every one-launch append
+// words of the two-launch path; sync: kOneSyncWords words of the one-launch
+// path, zero before the first append and left zero by every one-launch append
 hipError_t launch_append(float* const dst[6], int32_t* dst_count, int32_t cap,
                          float* const stage[6], const int32_t* stage_cnt,
                          const int32_t* src_count, int32_t* scan_tmp, int32_t* tmp2, uint32_t* sync,
