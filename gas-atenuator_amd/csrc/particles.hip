@@ -236,7 +236,7 @@ __device__ __forceinline__ void run_add(const TileBox& b, const DepRun& d, int n
 }
 
 #ifndef GASX_GATHER_BATCH
-#define GASX_GATHER_BATCH 1
+#define GASX_GATHER_BATCH 2
 #endif
 // particles of a lane whose field reads are in flight together (divides kPer)
 constexpr int kGatherBatch = GASX_GATHER_BATCH;
